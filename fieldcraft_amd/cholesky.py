@@ -1,0 +1,349 @@
+"""The solver crate's factor/solve API (crates/solver/src/cholesky.rs) over the C ABI of
+``libfc_chol.so`` (include/fc_chol.h): the symbolic phase on the host, the numeric phase and the
+solves on one MI355X.
+
+Names, argument meaning and error behaviour follow the reference:
+
+* ``UpperColumns`` (cholesky.rs:99-145) — the upper triangle by columns, ``from_symmetric``.
+* ``FactorError`` and its kinds (cholesky.rs:346-361) — raised, never a silent fallback.
+* ``Symbolic`` — ``Symbolic::new`` (cholesky.rs:1213): analyse once, factor many
+  (cholesky.rs:1390-1395); exports the reference's symbolic outputs for parity checks.
+* ``Factor`` — ``Factor::new`` / ``from_symbolic`` / ``entries`` / ``solve`` / ``solve_many``
+  (cholesky.rs:1532-1570), plus ``solve_refined`` (FP64 refinement, ldlt.rs:524-563 made
+  general) and ``values`` (the reference's trapezoid layout).
+
+Nothing here computes on the CPU: every numeric call goes to the HIP library, and a missing
+library raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _build
+
+FP64, MIXED, FP32 = 0, 1, 2
+PRECISIONS = {"fp64": FP64, "mixed": MIXED, "fp32": FP32}
+
+_OK, _NPD, _SINGULAR, _TOO_LARGE, _OOM, _INTERNAL, _CANCELLED, _INVALID, _DEVICE = range(9)
+
+
+class FactorError(Exception):
+    """Why a factorization or solve stopped (FactorError, cholesky.rs:346-361)."""
+
+    kind = "internal"
+
+    def __init__(self, message: str = "", column: int | None = None, entries: int | None = None,
+                 limit: int | None = None):
+        self.column, self.entries, self.limit = column, entries, limit
+        super().__init__(message)
+
+
+class NotPositiveDefinite(FactorError):
+    kind = "not_positive_definite"
+
+
+class Singular(FactorError):
+    kind = "singular"
+
+
+class TooLarge(FactorError):
+    kind = "too_large"
+
+
+class OutOfMemory(FactorError):
+    kind = "out_of_memory"
+
+
+class Internal(FactorError):
+    kind = "internal"
+
+
+class Cancelled(FactorError):
+    kind = "cancelled"
+
+
+class InvalidArgument(FactorError):
+    kind = "invalid_argument"
+
+
+class DeviceError(FactorError):
+    kind = "device"
+
+
+_KINDS = {_NPD: NotPositiveDefinite, _SINGULAR: Singular, _TOO_LARGE: TooLarge, _OOM: OutOfMemory,
+          _INTERNAL: Internal, _CANCELLED: Cancelled, _INVALID: InvalidArgument, _DEVICE: DeviceError}
+
+
+class _Error(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("pad", C.c_int32), ("column", C.c_uint64), ("entries", C.c_uint64),
+                ("limit", C.c_uint64), ("message", C.c_char * 256)]
+
+
+class _Opts(C.Structure):
+    _fields_ = [("precision", C.c_int32), ("device", C.c_int32), ("stream", C.c_void_p),
+                ("cancel", C.POINTER(C.c_int32)), ("bf16_terms", C.c_int32), ("use_graph", C.c_int32),
+                ("pivot_fallback", C.c_double)]
+
+
+class _Info(C.Structure):
+    _fields_ = [(k, C.c_uint64) for k in ["n", "supernodes", "entries", "rows", "updates", "tasks", "top", "fronts",
+                                           "levels", "stored"]] + \
+               [("flops", C.c_double), ("front_flops", C.c_double), ("l_bytes_fp64", C.c_uint64),
+                ("u_bytes_fp64", C.c_uint64), ("by_blocks", C.c_int32), ("pad", C.c_int32)]
+
+
+class _RefineOpts(C.Structure):
+    _fields_ = [("method", C.c_int32), ("max_iter", C.c_int32), ("target_berr", C.c_double)]
+
+
+class _RefineReport(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("converged", C.c_int32), ("berr", C.c_double), ("seconds", C.c_double),
+                ("berr_history", C.c_double * 64)]
+
+
+_lib = None
+_u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+
+#: every symbol include/fc_chol.h declares
+EXPORTS = ["fc_analyze", "fc_symbolic_info_get", "fc_symbolic_export", "fc_symbolic_free", "fc_factor_numeric",
+           "fc_factor_new", "fc_refactor", "fc_entries", "fc_solve", "fc_solve_many", "fc_solve_device",
+           "fc_factor_export", "fc_factor_seconds", "fc_factor_free", "fc_matrix_new", "fc_matrix_free",
+           "fc_solve_refined", "fc_version"]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        L = C.CDLL(_build.chol_library())
+        ep = C.POINTER(_Error)
+        vp = C.c_void_p
+        L.fc_analyze.argtypes = [C.c_uint64, _u64p, _u64p, C.c_uint64, C.POINTER(vp), ep]
+        L.fc_symbolic_info_get.argtypes = [vp, C.POINTER(_Info)]
+        L.fc_symbolic_export.argtypes = [vp] + [vp] * 8
+        L.fc_symbolic_free.argtypes = [vp]
+        L.fc_symbolic_free.restype = None
+        L.fc_factor_numeric.argtypes = [vp, _f64p, C.c_uint64, C.POINTER(_Opts), C.POINTER(vp), ep]
+        L.fc_refactor.argtypes = [vp, _f64p, C.c_uint64, ep]
+        L.fc_entries.argtypes = [vp]
+        L.fc_entries.restype = C.c_uint64
+        L.fc_solve.argtypes = [vp, _f64p, C.c_uint64, ep]
+        L.fc_solve_many.argtypes = [vp, C.POINTER(C.c_void_p), C.c_uint64, C.c_uint64, ep]
+        L.fc_solve_device.argtypes = [vp, vp, C.c_uint64, C.c_uint64, C.c_uint64, ep]
+        L.fc_factor_export.argtypes = [vp, _f64p, C.c_uint64, ep]
+        L.fc_factor_seconds.argtypes = [vp]
+        L.fc_factor_seconds.restype = C.c_double
+        L.fc_factor_free.argtypes = [vp]
+        L.fc_factor_free.restype = None
+        L.fc_matrix_new.argtypes = [C.c_uint64, _u64p, _u64p, _f64p, C.c_int32, C.POINTER(vp), ep]
+        L.fc_matrix_free.argtypes = [vp]
+        L.fc_matrix_free.restype = None
+        L.fc_solve_refined.argtypes = [vp, vp, _f64p, _f64p, C.c_uint64, C.POINTER(_RefineOpts),
+                                       C.POINTER(_RefineReport), ep]
+        L.fc_version.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def _raise(status: int, err: _Error):
+    if status == _OK:
+        return
+    cls = _KINDS.get(status, Internal)
+    msg = err.message.decode(errors="replace")
+    raise cls(msg or cls.kind, column=int(err.column), entries=int(err.entries), limit=int(err.limit))
+
+
+def _u64(a):
+    a = np.asarray(a)
+    return np.ascontiguousarray(a, dtype=np.uint64) if a.size else np.zeros(1, np.uint64)
+
+
+@dataclass
+class UpperColumns:
+    """The upper triangle by columns: column k holds A(i, k) for i <= k, rows sorted
+    (cholesky.rs:99-108)."""
+
+    n: int
+    ptr: np.ndarray
+    idx: np.ndarray
+    val: np.ndarray
+
+    @staticmethod
+    def from_symmetric(n: int, ptr, idx, val) -> "UpperColumns":
+        """From a symmetric matrix by full rows: row k's entries with column <= k are column k
+        (cholesky.rs:112-144)."""
+        ptr = np.asarray(ptr, np.int64)
+        idx = np.asarray(idx, np.int64)
+        val = np.asarray(val, np.float64)
+        rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(ptr))
+        keep = idx <= rows
+        counts = np.bincount(rows[keep], minlength=n)
+        uptr = np.zeros(n + 1, np.int64)
+        np.cumsum(counts, out=uptr[1:])
+        return UpperColumns(n, uptr, np.ascontiguousarray(idx[keep]), np.ascontiguousarray(val[keep]))
+
+
+class Symbolic:
+    """Symbolic::new (cholesky.rs:1213-1378) on the host, through fc_analyze."""
+
+    def __init__(self, a: UpperColumns, max_entries: int = 2 ** 63 - 1):
+        L = lib()
+        self.n = int(a.n)
+        self._h = C.c_void_p()
+        err = _Error()
+        st = L.fc_analyze(self.n, _u64(a.ptr), _u64(a.idx), int(max_entries), C.byref(self._h), C.byref(err))
+        if st != _OK:
+            self._h = None
+            _raise(st, err)
+        self.nnz = int(a.ptr[-1]) if self.n else 0
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().fc_symbolic_free(self._h)
+            self._h = None
+
+    def info(self) -> dict:
+        i = _Info()
+        lib().fc_symbolic_info_get(self._h, C.byref(i))
+        return {k: getattr(i, k) for k, _ in _Info._fields_ if k != "pad"}
+
+    def export(self) -> dict:
+        """The reference's symbolic outputs: perm, first, rptr, rows, uptr, updates (k, p1, p2),
+        tasks ([start, end)) and top."""
+        i = self.info()
+        n, ns = i["n"], i["supernodes"]
+        out = dict(perm=np.zeros(max(n, 1), np.uint64), first=np.zeros(ns + 1, np.uint64),
+                   rptr=np.zeros(ns + 1, np.uint64), rows=np.zeros(max(i["rows"], 1), np.uint64),
+                   uptr=np.zeros(ns + 1, np.uint64), updates=np.zeros(max(3 * i["updates"], 1), np.uint64),
+                   tasks=np.zeros(max(2 * i["tasks"], 1), np.uint64), top=np.zeros(max(i["top"], 1), np.uint64))
+        keys = ["perm", "first", "rptr", "rows", "uptr", "updates", "tasks", "top"]
+        lib().fc_symbolic_export(self._h, *[out[k].ctypes.data for k in keys])
+        res = {k: out[k].astype(np.int64) for k in keys}
+        res["perm"] = res["perm"][:n]
+        res["rows"] = res["rows"][: i["rows"]]
+        res["updates"] = res["updates"][: 3 * i["updates"]].reshape(-1, 3)
+        res["tasks"] = res["tasks"][: 2 * i["tasks"]].reshape(-1, 2)
+        res["top"] = res["top"][: i["top"]]
+        res["entries"] = i["entries"]
+        return res
+
+
+class Matrix:
+    """The matrix by full rows on the device, for FP64 residuals (Csr, sparse.rs)."""
+
+    def __init__(self, n: int, ptr, idx, val, device: int = 0):
+        self.n = int(n)
+        self._h = C.c_void_p()
+        err = _Error()
+        st = lib().fc_matrix_new(self.n, _u64(ptr), _u64(idx), np.ascontiguousarray(val, np.float64), int(device),
+                                 C.byref(self._h), C.byref(err))
+        if st != _OK:
+            self._h = None
+            _raise(st, err)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().fc_matrix_free(self._h)
+            self._h = None
+
+
+class Factor:
+    """A Cholesky factor on the GPU (Factor, cholesky.rs:1532-1570)."""
+
+    def __init__(self, symbolic: Symbolic, val, precision: str | int = "fp64", device: int = 0,
+                 bf16_terms: int = 1, cancel: C.c_int32 | None = None, pivot_fallback: float = 0.0,
+                 use_graph: bool = False):
+        L = lib()
+        self.symbolic = symbolic
+        self.n = symbolic.n
+        self.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        self._cancel = cancel
+        opts = _Opts(self.precision, int(device), None, C.pointer(cancel) if cancel is not None else None,
+                     int(bf16_terms), int(use_graph), float(pivot_fallback))
+        self._h = C.c_void_p()
+        err = _Error()
+        val = np.ascontiguousarray(val, np.float64)
+        st = L.fc_factor_numeric(symbolic._h, val if val.size else np.zeros(1), val.size, C.byref(opts),
+                                 C.byref(self._h), C.byref(err))
+        if st != _OK:
+            self._h = None
+            _raise(st, err)
+
+    @staticmethod
+    def new(a: UpperColumns, max_entries: int = 2 ** 63 - 1, **kw) -> "Factor":
+        """Factor::new (cholesky.rs:1545-1547)."""
+        return Factor(Symbolic(a, max_entries), a.val, **kw)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().fc_factor_free(self._h)
+            self._h = None
+
+    def refactor(self, val) -> None:
+        err = _Error()
+        val = np.ascontiguousarray(val, np.float64)
+        st = lib().fc_refactor(self._h, val if val.size else np.zeros(1), val.size, C.byref(err))
+        _raise(st, err)
+
+    def entries(self) -> int:
+        """Factor::entries (cholesky.rs:1557-1559)."""
+        return int(lib().fc_entries(self._h))
+
+    def seconds(self) -> float:
+        return float(lib().fc_factor_seconds(self._h))
+
+    def solve(self, b) -> np.ndarray:
+        """Factor::solve (cholesky.rs:1562-1564); returns x (b is not modified)."""
+        x = np.array(b, dtype=np.float64, copy=True)
+        err = _Error()
+        st = lib().fc_solve(self._h, x if x.size else np.zeros(1), x.shape[0], C.byref(err))
+        _raise(st, err)
+        return x
+
+    def solve_many(self, bs) -> list:
+        """Factor::solve_many (cholesky.rs:1568-1570): each the same bit for bit as solve."""
+        xs = [np.array(b, dtype=np.float64, copy=True) for b in bs]
+        n = xs[0].shape[0] if xs else self.n
+        ptrs = (C.c_void_p * max(len(xs), 1))(*[x.ctypes.data for x in xs])
+        err = _Error()
+        st = lib().fc_solve_many(self._h, ptrs, len(xs), n, C.byref(err))
+        _raise(st, err)
+        return xs
+
+    def solve_device(self, b_ptr: int, nrhs: int, ldb: int | None = None) -> None:
+        """Solve in place a column-major block of nrhs right-hand sides in device memory."""
+        err = _Error()
+        st = lib().fc_solve_device(self._h, C.c_void_p(b_ptr), self.n, nrhs, ldb or self.n, C.byref(err))
+        _raise(st, err)
+
+    def values(self) -> np.ndarray:
+        """The factor's values in the reference layout (trapezoids, cholesky.rs:393-420)."""
+        e = self.entries()
+        out = np.zeros(max(e, 1))
+        err = _Error()
+        st = lib().fc_factor_export(self._h, out, e, C.byref(err))
+        _raise(st, err)
+        return out[:e]
+
+    def solve_refined(self, matrix: Matrix, b, method: str = "auto", max_iter: int = 50,
+                      target_berr: float = 1e-12):
+        """x with A x = b to normwise backward error <= target_berr (FP64 residuals on the
+        device). Returns (x, report)."""
+        m = {"refine": 0, "cg": 1, "auto": 2}[method]
+        opts = _RefineOpts(m, int(max_iter), float(target_berr))
+        rep = _RefineReport()
+        b = np.ascontiguousarray(b, np.float64)
+        x = np.zeros_like(b)
+        err = _Error()
+        st = lib().fc_solve_refined(self._h, matrix._h, b, x, b.shape[0], C.byref(opts), C.byref(rep), C.byref(err))
+        _raise(st, err)
+        report = dict(iterations=rep.iterations, converged=bool(rep.converged), berr=rep.berr, seconds=rep.seconds,
+                      berr_history=list(rep.berr_history[: min(rep.iterations + 1, 64)]))
+        return x, report
+
+
+def version() -> str:
+    return lib().fc_version().decode()

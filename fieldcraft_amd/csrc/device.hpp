@@ -1,0 +1,52 @@
+// Device-side data of the numeric phase: front descriptors and work items, shared by the
+// factor and solve kernels (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace fc {
+
+// One front as the kernels see it. L part: M x W column-major at lpos with leading dimension
+// ld (rows 0..W-1 are the front's own columns, rows W..M-1 the rows below). Update matrix:
+// (M-W) x (M-W) column-major at upos with leading dimension M-W (lower triangle used).
+struct FrontDev {
+  int64_t lpos;
+  int64_t upos;
+  int64_t rel_off;   // relative map of the rows below into the parent front
+  int64_t rows_off;  // front rows (postorder column numbers) in the rows array
+  int32_t ld, w, m, col0;
+  int32_t parent, kid0, kid1, level;  // children are kids[kid0..kid1)
+};
+
+// A unit of work of one launch; the meaning of a, b, c, d depends on the kernel.
+struct Item {
+  int32_t f, a, b, c, d, e, pad0, pad1;
+};
+
+enum Kind : int32_t {
+  K_EXTEND_L = 0,  // children's update matrices into the L part: a = first column, b = end
+  K_EXTEND_U = 1,  // ... into the update matrix (after the Schur product): a, b columns
+  K_UPDATE = 2,    // panel update L[i0.., c0..c1) -= L[i0.., 0..c0) L[c0..c1, 0..c0)^T
+  K_POTRF = 3,     // diagonal block of a panel
+  K_TRSM = 4,      // rows below a panel against its diagonal block
+  K_SYRK = 5,      // update matrix U = -L21 L21^T
+  K_SMALL = 6,     // a whole small front in one workgroup
+};
+
+struct Launch {
+  int32_t kind;
+  int32_t level;
+  int64_t item0;  // first item in the item array
+  int32_t count;  // items (= workgroups)
+  int32_t pad;
+};
+
+constexpr int PANEL = 64;      // columns of a panel (the reference's PANEL, cholesky.rs:55)
+constexpr int TILE_M = 64;     // rows of a GEMM tile
+constexpr int TILE_N = 64;     // columns of a GEMM tile
+constexpr int EA_COLS = 8;     // columns of the parent one extend-add workgroup owns
+constexpr int TRSM_ROWS = 64;  // rows of a TRSM tile
+
+}  // namespace fc

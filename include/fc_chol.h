@@ -1,0 +1,181 @@
+/*
+ * fc_chol.h — C ABI of the MI355X supernodal Cholesky (numeric phase on one GPU, gfx950).
+ *
+ * The entry points mirror the factor/solve API of FieldCraft's solver crate
+ * (crates/solver/src/cholesky.rs), so a host program that calls Factor::new / solve /
+ * solve_many can call these instead. Plain pointers and sizes only; the library owns device
+ * memory, the caller owns every array it passes. Every entry point returns an fc_status and
+ * fills *err (when err is not NULL); none aborts.
+ *
+ * Matrix input is the reference's UpperColumns (cholesky.rs:99-108): column k holds the rows
+ * i <= k of the upper triangle, sorted, with 64-bit ptr/idx. Column numbers in errors are the
+ * caller's original columns, as FactorError names them (cholesky.rs:1506-1513).
+ */
+#ifndef FC_CHOL_H
+#define FC_CHOL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* FactorError (cholesky.rs:346-361), plus argument and device failures. */
+typedef enum {
+  FC_OK = 0,
+  FC_NOT_POSITIVE_DEFINITE = 1, /* FactorError::NotPositiveDefinite { column }            */
+  FC_SINGULAR = 2,              /* FactorError::Singular { column } (LDL^T only)            */
+  FC_TOO_LARGE = 3,             /* FactorError::TooLarge { entries, limit }                 */
+  FC_OUT_OF_MEMORY = 4,         /* FactorError::OutOfMemory { entries }                     */
+  FC_INTERNAL = 5,              /* FactorError::Internal(String)                            */
+  FC_CANCELLED = 6,             /* FactorError::Cancelled                                   */
+  FC_INVALID_ARGUMENT = 7,      /* a malformed pattern, length or option                    */
+  FC_DEVICE = 8                 /* a HIP runtime error                                      */
+} fc_status;
+
+typedef struct {
+  int32_t kind;      /* fc_status */
+  int32_t pad;
+  uint64_t column;   /* failing original column (NOT_POSITIVE_DEFINITE, SINGULAR)      */
+  uint64_t entries;  /* entries of L (TOO_LARGE, OUT_OF_MEMORY)                         */
+  uint64_t limit;    /* the cap (TOO_LARGE)                                             */
+  char message[256];
+} fc_error;
+
+typedef struct fc_symbolic fc_symbolic;
+typedef struct fc_factor fc_factor;
+typedef struct fc_matrix fc_matrix;
+
+/* Arithmetic of the numeric phase. */
+typedef enum {
+  FC_FP64 = 0,  /* the plain method: every step in FP64 on the FP64 matrix cores           */
+  FC_MIXED = 1, /* Schur updates in BF16 (power-of-two scaled, FP32 accumulation), diagonal
+                   blocks and TRSM in FP32 (FP64 where a scaled pivot is below threshold);
+                   the factor is stored in FP32; solutions are refined in FP64            */
+  FC_FP32 = 2   /* every step in FP32, factor stored in FP32                               */
+} fc_precision;
+
+typedef struct {
+  int32_t precision;              /* fc_precision                                           */
+  int32_t device;                 /* HIP device ordinal                                     */
+  void* stream;                   /* hipStream_t to run on; NULL: the library's own stream  */
+  const volatile int32_t* cancel; /* polled between levels of the front tree; NULL: none    */
+  int32_t bf16_terms;             /* FC_MIXED: 1 = one BF16 product per operand pair,
+                                     3 = split FP32 operands (hi*hi + hi*lo + lo*hi)       */
+  int32_t use_graph;              /* replay the level-ordered launches from a hipGraph      */
+  double pivot_fallback;          /* FC_MIXED: relative pivot below which a diagonal block
+                                     is refactored in FP64 (0: the default, 1e-4)           */
+} fc_factor_opts;
+
+typedef struct {
+  uint64_t n, supernodes, entries, rows, updates, tasks, top;
+  uint64_t fronts, levels, stored; /* execution plan: fronts, tree levels, stored L entries  */
+  double flops;                    /* sum over supernodes of sum_{c<w} (m - c)^2            */
+  double front_flops;              /* the same over fronts (padding rows included)          */
+  uint64_t l_bytes_fp64, u_bytes_fp64; /* device storage of L and of the update arena      */
+  int32_t by_blocks;               /* analysed on the 3x3 node blocks' graph                */
+  int32_t pad;
+} fc_symbolic_info;
+
+/* --- analysis ---------------------------------------------------------------------------- */
+
+/* Symbolic::new (cholesky.rs:1213-1378): elimination tree, column counts, postorder,
+ * fundamental supernodes, row structures, update lists and subtree tasks, on the host.
+ * Refuses with FC_TOO_LARGE before planning any numeric memory when the factor would hold
+ * more than max_entries entries (cholesky.rs:1216-1219). */
+int fc_analyze(uint64_t n, const uint64_t* ptr, const uint64_t* idx, uint64_t max_entries, fc_symbolic** out,
+               fc_error* err);
+
+int fc_symbolic_info_get(const fc_symbolic* s, fc_symbolic_info* info);
+
+/* The reference's symbolic outputs (postorder numbering; arrays sized by fc_symbolic_info):
+ * perm[n] (cholesky.rs:1197), first[ns+1], rptr[ns+1], rows[rows] (Structure, :392-397),
+ * uptr[ns+1] + updates[3*updates] as {k, p1, p2} (Update, :382-389), tasks[2*tasks] as
+ * [start, end) and top[top] (:1203-1205). Any pointer may be NULL. */
+int fc_symbolic_export(const fc_symbolic* s, uint64_t* perm, uint64_t* first, uint64_t* rptr, uint64_t* rows,
+                       uint64_t* uptr, uint64_t* updates, uint64_t* tasks, uint64_t* top);
+
+void fc_symbolic_free(fc_symbolic* s);
+
+/* --- numeric phase ----------------------------------------------------------------------- */
+
+/* Factor::from_symbolic (cholesky.rs:1549-1555): the numeric factorization on the GPU of a
+ * matrix with the analysed pattern (values of the upper columns, nnz = ptr[n]). Fails with
+ * FC_NOT_POSITIVE_DEFINITE naming the first failing column in postorder, translated to the
+ * original column (cholesky.rs:1506-1528), with FC_OUT_OF_MEMORY when device memory is short,
+ * and with FC_CANCELLED when *opts->cancel is set. */
+int fc_factor_numeric(const fc_symbolic* s, const double* upper_val, uint64_t nnz, const fc_factor_opts* opts,
+                      fc_factor** out, fc_error* err);
+
+/* Factor::new (cholesky.rs:1545-1547): fc_analyze then fc_factor_numeric. */
+int fc_factor_new(uint64_t n, const uint64_t* ptr, const uint64_t* idx, const double* val, uint64_t max_entries,
+                  const fc_factor_opts* opts, fc_factor** out, fc_error* err);
+
+/* The same factor's values refactored for a new matrix of the same pattern (one Symbolic
+ * serving many numeric factorizations, cholesky.rs:1390-1395). */
+int fc_refactor(fc_factor* f, const double* upper_val, uint64_t nnz, fc_error* err);
+
+/* Factor::entries (cholesky.rs:1557-1559). */
+uint64_t fc_entries(const fc_factor* f);
+
+/* Factor::solve (cholesky.rs:1562-1564): A x = b in place, host memory, FP64 result. An
+ * FC_MIXED or FC_FP32 factor solves with its FP32 factor (no refinement: fc_solve_refined). */
+int fc_solve(const fc_factor* f, double* b, uint64_t n, fc_error* err);
+
+/* Factor::solve_many (cholesky.rs:1568-1570): each of bs[0..nrhs) solved in place, each the
+ * same bit for bit as fc_solve. */
+int fc_solve_many(const fc_factor* f, double* const* bs, uint64_t nrhs, uint64_t n, fc_error* err);
+
+/* Device flavour: nrhs right-hand sides in device memory, column-major with leading dimension
+ * ldb, solved in place on the factor's stream. */
+int fc_solve_device(const fc_factor* f, double* b, uint64_t n, uint64_t nrhs, uint64_t ldb, fc_error* err);
+
+/* The factor values in the reference layout (cholesky.rs:393-420): every supernode's lower
+ * trapezoid, column by column (column c holds rows c..m of its row structure), supernodes in
+ * order; `out` holds `entries` doubles. For parity checks. */
+int fc_factor_export(const fc_factor* f, double* out, uint64_t len, fc_error* err);
+
+/* Seconds the last numeric factorization took on the device (events around the launches). */
+double fc_factor_seconds(const fc_factor* f);
+
+void fc_factor_free(fc_factor* f);
+
+/* --- refinement -------------------------------------------------------------------------- */
+
+/* The matrix by full rows (Csr, crates/solver/src/sparse.rs), uploaded once for the FP64
+ * residuals of refinement (Csr::mul, sparse.rs:42-50). */
+int fc_matrix_new(uint64_t n, const uint64_t* ptr, const uint64_t* idx, const double* val, int32_t device,
+                  fc_matrix** out, fc_error* err);
+void fc_matrix_free(fc_matrix* a);
+
+typedef struct {
+  int32_t method;        /* 0: refinement x += solve(b - A x); 1: conjugate gradients
+                            preconditioned by the factor; 2: 0, switching to 1 when a step
+                            does not shrink the backward error                                */
+  int32_t max_iter;      /* 0: 50                                                            */
+  double target_berr;    /* 0: 1e-12; berr = ||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf) */
+} fc_refine_opts;
+
+typedef struct {
+  int32_t iterations;    /* solves after the first                                         */
+  int32_t converged;
+  double berr;           /* final normwise backward error                                   */
+  double seconds;        /* device time of the refined solve                                */
+  double berr_history[64];
+} fc_refine_report;
+
+/* Symmetric::solve_refined (ldlt.rs:524-563) made general: x solves A x = b to the target
+ * normwise backward error, with FP64 residuals A x on the device. b and x are host arrays
+ * of n doubles (x may alias b). */
+int fc_solve_refined(const fc_factor* f, const fc_matrix* a, const double* b, double* x, uint64_t n,
+                     const fc_refine_opts* opts, fc_refine_report* report, fc_error* err);
+
+/* Library version string. */
+const char* fc_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FC_CHOL_H */

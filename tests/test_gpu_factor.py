@@ -1,0 +1,119 @@
+"""GPU parity of the numeric phase and solves against the oracle (the C restatement of
+cholesky.rs), through the C ABI. Tolerances: FP64 factor values and solutions within 1e-12
+(relative, normwise) of the oracle's; the reference's own solve-error bounds where it states
+them."""
+import numpy as np
+import pytest
+
+from fieldcraft_amd import cholesky as CH
+from fieldcraft_amd import problems as P
+from oracle import oracle as O
+
+from matrices import dense_to_csr, grid_7x5, random_spd, singular_laplacian, two_grids_and_separator, upper
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-12  # FP64 parity bound: factor values and solutions vs the oracle
+
+
+def both(n, ptr, idx, val, precision="fp64"):
+    up = upper(n, ptr, idx, val)
+    ref = O.Factor(n, *up)
+    sym = CH.Symbolic(CH.UpperColumns(n, *up))
+    f = CH.Factor(sym, up[2], precision=precision)
+    return ref, f, up
+
+
+def rel(a, b):
+    d = np.linalg.norm(np.asarray(a) - np.asarray(b))
+    s = np.linalg.norm(np.asarray(b))
+    return d / s if s > 0 else d
+
+
+def check_solve(n, ptr, idx, val, f, tol):
+    """check_solve (cholesky.rs:1680-1689): x = sin(0.37 i), b = A x, max |solve(b) - x| < tol."""
+    x = np.sin(np.arange(n) * 0.37)
+    b = O.csr_mul(n, ptr, idx, val, x)
+    err = np.abs(f.solve(b) - x).max()
+    assert err < tol, err
+
+
+def test_grid_7x5(gpu):
+    n, ptr, idx, val = dense_to_csr(grid_7x5())
+    ref, f, _ = both(n, ptr, idx, val)
+    check_solve(n, ptr, idx, val, f, 1e-13)
+    assert rel(f.values(), ref.values()) <= REL
+    assert f.entries() == ref.entries()
+
+
+def test_one_by_one_and_empty(gpu):
+    n, ptr, idx, val = dense_to_csr([[4.0]])
+    _, f, _ = both(n, ptr, idx, val)
+    assert f.solve(np.array([2.0])).tolist() == [0.5]  # cholesky.rs:1734-1737
+    up = (np.zeros(1, np.int64), np.zeros(0, np.int64), np.zeros(0))
+    sym = CH.Symbolic(CH.UpperColumns(0, *up), max_entries=10)
+    e = CH.Factor(sym, up[2])
+    assert e.entries() == 0
+    assert e.solve(np.zeros(0)).shape == (0,)
+
+
+def test_not_positive_definite_names_the_column(gpu):
+    n, ptr, idx, val = dense_to_csr(singular_laplacian())
+    up = upper(n, ptr, idx, val)
+    with pytest.raises(CH.NotPositiveDefinite) as e:
+        CH.Factor.new(CH.UpperColumns(n, *up))
+    assert e.value.column == n - 1  # cholesky.rs:1714-1720
+    with pytest.raises(O.OracleError) as o:
+        O.Factor(n, *up)
+    assert o.value.column == e.value.column
+
+
+def test_two_grids_and_dense_separator(gpu):
+    n, ptr, idx, val = two_grids_and_separator()
+    ref, f, _ = both(n, ptr, idx, val)
+    check_solve(n, ptr, idx, val, f, 1e-12)  # cholesky.rs:1815
+    assert rel(f.values(), ref.values()) <= REL
+    rhs = [np.sin((np.arange(n) * 7 + s * 13) * 0.11) for s in range(5)]
+    alone = [f.solve(b) for b in rhs]
+    together = f.solve_many(rhs)
+    for a, t in zip(alone, together):
+        assert np.array_equal(a.view(np.uint64), t.view(np.uint64))  # cholesky.rs:1816-1828
+    for a, b in zip(alone, rhs):
+        assert rel(a, ref.solve(b)) <= REL
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_random_spd_against_dense(gpu, seed):
+    a = random_spd(120, 0.05, seed)
+    n, ptr, idx, val = dense_to_csr(a)
+    ref, f, _ = both(n, ptr, idx, val)
+    b = np.cos(np.arange(n) * 0.3)
+    x = f.solve(b)
+    assert rel(x, np.linalg.solve(a, b)) <= 1e-12
+    assert rel(x, ref.solve(b)) <= REL
+    assert rel(f.values(), ref.values()) <= REL
+
+
+def test_cantilever_hex8_matches_the_oracle(gpu):
+    p = P.cantilever_hex8()
+    ref, f, up = both(p.n, p.ptr, p.idx, p.kval)
+    assert rel(f.values(), ref.values()) <= REL
+    x, xr = f.solve(p.rhs), ref.solve(p.rhs)
+    assert rel(x, xr) <= REL
+    r = p.mul(x) - p.rhs
+    berr = np.abs(r).max() / (p.norm_inf() * np.abs(x).max() + np.abs(p.rhs).max())
+    assert berr < 1e-14
+
+
+def test_refined_solve_reaches_the_target(gpu):
+    p = P.cantilever_hex8()
+    up = p.upper()
+    sym = CH.Symbolic(CH.UpperColumns(p.n, *up))
+    A = CH.Matrix(p.n, p.ptr, p.idx, p.kval)
+    for prec in ["fp64", "fp32"]:
+        f = CH.Factor(sym, up[2], precision=prec)
+        x, rep = f.solve_refined(A, p.rhs, target_berr=1e-12)
+        assert rep["converged"], rep
+        r = p.mul(x) - p.rhs
+        berr = np.abs(r).max() / (p.norm_inf() * np.abs(x).max() + np.abs(p.rhs).max())
+        assert berr <= 1e-12, (prec, berr, rep)
