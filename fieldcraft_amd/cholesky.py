@@ -85,14 +85,15 @@ class _Error(C.Structure):
 class _Opts(C.Structure):
     _fields_ = [("precision", C.c_int32), ("device", C.c_int32), ("stream", C.c_void_p),
                 ("cancel", C.POINTER(C.c_int32)), ("bf16_terms", C.c_int32), ("use_graph", C.c_int32),
-                ("pivot_fallback", C.c_double)]
+                ("pivot_fallback", C.c_double), ("profile_mask", C.c_int32), ("pad", C.c_int32)]
 
 
 class _Info(C.Structure):
     _fields_ = [(k, C.c_uint64) for k in ["n", "supernodes", "entries", "rows", "updates", "tasks", "top", "fronts",
                                            "levels", "stored"]] + \
                [("flops", C.c_double), ("front_flops", C.c_double), ("l_bytes_fp64", C.c_uint64),
-                ("u_bytes_fp64", C.c_uint64), ("by_blocks", C.c_int32), ("pad", C.c_int32)]
+                ("u_bytes_fp64", C.c_uint64), ("by_blocks", C.c_int32), ("pad", C.c_int32),
+                ("launches", C.c_uint64), ("kind_flops", C.c_double * 8), ("kind_entries", C.c_double * 8)]
 
 
 class _RefineOpts(C.Structure):
@@ -110,9 +111,13 @@ _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 
 #: every symbol include/fc_chol.h declares
 EXPORTS = ["fc_analyze", "fc_symbolic_info_get", "fc_symbolic_export", "fc_symbolic_free", "fc_factor_numeric",
-           "fc_factor_new", "fc_refactor", "fc_entries", "fc_solve", "fc_solve_many", "fc_solve_device",
-           "fc_factor_export", "fc_factor_seconds", "fc_factor_free", "fc_matrix_new", "fc_matrix_free",
-           "fc_solve_refined", "fc_version"]
+           "fc_factor_numeric_device", "fc_factor_new", "fc_refactor", "fc_refactor_device", "fc_entries", "fc_solve",
+           "fc_solve_many", "fc_solve_device", "fc_factor_export", "fc_factor_seconds", "fc_factor_profile",
+           "fc_factor_free", "fc_matrix_new", "fc_matrix_free", "fc_solve_refined", "fc_solve_refined_device",
+           "fc_version"]
+
+#: kernel kinds of the numeric phase (FC_KIND_* in include/fc_chol.h)
+KINDS = ["extend_l", "extend_u", "update", "potrf", "trsm", "syrk", "reduce"]
 
 
 def lib():
@@ -127,7 +132,10 @@ def lib():
         L.fc_symbolic_free.argtypes = [vp]
         L.fc_symbolic_free.restype = None
         L.fc_factor_numeric.argtypes = [vp, _f64p, C.c_uint64, C.POINTER(_Opts), C.POINTER(vp), ep]
+        L.fc_factor_numeric_device.argtypes = [vp, vp, C.c_uint64, C.POINTER(_Opts), C.POINTER(vp), ep]
         L.fc_refactor.argtypes = [vp, _f64p, C.c_uint64, ep]
+        L.fc_refactor_device.argtypes = [vp, vp, C.c_uint64, ep]
+        L.fc_factor_profile.argtypes = [vp, C.c_int32] + [C.POINTER(C.c_double)] * 4
         L.fc_entries.argtypes = [vp]
         L.fc_entries.restype = C.c_uint64
         L.fc_solve.argtypes = [vp, _f64p, C.c_uint64, ep]
@@ -143,6 +151,8 @@ def lib():
         L.fc_matrix_free.restype = None
         L.fc_solve_refined.argtypes = [vp, vp, _f64p, _f64p, C.c_uint64, C.POINTER(_RefineOpts),
                                        C.POINTER(_RefineReport), ep]
+        L.fc_solve_refined_device.argtypes = [vp, vp, vp, vp, C.c_uint64, C.POINTER(_RefineOpts),
+                                              C.POINTER(_RefineReport), ep]
         L.fc_version.restype = C.c_char_p
         _lib = L
     return _lib
@@ -208,7 +218,10 @@ class Symbolic:
     def info(self) -> dict:
         i = _Info()
         lib().fc_symbolic_info_get(self._h, C.byref(i))
-        return {k: getattr(i, k) for k, _ in _Info._fields_ if k != "pad"}
+        out = {k: getattr(i, k) for k, _ in _Info._fields_ if k not in ("pad", "kind_flops", "kind_entries")}
+        out["kind_flops"] = {KINDS[k]: i.kind_flops[k] for k in range(len(KINDS))}
+        out["kind_entries"] = {KINDS[k]: i.kind_entries[k] for k in range(len(KINDS))}
+        return out
 
     def export(self) -> dict:
         """The reference's symbolic outputs: perm, first, rptr, rows, uptr, updates (k, p1, p2),
@@ -253,21 +266,32 @@ class Matrix:
 class Factor:
     """A Cholesky factor on the GPU (Factor, cholesky.rs:1532-1570)."""
 
-    def __init__(self, symbolic: Symbolic, val, precision: str | int = "fp64", device: int = 0,
+    def __init__(self, symbolic: Symbolic, val=None, precision: str | int = "fp64", device: int = 0,
                  bf16_terms: int = 1, cancel: C.c_int32 | None = None, pivot_fallback: float = 0.0,
-                 use_graph: bool = False):
+                 use_graph: bool = False, profile: tuple = (), device_values: int | None = None,
+                 stream: int | None = None):
+        """Factor the matrix with values `val` (host array) or `device_values` (a device
+        pointer to the upper columns' values, nnz doubles)."""
         L = lib()
         self.symbolic = symbolic
         self.n = symbolic.n
         self.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         self._cancel = cancel
-        opts = _Opts(self.precision, int(device), None, C.pointer(cancel) if cancel is not None else None,
-                     int(bf16_terms), int(use_graph), float(pivot_fallback))
+        mask = 0
+        for k in profile:
+            mask |= 1 << (KINDS.index(k) if isinstance(k, str) else int(k))
+        opts = _Opts(self.precision, int(device), C.c_void_p(stream) if stream else None,
+                     C.pointer(cancel) if cancel is not None else None, int(bf16_terms), int(use_graph),
+                     float(pivot_fallback), mask, 0)
         self._h = C.c_void_p()
         err = _Error()
-        val = np.ascontiguousarray(val, np.float64)
-        st = L.fc_factor_numeric(symbolic._h, val if val.size else np.zeros(1), val.size, C.byref(opts),
-                                 C.byref(self._h), C.byref(err))
+        if device_values is not None:
+            st = L.fc_factor_numeric_device(symbolic._h, C.c_void_p(device_values), symbolic.nnz, C.byref(opts),
+                                            C.byref(self._h), C.byref(err))
+        else:
+            val = np.ascontiguousarray(val, np.float64)
+            st = L.fc_factor_numeric(symbolic._h, val if val.size else np.zeros(1), val.size, C.byref(opts),
+                                     C.byref(self._h), C.byref(err))
         if st != _OK:
             self._h = None
             _raise(st, err)
@@ -287,6 +311,19 @@ class Factor:
         val = np.ascontiguousarray(val, np.float64)
         st = lib().fc_refactor(self._h, val if val.size else np.zeros(1), val.size, C.byref(err))
         _raise(st, err)
+
+    def refactor_device(self, device_values: int) -> None:
+        err = _Error()
+        st = lib().fc_refactor_device(self._h, C.c_void_p(device_values), self.symbolic.nnz, C.byref(err))
+        _raise(st, err)
+
+    def profile(self, kind) -> dict:
+        """Kernel kind `kind` in the last factorization: launches timed, their device seconds, and
+        the kind's algorithmic flops and extend-add entries."""
+        k = KINDS.index(kind) if isinstance(kind, str) else int(kind)
+        v = [C.c_double() for _ in range(4)]
+        lib().fc_factor_profile(self._h, k, *[C.byref(x) for x in v])
+        return dict(launches=v[0].value, seconds=v[1].value, flops=v[2].value, entries=v[3].value)
 
     def entries(self) -> int:
         """Factor::entries (cholesky.rs:1557-1559)."""
@@ -343,6 +380,20 @@ class Factor:
         report = dict(iterations=rep.iterations, converged=bool(rep.converged), berr=rep.berr, seconds=rep.seconds,
                       berr_history=list(rep.berr_history[: min(rep.iterations + 1, 64)]))
         return x, report
+
+
+    def solve_refined_device(self, matrix: Matrix, b_ptr: int, x_ptr: int, method: str = "auto",
+                             max_iter: int = 50, target_berr: float = 1e-12) -> dict:
+        """Refined solve with b and x in device memory; returns the report."""
+        m = {"refine": 0, "cg": 1, "auto": 2}[method]
+        opts = _RefineOpts(m, int(max_iter), float(target_berr))
+        rep = _RefineReport()
+        err = _Error()
+        st = lib().fc_solve_refined_device(self._h, matrix._h, C.c_void_p(b_ptr), C.c_void_p(x_ptr), self.n,
+                                           C.byref(opts), C.byref(rep), C.byref(err))
+        _raise(st, err)
+        return dict(iterations=rep.iterations, converged=bool(rep.converged), berr=rep.berr, seconds=rep.seconds,
+                    berr_history=list(rep.berr_history[: min(rep.iterations + 1, 64)]))
 
 
 def version() -> str:

@@ -32,7 +32,7 @@ enum Kind : int32_t {
   K_POTRF = 3,     // diagonal block of a panel
   K_TRSM = 4,      // rows below a panel against its diagonal block
   K_SYRK = 5,      // update matrix U = -L21 L21^T
-  K_SMALL = 6,     // a whole small front in one workgroup
+  K_REDUCE = 6,    // sums of the pieces of a GEMM split along k
 };
 
 struct Launch {
@@ -41,7 +41,11 @@ struct Launch {
   int64_t item0;  // first item in the item array
   int32_t count;  // items (= workgroups)
   int32_t pad;
+  double flops;   // algorithmic flops of the launch (GEMM-like kinds)
+  double entries; // algorithmic entries moved (extend-add: child entries added)
 };
+
+constexpr int KINDS = 7;
 
 constexpr int PANEL = 64;      // columns of a panel (the reference's PANEL, cholesky.rs:55)
 constexpr int TILE_M = 64;     // rows of a GEMM tile

@@ -3,7 +3,6 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -13,8 +12,9 @@
 #include <string>
 #include <vector>
 
-#include "fc_chol.h"
 #include "factor_kernels.hpp"
+#include "fc_chol.h"
+#include "gemm_kernels.hpp"
 #include "solve_kernels.hpp"
 #include "symbolic.hpp"
 
@@ -29,10 +29,10 @@ struct DeviceError {
   const char* what;
 };
 
-#define HIP_CHECK(x)                                   \
-  do {                                                 \
-    hipError_t e_ = (x);                               \
-    if (e_ != hipSuccess) throw DeviceError{e_, #x};   \
+#define HIP_CHECK(x)                                 \
+  do {                                               \
+    hipError_t e_ = (x);                             \
+    if (e_ != hipSuccess) throw DeviceError{e_, #x}; \
   } while (0)
 
 template <typename T>
@@ -58,6 +58,28 @@ void dfree(void* p) {
   if (p) (void)hipFree(p);
 }
 
+// A device buffer that grows on demand and is kept (workspaces reused across calls).
+struct Buffer {
+  void* p = nullptr;
+  size_t bytes = 0;
+  void* get(size_t need) {
+    if (need == 0) need = 1;
+    if (need > bytes) {
+      dfree(p);
+      p = nullptr;
+      bytes = 0;
+      p = (void*)dalloc<char>(need);
+      bytes = need;
+    }
+    return p;
+  }
+  void release() {
+    dfree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+};
+
 void set_error(fc_error* err, int kind, const std::string& msg, uint64_t column = 0, uint64_t entries = 0,
                uint64_t limit = 0) {
   if (!err) return;
@@ -71,22 +93,105 @@ void set_error(fc_error* err, int kind, const std::string& msg, uint64_t column 
 
 // The launches of the numeric phase and of the solves, built once per analysis.
 struct Program {
-  std::vector<fc::Item> items;
+  std::vector<fc::Item> items;         // extend-add and diagonal-block items
+  std::vector<fc::GemmItem> gitems;    // GEMM tiles (UPDATE, TRSM, SYRK launches)
+  std::vector<fc::ReduceItem> ritems;  // split-K reductions
   std::vector<fc::Launch> launches;
-  std::vector<int32_t> solve_fronts;   // fronts by level
-  std::vector<i64> solve_lvl;          // level starts in solve_fronts
-  std::vector<i64> uoff;               // per front: start of its solve contribution
-  i64 u_solve = 0;                     // total contribution length (one right-hand side)
+  i64 inv_slots = 0;                   // diagonal-block inverses one launch keeps
+  i64 part_size = 0;                   // entries of the split-K workspace
+  std::vector<fc::Item> sitems;       // solve work items
+  std::vector<fc::Launch> sfwd, sbwd; // forward launches (leaves first), backward (root first)
+  std::vector<i64> uoff;              // per front: start of its solve contribution
+  i64 u_solve = 0;                    // total contribution length (one right-hand side)
+  double work[fc::KINDS][2] = {};     // per kind: flops, entries
 };
+
+// kinds of solve launches
+enum SolveKind : int32_t { S_GATHER = 0, S_FWD_BLOCK = 1, S_FWD_GEMV = 2, S_BWD_BELOW = 3, S_BWD_BLOCK = 4, S_BWD_GEMV = 5 };
+
+// lower pairs (i >= j) of rows [r0, r1) x columns [c0, c1)
+double lower_pairs(i64 r0, i64 r1, i64 c0, i64 c1) {
+  double s = 0;
+  for (i64 j = c0; j < c1; ++j) s += (double)std::max<i64>(0, r1 - std::max(r0, j));
+  return s;
+}
+
+// An output tile of a GEMM launch with its algorithmic flops.
+struct Tile {
+  fc::GemmItem g;
+  double flops;
+};
+
+// Output tiles a launch should have to fill the GPU (256 CUs, two workgroups each, and some
+// slack); a launch with fewer is split along k into pieces of at least SPLIT_MIN_K.
+constexpr i64 SPLIT_TARGET = 1024;
+constexpr i64 SPLIT_MIN_K = 512;
+
+// One GEMM launch of `tiles` (kind: UPDATE, TRSM or SYRK), split along k when the tiles are
+// too few, followed by the launch that reduces the pieces.
+void add_gemm(Program& p, const std::vector<Tile>& tiles, int kind, i64 level) {
+  if (tiles.empty()) return;
+  const i64 nt = (i64)tiles.size();
+  const i64 g0 = (i64)p.gitems.size(), r0 = (i64)p.ritems.size();
+  double flops = 0;
+  i64 part = 0;
+  for (const Tile& t : tiles) {
+    flops += t.flops;
+    i64 split = 1;
+    if (nt < SPLIT_TARGET && t.g.k >= 2 * SPLIT_MIN_K)
+      split = std::min<i64>((SPLIT_TARGET + nt - 1) / nt, t.g.k / SPLIT_MIN_K);
+    if (split <= 1) {
+      p.gitems.push_back(t.g);
+      continue;
+    }
+    i64 chunk = (t.g.k + split - 1) / split;
+    chunk = (chunk + fc::GK - 1) / fc::GK * fc::GK;
+    split = (t.g.k + chunk - 1) / chunk;
+    fc::ReduceItem r{};
+    r.c = t.g.c;
+    r.part0 = part;
+    r.ldc = t.g.ldc;
+    r.m = t.g.m;
+    r.n = t.g.n;
+    r.mode = t.g.mode;
+    r.diag = t.g.diag;
+    r.parts = (int32_t)split;
+    r.csrc = t.g.csrc;
+    p.ritems.push_back(r);
+    for (i64 q = 0; q < split; ++q) {
+      fc::GemmItem g = t.g;
+      g.a += q * chunk * (i64)t.g.lda;
+      g.b += q * chunk * (i64)t.g.ldb;
+      g.k = (int32_t)std::min<i64>(chunk, t.g.k - q * chunk);
+      g.mode = fc::G_PART;
+      g.c = part + q * (i64)fc::GB * fc::GB;
+      g.csrc = fc::S_PART;
+      p.gitems.push_back(g);
+    }
+    part += split * (i64)fc::GB * fc::GB;
+  }
+  p.launches.push_back(fc::Launch{kind, (int32_t)level, g0, (int32_t)((i64)p.gitems.size() - g0), 0, flops, 0.0});
+  p.work[kind][0] += flops;
+  if ((i64)p.ritems.size() > r0)
+    p.launches.push_back(fc::Launch{fc::K_REDUCE, (int32_t)level, r0, (int32_t)((i64)p.ritems.size() - r0), 0, 0.0, 0.0});
+  p.part_size = std::max(p.part_size, part);
+}
 
 Program build_program(const fc::Symbolic& s) {
   using fc::Item;
   using fc::Launch;
   Program p;
+  double flops = 0, entries = 0;
   auto add = [&](int kind, i64 level, i64 item0) {
     const i64 count = (i64)p.items.size() - item0;
-    if (count > 0) p.launches.push_back(Launch{kind, (int32_t)level, item0, (int32_t)count, 0});
+    if (count > 0) {
+      p.launches.push_back(Launch{kind, (int32_t)level, item0, (int32_t)count, 0, flops, entries});
+      p.work[kind][0] += flops;
+      p.work[kind][1] += entries;
+    }
+    flops = entries = 0;
   };
+  std::vector<double> colent;  // child entries landing in each parent column
   for (i64 l = 0; l < s.levels; ++l) {
     const i64* fl = &s.lvl_fronts[s.lvl_ptr[l]];
     const i64 nl = s.lvl_ptr[l + 1] - s.lvl_ptr[l];
@@ -95,69 +200,189 @@ Program build_program(const fc::Symbolic& s) {
         if (s.f_m[s.f_kids[q]] > s.f_w[s.f_kids[q]]) return true;
       return false;
     };
+    auto column_entries = [&](i64 f) {
+      colent.assign(s.f_m[f], 0.0);
+      for (i64 q = s.f_kptr[f]; q < s.f_kptr[f + 1]; ++q) {
+        const i64 c = s.f_kids[q], cmu = s.f_m[c] - s.f_w[c];
+        for (i64 jc = 0; jc < cmu; ++jc) colent[s.rel[s.f_rel_off[c] + jc]] += (double)(cmu - jc);
+      }
+    };
     // extend-add into the L parts
     i64 i0 = (i64)p.items.size();
     for (i64 t = 0; t < nl; ++t) {
       const i64 f = fl[t];
       if (!has_kids_below(f)) continue;
-      for (i64 a = 0; a < s.f_w[f]; a += fc::EA_COLS)
-        p.items.push_back(Item{(int32_t)f, (int32_t)a, (int32_t)std::min(a + fc::EA_COLS, s.f_w[f]), 0, 0, 0, 0, 0});
+      column_entries(f);
+      for (i64 a = 0; a < s.f_w[f]; a += fc::EA_COLS) {
+        const i64 b = std::min(a + fc::EA_COLS, s.f_w[f]);
+        p.items.push_back(Item{(int32_t)f, (int32_t)a, (int32_t)b, 0, 0, 0, 0, 0});
+        for (i64 j = a; j < b; ++j) entries += colent[j];
+      }
     }
     add(fc::K_EXTEND_L, l, i0);
-    // panels
+    // panels of NB columns, left-looking inside each front (cholesky.rs:1138-1186)
     i64 wmax = 0;
     for (i64 t = 0; t < nl; ++t) wmax = std::max(wmax, s.f_w[fl[t]]);
-    for (i64 c0 = 0; c0 < wmax; c0 += fc::PANEL) {
-      if (c0 > 0) {
-        i0 = (i64)p.items.size();
+    std::vector<Tile> tiles;
+    for (i64 c0 = 0; c0 < wmax; c0 += fc::NB) {
+      if (c0 > 0) {  // the panel minus its products with the front's columns before it
+        tiles.clear();
         for (i64 t = 0; t < nl; ++t) {
           const i64 f = fl[t];
           if (s.f_w[f] <= c0) continue;
-          const i64 c1 = std::min(c0 + fc::PANEL, s.f_w[f]);
-          for (i64 r = c0; r < s.f_m[f]; r += fc::TILE_M)
-            p.items.push_back(Item{(int32_t)f, (int32_t)r, (int32_t)c0, (int32_t)c1, 0, 0, 0, 0});
+          const i64 c1 = std::min(c0 + fc::NB, s.f_w[f]);
+          for (i64 r = c0; r < s.f_m[f]; r += fc::GB) {
+            const i64 m = std::min<i64>(fc::GB, s.f_m[f] - r);
+            fc::GemmItem g{};
+            g.a = s.f_lpos[f] + r;
+            g.b = s.f_lpos[f] + c0;
+            g.c = s.f_lpos[f] + c0 * s.f_ld[f] + r;
+            g.lda = g.ldb = g.ldc = (int32_t)s.f_ld[f];
+            g.m = (int32_t)m;
+            g.n = (int32_t)(c1 - c0);
+            g.k = (int32_t)c0;
+            g.mode = fc::G_SUB;
+            g.diag = (int32_t)(r - c0);
+            g.asrc = g.bsrc = g.csrc = fc::S_L;
+            tiles.push_back(Tile{g, 2.0 * (double)c0 * lower_pairs(r, r + m, c0, c1)});
+          }
         }
-        add(fc::K_UPDATE, l, i0);
+        add_gemm(p, tiles, fc::K_UPDATE, l);
       }
-      i0 = (i64)p.items.size();
-      for (i64 t = 0; t < nl; ++t) {
-        const i64 f = fl[t];
-        if (s.f_w[f] > c0) p.items.push_back(Item{(int32_t)f, (int32_t)c0, 0, 0, 0, 0, 0, 0});
-      }
-      add(fc::K_POTRF, l, i0);
+      i64 slot = 0;
       i0 = (i64)p.items.size();
       for (i64 t = 0; t < nl; ++t) {
         const i64 f = fl[t];
         if (s.f_w[f] <= c0) continue;
-        const i64 c1 = std::min(c0 + fc::PANEL, s.f_w[f]);
-        for (i64 r = c1; r < s.f_m[f]; r += fc::TRSM_ROWS)
-          p.items.push_back(Item{(int32_t)f, (int32_t)c0, (int32_t)r, 0, 0, 0, 0, 0});
+        const double pw = (double)std::min<i64>(fc::NB, s.f_w[f] - c0);
+        p.items.push_back(Item{(int32_t)f, (int32_t)c0, (int32_t)slot++, 0, 0, 0, 0, 0});
+        flops += pw * pw * pw / 3.0;
       }
-      add(fc::K_TRSM, l, i0);
+      p.inv_slots = std::max(p.inv_slots, slot);
+      add(fc::K_POTRF, l, i0);
+      // the rows below the diagonal block: X = B Linv^T, in place
+      tiles.clear();
+      slot = 0;
+      for (i64 t = 0; t < nl; ++t) {
+        const i64 f = fl[t];
+        if (s.f_w[f] <= c0) continue;
+        const i64 c1 = std::min(c0 + fc::NB, s.f_w[f]);
+        const i64 pw = c1 - c0;
+        for (i64 r = c1; r < s.f_m[f]; r += fc::GB) {
+          const i64 m = std::min<i64>(fc::GB, s.f_m[f] - r);
+          fc::GemmItem g{};
+          g.a = s.f_lpos[f] + c0 * s.f_ld[f] + r;
+          g.b = slot * (i64)fc::NB * fc::NB;
+          g.c = g.a;
+          g.lda = g.ldc = (int32_t)s.f_ld[f];
+          g.ldb = fc::NB;
+          g.m = (int32_t)m;
+          g.n = g.k = (int32_t)pw;
+          g.mode = fc::G_SET;
+          g.diag = fc::GB;
+          g.asrc = g.csrc = fc::S_L;
+          g.bsrc = fc::S_INV;
+          tiles.push_back(Tile{g, (double)m * (double)pw * (double)pw});
+        }
+        ++slot;
+      }
+      add_gemm(p, tiles, fc::K_TRSM, l);
     }
-    // update matrices
-    i0 = (i64)p.items.size();
+    // update matrices U = -L21 L21^T
+    tiles.clear();
     for (i64 t = 0; t < nl; ++t) {
       const i64 f = fl[t];
-      const i64 mu = s.f_m[f] - s.f_w[f];
-      for (i64 a = 0; a < mu; a += fc::TILE_M)
-        for (i64 b = 0; b <= a; b += fc::TILE_N) p.items.push_back(Item{(int32_t)f, (int32_t)a, (int32_t)b, 0, 0, 0, 0, 0});
+      const i64 w = s.f_w[f], mu = s.f_m[f] - w;
+      for (i64 a = 0; a < mu; a += fc::GB)
+        for (i64 b = 0; b <= a; b += fc::GB) {
+          const i64 m = std::min<i64>(fc::GB, mu - a), nn = std::min<i64>(fc::GB, mu - b);
+          fc::GemmItem g{};
+          g.a = s.f_lpos[f] + w + a;
+          g.b = s.f_lpos[f] + w + b;
+          g.c = s.f_upos[f] + b * mu + a;
+          g.lda = g.ldb = (int32_t)s.f_ld[f];
+          g.ldc = (int32_t)mu;
+          g.m = (int32_t)m;
+          g.n = (int32_t)nn;
+          g.k = (int32_t)w;
+          g.mode = fc::G_NEG;
+          g.diag = (int32_t)(a - b);
+          g.asrc = g.bsrc = fc::S_L;
+          g.csrc = fc::S_U;
+          tiles.push_back(Tile{g, 2.0 * (double)w * lower_pairs(a, a + m, b, b + nn)});
+        }
     }
-    add(fc::K_SYRK, l, i0);
+    add_gemm(p, tiles, fc::K_SYRK, l);
     i0 = (i64)p.items.size();
     for (i64 t = 0; t < nl; ++t) {
       const i64 f = fl[t];
       if (!has_kids_below(f)) continue;
-      for (i64 a = s.f_w[f]; a < s.f_m[f]; a += fc::EA_COLS)
-        p.items.push_back(Item{(int32_t)f, (int32_t)a, (int32_t)std::min(a + fc::EA_COLS, s.f_m[f]), 0, 0, 0, 0, 0});
+      column_entries(f);
+      for (i64 a = s.f_w[f]; a < s.f_m[f]; a += fc::EA_COLS) {
+        const i64 b = std::min(a + fc::EA_COLS, s.f_m[f]);
+        p.items.push_back(Item{(int32_t)f, (int32_t)a, (int32_t)b, 0, 0, 0, 0, 0});
+        for (i64 j = a; j < b; ++j) entries += colent[j];
+      }
     }
     add(fc::K_EXTEND_U, l, i0);
   }
-  // solves
-  p.solve_lvl.assign(1, 0);
+  // solves: per level, forward leaves first, backward root first
+  auto sadd = [&](std::vector<Launch>& list, int kind, i64 level, i64 item0) {
+    const i64 count = (i64)p.sitems.size() - item0;
+    if (count > 0) list.push_back(Launch{kind, (int32_t)level, item0, (int32_t)count, 0, 0.0, 0.0});
+  };
   for (i64 l = 0; l < s.levels; ++l) {
-    for (i64 t = s.lvl_ptr[l]; t < s.lvl_ptr[l + 1]; ++t) p.solve_fronts.push_back((int32_t)s.lvl_fronts[t]);
-    p.solve_lvl.push_back((i64)p.solve_fronts.size());
+    const i64* fl = &s.lvl_fronts[s.lvl_ptr[l]];
+    const i64 nl = s.lvl_ptr[l + 1] - s.lvl_ptr[l];
+    i64 wmax = 0;
+    for (i64 t = 0; t < nl; ++t) wmax = std::max(wmax, s.f_w[fl[t]]);
+    i64 i0 = (i64)p.sitems.size();
+    for (i64 t = 0; t < nl; ++t) p.sitems.push_back(Item{(int32_t)fl[t], 0, 0, 0, 0, 0, 0, 0});
+    sadd(p.sfwd, S_GATHER, l, i0);
+    for (i64 c0 = 0; c0 < wmax; c0 += fc::SB) {
+      i0 = (i64)p.sitems.size();
+      for (i64 t = 0; t < nl; ++t) {
+        const i64 f = fl[t];
+        if (s.f_w[f] > c0) p.sitems.push_back(Item{(int32_t)f, (int32_t)c0, s.f_w[f] <= fc::SB ? 1 : 0, 0, 0, 0, 0, 0});
+      }
+      sadd(p.sfwd, S_FWD_BLOCK, l, i0);
+      i0 = (i64)p.sitems.size();
+      for (i64 t = 0; t < nl; ++t) {
+        const i64 f = fl[t];
+        if (s.f_w[f] <= fc::SB || s.f_w[f] <= c0) continue;
+        for (i64 r = std::min(c0 + fc::SB, s.f_w[f]); r < s.f_m[f]; r += fc::SROWS)
+          p.sitems.push_back(Item{(int32_t)f, (int32_t)c0, (int32_t)r, 0, 0, 0, 0, 0});
+      }
+      sadd(p.sfwd, S_FWD_GEMV, l, i0);
+    }
+  }
+  for (i64 l = s.levels - 1; l >= 0; --l) {
+    const i64* fl = &s.lvl_fronts[s.lvl_ptr[l]];
+    const i64 nl = s.lvl_ptr[l + 1] - s.lvl_ptr[l];
+    i64 wmax = 0;
+    i64 i0 = (i64)p.sitems.size();
+    for (i64 t = 0; t < nl; ++t) {
+      const i64 f = fl[t];
+      wmax = std::max(wmax, s.f_w[f]);
+      if (s.f_m[f] == s.f_w[f]) continue;
+      for (i64 j = 0; j < s.f_w[f]; j += fc::SCOLS) p.sitems.push_back(Item{(int32_t)f, (int32_t)j, 0, 0, 0, 0, 0, 0});
+    }
+    sadd(p.sbwd, S_BWD_BELOW, l, i0);
+    const i64 nblocks = (wmax + fc::SB - 1) / fc::SB;
+    for (i64 b = nblocks - 1; b >= 0; --b) {
+      const i64 c0 = b * fc::SB;
+      i0 = (i64)p.sitems.size();
+      for (i64 t = 0; t < nl; ++t)
+        if (s.f_w[fl[t]] > c0) p.sitems.push_back(Item{(int32_t)fl[t], (int32_t)c0, 0, 0, 0, 0, 0, 0});
+      sadd(p.sbwd, S_BWD_BLOCK, l, i0);
+      i0 = (i64)p.sitems.size();
+      for (i64 t = 0; t < nl; ++t) {
+        const i64 f = fl[t];
+        if (s.f_w[f] <= c0) continue;
+        for (i64 j = 0; j < c0; j += fc::SCOLS) p.sitems.push_back(Item{(int32_t)f, (int32_t)c0, (int32_t)j, 0, 0, 0, 0, 0});
+      }
+      sadd(p.sbwd, S_BWD_GEMV, l, i0);
+    }
   }
   p.uoff.resize(s.nf);
   for (i64 f = 0; f < s.nf; ++f) {
@@ -178,7 +403,9 @@ struct DevicePlan {
   int64_t* diag_src = nullptr;
   int64_t* perm = nullptr;
   fc::Item* items = nullptr;
-  int32_t* solve_fronts = nullptr;
+  fc::GemmItem* gitems = nullptr;
+  fc::ReduceItem* ritems = nullptr;
+  fc::Item* sitems = nullptr;
   int64_t* uoff = nullptr;
   ~DevicePlan() {
     if (device < 0) return;
@@ -186,7 +413,7 @@ struct DevicePlan {
     (void)hipGetDevice(&cur);
     (void)hipSetDevice(device);
     for (void* q : {(void*)fronts, (void*)kids, (void*)rel, (void*)rows, (void*)a_pos, (void*)diag_src, (void*)perm,
-                    (void*)items, (void*)solve_fronts, (void*)uoff})
+                    (void*)items, (void*)gitems, (void*)ritems, (void*)sitems, (void*)uoff})
       dfree(q);
     (void)hipSetDevice(cur);
   }
@@ -234,7 +461,9 @@ struct fc_symbolic {
     p->diag_src = dupload(diag_src, st);
     p->perm = dupload(S.perm, st);
     p->items = dupload(prog.items, st);
-    p->solve_fronts = dupload(prog.solve_fronts, st);
+    p->gitems = dupload(prog.gitems, st);
+    p->ritems = dupload(prog.ritems, st);
+    p->sitems = dupload(prog.sitems, st);
     p->uoff = dupload(prog.uoff, st);
     HIP_CHECK(hipStreamSynchronize(st));
     plans.push_back(p);
@@ -251,10 +480,17 @@ struct fc_factor {
   fc_factor_opts opts{};
   hipStream_t stream = nullptr;
   bool own_stream = false;
-  void* L = nullptr;      // factor storage (double or float)
-  double* diag = nullptr; // A's diagonal (postorder)
+  void* L = nullptr;       // factor storage (double or float)
+  double* diag = nullptr;  // A's diagonal (postorder)
   unsigned long long* fail = nullptr;
+  Buffer U;                // update-matrix arena (kept between factorizations)
+  Buffer inv, part;        // diagonal-block inverses, split-K pieces
+  Buffer vals;             // the matrix values on the device (host-value entry points)
+  mutable std::mutex solve_mu;
+  mutable Buffer xp, usolve, refine_work;  // solve workspaces
   double seconds = 0.0;
+  double kind_seconds[fc::KINDS] = {};
+  double kind_launches[fc::KINDS] = {};
   bool valid = false;
   ~fc_factor() {
     int cur = 0;
@@ -263,6 +499,13 @@ struct fc_factor {
     dfree(L);
     dfree(diag);
     dfree(fail);
+    U.release();
+    inv.release();
+    part.release();
+    vals.release();
+    xp.release();
+    usolve.release();
+    refine_work.release();
     if (own_stream && stream) (void)hipStreamDestroy(stream);
     (void)hipSetDevice(cur);
   }
@@ -301,25 +544,38 @@ std::shared_ptr<fc_symbolic> keep(const fc_symbolic* s) {
 
 inline int blocks_for(i64 n, int bs = 256) { return (int)std::max<i64>(1, std::min<i64>((n + bs - 1) / bs, 1 << 16)); }
 
+struct Events {
+  std::vector<hipEvent_t> ev;
+  ~Events() {
+    for (auto e : ev) (void)hipEventDestroy(e);
+  }
+  hipEvent_t make() {
+    hipEvent_t e;
+    HIP_CHECK(hipEventCreate(&e));
+    ev.push_back(e);
+    return e;
+  }
+};
+
 template <typename T>
 void run_factor(fc_factor* F, const double* dval, i64 nnz, hipStream_t st) {
   const fc::Symbolic& s = F->sym->s;
   DevicePlan& P = *F->plan;
   T* L = (T*)F->L;
-  T* U = dalloc<T>((size_t)std::max<i64>(s.u_size, 1));
-  struct Free {
-    T* p;
-    ~Free() { dfree(p); }
-  } ufree{U};
+  T* U = (T*)F->U.get((size_t)std::max<i64>(s.u_size, 1) * sizeof(T));
+  const auto& pr = F->sym->prog;
+  T* Inv = (T*)F->inv.get((size_t)std::max<i64>(pr.inv_slots, 1) * fc::NB * fc::NB * sizeof(T));
+  T* Part = (T*)F->part.get((size_t)std::max<i64>(pr.part_size, 1) * sizeof(T));
+  Events ev;
+  hipEvent_t e0 = ev.make(), e1 = ev.make();
+  HIP_CHECK(hipEventRecord(e0, st));
   fc::k_fill<T><<<blocks_for(s.l_size), 256, 0, st>>>(L, s.l_size, T(0));
   fc::k_scatter_a<T><<<blocks_for(nnz), 256, 0, st>>>(dval, P.a_pos, nnz, L);
   fc::k_gather_diag<<<blocks_for(s.n), 256, 0, st>>>(dval, P.diag_src, s.n, F->diag);
-  HIP_CHECK(hipMemsetAsync(F->fail, 0xff, sizeof(unsigned long long), st));
-  hipEvent_t e0, e1;
-  HIP_CHECK(hipEventCreate(&e0));
-  HIP_CHECK(hipEventCreate(&e1));
-  HIP_CHECK(hipEventRecord(e0, st));
+  fc::k_fill<unsigned long long><<<1, 64, 0, st>>>(F->fail, 1, ~0ull);
   const auto& prog = F->sym->prog;
+  const int mask = F->opts.profile_mask;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> timed;
   int level = -1;
   for (const fc::Launch& l : prog.launches) {
     if (l.level != level) {
@@ -328,25 +584,34 @@ void run_factor(fc_factor* F, const double* dval, i64 nnz, hipStream_t st) {
     }
     const fc::Item* it = P.items + l.item0;
     dim3 g(l.count), b(256);
+    hipEvent_t a = nullptr, z = nullptr;
+    if (mask & (1 << l.kind)) {
+      a = ev.make();
+      z = ev.make();
+      HIP_CHECK(hipEventRecord(a, st));
+    }
     switch (l.kind) {
       case fc::K_EXTEND_L:
       case fc::K_EXTEND_U:
         fc::k_extend_add<T><<<g, b, 0, st>>>(it, P.fronts, P.kids, P.rel, L, U, U);
         break;
       case fc::K_UPDATE:
-        fc::k_update<T><<<g, b, 0, st>>>(it, P.fronts, L);
+      case fc::K_TRSM:
+      case fc::K_SYRK:
+        fc::k_gemm<T><<<g, b, 0, st>>>(P.gitems + l.item0, L, U, Inv, Part);
+        break;
+      case fc::K_REDUCE:
+        fc::k_reduce_parts<T><<<g, b, 0, st>>>(P.ritems + l.item0, L, U, Part);
         break;
       case fc::K_POTRF:
-        fc::k_potrf<T><<<g, b, 0, st>>>(it, P.fronts, L, F->diag, F->fail);
-        break;
-      case fc::K_TRSM:
-        fc::k_trsm<T><<<g, b, 0, st>>>(it, P.fronts, L);
-        break;
-      case fc::K_SYRK:
-        fc::k_syrk<T><<<g, b, 0, st>>>(it, P.fronts, L, U);
+        fc::k_potrf_inv<T><<<g, b, 0, st>>>(it, P.fronts, L, F->diag, F->fail, Inv);
         break;
       default:
         throw fc::Failure{fc::INTERNAL, -1, 0, 0, "unknown launch"};
+    }
+    if (a) {
+      HIP_CHECK(hipEventRecord(z, st));
+      timed.push_back({l.kind, {a, z}});
     }
   }
   HIP_CHECK(hipGetLastError());
@@ -355,12 +620,17 @@ void run_factor(fc_factor* F, const double* dval, i64 nnz, hipStream_t st) {
   float ms = 0.f;
   HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
   F->seconds = ms * 1e-3;
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  for (int k = 0; k < fc::KINDS; ++k) F->kind_seconds[k] = F->kind_launches[k] = 0;
+  for (auto& t : timed) {
+    float m = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&m, t.second.first, t.second.second));
+    F->kind_seconds[t.first] += m * 1e-3;
+    F->kind_launches[t.first] += 1;
+  }
 }
 
-// numeric phase, with failures as values
-int factor_numeric(fc_factor* F, const double* upper_val, uint64_t nnz, fc_error* err) {
+// numeric phase from device values, failures as values
+int factor_device(fc_factor* F, const double* dval, uint64_t nnz, fc_error* err) {
   const fc::Symbolic& s = F->sym->s;
   if ((i64)nnz != (i64)s.upper_idx.size()) {
     set_error(err, FC_INVALID_ARGUMENT, "the matrix's pattern differs from the analysed one (entry count)");
@@ -369,12 +639,6 @@ int factor_numeric(fc_factor* F, const double* upper_val, uint64_t nnz, fc_error
   try {
     HIP_CHECK(hipSetDevice(F->device));
     hipStream_t st = F->stream;
-    double* dval = dalloc<double>(std::max<uint64_t>(nnz, 1));
-    struct Free {
-      double* p;
-      ~Free() { dfree(p); }
-    } vfree{dval};
-    if (nnz) HIP_CHECK(hipMemcpyAsync(dval, upper_val, nnz * sizeof(double), hipMemcpyHostToDevice, st));
     F->valid = false;
     if (F->precision == FC_FP64) run_factor<double>(F, dval, (i64)nnz, st);
     else run_factor<float>(F, dval, (i64)nnz, st);
@@ -401,46 +665,189 @@ int factor_numeric(fc_factor* F, const double* upper_val, uint64_t nnz, fc_error
   }
 }
 
+int factor_host(fc_factor* F, const double* upper_val, uint64_t nnz, fc_error* err) {
+  try {
+    HIP_CHECK(hipSetDevice(F->device));
+    double* dval = (double*)F->vals.get(std::max<uint64_t>(nnz, 1) * sizeof(double));
+    if (nnz) HIP_CHECK(hipMemcpyAsync(dval, upper_val, nnz * sizeof(double), hipMemcpyHostToDevice, F->stream));
+    return factor_device(F, dval, nnz, err);
+  } catch (const std::bad_alloc&) {
+    set_error(err, FC_OUT_OF_MEMORY, "not enough device memory for the matrix values", 0, F->sym->s.entries);
+    return FC_OUT_OF_MEMORY;
+  } catch (const DeviceError& d) {
+    set_error(err, FC_DEVICE, std::string(d.what) + ": " + hipGetErrorString(d.e));
+    return FC_DEVICE;
+  }
+}
+
 template <typename T>
 void run_solve(const fc_factor* F, double* xp, i64 ldx, int nrhs, hipStream_t st) {
-  const fc::Symbolic& s = F->sym->s;
   const Program& prog = F->sym->prog;
   DevicePlan& P = *F->plan;
   const T* L = (const T*)F->L;
-  double* u = dalloc<double>((size_t)std::max<i64>(prog.u_solve * nrhs, 1));
-  struct Free {
-    double* p;
-    ~Free() { dfree(p); }
-  } ufree{u};
-  const i64 nl = (i64)prog.solve_lvl.size() - 1;
-  for (i64 l = 0; l < nl; ++l) {
-    const i64 a = prog.solve_lvl[l], b = prog.solve_lvl[l + 1];
-    if (b > a)
-      fc::k_forward<T><<<(unsigned)(b - a), 256, 0, st>>>(P.solve_fronts + a, P.fronts, P.kids, P.rel, L, xp, ldx, nrhs, u,
-                                                           P.uoff);
-  }
-  for (i64 l = nl - 1; l >= 0; --l) {
-    const i64 a = prog.solve_lvl[l], b = prog.solve_lvl[l + 1];
-    if (b > a)
-      fc::k_backward<T><<<(unsigned)(b - a), 256, 0, st>>>(P.solve_fronts + a, P.fronts, P.rows, L, xp, ldx, nrhs);
-  }
+  double* u = (double*)F->usolve.get((size_t)std::max<i64>(prog.u_solve * nrhs, 1) * sizeof(double));
+  auto run = [&](const fc::Launch& l) {
+    const fc::Item* it = P.sitems + l.item0;
+    dim3 g(l.count), b(256);
+    switch (l.kind) {
+      case S_GATHER:
+        fc::k_solve_gather<<<g, b, 0, st>>>(it, P.fronts, P.kids, P.rel, xp, ldx, nrhs, u, P.uoff);
+        break;
+      case S_FWD_BLOCK:
+        fc::k_forward_block<T><<<g, b, 0, st>>>(it, P.fronts, L, xp, ldx, nrhs, u, P.uoff);
+        break;
+      case S_FWD_GEMV:
+        fc::k_forward_gemv<T><<<g, b, 0, st>>>(it, P.fronts, L, xp, ldx, nrhs, u, P.uoff);
+        break;
+      case S_BWD_BELOW:
+        fc::k_backward_below<T><<<g, b, 0, st>>>(it, P.fronts, P.rows, L, xp, ldx, nrhs);
+        break;
+      case S_BWD_BLOCK:
+        fc::k_backward_block<T><<<g, b, 0, st>>>(it, P.fronts, L, xp, ldx, nrhs);
+        break;
+      case S_BWD_GEMV:
+        fc::k_backward_gemv<T><<<g, b, 0, st>>>(it, P.fronts, L, xp, ldx, nrhs);
+        break;
+    }
+  };
+  for (const fc::Launch& l : prog.sfwd) run(l);
+  for (const fc::Launch& l : prog.sbwd) run(l);
   HIP_CHECK(hipGetLastError());
-  (void)s;
 }
 
-// b (device, n x nrhs, ldb) solved in place
+// b (device, n x nrhs, ldb) solved in place, MAX_RHS right-hand sides at a time; the caller
+// holds solve_mu
 void solve_device(const fc_factor* F, double* b, i64 n, int nrhs, i64 ldb, hipStream_t st) {
   if (n == 0 || nrhs == 0) return;
-  double* xp = dalloc<double>((size_t)n * nrhs);
-  struct Free {
-    double* p;
-    ~Free() { dfree(p); }
-  } xfree{xp};
-  fc::k_permute<<<blocks_for(n), 256, 0, st>>>(b, xp, F->plan->perm, n, nrhs, ldb, n, 0);
-  if (F->precision == FC_FP64) run_solve<double>(F, xp, n, nrhs, st);
-  else run_solve<float>(F, xp, n, nrhs, st);
-  fc::k_permute<<<blocks_for(n), 256, 0, st>>>(xp, b, F->plan->perm, n, nrhs, n, ldb, 1);
+  for (int r0 = 0; r0 < nrhs; r0 += fc::MAX_RHS) {
+    const int nr = std::min(fc::MAX_RHS, nrhs - r0);
+    double* br = b + (i64)r0 * ldb;
+    double* xp = (double*)F->xp.get((size_t)n * nr * sizeof(double));
+    fc::k_permute<<<blocks_for(n), 256, 0, st>>>(br, xp, F->plan->perm, n, nr, ldb, n, 0);
+    if (F->precision == FC_FP64) run_solve<double>(F, xp, n, nr, st);
+    else run_solve<float>(F, xp, n, nr, st);
+    fc::k_permute<<<blocks_for(n), 256, 0, st>>>(xp, br, F->plan->perm, n, nr, n, ldb, 1);
+  }
   HIP_CHECK(hipGetLastError());
+}
+
+struct Reducer {
+  double* partial = nullptr;
+  std::vector<double> host = std::vector<double>(fc::RED_BLOCKS);
+  double run(hipStream_t st, i64 n, const double* a, const double* b, int mode) {
+    fc::k_reduce<<<fc::RED_BLOCKS, 256, 0, st>>>(n, a, b, mode, partial);
+    HIP_CHECK(hipMemcpyAsync(host.data(), partial, fc::RED_BLOCKS * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    double s = 0.0;
+    for (double v : host) s = mode == 0 ? s + v : std::max(s, v);
+    return s;
+  }
+};
+
+// Refinement on device vectors db (right-hand side) and dx (solution); the caller holds solve_mu.
+void refine_device(const fc_factor* f, const fc_matrix* a, const double* db, double* dx, i64 N, fc_refine_opts o,
+                   fc_refine_report& rep, Buffer& work) {
+  hipStream_t st = f->stream;
+  double* buf = (double*)work.get(((size_t)N * 4 + fc::RED_BLOCKS) * sizeof(double));
+  double *dr = buf, *dz = buf + N, *dp = buf + 2 * N, *dq = buf + 3 * N;
+  Reducer red;
+  red.partial = buf + 4 * N;
+  const int g = blocks_for(N);
+  Events ev;
+  hipEvent_t e0 = ev.make(), e1 = ev.make();
+  HIP_CHECK(hipEventRecord(e0, st));
+  const double bnorm = red.run(st, N, db, db, 1);
+  auto spmv = [&](const double* in, double* out) {
+    fc::k_spmv<<<blocks_for(N), 256, 0, st>>>(N, a->ptr, a->idx, a->val, in, out);
+  };
+  auto berr_of = [&](const double* r) {
+    const double rn = red.run(st, N, r, r, 1), xn = red.run(st, N, dx, dx, 1);
+    const double den = a->norm_inf * xn + bnorm;
+    return den > 0 ? rn / den : 0.0;
+  };
+  auto residual = [&]() {  // dr = b - A x
+    spmv(dx, dq);
+    fc::k_residual<<<g, 256, 0, st>>>(N, db, dq, dr);
+    return berr_of(dr);
+  };
+  HIP_CHECK(hipMemcpyAsync(dx, db, N * sizeof(double), hipMemcpyDeviceToDevice, st));
+  solve_device(f, dx, N, 1, N, st);
+  double berr = residual();
+  rep.berr_history[0] = berr;
+  int it = 0;
+  bool cg = o.method == 1;
+  double prev = berr;
+  // plain refinement while it contracts
+  while (!cg && berr > o.target_berr && it < o.max_iter) {
+    HIP_CHECK(hipMemcpyAsync(dz, dr, N * sizeof(double), hipMemcpyDeviceToDevice, st));
+    solve_device(f, dz, N, 1, N, st);
+    fc::k_axpy<<<g, 256, 0, st>>>(N, 1.0, dz, dx);
+    ++it;
+    berr = residual();
+    if (it < 64) rep.berr_history[it] = berr;
+    if (o.method == 2 && berr > 0.5 * prev) cg = true;  // not contracting: conjugate gradients
+    prev = berr;
+  }
+  if (cg && berr > o.target_berr && it < o.max_iter) {
+    // conjugate gradients on A d = r from the current x, preconditioned by the factor
+    HIP_CHECK(hipMemcpyAsync(dz, dr, N * sizeof(double), hipMemcpyDeviceToDevice, st));
+    solve_device(f, dz, N, 1, N, st);
+    HIP_CHECK(hipMemcpyAsync(dp, dz, N * sizeof(double), hipMemcpyDeviceToDevice, st));
+    double rz = red.run(st, N, dr, dz, 0);
+    while (it < o.max_iter) {
+      spmv(dp, dq);
+      const double pq = red.run(st, N, dp, dq, 0);
+      if (!(pq > 0)) break;
+      const double alpha = rz / pq;
+      fc::k_axpy<<<g, 256, 0, st>>>(N, alpha, dp, dx);
+      fc::k_axpy<<<g, 256, 0, st>>>(N, -alpha, dq, dr);
+      ++it;
+      // the backward error from the true residual, into dq (free until the next A p)
+      spmv(dx, dq);
+      fc::k_residual<<<g, 256, 0, st>>>(N, db, dq, dq);
+      berr = berr_of(dq);
+      if (it < 64) rep.berr_history[it] = berr;
+      if (berr <= o.target_berr) break;
+      HIP_CHECK(hipMemcpyAsync(dz, dr, N * sizeof(double), hipMemcpyDeviceToDevice, st));
+      solve_device(f, dz, N, 1, N, st);
+      const double rz2 = red.run(st, N, dr, dz, 0);
+      fc::k_xpby<<<g, 256, 0, st>>>(N, dz, rz2 / rz, dp);
+      rz = rz2;
+    }
+  }
+  HIP_CHECK(hipEventRecord(e1, st));
+  HIP_CHECK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  rep.iterations = it;
+  rep.berr = berr;
+  rep.converged = berr <= o.target_berr;
+  rep.seconds = ms * 1e-3;
+}
+
+fc_refine_opts refine_defaults(const fc_refine_opts* opts) {
+  fc_refine_opts o{2, 50, 1e-12};
+  if (opts) {
+    o = *opts;
+    if (o.max_iter <= 0) o.max_iter = 50;
+    if (!(o.target_berr > 0)) o.target_berr = 1e-12;
+  }
+  return o;
+}
+
+template <typename Body>
+int guarded(fc_error* err, const char* oom, Body body) {
+  try {
+    body();
+  } catch (const std::bad_alloc&) {
+    set_error(err, FC_OUT_OF_MEMORY, oom);
+    return FC_OUT_OF_MEMORY;
+  } catch (const DeviceError& d) {
+    set_error(err, FC_DEVICE, std::string(d.what) + ": " + hipGetErrorString(d.e));
+    return FC_DEVICE;
+  }
+  set_error(err, FC_OK, "");
+  return FC_OK;
 }
 
 }  // namespace
@@ -509,6 +916,11 @@ int fc_symbolic_info_get(const fc_symbolic* h, fc_symbolic_info* info) {
   info->l_bytes_fp64 = (uint64_t)s.l_size * 8;
   info->u_bytes_fp64 = (uint64_t)s.u_size * 8;
   info->by_blocks = s.by_blocks;
+  info->launches = h->prog.launches.size();
+  for (int k = 0; k < fc::KINDS && k < 8; ++k) {
+    info->kind_flops[k] = h->prog.work[k][0];
+    info->kind_entries[k] = h->prog.work[k][1];
+  }
   return FC_OK;
 }
 
@@ -543,20 +955,15 @@ void fc_symbolic_free(fc_symbolic* h) {
                     g_symbolics.end());
 }
 
-int fc_factor_numeric(const fc_symbolic* h, const double* upper_val, uint64_t nnz, const fc_factor_opts* opts,
-                      fc_factor** out, fc_error* err) {
-  if (!h || !out || (nnz && !upper_val)) {
-    set_error(err, FC_INVALID_ARGUMENT, "null argument");
-    return FC_INVALID_ARGUMENT;
-  }
-  *out = nullptr;
+static int factor_create(const fc_symbolic* h, const fc_factor_opts* opts, std::unique_ptr<fc_factor>& F,
+                         fc_error* err) {
   fc_factor_opts o{};
   if (opts) o = *opts;
   if (o.precision < FC_FP64 || o.precision > FC_FP32) {
     set_error(err, FC_INVALID_ARGUMENT, "unknown precision");
     return FC_INVALID_ARGUMENT;
   }
-  std::unique_ptr<fc_factor> F(new fc_factor());
+  F.reset(new fc_factor());
   F->sym_keep = keep(h);
   F->sym = h;
   F->device = o.device;
@@ -583,7 +990,36 @@ int fc_factor_numeric(const fc_symbolic* h, const double* upper_val, uint64_t nn
     set_error(err, FC_DEVICE, std::string(d.what) + ": " + hipGetErrorString(d.e));
     return FC_DEVICE;
   }
-  int st = factor_numeric(F.get(), upper_val, nnz, err);
+  return FC_OK;
+}
+
+int fc_factor_numeric(const fc_symbolic* h, const double* upper_val, uint64_t nnz, const fc_factor_opts* opts,
+                      fc_factor** out, fc_error* err) {
+  if (!h || !out || (nnz && !upper_val)) {
+    set_error(err, FC_INVALID_ARGUMENT, "null argument");
+    return FC_INVALID_ARGUMENT;
+  }
+  *out = nullptr;
+  std::unique_ptr<fc_factor> F;
+  int st = factor_create(h, opts, F, err);
+  if (st != FC_OK) return st;
+  st = factor_host(F.get(), upper_val, nnz, err);
+  if (st != FC_OK) return st;
+  *out = F.release();
+  return FC_OK;
+}
+
+int fc_factor_numeric_device(const fc_symbolic* h, const double* upper_val_device, uint64_t nnz,
+                             const fc_factor_opts* opts, fc_factor** out, fc_error* err) {
+  if (!h || !out || (nnz && !upper_val_device)) {
+    set_error(err, FC_INVALID_ARGUMENT, "null argument");
+    return FC_INVALID_ARGUMENT;
+  }
+  *out = nullptr;
+  std::unique_ptr<fc_factor> F;
+  int st = factor_create(h, opts, F, err);
+  if (st != FC_OK) return st;
+  st = factor_device(F.get(), upper_val_device, nnz, err);
   if (st != FC_OK) return st;
   *out = F.release();
   return FC_OK;
@@ -600,16 +1036,34 @@ int fc_factor_new(uint64_t n, const uint64_t* ptr, const uint64_t* idx, const do
 }
 
 int fc_refactor(fc_factor* f, const double* upper_val, uint64_t nnz, fc_error* err) {
-  if (!f) {
-    set_error(err, FC_INVALID_ARGUMENT, "null factor");
+  if (!f || (nnz && !upper_val)) {
+    set_error(err, FC_INVALID_ARGUMENT, "null argument");
     return FC_INVALID_ARGUMENT;
   }
-  return factor_numeric(f, upper_val, nnz, err);
+  return factor_host(f, upper_val, nnz, err);
+}
+
+int fc_refactor_device(fc_factor* f, const double* upper_val_device, uint64_t nnz, fc_error* err) {
+  if (!f || (nnz && !upper_val_device)) {
+    set_error(err, FC_INVALID_ARGUMENT, "null argument");
+    return FC_INVALID_ARGUMENT;
+  }
+  return factor_device(f, upper_val_device, nnz, err);
 }
 
 uint64_t fc_entries(const fc_factor* f) { return f ? (uint64_t)f->sym->s.entries : 0; }
 
 double fc_factor_seconds(const fc_factor* f) { return f ? f->seconds : 0.0; }
+
+int fc_factor_profile(const fc_factor* f, int32_t kind, double* launches, double* seconds, double* flops,
+                      double* entries) {
+  if (!f || kind < 0 || kind >= fc::KINDS) return FC_INVALID_ARGUMENT;
+  if (launches) *launches = f->kind_launches[kind];
+  if (seconds) *seconds = f->kind_seconds[kind];
+  if (flops) *flops = f->sym->prog.work[kind][0];
+  if (entries) *entries = f->sym->prog.work[kind][1];
+  return FC_OK;
+}
 
 void fc_factor_free(fc_factor* f) { delete f; }
 
@@ -622,19 +1076,12 @@ int fc_solve_device(const fc_factor* f, double* b, uint64_t n, uint64_t nrhs, ui
     set_error(err, FC_INTERNAL, "right-hand side length");
     return FC_INTERNAL;
   }
-  try {
+  std::lock_guard<std::mutex> g(f->solve_mu);
+  return guarded(err, "not enough device memory for the solve", [&] {
     HIP_CHECK(hipSetDevice(f->device));
     solve_device(f, b, (i64)n, (int)nrhs, (i64)ldb, f->stream);
     HIP_CHECK(hipStreamSynchronize(f->stream));
-  } catch (const std::bad_alloc&) {
-    set_error(err, FC_OUT_OF_MEMORY, "not enough device memory for the solve");
-    return FC_OUT_OF_MEMORY;
-  } catch (const DeviceError& d) {
-    set_error(err, FC_DEVICE, std::string(d.what) + ": " + hipGetErrorString(d.e));
-    return FC_DEVICE;
-  }
-  set_error(err, FC_OK, "");
-  return FC_OK;
+  });
 }
 
 int fc_solve_many(const fc_factor* f, double* const* bs, uint64_t nrhs, uint64_t n, fc_error* err) {
@@ -650,28 +1097,20 @@ int fc_solve_many(const fc_factor* f, double* const* bs, uint64_t nrhs, uint64_t
     set_error(err, FC_OK, "");
     return FC_OK;
   }
-  try {
+  std::lock_guard<std::mutex> g(f->solve_mu);
+  Buffer tmp;
+  int st = guarded(err, "not enough device memory for the solve", [&] {
     HIP_CHECK(hipSetDevice(f->device));
-    double* d = dalloc<double>((size_t)n * nrhs);
-    struct Free {
-      double* p;
-      ~Free() { dfree(p); }
-    } dfree_{d};
+    double* d = (double*)tmp.get((size_t)n * nrhs * sizeof(double));
     for (uint64_t r = 0; r < nrhs; ++r)
       HIP_CHECK(hipMemcpyAsync(d + r * n, bs[r], n * sizeof(double), hipMemcpyHostToDevice, f->stream));
     solve_device(f, d, (i64)n, (int)nrhs, (i64)n, f->stream);
     for (uint64_t r = 0; r < nrhs; ++r)
       HIP_CHECK(hipMemcpyAsync(bs[r], d + r * n, n * sizeof(double), hipMemcpyDeviceToHost, f->stream));
     HIP_CHECK(hipStreamSynchronize(f->stream));
-  } catch (const std::bad_alloc&) {
-    set_error(err, FC_OUT_OF_MEMORY, "not enough device memory for the solve");
-    return FC_OUT_OF_MEMORY;
-  } catch (const DeviceError& d) {
-    set_error(err, FC_DEVICE, std::string(d.what) + ": " + hipGetErrorString(d.e));
-    return FC_DEVICE;
-  }
-  set_error(err, FC_OK, "");
-  return FC_OK;
+  });
+  tmp.release();
+  return st;
 }
 
 int fc_solve(const fc_factor* f, double* b, uint64_t n, fc_error* err) {
@@ -689,11 +1128,10 @@ int fc_factor_export(const fc_factor* f, double* out, uint64_t len, fc_error* er
     set_error(err, FC_INVALID_ARGUMENT, "export length differs from the factor's entries");
     return FC_INVALID_ARGUMENT;
   }
-  try {
+  return guarded(err, "not enough host memory for the export", [&] {
     HIP_CHECK(hipSetDevice(f->device));
-    const bool dbl = f->precision == FC_FP64;
     std::vector<double> Lh((size_t)s.l_size);
-    if (dbl) {
+    if (f->precision == FC_FP64) {
       HIP_CHECK(hipMemcpy(Lh.data(), f->L, s.l_size * sizeof(double), hipMemcpyDeviceToHost));
     } else {
       std::vector<float> tmp((size_t)s.l_size);
@@ -709,18 +1147,12 @@ int fc_factor_export(const fc_factor* f, double* out, uint64_t len, fc_error* er
       for (i64 c = s.first[q]; c < s.first[q + 1]; ++c) {
         const i64 col = (c - s.f_col0[fr]) * s.f_ld[fr] + s.f_lpos[fr];
         for (i64 t = s.rptr[q] + (c - s.first[q]); t < s.rptr[q] + m; ++t) {
-          const i64 r = s.rows[t];
-          const i64 pos = std::lower_bound(frows, frows + fm, r) - frows;
+          const i64 pos = std::lower_bound(frows, frows + fm, s.rows[t]) - frows;
           out[at++] = Lh[col + pos];
         }
       }
     }
-  } catch (const DeviceError& d) {
-    set_error(err, FC_DEVICE, std::string(d.what) + ": " + hipGetErrorString(d.e));
-    return FC_DEVICE;
-  }
-  set_error(err, FC_OK, "");
-  return FC_OK;
+  });
 }
 
 int fc_matrix_new(uint64_t n, const uint64_t* ptr, const uint64_t* idx, const double* val, int32_t device,
@@ -729,11 +1161,15 @@ int fc_matrix_new(uint64_t n, const uint64_t* ptr, const uint64_t* idx, const do
     set_error(err, FC_INVALID_ARGUMENT, "null argument");
     return FC_INVALID_ARGUMENT;
   }
+  if (n >= (uint64_t)INT32_MAX) {
+    set_error(err, FC_INVALID_ARGUMENT, "the residual matrix keeps 32-bit column indices");
+    return FC_INVALID_ARGUMENT;
+  }
   std::unique_ptr<fc_matrix> A(new fc_matrix());
   A->device = device;
   A->n = (i64)n;
   A->nnz = n ? (i64)ptr[n] : 0;
-  try {
+  int st = guarded(err, "not enough device memory for the matrix", [&] {
     HIP_CHECK(hipSetDevice(device));
     std::vector<int64_t> p(ptr, ptr + n + 1);
     std::vector<int32_t> ix(A->nnz);
@@ -750,41 +1186,16 @@ int fc_matrix_new(uint64_t n, const uint64_t* ptr, const uint64_t* idx, const do
     A->idx = dupload(ix, nullptr);
     A->val = dupload(v, nullptr);
     HIP_CHECK(hipDeviceSynchronize());
-  } catch (const std::bad_alloc&) {
-    set_error(err, FC_OUT_OF_MEMORY, "not enough device memory for the matrix");
-    return FC_OUT_OF_MEMORY;
-  } catch (const DeviceError& d) {
-    set_error(err, FC_DEVICE, std::string(d.what) + ": " + hipGetErrorString(d.e));
-    return FC_DEVICE;
-  }
+  });
+  if (st != FC_OK) return st;
   *out = A.release();
-  set_error(err, FC_OK, "");
   return FC_OK;
 }
 
 void fc_matrix_free(fc_matrix* a) { delete a; }
 
-}  // extern "C"
-
-namespace {
-
-struct Reducer {
-  double* partial = nullptr;
-  std::vector<double> host = std::vector<double>(fc::RED_BLOCKS);
-  double run(hipStream_t st, i64 n, const double* a, const double* b, int mode) {
-    fc::k_reduce<<<fc::RED_BLOCKS, 256, 0, st>>>(n, a, b, mode, partial);
-    HIP_CHECK(hipMemcpyAsync(host.data(), partial, fc::RED_BLOCKS * sizeof(double), hipMemcpyDeviceToHost, st));
-    HIP_CHECK(hipStreamSynchronize(st));
-    double s = 0.0;
-    for (double v : host) s = mode == 0 ? s + v : std::max(s, v);
-    return s;
-  }
-};
-
-}  // namespace
-
-extern "C" int fc_solve_refined(const fc_factor* f, const fc_matrix* a, const double* b, double* x, uint64_t n,
-                                const fc_refine_opts* opts, fc_refine_report* report, fc_error* err) {
+int fc_solve_refined(const fc_factor* f, const fc_matrix* a, const double* b, double* x, uint64_t n,
+                     const fc_refine_opts* opts, fc_refine_report* report, fc_error* err) {
   if (!f || !f->valid || !a || !b || !x) {
     set_error(err, FC_INVALID_ARGUMENT, "null argument or invalid factor");
     return FC_INVALID_ARGUMENT;
@@ -793,116 +1204,55 @@ extern "C" int fc_solve_refined(const fc_factor* f, const fc_matrix* a, const do
     set_error(err, FC_INTERNAL, "right-hand side length");
     return FC_INTERNAL;
   }
-  fc_refine_opts o{2, 50, 1e-12};
-  if (opts) {
-    o = *opts;
-    if (o.max_iter <= 0) o.max_iter = 50;
-    if (!(o.target_berr > 0)) o.target_berr = 1e-12;
-  }
+  const fc_refine_opts o = refine_defaults(opts);
   fc_refine_report rep;
   std::memset(&rep, 0, sizeof rep);
-  if (n == 0) {
-    rep.converged = 1;
-    if (report) *report = rep;
-    set_error(err, FC_OK, "");
-    return FC_OK;
-  }
-  try {
+  std::lock_guard<std::mutex> g(f->solve_mu);
+  Buffer vec, work;
+  int st = guarded(err, "not enough device memory for refinement", [&] {
+    if (n == 0) {
+      rep.converged = 1;
+      return;
+    }
     HIP_CHECK(hipSetDevice(f->device));
-    hipStream_t st = f->stream;
     const i64 N = (i64)n;
-    double* buf = dalloc<double>((size_t)N * 6 + fc::RED_BLOCKS);
-    struct Free {
-      double* p;
-      ~Free() { dfree(p); }
-    } bfree{buf};
-    double *db = buf, *dx = buf + N, *dr = buf + 2 * N, *dz = buf + 3 * N, *dp = buf + 4 * N, *dq = buf + 5 * N;
-    Reducer red;
-    red.partial = buf + 6 * N;
-    const int g = blocks_for(N);
-    hipEvent_t e0, e1;
-    HIP_CHECK(hipEventCreate(&e0));
-    HIP_CHECK(hipEventCreate(&e1));
-    HIP_CHECK(hipMemcpyAsync(db, b, N * sizeof(double), hipMemcpyHostToDevice, st));
-    HIP_CHECK(hipEventRecord(e0, st));
-    const double bnorm = red.run(st, N, db, db, 1);
-    auto spmv = [&](const double* in, double* out) {
-      fc::k_spmv<<<blocks_for(N), 256, 0, st>>>(N, a->ptr, a->idx, a->val, in, out);
-    };
-    auto residual = [&]() {  // dr = b - A x, returns the backward error
-      spmv(dx, dq);
-      fc::k_residual<<<g, 256, 0, st>>>(N, db, dq, dr);
-      const double rn = red.run(st, N, dr, dr, 1), xn = red.run(st, N, dx, dx, 1);
-      const double den = a->norm_inf * xn + bnorm;
-      return den > 0 ? rn / den : 0.0;
-    };
-    HIP_CHECK(hipMemcpyAsync(dx, db, N * sizeof(double), hipMemcpyDeviceToDevice, st));
-    solve_device(f, dx, N, 1, N, st);
-    double berr = residual();
-    rep.berr_history[0] = berr;
-    int it = 0;
-    bool cg = o.method == 1;
-    double prev = berr;
-    // plain refinement while it contracts
-    while (!cg && berr > o.target_berr && it < o.max_iter) {
-      HIP_CHECK(hipMemcpyAsync(dz, dr, N * sizeof(double), hipMemcpyDeviceToDevice, st));
-      solve_device(f, dz, N, 1, N, st);
-      fc::k_axpy<<<g, 256, 0, st>>>(N, 1.0, dz, dx);
-      ++it;
-      berr = residual();
-      if (it < 64) rep.berr_history[it] = berr;
-      if (o.method == 2 && berr > 0.5 * prev) cg = true;  // not contracting: conjugate gradients
-      prev = berr;
-    }
-    if (cg && berr > o.target_berr && it < o.max_iter) {
-      // conjugate gradients on A d = r from the current x, preconditioned by the factor
-      HIP_CHECK(hipMemcpyAsync(dz, dr, N * sizeof(double), hipMemcpyDeviceToDevice, st));
-      solve_device(f, dz, N, 1, N, st);
-      HIP_CHECK(hipMemcpyAsync(dp, dz, N * sizeof(double), hipMemcpyDeviceToDevice, st));
-      double rz = red.run(st, N, dr, dz, 0);
-      // keep a copy of the recursive residual in dq's partner: use dz for z, dr for r
-      double* rr = dr;
-      while (it < o.max_iter) {
-        spmv(dp, dq);
-        const double pq = red.run(st, N, dp, dq, 0);
-        if (!(pq > 0)) break;
-        const double alpha = rz / pq;
-        fc::k_axpy<<<g, 256, 0, st>>>(N, alpha, dp, dx);
-        fc::k_axpy<<<g, 256, 0, st>>>(N, -alpha, dq, rr);
-        ++it;
-        // the backward error from the true residual, into dq (dq is free until the next A p)
-        spmv(dx, dq);
-        fc::k_residual<<<g, 256, 0, st>>>(N, db, dq, dq);
-        const double rn = red.run(st, N, dq, dq, 1), xn = red.run(st, N, dx, dx, 1);
-        berr = rn / (a->norm_inf * xn + bnorm);
-        if (it < 64) rep.berr_history[it] = berr;
-        if (berr <= o.target_berr) break;
-        HIP_CHECK(hipMemcpyAsync(dz, rr, N * sizeof(double), hipMemcpyDeviceToDevice, st));
-        solve_device(f, dz, N, 1, N, st);
-        const double rz2 = red.run(st, N, rr, dz, 0);
-        fc::k_xpby<<<g, 256, 0, st>>>(N, dz, rz2 / rz, dp);
-        rz = rz2;
-      }
-    }
-    HIP_CHECK(hipEventRecord(e1, st));
-    HIP_CHECK(hipMemcpyAsync(x, dx, N * sizeof(double), hipMemcpyDeviceToHost, st));
-    HIP_CHECK(hipStreamSynchronize(st));
-    float ms = 0.f;
-    HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    rep.iterations = it;
-    rep.berr = berr;
-    rep.converged = berr <= o.target_berr;
-    rep.seconds = ms * 1e-3;
-  } catch (const std::bad_alloc&) {
-    set_error(err, FC_OUT_OF_MEMORY, "not enough device memory for refinement");
-    return FC_OUT_OF_MEMORY;
-  } catch (const DeviceError& d) {
-    set_error(err, FC_DEVICE, std::string(d.what) + ": " + hipGetErrorString(d.e));
-    return FC_DEVICE;
-  }
+    double* db = (double*)vec.get((size_t)2 * N * sizeof(double));
+    double* dx = db + N;
+    HIP_CHECK(hipMemcpyAsync(db, b, N * sizeof(double), hipMemcpyHostToDevice, f->stream));
+    refine_device(f, a, db, dx, N, o, rep, work);
+    HIP_CHECK(hipMemcpyAsync(x, dx, N * sizeof(double), hipMemcpyDeviceToHost, f->stream));
+    HIP_CHECK(hipStreamSynchronize(f->stream));
+  });
+  vec.release();
+  work.release();
   if (report) *report = rep;
-  set_error(err, FC_OK, "");
-  return FC_OK;
+  return st;
 }
+
+int fc_solve_refined_device(const fc_factor* f, const fc_matrix* a, const double* b, double* x, uint64_t n,
+                            const fc_refine_opts* opts, fc_refine_report* report, fc_error* err) {
+  if (!f || !f->valid || !a || !b || !x) {
+    set_error(err, FC_INVALID_ARGUMENT, "null argument or invalid factor");
+    return FC_INVALID_ARGUMENT;
+  }
+  if ((i64)n != f->sym->s.n || a->n != (i64)n) {
+    set_error(err, FC_INTERNAL, "right-hand side length");
+    return FC_INTERNAL;
+  }
+  const fc_refine_opts o = refine_defaults(opts);
+  fc_refine_report rep;
+  std::memset(&rep, 0, sizeof rep);
+  std::lock_guard<std::mutex> g(f->solve_mu);
+  int st = guarded(err, "not enough device memory for refinement", [&] {
+    if (n == 0) {
+      rep.converged = 1;
+      return;
+    }
+    HIP_CHECK(hipSetDevice(f->device));
+    refine_device(f, a, b, x, (i64)n, o, rep, f->refine_work);
+  });
+  if (report) *report = rep;
+  return st;
+}
+
+}  // extern "C"

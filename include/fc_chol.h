@@ -66,7 +66,22 @@ typedef struct {
   int32_t use_graph;              /* replay the level-ordered launches from a hipGraph      */
   double pivot_fallback;          /* FC_MIXED: relative pivot below which a diagonal block
                                      is refactored in FP64 (0: the default, 1e-4)           */
+  int32_t profile_mask;           /* bit k: time every launch of kernel kind k with HIP events
+                                     (fc_factor_profile); 0: none                           */
+  int32_t pad;
 } fc_factor_opts;
+
+/* Kernel kinds of the numeric phase (fc_factor_profile, fc_symbolic_info.kind_*). */
+enum {
+  FC_KIND_EXTEND_L = 0, /* children's update matrices into the fronts' L parts               */
+  FC_KIND_EXTEND_U = 1, /* ... into the fronts' update matrices                               */
+  FC_KIND_UPDATE = 2,   /* panel minus its products with the front's earlier columns (GEMM)  */
+  FC_KIND_POTRF = 3,    /* diagonal blocks                                                   */
+  FC_KIND_TRSM = 4,     /* rows below a panel against its diagonal block                     */
+  FC_KIND_SYRK = 5,     /* update matrices U = -L21 L21^T                                    */
+  FC_KIND_REDUCE = 6,   /* sums of the pieces of a GEMM split along k                        */
+  FC_KINDS = 7
+};
 
 typedef struct {
   uint64_t n, supernodes, entries, rows, updates, tasks, top;
@@ -76,6 +91,9 @@ typedef struct {
   uint64_t l_bytes_fp64, u_bytes_fp64; /* device storage of L and of the update arena      */
   int32_t by_blocks;               /* analysed on the 3x3 node blocks' graph                */
   int32_t pad;
+  uint64_t launches;               /* kernel launches of one numeric factorization          */
+  double kind_flops[8];            /* algorithmic flops per kernel kind (FC_KIND_*)          */
+  double kind_entries[8];          /* update-matrix entries added per extend-add kind       */
 } fc_symbolic_info;
 
 /* --- analysis ---------------------------------------------------------------------------- */
@@ -108,6 +126,10 @@ void fc_symbolic_free(fc_symbolic* s);
 int fc_factor_numeric(const fc_symbolic* s, const double* upper_val, uint64_t nnz, const fc_factor_opts* opts,
                       fc_factor** out, fc_error* err);
 
+/* fc_factor_numeric with the values already in device memory (no host transfer). */
+int fc_factor_numeric_device(const fc_symbolic* s, const double* upper_val_device, uint64_t nnz,
+                             const fc_factor_opts* opts, fc_factor** out, fc_error* err);
+
 /* Factor::new (cholesky.rs:1545-1547): fc_analyze then fc_factor_numeric. */
 int fc_factor_new(uint64_t n, const uint64_t* ptr, const uint64_t* idx, const double* val, uint64_t max_entries,
                   const fc_factor_opts* opts, fc_factor** out, fc_error* err);
@@ -115,6 +137,7 @@ int fc_factor_new(uint64_t n, const uint64_t* ptr, const uint64_t* idx, const do
 /* The same factor's values refactored for a new matrix of the same pattern (one Symbolic
  * serving many numeric factorizations, cholesky.rs:1390-1395). */
 int fc_refactor(fc_factor* f, const double* upper_val, uint64_t nnz, fc_error* err);
+int fc_refactor_device(fc_factor* f, const double* upper_val_device, uint64_t nnz, fc_error* err);
 
 /* Factor::entries (cholesky.rs:1557-1559). */
 uint64_t fc_entries(const fc_factor* f);
@@ -138,6 +161,12 @@ int fc_factor_export(const fc_factor* f, double* out, uint64_t len, fc_error* er
 
 /* Seconds the last numeric factorization took on the device (events around the launches). */
 double fc_factor_seconds(const fc_factor* f);
+
+/* Kernel kind `kind` in the last numeric factorization: launches timed and their summed
+ * device seconds (when opts.profile_mask selected the kind), and the kind's algorithmic flops
+ * and extend-add entries over the whole factorization. */
+int fc_factor_profile(const fc_factor* f, int32_t kind, double* launches, double* seconds, double* flops,
+                      double* entries);
 
 void fc_factor_free(fc_factor* f);
 
@@ -170,6 +199,10 @@ typedef struct {
  * of n doubles (x may alias b). */
 int fc_solve_refined(const fc_factor* f, const fc_matrix* a, const double* b, double* x, uint64_t n,
                      const fc_refine_opts* opts, fc_refine_report* report, fc_error* err);
+
+/* fc_solve_refined with b and x in device memory (n doubles each, distinct). */
+int fc_solve_refined_device(const fc_factor* f, const fc_matrix* a, const double* b, double* x, uint64_t n,
+                            const fc_refine_opts* opts, fc_refine_report* report, fc_error* err);
 
 /* Library version string. */
 const char* fc_version(void);
