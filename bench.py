@@ -103,7 +103,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cells", type=int, default=80)
     ap.add_argument("--form", default="mixed", choices=["mixed", "fp32", "fp64"])
-    ap.add_argument("--bf16-terms", type=int, default=1)
+    ap.add_argument("--bf16-terms", type=int, default=3)
     ap.add_argument("--cpu-cells", type=int, default=28)
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count())
     ap.add_argument("--no-cpu", action="store_true")
@@ -168,10 +168,14 @@ def main():
         roof = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK, unit="GB/s", frac=achieved / HBM_PEAK,
                     traffic=None, kernel=f"k_extend_add ({dom})", launches=dk["launches"])
     else:
-        peak = PEAK["fp64"] if form == "fp64" else PEAK["fp32"]
+        bf16 = form == "mixed" and dom in ("syrk", "update")
+        peak = PEAK["fp64"] if form == "fp64" else (PEAK["bf16"] if bf16 else PEAK["fp32"])
         achieved = dk["flops"] / dk["seconds"] / 1e12
+        kname = (f"k_gemm_bf16<{args.bf16_terms}> ({dom})" if bf16 else
+                 f"k_gemm<{'double' if form == 'fp64' else 'float'}> ({dom})" if dom in ("syrk", "update", "trsm") else
+                 f"{dom}<{'double' if form == 'fp64' else 'float'}>")
         roof = dict(bound="mfma", achieved=achieved, peak=peak, unit="TFLOP/s", frac=achieved / peak, traffic=None,
-                    kernel=f"k_{dom}<{'double' if form == 'fp64' else 'float'}>", launches=dk["launches"],
+                    kernel=kname, launches=dk["launches"], mfma_products_per_flop=args.bf16_terms if bf16 else 1,
                     flops_per_launch=dk["flops"] / max(dk["launches"], 1),
                     avg_launch_s=dk["seconds"] / max(dk["launches"], 1))
     line = dict(metric=METRIC, value=F / t_step / 1e12, unit="TFLOP/s", n_gpus=1, steps=args.steps,

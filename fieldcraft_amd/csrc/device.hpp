@@ -16,9 +16,11 @@ struct FrontDev {
   int64_t upos;
   int64_t rel_off;   // relative map of the rows below into the parent front
   int64_t rows_off;  // front rows (postorder column numbers) in the rows array
+  int64_t inv0;      // inverses of the diagonal blocks: panel at c0 at inv0 + (c0 / NB) NB^2, ld = its width
   int32_t ld, w, m, col0;
   int32_t parent, kid0, kid1, level;  // children are kids[kid0..kid1)
 };
+static_assert(sizeof(FrontDev) == 72, "FrontDev layout");
 
 // A unit of work of one launch; the meaning of a, b, c, d depends on the kernel.
 struct Item {

@@ -34,6 +34,35 @@ __global__ void k_gather_diag(const double* __restrict__ aval, const int64_t* __
     diag[c] = diag_src[c] >= 0 ? aval[diag_src[c]] : 0.0;
 }
 
+// Per front: the exponent e of the smallest power of two with sqrt(A_ii) <= 2^e over the
+// front's rows. For an SPD matrix every factor entry satisfies |L_ij| <= sqrt(A_ii) (row i of L
+// has squared norm A_ii), so 2^-e scales the front's columns into [-1, 1]. 0 when no row has a
+// positive finite diagonal.
+__global__ __launch_bounds__(256) void k_front_scale(const FrontDev* __restrict__ fronts, const int32_t* __restrict__ rows,
+                                                     const double* __restrict__ diag, int64_t nf,
+                                                     int32_t* __restrict__ fexp) {
+  __shared__ double red[4];
+  for (int64_t f = blockIdx.x; f < nf; f += gridDim.x) {
+    const FrontDev F = fronts[f];
+    const int32_t* fr = rows + F.rows_off;
+    double mx = 0.0;
+    for (int i = threadIdx.x; i < F.m; i += blockDim.x) {
+      const double d = diag[fr[i]];
+      if (isfinite(d) && d > mx) mx = d;
+    }
+    for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      mx = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+      int e = 0;
+      if (mx > 0.0) (void)frexp(sqrt(mx), &e);
+      fexp[f] = e;
+    }
+    __syncthreads();
+  }
+}
+
 template <typename T>
 __global__ void k_fill(T* __restrict__ p, int64_t len, T v) {
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < len; e += (int64_t)gridDim.x * blockDim.x) p[e] = v;

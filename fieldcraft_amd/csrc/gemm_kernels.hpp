@@ -36,7 +36,7 @@ struct GemmItem {
   int32_t mode;           // GemmMode
   int32_t diag;           // store C(i, j) only when i - j + diag >= 0
   int8_t asrc, bsrc, csrc, pad0;
-  int32_t pad1;
+  int32_t front;          // the front both operands come from (its scale in the mixed form)
 };
 static_assert(sizeof(GemmItem) == 64, "GemmItem layout");
 
@@ -160,13 +160,159 @@ __global__ __launch_bounds__(256, 2) void k_gemm(const GemmItem* __restrict__ it
       }
 }
 
-// Sum of the partial tiles of a split product, in order, applied to C as its mode says.
+// ------------------------------------------------------------------------------------------
+// The Schur products of the mixed form: FP32 operands in HBM, multiplied as BF16 on the
+// 16x16x32 BF16 matrix cores with FP32 accumulation. Both operands of a product come from one
+// front and are scaled by that front's power of two, 2^-fexp[front], which bounds them by 1
+// (|L_ij| <= sqrt(A_ii) for an SPD matrix); the result is scaled back exactly. With TERMS = 3
+// each scaled operand x is split into hi = bf16(x) and lo = bf16(x - hi) and the product is
+// hi*hi + hi*lo + lo*hi, which carries about 16 significant bits; TERMS = 1 keeps hi*hi.
+//
+// Same tile and item layout as k_gemm (128 x 128, 4 waves of 64 x 64, modes and split-K
+// pieces alike). k is staged BK = 32 at a time: a loader thread reads 16 consecutive k of one
+// row (coalesced across threads along the rows), converts and writes them to LDS as k-major
+// rows, from which each lane reads its 8 k of a fragment as one 16-byte load.
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+constexpr int BK = 32;
+constexpr int BROW = BK + 8;  // LDS row stride (80 bytes: 16-byte aligned, rows spread over the banks)
+
+template <int TERMS>
+__global__ __launch_bounds__(256, 2) void k_gemm_bf16(const GemmItem* __restrict__ items, float* L, float* U,
+                                                      float* Part, const int32_t* __restrict__ fexp) {
+  static_assert(TERMS == 1 || TERMS == 3, "one or three BF16 products");
+  constexpr int NS = TERMS == 3 ? 2 : 1;  // hi (and lo) copies of each operand
+  __shared__ __attribute__((aligned(16))) __bf16 As[2][NS][GB][BROW];
+  __shared__ __attribute__((aligned(16))) __bf16 Bs[2][NS][GB][BROW];
+  const GemmItem it = items[blockIdx.x];
+  const int e = fexp[it.front];
+  const float sc = ldexpf(1.0f, -e);  // exact power of two
+  auto base = [&](int8_t src) { return src == S_L ? L : (src == S_U ? U : Part); };
+  const float* __restrict__ A = base(it.asrc) + it.a;
+  const float* __restrict__ B = base(it.bsrc) + it.b;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wi = (wave >> 1) * 64, wj = (wave & 1) * 64;
+  const int lrow = tid & (GB - 1), lk = (tid >> 7) * 16;  // loader: one row, 16 consecutive k
+  const bool arow = lrow < it.m, brow = lrow < it.n;
+  float ra[16], rb[16];
+  auto load = [&](int ks) {
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int k = ks + lk + r;
+      const bool kin = k < it.k;
+      ra[r] = (arow && kin) ? A[(int64_t)k * it.lda + lrow] : 0.0f;
+      rb[r] = (brow && kin) ? B[(int64_t)k * it.ldb + lrow] : 0.0f;
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      v8bf ah, bh, al, bl;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const float xa = ra[8 * h + q] * sc, xb = rb[8 * h + q] * sc;
+        ah[q] = (__bf16)xa;
+        bh[q] = (__bf16)xb;
+        if (TERMS == 3) {
+          al[q] = (__bf16)(xa - (float)ah[q]);
+          bl[q] = (__bf16)(xb - (float)bh[q]);
+        }
+      }
+      *(v8bf*)&As[buf][0][lrow][lk + 8 * h] = ah;
+      *(v8bf*)&Bs[buf][0][lrow][lk + 8 * h] = bh;
+      if (TERMS == 3) {
+        *(v8bf*)&As[buf][NS - 1][lrow][lk + 8 * h] = al;
+        *(v8bf*)&Bs[buf][NS - 1][lrow][lk + 8 * h] = bl;
+      }
+    }
+  };
+  v4f acc[4][4];
+#pragma unroll
+  for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 4; ++bj) acc[bi][bj] = v4f{0, 0, 0, 0};
+  const int stages = (it.k + BK - 1) / BK;
+  const int fr = lane & 15, fk = 8 * (lane >> 4);  // a lane's fragment row and first k
+  load(0);
+  store(0);
+  __syncthreads();
+  for (int s = 0; s < stages; ++s) {
+    const int cur = s & 1;
+    if (s + 1 < stages) load((s + 1) * BK);
+    v8bf fj[4], fi[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      fj[b] = *(const v8bf*)&Bs[cur][0][wj + 16 * b + fr][fk];
+      fi[b] = *(const v8bf*)&As[cur][0][wi + 16 * b + fr][fk];
+    }
+    if (TERMS == 3) {
+      v8bf gj[4], gi[4];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        gj[b] = *(const v8bf*)&Bs[cur][NS - 1][wj + 16 * b + fr][fk];
+        gi[b] = *(const v8bf*)&As[cur][NS - 1][wi + 16 * b + fr][fk];
+      }
+#pragma unroll
+      for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 4; ++bj) {
+          acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(gj[bj], fi[bi], acc[bi][bj], 0, 0, 0);
+          acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fj[bj], gi[bi], acc[bi][bj], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+      for (int bj = 0; bj < 4; ++bj)
+        acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fj[bj], fi[bi], acc[bi][bj], 0, 0, 0);
+    if (s + 1 < stages) store(cur ^ 1);
+    __syncthreads();
+  }
+  // acc[bi][bj][v] = C(wi + 16 bi + lane % 16, wj + 16 bj + 4 (lane / 16) + v) / sc^2
+  const float unscale = ldexpf(1.0f, 2 * e);
+  const int il = lane & 15;
+  if (it.mode == G_PART) {
+    float* P = Part + it.c;
+#pragma unroll
+    for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+      for (int bj = 0; bj < 4; ++bj)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int i = wi + 16 * bi + il, j = wj + 16 * bj + Mma<float>::row(lane, v);
+          P[(int64_t)j * GB + i] = acc[bi][bj][v] * unscale;
+        }
+    return;
+  }
+  float* C = base(it.csrc) + it.c;
+#pragma unroll
+  for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 4; ++bj)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = wi + 16 * bi + il, j = wj + 16 * bj + Mma<float>::row(lane, v);
+        if (i < it.m && j < it.n && i - j + it.diag >= 0) {
+          float* p = C + (int64_t)j * it.ldc + i;
+          const float x = acc[bi][bj][v] * unscale;
+          if (it.mode == G_SUB) *p -= x;
+          else if (it.mode == G_NEG) *p = -x;
+          else *p = x;
+        }
+      }
+}
+
+// Sum of the partial tiles of a split product, in order, applied to C as its mode says. Each
+// tile is shared by RSPLIT workgroups (GB * GB / RSPLIT entries each).
+constexpr int RSPLIT = 16;
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_reduce_parts(const ReduceItem* __restrict__ items, T* L, T* U,
                                                       const T* __restrict__ Part) {
-  const ReduceItem it = items[blockIdx.x];
+  const ReduceItem it = items[blockIdx.x / RSPLIT];
   T* C = (it.csrc == S_L ? L : U) + it.c;
-  for (int e = threadIdx.x; e < GB * GB; e += blockDim.x) {
+  constexpr int CHUNK = GB * GB / RSPLIT;
+  const int e0 = (blockIdx.x % RSPLIT) * CHUNK;
+  for (int e = e0 + threadIdx.x; e < e0 + CHUNK; e += blockDim.x) {
     const int i = e % GB, j = e / GB;
     if (i >= it.m || j >= it.n || i - j + it.diag < 0) continue;
     T s = 0;
@@ -185,63 +331,193 @@ __global__ __launch_bounds__(256) void k_reduce_parts(const ReduceItem* __restri
 
 constexpr double PIVOT_TOL = 1e-10;  // cholesky.rs:48
 
-template <typename T>
-__global__ __launch_bounds__(256, 1) void k_potrf_inv(const Item* __restrict__ items, const FrontDev* __restrict__ fronts,
-                                                      T* __restrict__ L, const double* __restrict__ diag,
-                                                      unsigned long long* __restrict__ fail, T* __restrict__ Inv) {
-  __shared__ double S[NB][NB + 1];  // lower: L11; strict upper S[j][i] (j < i): Linv(i, j)
-  __shared__ double dinv[NB];
-  __shared__ int bad;
+// K: the widest block the variant takes (32, 64 or NB); the host sends each panel to the
+// narrowest variant that holds it. One barrier per column: column c's trailing update reads
+// column c unscaled (times 1 / d_c^2) while column c - 1 is scaled in the same phase.
+template <typename T, int K>
+__global__ __launch_bounds__(K == NB ? 256 : 64 * (K / 32), 1) void k_potrf_inv(
+    const Item* __restrict__ items, const FrontDev* __restrict__ fronts, T* __restrict__ L,
+    const double* __restrict__ diag, unsigned long long* __restrict__ fail, T* __restrict__ Inv) {
+  __shared__ double S[K][K + 1];  // lower: L11; strict upper S[j][i] (j < i): Linv(i, j)
+  __shared__ double dv[K];        // the pivots' square roots, then their inverses
   const Item it = items[blockIdx.x];
   const FrontDev F = fronts[it.f];
-  const int c0 = it.a, pw = min(NB, F.w - c0);
+  const int c0 = it.a, pw = min(K, F.w - c0);
   T* base = L + F.lpos + (int64_t)c0 * F.ld + c0;
-  for (int e = threadIdx.x; e < NB * NB; e += blockDim.x) {
-    const int i = e % NB, j = e / NB;
+  for (int e = threadIdx.x; e < K * K; e += blockDim.x) {
+    const int i = e % K, j = e / K;
     if (i >= j) S[i][j] = (i < pw && j < pw) ? (double)base[(int64_t)j * F.ld + i] : 0.0;
   }
-  if (threadIdx.x == 0) bad = 0;
   __syncthreads();
-  for (int c = 0; c < pw; ++c) {
-    if (threadIdx.x == 0) {
-      const double d = S[c][c];
-      const double a = diag[F.col0 + c0 + c];
-      if (!(d > PIVOT_TOL * fabs(a)) || !isfinite(d)) {
-        if (!bad) atomicMin(fail, (unsigned long long)(F.col0 + c0 + c));
-        bad = 1;
-      }
-      S[c][c] = sqrt(d);
+  bool bad = false;
+  for (int c = 0; c <= pw; ++c) {
+    if (c > 0) {  // scale the previous column
+      const double d = dv[c - 1];
+      for (int i = c + threadIdx.x; i < pw; i += blockDim.x) S[i][c - 1] /= d;
+      if (threadIdx.x == 0) S[c - 1][c - 1] = d;
     }
-    __syncthreads();
-    const double d = S[c][c];
-    for (int i = c + 1 + threadIdx.x; i < pw; i += blockDim.x) S[i][c] /= d;
-    __syncthreads();
-    const int rest = pw - c - 1;
-    for (int e = threadIdx.x; e < rest * rest; e += blockDim.x) {
-      const int i = c + 1 + e % rest, j = c + 1 + e / rest;
-      if (i >= j) S[i][j] -= S[i][c] * S[j][c];
+    if (c < pw) {
+      const double p = S[c][c];
+      const double d = sqrt(p);
+      if (threadIdx.x == 0) {
+        dv[c] = d;
+        const double a = diag[F.col0 + c0 + c];
+        if (!bad && (!(p > PIVOT_TOL * fabs(a)) || !isfinite(p))) {
+          atomicMin(fail, (unsigned long long)(F.col0 + c0 + c));
+          bad = true;
+        }
+      }
+      const double q = 1.0 / (d * d);
+      const int rest = pw - c - 1;
+      for (int e = threadIdx.x; e < rest * rest; e += blockDim.x) {
+        const int i = c + 1 + e % rest, j = c + 1 + e / rest;
+        if (i >= j) S[i][j] -= S[i][c] * S[j][c] * q;
+      }
     }
     __syncthreads();
   }
   // Linv row by row: Linv(i, j) = -(sum_{k=j}^{i-1} L(i, k) Linv(k, j)) / L(i, i), j < i
-  for (int i = threadIdx.x; i < pw; i += blockDim.x) dinv[i] = 1.0 / S[i][i];
+  for (int i = threadIdx.x; i < pw; i += blockDim.x) dv[i] = 1.0 / S[i][i];
   __syncthreads();
   for (int i = 1; i < pw; ++i) {
     for (int j = threadIdx.x; j < i; j += blockDim.x) {
-      double s = S[i][j] * dinv[j];
+      double s = S[i][j] * dv[j];
       for (int k = j + 1; k < i; ++k) s += S[i][k] * S[j][k];  // S[j][k] = Linv(k, j)
-      S[j][i] = -s * dinv[i];
+      S[j][i] = -s * dv[i];
     }
     __syncthreads();
   }
-  T* inv = Inv + (int64_t)it.b * NB * NB;
-  for (int e = threadIdx.x; e < NB * NB; e += blockDim.x) {
-    const int i = e % NB, j = e / NB;
-    double v = 0.0;
-    if (i < pw && j < pw) v = i == j ? dinv[i] : (i > j ? S[j][i] : 0.0);
-    inv[e] = (T)v;
-    if (i < pw && j < pw && i >= j) base[(int64_t)j * F.ld + i] = (T)S[i][j];
+  T* inv = Inv + F.inv0 + (int64_t)(c0 / NB) * NB * NB;  // ld = pw
+  for (int e = threadIdx.x; e < K * K; e += blockDim.x) {
+    const int i = e % K, j = e / K;
+    if (i < pw && j < pw) {
+      inv[(int64_t)j * pw + i] = (T)(i == j ? dv[i] : (i > j ? S[j][i] : 0.0));
+      if (i >= j) base[(int64_t)j * F.ld + i] = (T)S[i][j];
+    }
   }
+}
+
+// The same for blocks up to NB = 128 wide, with the block in registers: 256 threads as a
+// 16 x 16 grid, thread (ti, tj) holding entries (ti + 16 a, tj + 16 b), a, b < 8. A column
+// step publishes only its pivot and its column through LDS, and every thread updates its own
+// 64 entries, so the rank-1 updates cost no LDS traffic. The inverse is formed the same way
+// (right-looking on L X = I) with L read from LDS. Two barriers per column in each part.
+template <typename T>
+__global__ __launch_bounds__(256, 1) void k_potrf_inv128(const Item* __restrict__ items, const FrontDev* __restrict__ fronts,
+                                                         T* __restrict__ L, const double* __restrict__ diag,
+                                                         unsigned long long* __restrict__ fail, T* __restrict__ Inv) {
+  constexpr int G = 16, R = NB / G;
+  __shared__ double Ls[NB][NB + 1];
+  __shared__ double col[NB], row[NB], piv[NB + 1];
+  const Item it = items[blockIdx.x];
+  const FrontDev F = fronts[it.f];
+  const int c0 = it.a, pw = min(NB, F.w - c0);
+  T* base = L + F.lpos + (int64_t)c0 * F.ld + c0;
+  const int ti = threadIdx.x % G, tj = threadIdx.x / G;
+  double s[R][R];
+#pragma unroll
+  for (int a = 0; a < R; ++a)
+#pragma unroll
+    for (int b = 0; b < R; ++b) {
+      const int i = ti + G * a, j = tj + G * b;
+      s[a][b] = (i < pw && j < pw && i >= j) ? (double)base[(int64_t)j * F.ld + i] : (i == j ? 1.0 : 0.0);
+    }
+  if (threadIdx.x == 0) piv[0] = s[0][0];
+  __syncthreads();
+  bool bad = false;
+#pragma unroll
+  for (int cb = 0; cb < R; ++cb) {
+    for (int cc = 0; cc < G; ++cc) {
+      const int c = G * cb + cc;
+      if (c >= pw) break;
+      const double p = piv[c];
+      const double d = sqrt(p);
+      if (threadIdx.x == 0 && !bad) {
+        const double a0 = diag[F.col0 + c0 + c];
+        if (!(p > PIVOT_TOL * fabs(a0)) || !isfinite(p)) {
+          atomicMin(fail, (unsigned long long)(F.col0 + c0 + c));
+          bad = true;
+        }
+      }
+      if (tj == cc) {  // the owners of column c scale it and publish it
+#pragma unroll
+        for (int a = 0; a < R; ++a) {
+          const int i = ti + G * a;
+          if (i > c) {
+            s[a][cb] /= d;
+            col[i] = s[a][cb];
+          } else if (i == c) {
+            s[a][cb] = d;
+          }
+        }
+      }
+      __syncthreads();
+      double li[R], lj[R];
+#pragma unroll
+      for (int a = 0; a < R; ++a) li[a] = (ti + G * a > c) ? col[ti + G * a] : 0.0;
+#pragma unroll
+      for (int b = 0; b < R; ++b) lj[b] = (tj + G * b > c) ? col[tj + G * b] : 0.0;
+#pragma unroll
+      for (int a = 0; a < R; ++a)
+#pragma unroll
+        for (int b = 0; b < R; ++b)
+          if (ti + G * a >= tj + G * b) s[a][b] -= li[a] * lj[b];
+      // the next pivot, from its owner
+      const int n1 = c + 1;
+      if (n1 < pw && ti == n1 % G && tj == n1 % G) piv[n1] = (cc + 1 < G) ? s[cb][cb] : s[(cb + 1) % R][(cb + 1) % R];
+      __syncthreads();
+    }
+  }
+  // L11 to LDS (and back to the front), then X = L11^-1 in registers
+#pragma unroll
+  for (int a = 0; a < R; ++a)
+#pragma unroll
+    for (int b = 0; b < R; ++b) {
+      const int i = ti + G * a, j = tj + G * b;
+      if (i >= j) {
+        Ls[i][j] = s[a][b];
+        if (i < pw && j < pw) base[(int64_t)j * F.ld + i] = (T)s[a][b];
+      }
+    }
+#pragma unroll
+  for (int a = 0; a < R; ++a)
+#pragma unroll
+    for (int b = 0; b < R; ++b) s[a][b] = (ti + G * a == tj + G * b) ? 1.0 : 0.0;  // X = I
+  __syncthreads();
+#pragma unroll
+  for (int kb = 0; kb < R; ++kb) {
+    for (int kc = 0; kc < G; ++kc) {
+      const int k = G * kb + kc;
+      if (k >= pw) break;
+      const double dk = Ls[k][k];
+      if (ti == kc) {  // the owners of row k of X finish it and publish it
+#pragma unroll
+        for (int b = 0; b < R; ++b) {
+          s[kb][b] /= dk;
+          row[tj + G * b] = s[kb][b];
+        }
+      }
+      __syncthreads();
+      double lk[R], xk[R];
+#pragma unroll
+      for (int a = 0; a < R; ++a) lk[a] = (ti + G * a > k) ? Ls[ti + G * a][k] : 0.0;
+#pragma unroll
+      for (int b = 0; b < R; ++b) xk[b] = row[tj + G * b];
+#pragma unroll
+      for (int a = 0; a < R; ++a)
+#pragma unroll
+        for (int b = 0; b < R; ++b) s[a][b] -= lk[a] * xk[b];
+      __syncthreads();
+    }
+  }
+  T* inv = Inv + F.inv0 + (int64_t)(c0 / NB) * NB * NB;  // ld = pw
+#pragma unroll
+  for (int a = 0; a < R; ++a)
+#pragma unroll
+    for (int b = 0; b < R; ++b) {
+      const int i = ti + G * a, j = tj + G * b;
+      if (i < pw && j < pw) inv[(int64_t)j * pw + i] = (T)(i >= j ? s[a][b] : 0.0);
+    }
 }
 
 }  // namespace fc

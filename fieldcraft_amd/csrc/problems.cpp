@@ -318,7 +318,7 @@ struct fcp_problem;
 //   kind        0 = hex8, 1 = tet10 (six Kuhn tetrahedra a cell, mid-side nodes on the doubled
 //               lattice, as the reference's test grids, solver tests/common/mod.rs:23-98)
 //   young       per element (n_young == number of elements) or one value (n_young == 1)
-//   clamp_face  0..5 = x=0, x=L, y=0, y=L, z=0, z=L: every dof of its nodes is fixed
+//   clamp_face  0..5 = x=0, x=L, y=0, y=L, z=0, z=L: every dof of its nodes is fixed; -1: none
 //   jitter      fraction of the cell size by which corner nodes off the clamped face move
 //               (uniform in +-jitter per direction, seeded); mid-side nodes stay at the
 //               midpoints of their corners, so elements stay straight-sided
@@ -343,6 +343,7 @@ fcp_problem* fcp_block(int kind, i64 nx, i64 ny, i64 nz, double lx, double ly, d
       }
   auto on_face = [&](i64 i, i64 j, i64 k, int face) {
     switch (face) {
+      case -1: return false;  // no supports
       case 0: return i == 0;
       case 1: return i == sx - 1;
       case 2: return j == 0;

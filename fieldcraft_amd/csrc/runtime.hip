@@ -10,6 +10,7 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "factor_kernels.hpp"
@@ -97,7 +98,8 @@ struct Program {
   std::vector<fc::GemmItem> gitems;    // GEMM tiles (UPDATE, TRSM, SYRK launches)
   std::vector<fc::ReduceItem> ritems;  // split-K reductions
   std::vector<fc::Launch> launches;
-  i64 inv_slots = 0;                   // diagonal-block inverses one launch keeps
+  std::vector<i64> f_inv0;             // per front: its diagonal-block inverses in the Inv buffer
+  i64 inv_size = 0;                    // entries of the Inv buffer (kept with the factor)
   i64 part_size = 0;                   // entries of the split-K workspace
   std::vector<fc::Item> sitems;       // solve work items
   std::vector<fc::Launch> sfwd, sbwd; // forward launches (leaves first), backward (root first)
@@ -107,7 +109,8 @@ struct Program {
 };
 
 // kinds of solve launches
-enum SolveKind : int32_t { S_GATHER = 0, S_FWD_BLOCK = 1, S_FWD_GEMV = 2, S_BWD_BELOW = 3, S_BWD_BLOCK = 4, S_BWD_GEMV = 5 };
+enum SolveKind : int32_t { S_GATHER = 0, S_FWD_STEP = 1, S_BWD_BELOW = 2, S_BWD_STEP = 3 };
+static_assert(fc::SNB == fc::NB, "solve blocks are the factorization's panels");
 
 // lower pairs (i >= j) of rows [r0, r1) x columns [c0, c1)
 double lower_pairs(i64 r0, i64 r1, i64 c0, i64 c1) {
@@ -145,7 +148,7 @@ void add_gemm(Program& p, const std::vector<Tile>& tiles, int kind, i64 level) {
       continue;
     }
     i64 chunk = (t.g.k + split - 1) / split;
-    chunk = (chunk + fc::GK - 1) / fc::GK * fc::GK;
+    chunk = (chunk + fc::BK - 1) / fc::BK * fc::BK;  // whole LDS stages of either kernel
     split = (t.g.k + chunk - 1) / chunk;
     fc::ReduceItem r{};
     r.c = t.g.c;
@@ -191,6 +194,14 @@ Program build_program(const fc::Symbolic& s) {
     }
     flops = entries = 0;
   };
+  // the inverses of every front's diagonal blocks (NB x NB, the last one pw x pw), kept for
+  // the solves
+  p.f_inv0.resize(s.nf);
+  for (i64 f = 0; f < s.nf; ++f) {
+    p.f_inv0[f] = p.inv_size;
+    const i64 full = s.f_w[f] / fc::NB, last = s.f_w[f] % fc::NB;
+    p.inv_size += full * fc::NB * fc::NB + last * last;
+  }
   std::vector<double> colent;  // child entries landing in each parent column
   for (i64 l = 0; l < s.levels; ++l) {
     const i64* fl = &s.lvl_fronts[s.lvl_ptr[l]];
@@ -244,25 +255,31 @@ Program build_program(const fc::Symbolic& s) {
             g.mode = fc::G_SUB;
             g.diag = (int32_t)(r - c0);
             g.asrc = g.bsrc = g.csrc = fc::S_L;
+            g.front = (int32_t)f;
             tiles.push_back(Tile{g, 2.0 * (double)c0 * lower_pairs(r, r + m, c0, c1)});
           }
         }
         add_gemm(p, tiles, fc::K_UPDATE, l);
       }
-      i64 slot = 0;
-      i0 = (i64)p.items.size();
-      for (i64 t = 0; t < nl; ++t) {
-        const i64 f = fl[t];
-        if (s.f_w[f] <= c0) continue;
-        const double pw = (double)std::min<i64>(fc::NB, s.f_w[f] - c0);
-        p.items.push_back(Item{(int32_t)f, (int32_t)c0, (int32_t)slot++, 0, 0, 0, 0, 0});
-        flops += pw * pw * pw / 3.0;
+      // diagonal blocks, each to the narrowest kernel variant that holds it
+      for (int variant : {32, 64, (int)fc::NB}) {
+        i0 = (i64)p.items.size();
+        for (i64 t = 0; t < nl; ++t) {
+          const i64 f = fl[t];
+          if (s.f_w[f] <= c0) continue;
+          const i64 pw = std::min<i64>(fc::NB, s.f_w[f] - c0);
+          const int need = pw <= 32 ? 32 : (pw <= 64 ? 64 : (int)fc::NB);
+          if (need == variant) {
+            p.items.push_back(Item{(int32_t)f, (int32_t)c0, 0, 0, 0, 0, 0, 0});
+            flops += (double)pw * (pw + 1) * (2 * pw + 1) / 6.0;  // sum_{t <= pw} t^2
+          }
+        }
+        add(fc::K_POTRF, l, i0);
+        if (!p.launches.empty() && p.launches.back().kind == fc::K_POTRF && p.launches.back().item0 == i0)
+          p.launches.back().pad = variant;
       }
-      p.inv_slots = std::max(p.inv_slots, slot);
-      add(fc::K_POTRF, l, i0);
       // the rows below the diagonal block: X = B Linv^T, in place
       tiles.clear();
-      slot = 0;
       for (i64 t = 0; t < nl; ++t) {
         const i64 f = fl[t];
         if (s.f_w[f] <= c0) continue;
@@ -272,19 +289,19 @@ Program build_program(const fc::Symbolic& s) {
           const i64 m = std::min<i64>(fc::GB, s.f_m[f] - r);
           fc::GemmItem g{};
           g.a = s.f_lpos[f] + c0 * s.f_ld[f] + r;
-          g.b = slot * (i64)fc::NB * fc::NB;
+          g.b = p.f_inv0[f] + (c0 / fc::NB) * (i64)fc::NB * fc::NB;
           g.c = g.a;
           g.lda = g.ldc = (int32_t)s.f_ld[f];
-          g.ldb = fc::NB;
+          g.ldb = (int32_t)pw;
           g.m = (int32_t)m;
           g.n = g.k = (int32_t)pw;
           g.mode = fc::G_SET;
           g.diag = fc::GB;
           g.asrc = g.csrc = fc::S_L;
           g.bsrc = fc::S_INV;
+          g.front = (int32_t)f;
           tiles.push_back(Tile{g, (double)m * (double)pw * (double)pw});
         }
-        ++slot;
       }
       add_gemm(p, tiles, fc::K_TRSM, l);
     }
@@ -309,6 +326,7 @@ Program build_program(const fc::Symbolic& s) {
           g.diag = (int32_t)(a - b);
           g.asrc = g.bsrc = fc::S_L;
           g.csrc = fc::S_U;
+          g.front = (int32_t)f;
           tiles.push_back(Tile{g, 2.0 * (double)w * lower_pairs(a, a + m, b, b + nn)});
         }
     }
@@ -339,21 +357,20 @@ Program build_program(const fc::Symbolic& s) {
     i64 i0 = (i64)p.sitems.size();
     for (i64 t = 0; t < nl; ++t) p.sitems.push_back(Item{(int32_t)fl[t], 0, 0, 0, 0, 0, 0, 0});
     sadd(p.sfwd, S_GATHER, l, i0);
-    for (i64 c0 = 0; c0 < wmax; c0 += fc::SB) {
+    for (i64 c0 = 0; c0 < wmax; c0 += fc::SNB) {
       i0 = (i64)p.sitems.size();
       for (i64 t = 0; t < nl; ++t) {
         const i64 f = fl[t];
-        if (s.f_w[f] > c0) p.sitems.push_back(Item{(int32_t)f, (int32_t)c0, s.f_w[f] <= fc::SB ? 1 : 0, 0, 0, 0, 0, 0});
+        if (s.f_w[f] <= c0) continue;
+        const i64 c1 = std::min(c0 + fc::SNB, s.f_w[f]);
+        // one item stores the block's solution even when no rows lie below it
+        i64 r = c1;
+        do {
+          p.sitems.push_back(Item{(int32_t)f, (int32_t)c0, (int32_t)r, r == c1 ? 1 : 0, 0, 0, 0, 0});
+          r += fc::SROWS;
+        } while (r < s.f_m[f]);
       }
-      sadd(p.sfwd, S_FWD_BLOCK, l, i0);
-      i0 = (i64)p.sitems.size();
-      for (i64 t = 0; t < nl; ++t) {
-        const i64 f = fl[t];
-        if (s.f_w[f] <= fc::SB || s.f_w[f] <= c0) continue;
-        for (i64 r = std::min(c0 + fc::SB, s.f_w[f]); r < s.f_m[f]; r += fc::SROWS)
-          p.sitems.push_back(Item{(int32_t)f, (int32_t)c0, (int32_t)r, 0, 0, 0, 0, 0});
-      }
-      sadd(p.sfwd, S_FWD_GEMV, l, i0);
+      sadd(p.sfwd, S_FWD_STEP, l, i0);
     }
   }
   for (i64 l = s.levels - 1; l >= 0; --l) {
@@ -368,20 +385,21 @@ Program build_program(const fc::Symbolic& s) {
       for (i64 j = 0; j < s.f_w[f]; j += fc::SCOLS) p.sitems.push_back(Item{(int32_t)f, (int32_t)j, 0, 0, 0, 0, 0, 0});
     }
     sadd(p.sbwd, S_BWD_BELOW, l, i0);
-    const i64 nblocks = (wmax + fc::SB - 1) / fc::SB;
+    const i64 nblocks = (wmax + fc::SNB - 1) / fc::SNB;
     for (i64 b = nblocks - 1; b >= 0; --b) {
-      const i64 c0 = b * fc::SB;
-      i0 = (i64)p.sitems.size();
-      for (i64 t = 0; t < nl; ++t)
-        if (s.f_w[fl[t]] > c0) p.sitems.push_back(Item{(int32_t)fl[t], (int32_t)c0, 0, 0, 0, 0, 0, 0});
-      sadd(p.sbwd, S_BWD_BLOCK, l, i0);
+      const i64 c0 = b * fc::SNB;
       i0 = (i64)p.sitems.size();
       for (i64 t = 0; t < nl; ++t) {
         const i64 f = fl[t];
         if (s.f_w[f] <= c0) continue;
-        for (i64 j = 0; j < c0; j += fc::SCOLS) p.sitems.push_back(Item{(int32_t)f, (int32_t)c0, (int32_t)j, 0, 0, 0, 0, 0});
+        // one item stores the block's solution even when no columns lie before it
+        i64 j = 0;
+        do {
+          p.sitems.push_back(Item{(int32_t)f, (int32_t)c0, (int32_t)j, j == 0 ? 1 : 0, 0, 0, 0, 0});
+          j += fc::SCOLS;
+        } while (j < c0);
       }
-      sadd(p.sbwd, S_BWD_GEMV, l, i0);
+      sadd(p.sbwd, S_BWD_STEP, l, i0);
     }
   }
   p.uoff.resize(s.nf);
@@ -442,6 +460,7 @@ struct fc_symbolic {
       d.upos = S.f_upos[f] < 0 ? 0 : S.f_upos[f];
       d.rel_off = S.f_rel_off[f];
       d.rows_off = S.f_rptr[f];
+      d.inv0 = prog.f_inv0[f];
       d.ld = (int32_t)S.f_ld[f];
       d.w = (int32_t)S.f_w[f];
       d.m = (int32_t)S.f_m[f];
@@ -485,9 +504,10 @@ struct fc_factor {
   unsigned long long* fail = nullptr;
   Buffer U;                // update-matrix arena (kept between factorizations)
   Buffer inv, part;        // diagonal-block inverses, split-K pieces
+  Buffer fexp;             // mixed form: per front, the exponent of its power-of-two scale
   Buffer vals;             // the matrix values on the device (host-value entry points)
   mutable std::mutex solve_mu;
-  mutable Buffer xp, usolve, refine_work;  // solve workspaces
+  mutable Buffer xp, yp, usolve, refine_work;  // solve workspaces
   double seconds = 0.0;
   double kind_seconds[fc::KINDS] = {};
   double kind_launches[fc::KINDS] = {};
@@ -502,8 +522,10 @@ struct fc_factor {
     U.release();
     inv.release();
     part.release();
+    fexp.release();
     vals.release();
     xp.release();
+    yp.release();
     usolve.release();
     refine_work.release();
     if (own_stream && stream) (void)hipStreamDestroy(stream);
@@ -564,7 +586,7 @@ void run_factor(fc_factor* F, const double* dval, i64 nnz, hipStream_t st) {
   T* L = (T*)F->L;
   T* U = (T*)F->U.get((size_t)std::max<i64>(s.u_size, 1) * sizeof(T));
   const auto& pr = F->sym->prog;
-  T* Inv = (T*)F->inv.get((size_t)std::max<i64>(pr.inv_slots, 1) * fc::NB * fc::NB * sizeof(T));
+  T* Inv = (T*)F->inv.get((size_t)std::max<i64>(pr.inv_size, 1) * sizeof(T));
   T* Part = (T*)F->part.get((size_t)std::max<i64>(pr.part_size, 1) * sizeof(T));
   Events ev;
   hipEvent_t e0 = ev.make(), e1 = ev.make();
@@ -573,6 +595,13 @@ void run_factor(fc_factor* F, const double* dval, i64 nnz, hipStream_t st) {
   fc::k_scatter_a<T><<<blocks_for(nnz), 256, 0, st>>>(dval, P.a_pos, nnz, L);
   fc::k_gather_diag<<<blocks_for(s.n), 256, 0, st>>>(dval, P.diag_src, s.n, F->diag);
   fc::k_fill<unsigned long long><<<1, 64, 0, st>>>(F->fail, 1, ~0ull);
+  const bool mixed = F->precision == FC_MIXED;
+  int32_t* fexp = nullptr;
+  if (mixed) {
+    fexp = (int32_t*)F->fexp.get((size_t)std::max<i64>(s.nf, 1) * sizeof(int32_t));
+    fc::k_front_scale<<<(int)std::min<i64>(std::max<i64>(s.nf, 1), 1 << 16), 256, 0, st>>>(P.fronts, P.rows, F->diag,
+                                                                                          s.nf, fexp);
+  }
   const auto& prog = F->sym->prog;
   const int mask = F->opts.profile_mask;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> timed;
@@ -596,15 +625,26 @@ void run_factor(fc_factor* F, const double* dval, i64 nnz, hipStream_t st) {
         fc::k_extend_add<T><<<g, b, 0, st>>>(it, P.fronts, P.kids, P.rel, L, U, U);
         break;
       case fc::K_UPDATE:
-      case fc::K_TRSM:
       case fc::K_SYRK:
+        if constexpr (std::is_same<T, float>::value) {
+          if (mixed) {
+            if (F->opts.bf16_terms == 1) fc::k_gemm_bf16<1><<<g, b, 0, st>>>(P.gitems + l.item0, L, U, Part, fexp);
+            else fc::k_gemm_bf16<3><<<g, b, 0, st>>>(P.gitems + l.item0, L, U, Part, fexp);
+            break;
+          }
+        }
+        fc::k_gemm<T><<<g, b, 0, st>>>(P.gitems + l.item0, L, U, Inv, Part);
+        break;
+      case fc::K_TRSM:
         fc::k_gemm<T><<<g, b, 0, st>>>(P.gitems + l.item0, L, U, Inv, Part);
         break;
       case fc::K_REDUCE:
-        fc::k_reduce_parts<T><<<g, b, 0, st>>>(P.ritems + l.item0, L, U, Part);
+        fc::k_reduce_parts<T><<<dim3(l.count * fc::RSPLIT), b, 0, st>>>(P.ritems + l.item0, L, U, Part);
         break;
       case fc::K_POTRF:
-        fc::k_potrf_inv<T><<<g, b, 0, st>>>(it, P.fronts, L, F->diag, F->fail, Inv);
+        if (l.pad == 32) fc::k_potrf_inv<T, 32><<<g, dim3(64), 0, st>>>(it, P.fronts, L, F->diag, F->fail, Inv);
+        else if (l.pad == 64) fc::k_potrf_inv<T, 64><<<g, dim3(128), 0, st>>>(it, P.fronts, L, F->diag, F->fail, Inv);
+        else fc::k_potrf_inv128<T><<<g, dim3(256), 0, st>>>(it, P.fronts, L, F->diag, F->fail, Inv);
         break;
       default:
         throw fc::Failure{fc::INTERNAL, -1, 0, 0, "unknown launch"};
@@ -681,10 +721,11 @@ int factor_host(fc_factor* F, const double* upper_val, uint64_t nnz, fc_error* e
 }
 
 template <typename T>
-void run_solve(const fc_factor* F, double* xp, i64 ldx, int nrhs, hipStream_t st) {
+void run_solve(const fc_factor* F, double* xp, double* yp, i64 ldx, int nrhs, hipStream_t st) {
   const Program& prog = F->sym->prog;
   DevicePlan& P = *F->plan;
   const T* L = (const T*)F->L;
+  const T* Inv = (const T*)F->inv.p;
   double* u = (double*)F->usolve.get((size_t)std::max<i64>(prog.u_solve * nrhs, 1) * sizeof(double));
   auto run = [&](const fc::Launch& l) {
     const fc::Item* it = P.sitems + l.item0;
@@ -693,20 +734,14 @@ void run_solve(const fc_factor* F, double* xp, i64 ldx, int nrhs, hipStream_t st
       case S_GATHER:
         fc::k_solve_gather<<<g, b, 0, st>>>(it, P.fronts, P.kids, P.rel, xp, ldx, nrhs, u, P.uoff);
         break;
-      case S_FWD_BLOCK:
-        fc::k_forward_block<T><<<g, b, 0, st>>>(it, P.fronts, L, xp, ldx, nrhs, u, P.uoff);
-        break;
-      case S_FWD_GEMV:
-        fc::k_forward_gemv<T><<<g, b, 0, st>>>(it, P.fronts, L, xp, ldx, nrhs, u, P.uoff);
+      case S_FWD_STEP:
+        fc::k_fwd_step<T><<<g, b, 0, st>>>(it, P.fronts, L, Inv, xp, yp, ldx, nrhs, u, P.uoff);
         break;
       case S_BWD_BELOW:
-        fc::k_backward_below<T><<<g, b, 0, st>>>(it, P.fronts, P.rows, L, xp, ldx, nrhs);
+        fc::k_bwd_below<T><<<g, b, 0, st>>>(it, P.fronts, P.rows, L, xp, yp, ldx, nrhs);
         break;
-      case S_BWD_BLOCK:
-        fc::k_backward_block<T><<<g, b, 0, st>>>(it, P.fronts, L, xp, ldx, nrhs);
-        break;
-      case S_BWD_GEMV:
-        fc::k_backward_gemv<T><<<g, b, 0, st>>>(it, P.fronts, L, xp, ldx, nrhs);
+      case S_BWD_STEP:
+        fc::k_bwd_step<T><<<g, b, 0, st>>>(it, P.fronts, L, Inv, yp, xp, ldx, nrhs);
         break;
     }
   };
@@ -723,9 +758,10 @@ void solve_device(const fc_factor* F, double* b, i64 n, int nrhs, i64 ldb, hipSt
     const int nr = std::min(fc::MAX_RHS, nrhs - r0);
     double* br = b + (i64)r0 * ldb;
     double* xp = (double*)F->xp.get((size_t)n * nr * sizeof(double));
+    double* yp = (double*)F->yp.get((size_t)n * nr * sizeof(double));
     fc::k_permute<<<blocks_for(n), 256, 0, st>>>(br, xp, F->plan->perm, n, nr, ldb, n, 0);
-    if (F->precision == FC_FP64) run_solve<double>(F, xp, n, nr, st);
-    else run_solve<float>(F, xp, n, nr, st);
+    if (F->precision == FC_FP64) run_solve<double>(F, xp, yp, n, nr, st);
+    else run_solve<float>(F, xp, yp, n, nr, st);
     fc::k_permute<<<blocks_for(n), 256, 0, st>>>(xp, br, F->plan->perm, n, nr, n, ldb, 1);
   }
   HIP_CHECK(hipGetLastError());
@@ -961,6 +997,11 @@ static int factor_create(const fc_symbolic* h, const fc_factor_opts* opts, std::
   if (opts) o = *opts;
   if (o.precision < FC_FP64 || o.precision > FC_FP32) {
     set_error(err, FC_INVALID_ARGUMENT, "unknown precision");
+    return FC_INVALID_ARGUMENT;
+  }
+  if (o.bf16_terms == 0) o.bf16_terms = 3;
+  if (o.bf16_terms != 1 && o.bf16_terms != 3) {
+    set_error(err, FC_INVALID_ARGUMENT, "bf16_terms must be 1 or 3 (0: the default, 3)");
     return FC_INVALID_ARGUMENT;
   }
   F.reset(new fc_factor());

@@ -8,23 +8,27 @@
 // read from the solution. Contributions travel up the tree instead of being scattered into
 // ancestors, so no two workgroups write one entry and no atomics are used.
 //
-// A front of at most SB columns is solved by one workgroup. A wider front is solved by blocks
-// of SB columns: the block's triangle by one workgroup, then the rows below the block (forward)
-// or the columns before it (backward) by many workgroups. The right-hand sides are the columns
-// of x (postorder numbering, leading dimension ldx); every entry of L is read once per solve
-// whatever their number.
+// A front is solved by blocks of NB columns, its factorization's panels, whose diagonal blocks'
+// inverses the factorization keeps. One launch per block: every workgroup forms the block's
+// solution with the inverse (a 128 x 128 product, cheap to repeat), the first one stores it,
+// and each updates its own share of the rows below (forward) or of the columns before
+// (backward). The forward pass reads v from x and writes y to a second vector; the backward
+// pass reads z from that vector and writes x, so no workgroup reads what another writes. The
+// right-hand sides are columns (postorder numbering, leading dimension ldx); every entry of L
+// is read once per RC right-hand sides.
 #pragma once
 
 #include "device.hpp"
 
 namespace fc {
 
-constexpr int SB = 64;        // columns of a solve block
-constexpr int SROWS = 256;    // rows of a forward GEMV workgroup
-constexpr int SCOLS = 32;     // columns of a backward GEMV workgroup
+constexpr int SNB = 128;      // columns of a solve block (the factorization's panel width)
+constexpr int SROWS = 256;    // rows of a forward workgroup
+constexpr int SCOLS = 32;     // columns of a backward workgroup
+constexpr int RC = 16;        // right-hand sides a workgroup keeps in LDS at a time
 constexpr int MAX_RHS = 64;   // right-hand sides one launch handles (the host splits larger blocks)
 
-// Item: f. Zero the front's contribution block and add its children's into it and into x.
+// Item: f. Zero the front's contribution block and add its children's into it and into v.
 __global__ __launch_bounds__(256) void k_solve_gather(const Item* __restrict__ items, const FrontDev* __restrict__ fronts,
                                                       const int32_t* __restrict__ kids, const int32_t* __restrict__ rel,
                                                       double* __restrict__ x, int64_t ldx, int nrhs, double* u,
@@ -51,124 +55,66 @@ __global__ __launch_bounds__(256) void k_solve_gather(const Item* __restrict__ i
   }
 }
 
-// The triangle of block [c0, c1) of a front's L11 into LDS (lower part).
+// Forward step of block [c0, c1): y_b = Linv v_b (stored by the item with c = 1), then rows
+// [r0, r0 + SROWS) at or below c1 lose L(r, block) y_b: rows of the front's columns in v (x),
+// rows below in the front's contribution u. Item: f, a = c0, b = r0, c = store flag.
 template <typename T>
-__device__ __forceinline__ void load_block(const T* __restrict__ lf, int ld, int c0, int bw, double (*D)[SB + 1]) {
-  for (int e = threadIdx.x; e < SB * SB; e += blockDim.x) {
-    const int i = e % SB, j = e / SB;
-    D[i][j] = (i < bw && j < bw && i >= j) ? (double)lf[(int64_t)(c0 + j) * ld + c0 + i] : 0.0;
-  }
-}
-
-// y = L_block^-1 v (forward) on x rows [c0, c0 + bw) of every right-hand side, in place.
-__device__ __forceinline__ void block_forward(double (*D)[SB + 1], double* xo, int64_t ldx, int nrhs, int bw,
-                                              double (*Y)[MAX_RHS]) {
-  for (int e = threadIdx.x; e < bw * nrhs; e += blockDim.x) Y[e % bw][e / bw] = xo[(e / bw) * ldx + e % bw];
-  __syncthreads();
-  for (int j = 0; j < bw; ++j) {
-    if (threadIdx.x < nrhs) Y[j][threadIdx.x] /= D[j][j];
-    __syncthreads();
-    for (int e = threadIdx.x; e < (bw - j - 1) * nrhs; e += blockDim.x) {
-      const int i = j + 1 + e % (bw - j - 1), r = e / (bw - j - 1);
-      Y[i][r] -= D[i][j] * Y[j][r];
-    }
-    __syncthreads();
-  }
-  for (int e = threadIdx.x; e < bw * nrhs; e += blockDim.x) xo[(e / bw) * ldx + e % bw] = Y[e % bw][e / bw];
-  __syncthreads();
-}
-
-// x = L_block^-T z (backward) on x rows [c0, c0 + bw), in place.
-__device__ __forceinline__ void block_backward(double (*D)[SB + 1], double* xo, int64_t ldx, int nrhs, int bw,
-                                               double (*Y)[MAX_RHS]) {
-  for (int e = threadIdx.x; e < bw * nrhs; e += blockDim.x) Y[e % bw][e / bw] = xo[(e / bw) * ldx + e % bw];
-  __syncthreads();
-  for (int j = bw - 1; j >= 0; --j) {
-    if (threadIdx.x < nrhs) Y[j][threadIdx.x] /= D[j][j];
-    __syncthreads();
-    for (int e = threadIdx.x; e < j * nrhs; e += blockDim.x) {
-      const int i = e % j, r = e / j;
-      Y[i][r] -= D[j][i] * Y[j][r];
-    }
-    __syncthreads();
-  }
-  for (int e = threadIdx.x; e < bw * nrhs; e += blockDim.x) xo[(e / bw) * ldx + e % bw] = Y[e % bw][e / bw];
-  __syncthreads();
-}
-
-// Forward triangle of a block: item f, a = c0. A front of at most SB columns also hands up
-// its contribution (b = 1): u -= L21 y.
-template <typename T>
-__global__ __launch_bounds__(256) void k_forward_block(const Item* __restrict__ items, const FrontDev* __restrict__ fronts,
-                                                       const T* __restrict__ L, double* __restrict__ x, int64_t ldx,
-                                                       int nrhs, double* __restrict__ u, const int64_t* __restrict__ uoff) {
-  __shared__ double D[SB][SB + 1];
-  __shared__ double Y[SB][MAX_RHS];
+__global__ __launch_bounds__(256) void k_fwd_step(const Item* __restrict__ items, const FrontDev* __restrict__ fronts,
+                                                  const T* __restrict__ L, const T* __restrict__ Inv, double* v,
+                                                  double* __restrict__ y, int64_t ldx, int nrhs, double* __restrict__ u,
+                                                  const int64_t* __restrict__ uoff) {
+  __shared__ double Vb[SNB][RC + 1];
+  __shared__ double Yb[SNB][RC + 1];
   const Item it = items[blockIdx.x];
   const FrontDev F = fronts[it.f];
-  const int c0 = it.a, bw = min(SB, F.w - c0);
+  const int c0 = it.a, pw = min(SNB, F.w - c0), c1 = c0 + pw, r0 = it.b;
+  const T* inv = Inv + F.inv0 + (int64_t)(c0 / SNB) * SNB * SNB;  // ld = pw
   const T* lf = L + F.lpos;
-  load_block(lf, F.ld, c0, bw, D);
-  __syncthreads();
-  double* xo = x + F.col0 + c0;
-  block_forward(D, xo, ldx, nrhs, bw, Y);
-  if (it.b) {  // the whole front: u = v_below - L21 y
-    const int mu = F.m - F.w;
-    double* uf = u + uoff[it.f] * nrhs;
-    for (int i = threadIdx.x; i < mu; i += blockDim.x) {
-      double s[8];
-      for (int r0 = 0; r0 < nrhs; r0 += 8) {
-        const int nr = min(8, nrhs - r0);
-        for (int r = 0; r < 8; ++r) s[r] = 0.0;
-        for (int j = 0; j < bw; ++j) {
-          const double l = (double)lf[(int64_t)j * F.ld + F.w + i];
-          for (int r = 0; r < nr; ++r) s[r] += l * Y[j][r0 + r];
-        }
-        for (int r = 0; r < nr; ++r) uf[(r0 + r) * mu + i] -= s[r];
-      }
-    }
-  }
-}
-
-// Forward update of the rows below a solved block: item f, a = c0, b = r0. Rows r0..r0+SROWS
-// (>= c0 + SB) lose L(r, block) y_block: rows of the front's columns in x, rows below in u.
-template <typename T>
-__global__ __launch_bounds__(256) void k_forward_gemv(const Item* __restrict__ items, const FrontDev* __restrict__ fronts,
-                                                      const T* __restrict__ L, double* __restrict__ x, int64_t ldx,
-                                                      int nrhs, double* __restrict__ u, const int64_t* __restrict__ uoff) {
-  __shared__ double Y[SB][MAX_RHS];
-  const Item it = items[blockIdx.x];
-  const FrontDev F = fronts[it.f];
-  const int c0 = it.a, bw = min(SB, F.w - c0), r0 = it.b;
-  const T* lf = L + F.lpos;
-  for (int e = threadIdx.x; e < bw * nrhs; e += blockDim.x) Y[e % bw][e / bw] = x[(e / bw) * ldx + F.col0 + c0 + e % bw];
-  __syncthreads();
   const int mu = F.m - F.w;
   double* uf = u + uoff[it.f] * nrhs;
   const int i = r0 + threadIdx.x;
-  if (i < F.m && threadIdx.x < SROWS) {
-    double s[8];
-    for (int q0 = 0; q0 < nrhs; q0 += 8) {
-      const int nr = min(8, nrhs - q0);
-      for (int r = 0; r < 8; ++r) s[r] = 0.0;
-      for (int j = 0; j < bw; ++j) {
-        const double l = (double)lf[(int64_t)(c0 + j) * F.ld + i];
-        for (int r = 0; r < nr; ++r) s[r] += l * Y[j][q0 + r];
-      }
-      if (i < F.w)
-        for (int r = 0; r < nr; ++r) x[(q0 + r) * ldx + F.col0 + i] -= s[r];
-      else
-        for (int r = 0; r < nr; ++r) uf[(q0 + r) * mu + i - F.w] -= s[r];
+  const bool row = threadIdx.x < SROWS && i >= c1 && i < F.m;
+  for (int q0 = 0; q0 < nrhs; q0 += RC) {
+    const int nr = min(RC, nrhs - q0);
+    for (int e = threadIdx.x; e < pw * nr; e += blockDim.x) {
+      const int k = e % pw, r = e / pw;
+      Vb[k][r] = v[(int64_t)(q0 + r) * ldx + F.col0 + c0 + k];
     }
+    __syncthreads();
+    for (int e = threadIdx.x; e < pw * nr; e += blockDim.x) {
+      const int b = e % pw, r = e / pw;
+      double s = 0.0;
+      for (int k = 0; k <= b; ++k) s += (double)inv[(int64_t)k * pw + b] * Vb[k][r];
+      Yb[b][r] = s;
+      if (it.c) y[(int64_t)(q0 + r) * ldx + F.col0 + c0 + b] = s;
+    }
+    __syncthreads();
+    if (row) {
+      double s[RC];
+#pragma unroll
+      for (int r = 0; r < RC; ++r) s[r] = 0.0;
+      for (int j = 0; j < pw; ++j) {
+        const double l = (double)lf[(int64_t)(c0 + j) * F.ld + i];
+#pragma unroll
+        for (int r = 0; r < RC; ++r) s[r] += l * Yb[j][r];
+      }
+      if (i < F.w) {
+        for (int r = 0; r < nr; ++r) v[(int64_t)(q0 + r) * ldx + F.col0 + i] -= s[r];
+      } else {
+        for (int r = 0; r < nr; ++r) uf[(int64_t)(q0 + r) * mu + i - F.w] -= s[r];
+      }
+    }
+    __syncthreads();
   }
 }
 
-// Backward: z_j -= L21(:, j)^T x_below for the columns [a, a + SCOLS) of a front; a wave per
-// column at a time. Item f, a = j0.
+// Backward: z_j -= L21(:, j)^T x_below for the columns [a, a + SCOLS) of a front, z in y and
+// x_below in x; a wave per column at a time. Item f, a = j0.
 template <typename T>
-__global__ __launch_bounds__(256) void k_backward_below(const Item* __restrict__ items, const FrontDev* __restrict__ fronts,
-                                                        const int32_t* __restrict__ rows, const T* __restrict__ L,
-                                                        double* __restrict__ x, int64_t ldx, int nrhs) {
+__global__ __launch_bounds__(256) void k_bwd_below(const Item* __restrict__ items, const FrontDev* __restrict__ fronts,
+                                                   const int32_t* __restrict__ rows, const T* __restrict__ L,
+                                                   const double* __restrict__ x, double* __restrict__ z, int64_t ldx,
+                                                   int nrhs) {
   const Item it = items[blockIdx.x];
   const FrontDev F = fronts[it.f];
   const int mu = F.m - F.w;
@@ -179,52 +125,59 @@ __global__ __launch_bounds__(256) void k_backward_below(const Item* __restrict__
   for (int j = it.a + wave; j < j1; j += 4) {
     const T* col = lf + (int64_t)j * F.ld + F.w;
     for (int r = 0; r < nrhs; ++r) {
-      const double* xr = x + r * ldx;
+      const double* xr = x + (int64_t)r * ldx;
       double s = 0.0;
       for (int i = lane; i < mu; i += 64) s += (double)col[i] * xr[below[i]];
       for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-      if (lane == 0) x[r * ldx + F.col0 + j] -= s;
+      if (lane == 0) z[(int64_t)r * ldx + F.col0 + j] -= s;
     }
   }
 }
 
-// Backward triangle of a block: item f, a = c0.
+// Backward step of block [c0, c1): x_b = Linv^T z_b (stored by the item with c = 1), then the
+// columns [j0, j0 + SCOLS) before c0 lose L(block, j)^T x_b in z. Item: f, a = c0, b = j0,
+// c = store flag.
 template <typename T>
-__global__ __launch_bounds__(256) void k_backward_block(const Item* __restrict__ items, const FrontDev* __restrict__ fronts,
-                                                        const T* __restrict__ L, double* __restrict__ x, int64_t ldx,
-                                                        int nrhs) {
-  __shared__ double D[SB][SB + 1];
-  __shared__ double Y[SB][MAX_RHS];
+__global__ __launch_bounds__(256) void k_bwd_step(const Item* __restrict__ items, const FrontDev* __restrict__ fronts,
+                                                  const T* __restrict__ L, const T* __restrict__ Inv, double* z,
+                                                  double* __restrict__ x, int64_t ldx, int nrhs) {
+  __shared__ double Zb[SNB][RC + 1];
+  __shared__ double Xb[SNB][RC + 1];
   const Item it = items[blockIdx.x];
   const FrontDev F = fronts[it.f];
-  const int c0 = it.a, bw = min(SB, F.w - c0);
-  load_block(L + F.lpos, F.ld, c0, bw, D);
-  __syncthreads();
-  block_backward(D, x + F.col0 + c0, ldx, nrhs, bw, Y);
-}
-
-// Backward update of the columns before a solved block: z_j -= L(block, j)^T x_block for
-// columns [b, b + SCOLS) below c0 = a. Item f, a = c0, b = j0.
-template <typename T>
-__global__ __launch_bounds__(256) void k_backward_gemv(const Item* __restrict__ items, const FrontDev* __restrict__ fronts,
-                                                       const T* __restrict__ L, double* __restrict__ x, int64_t ldx,
-                                                       int nrhs) {
-  __shared__ double Y[SB][MAX_RHS];
-  const Item it = items[blockIdx.x];
-  const FrontDev F = fronts[it.f];
-  const int c0 = it.a, bw = min(SB, F.w - c0);
+  const int c0 = it.a, pw = min(SNB, F.w - c0), j0 = it.b;
+  const T* inv = Inv + F.inv0 + (int64_t)(c0 / SNB) * SNB * SNB;  // ld = pw
   const T* lf = L + F.lpos;
-  for (int e = threadIdx.x; e < bw * nrhs; e += blockDim.x) Y[e % bw][e / bw] = x[(e / bw) * ldx + F.col0 + c0 + e % bw];
-  __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int j1 = min(c0, it.b + SCOLS);
-  for (int j = it.b + wave; j < j1; j += 4) {
-    const double l = lane < bw ? (double)lf[(int64_t)j * F.ld + c0 + lane] : 0.0;
-    for (int r = 0; r < nrhs; ++r) {
-      double s = l * Y[lane < bw ? lane : 0][r];
-      for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-      if (lane == 0) x[r * ldx + F.col0 + j] -= s;
+  const int j1 = min(c0, j0 + SCOLS);
+  for (int q0 = 0; q0 < nrhs; q0 += RC) {
+    const int nr = min(RC, nrhs - q0);
+    for (int e = threadIdx.x; e < pw * nr; e += blockDim.x) {
+      const int k = e % pw, r = e / pw;
+      Zb[k][r] = z[(int64_t)(q0 + r) * ldx + F.col0 + c0 + k];
     }
+    __syncthreads();
+    for (int e = threadIdx.x; e < pw * nr; e += blockDim.x) {
+      const int b = e % pw, r = e / pw;
+      double s = 0.0;
+      const T* col = inv + (int64_t)b * pw;
+      for (int k = b; k < pw; ++k) s += (double)col[k] * Zb[k][r];
+      Xb[b][r] = s;
+      if (it.c) x[(int64_t)(q0 + r) * ldx + F.col0 + c0 + b] = s;
+    }
+    __syncthreads();
+    for (int j = j0 + wave; j < j1; j += 4) {
+      const T* col = lf + (int64_t)j * F.ld + c0;
+      const bool h0 = lane < pw, h1 = lane + 64 < pw;  // Xb holds pw rows only
+      const double l0 = h0 ? (double)col[lane] : 0.0;
+      const double l1 = h1 ? (double)col[lane + 64] : 0.0;
+      for (int r = 0; r < nr; ++r) {
+        double s = (h0 ? l0 * Xb[lane][r] : 0.0) + (h1 ? l1 * Xb[lane + 64][r] : 0.0);
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+        if (lane == 0) z[(int64_t)(q0 + r) * ldx + F.col0 + j] -= s;
+      }
+    }
+    __syncthreads();
   }
 }
 

@@ -105,6 +105,21 @@ def test_cantilever_hex8_matches_the_oracle(gpu):
     assert berr < 1e-14
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_cube_matches_the_oracle(gpu, precision):
+    """A 16^3 hex8 cube (14 688 equations): many levels, wide fronts (128-column panels, split
+    products). FP64 within 1e-12 of the oracle; FP32 within its own precision."""
+    p = P.cube_hex8(16)
+    ref, f, up = both(p.n, p.ptr, p.idx, p.kval, precision=precision)
+    x, xr = f.solve(p.rhs), ref.solve(p.rhs)
+    if precision == "fp64":
+        assert rel(f.values(), ref.values()) <= REL
+        assert rel(x, xr) <= REL
+    else:
+        assert rel(f.values(), ref.values()) <= 1e-5
+        assert rel(x, xr) <= 1e-3
+
+
 def test_refined_solve_reaches_the_target(gpu):
     p = P.cantilever_hex8()
     up = p.upper()
