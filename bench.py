@@ -16,6 +16,14 @@ around its launches on the factor's stream during the timed steps; `cpu_baseline
 oracle (the C restatement of the reference's CPU factorization) on a bounded sample.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--cells 80] [--form mixed|fp32|fp64]
+                      [--contrast] [--workload cube|batch|shift]
+
+--contrast adds the harder variant of the same pattern (E in {210 GPa, 21 MPa} per element,
+nu = 0.49) with the same symbolic analysis. --workload batch runs 512 independent hex8/tet10
+models (2k-60k equations) as one block-diagonal factorization in FP32 with FP64 refinement of
+64 load cases per model; --workload shift factors K - sigma M of a tet10 steel block
+(sigma = (2 pi 50 Hz)^2) in the mixed form and times 40 refined block solves of 16 vectors.
+Those two print their own metric lines.
 """
 from __future__ import annotations
 
@@ -108,6 +116,10 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count())
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--contrast", action="store_true")
+    ap.add_argument("--workload", default="cube", choices=["cube", "batch", "shift"])
+    ap.add_argument("--models", type=int, default=512)
+    ap.add_argument("--tet-cells", type=int, default=50)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU: the factorization of one matrix is not split across GPUs here "
@@ -123,6 +135,10 @@ def main():
         print("no GPU", file=sys.stderr)
         sys.exit(1)
     torch.cuda.init()
+    if args.workload == "batch":
+        return batch_workload(args, CH, P)
+    if args.workload == "shift":
+        return shift_workload(args, CH, P)
     t0 = time.perf_counter()
     p = P.cube_hex8(args.cells)
     up = p.upper()
@@ -158,26 +174,42 @@ def main():
     head = legs[form]
     F = info["flops"]
     t_step = head["factor_s"] + head["solve_s"]
-    # the dominant kernel of the headline leg's factorization
-    dom = max(head["kinds"], key=lambda k: head["kinds"][k]["seconds"])
-    dk = head["kinds"][dom]
-    if dom in ("extend_l", "extend_u"):
+    # the dominant kernel function of the headline leg's factorization (kinds grouped by the
+    # kernel that runs them, so the figure matches a rocprofv3 per-kernel summary)
+    el = "double" if form == "fp64" else "float"
+    funcs = {}
+    for kind, v in head["kinds"].items():
+        if kind in ("update", "syrk") and form == "mixed":
+            name, bound, peak = f"k_gemm_bf16<{args.bf16_terms}>", "mfma", PEAK["bf16"]
+        elif kind in ("update", "syrk", "trsm"):
+            name, bound, peak = f"k_gemm<{el}>", "mfma", PEAK["fp64" if form == "fp64" else "fp32"]
+        elif kind in ("extend_l", "extend_u"):
+            name, bound, peak = f"k_extend_add<{el}>", "hbm", HBM_PEAK
+        elif kind == "potrf":
+            name, bound, peak = f"k_potrf_blk<{el}> + k_potrf_inv<{el},32>", "latency", None
+        else:
+            name, bound, peak = f"k_reduce_parts<{el}>", "hbm", HBM_PEAK
+        g = funcs.setdefault(name, dict(kinds=[], seconds=0.0, launches=0.0, flops=0.0, entries=0.0, bound=bound,
+                                        peak=peak))
+        g["kinds"].append(kind)
+        for key in ("seconds", "launches", "flops", "entries"):
+            g[key] += v[key]
+    dom = max((k for k in funcs if funcs[k]["bound"] != "latency"), key=lambda k: funcs[k]["seconds"])
+    g = funcs[dom]
+    avg = g["seconds"] / max(g["launches"], 1)
+    if g["bound"] == "hbm":
         es = 8 if form == "fp64" else 4
-        per_launch_bytes = dk["entries"] * (3 * es + 4) / max(dk["launches"], 1)
-        achieved = per_launch_bytes / (dk["seconds"] / max(dk["launches"], 1)) / 1e9
+        per_launch = g["entries"] * (3 * es + 4) / max(g["launches"], 1)  # child entry read, target RMW, row map
+        achieved = per_launch / avg / 1e9
         roof = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK, unit="GB/s", frac=achieved / HBM_PEAK,
-                    traffic=None, kernel=f"k_extend_add ({dom})", launches=dk["launches"])
+                    traffic=None, kernel=dom, kinds=g["kinds"], launches=g["launches"], bytes_per_launch=per_launch,
+                    avg_launch_s=avg)
     else:
-        bf16 = form == "mixed" and dom in ("syrk", "update")
-        peak = PEAK["fp64"] if form == "fp64" else (PEAK["bf16"] if bf16 else PEAK["fp32"])
-        achieved = dk["flops"] / dk["seconds"] / 1e12
-        kname = (f"k_gemm_bf16<{args.bf16_terms}> ({dom})" if bf16 else
-                 f"k_gemm<{'double' if form == 'fp64' else 'float'}> ({dom})" if dom in ("syrk", "update", "trsm") else
-                 f"{dom}<{'double' if form == 'fp64' else 'float'}>")
-        roof = dict(bound="mfma", achieved=achieved, peak=peak, unit="TFLOP/s", frac=achieved / peak, traffic=None,
-                    kernel=kname, launches=dk["launches"], mfma_products_per_flop=args.bf16_terms if bf16 else 1,
-                    flops_per_launch=dk["flops"] / max(dk["launches"], 1),
-                    avg_launch_s=dk["seconds"] / max(dk["launches"], 1))
+        achieved = g["flops"] / g["seconds"] / 1e12
+        roof = dict(bound="mfma", achieved=achieved, peak=g["peak"], unit="TFLOP/s", frac=achieved / g["peak"],
+                    traffic=None, kernel=dom, kinds=g["kinds"], launches=g["launches"],
+                    flops_per_launch=g["flops"] / max(g["launches"], 1), avg_launch_s=avg,
+                    mfma_products_per_flop=args.bf16_terms if "bf16" in dom else 1)
     line = dict(metric=METRIC, value=F / t_step / 1e12, unit="TFLOP/s", n_gpus=1, steps=args.steps,
                 warmup=args.warmup, ms_per_step=t_step * 1e3, higher_is_better=True, scaling="weak", vs_baseline=None,
                 dtype={"mixed": "bf16/fp32 factor, fp64 refinement", "fp32": "fp32 factor, fp64 refinement",
@@ -200,9 +232,129 @@ def main():
     line["kernels"] = {leg: {k: dict(seconds=v["seconds"], launches=v["launches"],
                                      tflops=(v["flops"] / v["seconds"] / 1e12) if v["seconds"] and v["flops"] else None)
                              for k, v in legs[leg]["kinds"].items()} for leg in legs}
+    if args.contrast:
+        pc = P.cube_hex8_contrast(args.cells)
+        dv_c = torch.from_numpy(pc.upper()[2]).to(dev)
+        Ac = CH.Matrix(pc.n, pc.ptr, pc.idx, pc.kval)
+        bc = torch.from_numpy(pc.rhs).to(dev)
+        xc = torch.zeros_like(bc)
+        out = {}
+        for leg in (["fp64"] if not args.no_fp64 else []) + [form]:
+            r = run_leg(CH, sym, dv_c.data_ptr(), Ac, bc.data_ptr(), xc.data_ptr(), pc.n, leg, 1, 1, [],
+                        bf16_terms=args.bf16_terms)
+            out[leg] = dict(factor_s=r["factor_s"], solve_s=r["solve_s"], iterations=r["iterations"], berr=r["berr"],
+                            converged=r["converged"])
+            if leg == "fp64":
+                xc64 = xc.cpu().numpy().copy()
+            else:
+                if not args.no_fp64:
+                    out[leg]["rel_err_vs_fp64"] = float(np.linalg.norm(xc.cpu().numpy() - xc64) / np.linalg.norm(xc64))
+            log(f"bench: contrast {leg} {out[leg]}")
+        line["contrast"] = dict(workload=pc.meta.get("workload", "contrast"), **out)
     if not args.no_cpu:
         log("bench: cpu baseline")
         line["cpu_baseline"] = cpu_baseline(args.cpu_cells, args.cpu_threads)
+    print(json.dumps(line), flush=True)
+
+
+def batch_workload(args, CH, P):
+    """512 independent models as one block-diagonal matrix: FP32 factor, then 64 load cases per
+    model refined in FP64 to berr <= 1e-12 per model (each model's own norms)."""
+    import torch
+
+    t0 = time.perf_counter()
+    models = [P.random_model(s) for s in range(args.models)]
+    bt = P.batch(models, nrhs=64)
+    up = bt.upper()
+    sym = CH.Symbolic(CH.UpperColumns(bt.n, *up))
+    info = sym.info()
+    log(f"bench: batch of {len(models)} models, {bt.n} equations, generated and analysed in "
+        f"{time.perf_counter() - t0:.1f}s; fronts {info['fronts']} levels {info['levels']} F={info['flops']:.3e}")
+    dev = torch.device("cuda:0")
+    dv = torch.from_numpy(up[2]).to(dev)
+    A = CH.Matrix(bt.n, bt.ptr, bt.idx, bt.kval)
+    A.set_segments(bt.segments)
+    B = torch.from_numpy(np.ascontiguousarray(bt.rhs.T)).to(dev)
+    X = torch.zeros_like(B)
+    f = CH.Factor(sym, precision="fp32", device_values=dv.data_ptr())
+    rep, berr = f.solve_refined_many_device(A, B.data_ptr(), X.data_ptr(), 64)
+    times = []
+    for _ in range(args.steps):
+        torch.cuda.synchronize()
+        f.refactor_device(dv.data_ptr())
+        tf = f.seconds()
+        rep, berr = f.solve_refined_many_device(A, B.data_ptr(), X.data_ptr(), 64)
+        times.append((tf, rep["seconds"]))
+    tf, ts = (float(np.mean([t[i] for t in times])) for i in (0, 1))
+    F = info["flops"]
+    line = dict(metric="batched factor + 64 refined load cases per model, 512 hex8/tet10 models (2k-60k dof)",
+                value=F / (tf + ts) / 1e12, unit="TFLOP/s", n_gpus=1, steps=args.steps, warmup=1,
+                ms_per_step=(tf + ts) * 1e3, higher_is_better=True, scaling="weak", vs_baseline=None,
+                dtype="fp32 factor, fp64 refinement", data="synthetic",
+                config=dict(workload=f"{len(models)} random hex8/tet10 models as one block-diagonal matrix",
+                            equations=bt.n, flops=F, factor_entries=info["entries"], rhs_per_model=64),
+                factor_s=tf, solve_s=ts, models_per_s=len(models) / (tf + ts), iterations=rep["iterations"],
+                berr_max=float(np.max(berr)), converged=rep["converged"])
+    print(json.dumps(line), flush=True)
+
+
+def shift_workload(args, CH, P):
+    """K - sigma M of a tet10 steel block (sigma = (2 pi 50)^2): FP64 factor for the reference
+    solutions of the first block, then the mixed factor and 40 refined solves of 16 vectors."""
+    import torch
+
+    t0 = time.perf_counter()
+    k = args.tet_cells
+    p = P.block_tet10((k, k, k), (k / 100, k / 100, k / 100))
+    sigma = (2 * np.pi * 50.0) ** 2
+    vals = p.kval - sigma * p.mval
+    up = p.upper(vals)
+    sym = CH.Symbolic(CH.UpperColumns(p.n, *up))
+    info = sym.info()
+    log(f"bench: tet10 {k}^3 shift-invert, {p.n} equations, set up in {time.perf_counter() - t0:.1f}s; "
+        f"F={info['flops']:.3e} entries {info['entries']:.3e}")
+    dev = torch.device("cuda:0")
+    dv = torch.from_numpy(up[2]).to(dev)
+    A = CH.Matrix(p.n, p.ptr, p.idx, vals)
+    rng = np.random.default_rng(20244)
+    blocks = [torch.from_numpy(rng.standard_normal((16, p.n))).to(dev) for _ in range(40)]
+    X = torch.zeros_like(blocks[0])
+    out = {}
+    if not args.no_fp64:
+        f64 = CH.Factor(sym, precision="fp64", device_values=dv.data_ptr())
+        out["fp64_factor_s"] = f64.seconds()
+        X.copy_(blocks[0])
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        f64.solve_device(X.data_ptr(), 16)
+        torch.cuda.synchronize()
+        out["fp64_solve_s"] = time.perf_counter() - t
+        x64 = X.cpu().numpy().copy()
+        del f64
+        torch.cuda.empty_cache()
+    f = CH.Factor(sym, precision="mixed", device_values=dv.data_ptr(), bf16_terms=args.bf16_terms)
+    out["mixed_factor_s"] = f.seconds()
+    lat, its, worst = [], [], 0.0
+    for q, Bq in enumerate(blocks):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        rep, berr = f.solve_refined_many_device(A, Bq.data_ptr(), X.data_ptr(), 16)
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - t)
+        its.append(rep["iterations"])
+        worst = max(worst, float(np.max(berr)))
+        if q == 0 and not args.no_fp64:
+            xm = X.cpu().numpy()
+            out["rel_err_vs_fp64"] = float(np.linalg.norm(xm - x64) / np.linalg.norm(x64))
+    F = info["flops"]
+    line = dict(metric="shift-invert: mixed factor + 40 refined block solves (16 vectors), tet10 ~3M dof",
+                value=float(np.median(lat)) * 1e3, unit="ms per 16-vector refined solve", n_gpus=1, steps=40,
+                warmup=0, ms_per_step=float(np.median(lat)) * 1e3, higher_is_better=False, scaling="weak",
+                vs_baseline=None, dtype="bf16/fp32 factor, fp64 refinement", data="synthetic",
+                config=dict(workload=f"tet10 {k}^3 Kuhn block, steel, K - (2 pi 50)^2 M", equations=p.n, flops=F,
+                            factor_entries=info["entries"]),
+                factor_tflops=F / out["mixed_factor_s"] / 1e12, iterations_median=float(np.median(its)),
+                berr_max=worst, **out)
     print(json.dumps(line), flush=True)
 
 

@@ -113,11 +113,11 @@ _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 EXPORTS = ["fc_analyze", "fc_symbolic_info_get", "fc_symbolic_export", "fc_symbolic_free", "fc_factor_numeric",
            "fc_factor_numeric_device", "fc_factor_new", "fc_refactor", "fc_refactor_device", "fc_entries", "fc_solve",
            "fc_solve_many", "fc_solve_device", "fc_factor_export", "fc_factor_seconds", "fc_factor_profile",
-           "fc_factor_free", "fc_matrix_new", "fc_matrix_free", "fc_solve_refined", "fc_solve_refined_device",
-           "fc_version"]
+           "fc_factor_free", "fc_matrix_new", "fc_matrix_free", "fc_matrix_set_segments", "fc_solve_refined",
+           "fc_solve_refined_device", "fc_solve_refined_many_device", "fc_version"]
 
 #: kernel kinds of the numeric phase (FC_KIND_* in include/fc_chol.h)
-KINDS = ["extend_l", "extend_u", "update", "potrf", "trsm", "syrk", "reduce"]
+KINDS = ["extend_l", "extend_u", "update", "potrf", "trsm", "syrk", "reduce", "split"]
 
 
 def lib():
@@ -153,6 +153,9 @@ def lib():
                                        C.POINTER(_RefineReport), ep]
         L.fc_solve_refined_device.argtypes = [vp, vp, vp, vp, C.c_uint64, C.POINTER(_RefineOpts),
                                               C.POINTER(_RefineReport), ep]
+        L.fc_matrix_set_segments.argtypes = [vp, C.c_uint64, _u64p, ep]
+        L.fc_solve_refined_many_device.argtypes = [vp, vp, vp, vp, C.c_uint64, C.c_uint64, C.POINTER(_RefineOpts),
+                                                   C.POINTER(_RefineReport), _f64p, ep]
         L.fc_version.restype = C.c_char_p
         _lib = L
     return _lib
@@ -211,8 +214,10 @@ class Symbolic:
         self.nnz = int(a.ptr[-1]) if self.n else 0
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            lib().fc_symbolic_free(self._h)
+        # the library object itself, not the module's lib(): at interpreter exit module
+        # globals may already be gone
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.fc_symbolic_free(self._h)
             self._h = None
 
     def info(self) -> dict:
@@ -257,9 +262,18 @@ class Matrix:
             self._h = None
             _raise(st, err)
 
+    def set_segments(self, offsets) -> None:
+        """Rows [offsets[s], offsets[s+1]) are independent models of a batch: refinement
+        measures each one's backward error with its own norms."""
+        off = _u64(offsets)
+        err = _Error()
+        _raise(lib().fc_matrix_set_segments(self._h, len(off) - 1, off, C.byref(err)), err)
+
     def __del__(self):
-        if getattr(self, "_h", None):
-            lib().fc_matrix_free(self._h)
+        # the library object itself, not the module's lib(): at interpreter exit module
+        # globals may already be gone
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.fc_matrix_free(self._h)
             self._h = None
 
 
@@ -302,8 +316,10 @@ class Factor:
         return Factor(Symbolic(a, max_entries), a.val, **kw)
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            lib().fc_factor_free(self._h)
+        # the library object itself, not the module's lib(): at interpreter exit module
+        # globals may already be gone
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.fc_factor_free(self._h)
             self._h = None
 
     def refactor(self, val) -> None:
@@ -392,8 +408,26 @@ class Factor:
         st = lib().fc_solve_refined_device(self._h, matrix._h, C.c_void_p(b_ptr), C.c_void_p(x_ptr), self.n,
                                            C.byref(opts), C.byref(rep), C.byref(err))
         _raise(st, err)
-        return dict(iterations=rep.iterations, converged=bool(rep.converged), berr=rep.berr, seconds=rep.seconds,
-                    berr_history=list(rep.berr_history[: min(rep.iterations + 1, 64)]))
+        return _report(rep)
+
+    def solve_refined_many_device(self, matrix: Matrix, b_ptr: int, x_ptr: int, nrhs: int, method: str = "auto",
+                                  max_iter: int = 50, target_berr: float = 1e-12):
+        """Refined solves of nrhs columns (n x nrhs, column-major, device memory); returns the
+        report (the worst column) and each column's backward error."""
+        m = {"refine": 0, "cg": 1, "auto": 2}[method]
+        opts = _RefineOpts(m, int(max_iter), float(target_berr))
+        rep = _RefineReport()
+        err = _Error()
+        berr = np.zeros(max(int(nrhs), 1))
+        st = lib().fc_solve_refined_many_device(self._h, matrix._h, C.c_void_p(b_ptr), C.c_void_p(x_ptr), self.n,
+                                                int(nrhs), C.byref(opts), C.byref(rep), berr, C.byref(err))
+        _raise(st, err)
+        return _report(rep), berr[: int(nrhs)]
+
+
+def _report(rep) -> dict:
+    return dict(iterations=rep.iterations, converged=bool(rep.converged), berr=rep.berr, seconds=rep.seconds,
+                berr_history=list(rep.berr_history[: min(rep.iterations + 1, 64)]))
 
 
 def version() -> str:

@@ -17,10 +17,11 @@ struct FrontDev {
   int64_t rel_off;   // relative map of the rows below into the parent front
   int64_t rows_off;  // front rows (postorder column numbers) in the rows array
   int64_t inv0;      // inverses of the diagonal blocks: panel at c0 at inv0 + (c0 / NB) NB^2, ld = its width
+  int64_t bpos;      // mixed form: BF16 copies of the L part, (i, j) at bpos + ((j / 32) ld + i) 32 + j % 32
   int32_t ld, w, m, col0;
   int32_t parent, kid0, kid1, level;  // children are kids[kid0..kid1)
 };
-static_assert(sizeof(FrontDev) == 72, "FrontDev layout");
+static_assert(sizeof(FrontDev) == 80, "FrontDev layout");
 
 // A unit of work of one launch; the meaning of a, b, c, d depends on the kernel.
 struct Item {
@@ -35,6 +36,7 @@ enum Kind : int32_t {
   K_TRSM = 4,      // rows below a panel against its diagonal block
   K_SYRK = 5,      // update matrix U = -L21 L21^T
   K_REDUCE = 6,    // sums of the pieces of a GEMM split along k
+  K_SPLIT = 7,     // mixed form: a finished panel's BF16 (hi, lo) copies
 };
 
 struct Launch {
@@ -47,7 +49,7 @@ struct Launch {
   double entries; // algorithmic entries moved (extend-add: child entries added)
 };
 
-constexpr int KINDS = 7;
+constexpr int KINDS = 8;
 
 constexpr int PANEL = 64;      // columns of a panel (the reference's PANEL, cholesky.rs:55)
 constexpr int TILE_M = 64;     // rows of a GEMM tile

@@ -161,67 +161,95 @@ __global__ __launch_bounds__(256, 2) void k_gemm(const GemmItem* __restrict__ it
 }
 
 // ------------------------------------------------------------------------------------------
-// The Schur products of the mixed form: FP32 operands in HBM, multiplied as BF16 on the
-// 16x16x32 BF16 matrix cores with FP32 accumulation. Both operands of a product come from one
-// front and are scaled by that front's power of two, 2^-fexp[front], which bounds them by 1
-// (|L_ij| <= sqrt(A_ii) for an SPD matrix); the result is scaled back exactly. With TERMS = 3
-// each scaled operand x is split into hi = bf16(x) and lo = bf16(x - hi) and the product is
-// hi*hi + hi*lo + lo*hi, which carries about 16 significant bits; TERMS = 1 keeps hi*hi.
+// The Schur products of the mixed form on the 16x16x32 BF16 matrix cores with FP32
+// accumulation. Both operands of a product come from one front, whose columns are scaled by
+// that front's power of two 2^-fexp[front] (which bounds them by 1: |L_ij| <= sqrt(A_ii) for an
+// SPD matrix); the result is scaled back exactly. With TERMS = 3 each scaled value x is carried
+// as hi = bf16(x) and lo = bf16(x - hi) and a product is hi*hi + hi*lo + lo*hi, about 16
+// significant bits; TERMS = 1 keeps hi*hi.
 //
-// Same tile and item layout as k_gemm (128 x 128, 4 waves of 64 x 64, modes and split-K
-// pieces alike). k is staged BK = 32 at a time: a loader thread reads 16 consecutive k of one
-// row (coalesced across threads along the rows), converts and writes them to LDS as k-major
-// rows, from which each lane reads its 8 k of a fragment as one 16-byte load.
+// The BF16 copies are made once per panel, when it is finished (k_split_panel), in a k-chunked
+// layout: 32 consecutive columns of a row are contiguous, so a 128-row x 32-column stage of an
+// operand is one contiguous 8 KB block per copy, loaded as 16-byte vectors and stored to LDS as
+// it is; each lane then reads its 8 k of a fragment as one 16-byte LDS load. Tiles, modes and
+// split-k pieces are those of k_gemm (128 x 128, 4 waves of 64 x 64).
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
-constexpr int BK = 32;
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+constexpr int BK = 32;        // k per stage: one MFMA depth, one chunk of the copies
 constexpr int BROW = BK + 8;  // LDS row stride (80 bytes: 16-byte aligned, rows spread over the banks)
 
+// A finished panel [c0, c0 + NB) of a front into the BF16 copies, rows [r0, r0 + 256) (at or
+// below c0: the rows any later product reads). Columns past the front's width up to the next
+// multiple of 32 get zeros, so a product's last stage needs no bounds. Item: f, a = c0, b = r0.
+__global__ __launch_bounds__(256) void k_split_panel(const Item* __restrict__ items, const FrontDev* __restrict__ fronts,
+                                                     const float* __restrict__ L, __bf16* __restrict__ Lh,
+                                                     __bf16* __restrict__ Ll, const int32_t* __restrict__ fexp) {
+  const Item it = items[blockIdx.x];
+  const FrontDev F = fronts[it.f];
+  const int c0 = it.a, c1 = min(c0 + NB, F.w), cend = min(c0 + NB, (F.w + BK - 1) / BK * BK);
+  const int i = it.b + threadIdx.x;
+  if (i >= F.m) return;
+  const float sc = ldexpf(1.0f, -fexp[it.f]);
+  const float* row = L + F.lpos + i;
+  for (int j0 = c0; j0 < cend; j0 += 8) {
+    float x[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) x[q] = j0 + q < c1 ? row[(int64_t)(j0 + q) * F.ld] * sc : 0.0f;
+    v8bf h, l;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      h[q] = (__bf16)x[q];
+      l[q] = (__bf16)(x[q] - (float)h[q]);
+    }
+    const int64_t o = F.bpos + ((int64_t)(j0 / BK) * F.ld + i) * BK + j0 % BK;
+    *(v8bf*)(Lh + o) = h;
+    *(v8bf*)(Ll + o) = l;
+  }
+}
+
 template <int TERMS>
-__global__ __launch_bounds__(256, 2) void k_gemm_bf16(const GemmItem* __restrict__ items, float* L, float* U,
-                                                      float* Part, const int32_t* __restrict__ fexp) {
+__global__ __launch_bounds__(256, 2) void k_gemm_b16(const GemmItem* __restrict__ items, const FrontDev* __restrict__ fronts,
+                                                     const __bf16* __restrict__ Lh, const __bf16* __restrict__ Ll,
+                                                     float* L, float* U, float* Part, const int32_t* __restrict__ fexp) {
   static_assert(TERMS == 1 || TERMS == 3, "one or three BF16 products");
   constexpr int NS = TERMS == 3 ? 2 : 1;  // hi (and lo) copies of each operand
   __shared__ __attribute__((aligned(16))) __bf16 As[2][NS][GB][BROW];
   __shared__ __attribute__((aligned(16))) __bf16 Bs[2][NS][GB][BROW];
   const GemmItem it = items[blockIdx.x];
+  const FrontDev F = fronts[it.front];
   const int e = fexp[it.front];
-  const float sc = ldexpf(1.0f, -e);  // exact power of two
+  // the operands' first rows and columns in the front, and their origins in the copies
+  const int64_t oa = it.a - F.lpos, ob = it.b - F.lpos;
+  const int64_t chunk = (int64_t)F.ld * BK;  // elements from one k-chunk of a row to the next
+  const int64_t a0 = F.bpos + (oa / F.ld) / BK * chunk + (oa % F.ld) * BK;
+  const int64_t b0 = F.bpos + (ob / F.ld) / BK * chunk + (ob % F.ld) * BK;
   auto base = [&](int8_t src) { return src == S_L ? L : (src == S_U ? U : Part); };
-  const float* __restrict__ A = base(it.asrc) + it.a;
-  const float* __restrict__ B = base(it.bsrc) + it.b;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wi = (wave >> 1) * 64, wj = (wave & 1) * 64;
-  const int lrow = tid & (GB - 1), lk = (tid >> 7) * 16;  // loader: one row, 16 consecutive k
-  const bool arow = lrow < it.m, brow = lrow < it.n;
-  float ra[16], rb[16];
-  auto load = [&](int ks) {
+  // loader: 16-byte vectors v = tid + 256 q of a stage (row v / 4, k 8 (v % 4) + [0, 8))
+  u32x4 ra[2][NS], rb[2][NS];
+  auto load = [&](int s) {
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const int k = ks + lk + r;
-      const bool kin = k < it.k;
-      ra[r] = (arow && kin) ? A[(int64_t)k * it.lda + lrow] : 0.0f;
-      rb[r] = (brow && kin) ? B[(int64_t)k * it.ldb + lrow] : 0.0f;
+    for (int q = 0; q < 2; ++q) {
+      const int v = tid + 256 * q, row = v >> 2, part = v & 3;
+      const int64_t oa2 = a0 + s * chunk + row * BK + part * 8, ob2 = b0 + s * chunk + row * BK + part * 8;
+      const u32x4 z = {0, 0, 0, 0};
+      ra[q][0] = row < it.m ? *(const u32x4*)(Lh + oa2) : z;
+      rb[q][0] = row < it.n ? *(const u32x4*)(Lh + ob2) : z;
+      if (TERMS == 3) {
+        ra[q][NS - 1] = row < it.m ? *(const u32x4*)(Ll + oa2) : z;
+        rb[q][NS - 1] = row < it.n ? *(const u32x4*)(Ll + ob2) : z;
+      }
     }
   };
   auto store = [&](int buf) {
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      v8bf ah, bh, al, bl;
+    for (int q = 0; q < 2; ++q) {
+      const int v = tid + 256 * q, row = v >> 2, part = v & 3;
 #pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        const float xa = ra[8 * h + q] * sc, xb = rb[8 * h + q] * sc;
-        ah[q] = (__bf16)xa;
-        bh[q] = (__bf16)xb;
-        if (TERMS == 3) {
-          al[q] = (__bf16)(xa - (float)ah[q]);
-          bl[q] = (__bf16)(xb - (float)bh[q]);
-        }
-      }
-      *(v8bf*)&As[buf][0][lrow][lk + 8 * h] = ah;
-      *(v8bf*)&Bs[buf][0][lrow][lk + 8 * h] = bh;
-      if (TERMS == 3) {
-        *(v8bf*)&As[buf][NS - 1][lrow][lk + 8 * h] = al;
-        *(v8bf*)&Bs[buf][NS - 1][lrow][lk + 8 * h] = bl;
+      for (int t = 0; t < NS; ++t) {
+        *(u32x4*)&As[buf][t][row][part * 8] = ra[q][t];
+        *(u32x4*)&Bs[buf][t][row][part * 8] = rb[q][t];
       }
     }
   };
@@ -237,7 +265,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_bf16(const GemmItem* __restrict
   __syncthreads();
   for (int s = 0; s < stages; ++s) {
     const int cur = s & 1;
-    if (s + 1 < stages) load((s + 1) * BK);
+    if (s + 1 < stages) load(s + 1);
     v8bf fj[4], fi[4];
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
@@ -331,8 +359,8 @@ __global__ __launch_bounds__(256) void k_reduce_parts(const ReduceItem* __restri
 
 constexpr double PIVOT_TOL = 1e-10;  // cholesky.rs:48
 
-// K: the widest block the variant takes (32, 64 or NB); the host sends each panel to the
-// narrowest variant that holds it. One barrier per column: column c's trailing update reads
+// K: the widest block the variant takes (panels up to 32 wide, one wave, use it; wider ones go
+// to k_potrf_blk). One barrier per column: column c's trailing update reads
 // column c unscaled (times 1 / d_c^2) while column c - 1 is scaled in the same phase.
 template <typename T, int K>
 __global__ __launch_bounds__(K == NB ? 256 : 64 * (K / 32), 1) void k_potrf_inv(
@@ -340,10 +368,12 @@ __global__ __launch_bounds__(K == NB ? 256 : 64 * (K / 32), 1) void k_potrf_inv(
     const double* __restrict__ diag, unsigned long long* __restrict__ fail, T* __restrict__ Inv) {
   __shared__ double S[K][K + 1];  // lower: L11; strict upper S[j][i] (j < i): Linv(i, j)
   __shared__ double dv[K];        // the pivots' square roots, then their inverses
+  __shared__ double ad[K];        // |A_jj| of the block's columns, for the pivot test
   const Item it = items[blockIdx.x];
   const FrontDev F = fronts[it.f];
   const int c0 = it.a, pw = min(K, F.w - c0);
   T* base = L + F.lpos + (int64_t)c0 * F.ld + c0;
+  for (int j = threadIdx.x; j < pw; j += blockDim.x) ad[j] = fabs(diag[F.col0 + c0 + j]);
   for (int e = threadIdx.x; e < K * K; e += blockDim.x) {
     const int i = e % K, j = e / K;
     if (i >= j) S[i][j] = (i < pw && j < pw) ? (double)base[(int64_t)j * F.ld + i] : 0.0;
@@ -361,8 +391,7 @@ __global__ __launch_bounds__(K == NB ? 256 : 64 * (K / 32), 1) void k_potrf_inv(
       const double d = sqrt(p);
       if (threadIdx.x == 0) {
         dv[c] = d;
-        const double a = diag[F.col0 + c0 + c];
-        if (!bad && (!(p > PIVOT_TOL * fabs(a)) || !isfinite(p))) {
+        if (!bad && (!(p > PIVOT_TOL * ad[c]) || !isfinite(p))) {
           atomicMin(fail, (unsigned long long)(F.col0 + c0 + c));
           bad = true;
         }
@@ -397,127 +426,233 @@ __global__ __launch_bounds__(K == NB ? 256 : 64 * (K / 32), 1) void k_potrf_inv(
   }
 }
 
-// The same for blocks up to NB = 128 wide, with the block in registers: 256 threads as a
-// 16 x 16 grid, thread (ti, tj) holding entries (ti + 16 a, tj + 16 b), a, b < 8. A column
-// step publishes only its pivot and its column through LDS, and every thread updates its own
-// 64 entries, so the rank-1 updates cost no LDS traffic. The inverse is formed the same way
-// (right-looking on L X = I) with L read from LDS. Two barriers per column in each part.
+// ------------------------------------------------------------------------------------------
+// Blocked variant for panels of 33..NB columns: the block splits into halves of h0 = 64 and
+// h1 = pw - 64 columns (h1 = 0 for panels up to 64 wide),
+//   [A11    ]   [L11    ] [L11^T L21^T]          [X11    ]
+//   [A21 A22] = [L21 L22] [      L22^T],  Linv = [X21 X22],  X21 = -X22 L21 X11,
+// Each half is factored and inverted inside one wave (wave_potrf_inv): the column values pass
+// between lanes by shuffles, so the 128 sequential column steps need no workgroup barrier and
+// wait on no memory. Everything else is a small product over LDS with independent loads:
+// L21 = A21 X11^T, A22 -= L21 L21^T, T = L21 X11, X21 = -X22 T.
+//
+// LDS holds the block in FP64: S[i][j] = L(i, j) for i >= j and Linv(j, i) for i < j, the
+// inverse's diagonal in xd. Same outputs as k_potrf_inv; the pivot test is the reference's.
+
+// Wave-local Cholesky and inverse of the h x h block at S[o.., o..] (h <= 64), the wave as an
+// 8 x 8 lane grid: lane (ti, tk) = (lane % 8, lane / 8) holds a[r][c] = A(ti + 8 r, tk + 8 c)
+// and x[r][c] = X(ti + 8 r, tk + 8 c). Step j scales column j of L and broadcasts it by
+// shuffles (the trailing update is then local), and takes the same step of the inverse,
+// right-looking on L X = I: row j of X is divided by L(j, j), then X(i, :) -= L(i, j) X(j, :)
+// for i > j. Lane 0 applies the pivot test and records the first failing column.
+// Steps 8 CB .. 8 CB + 7 (a template, so that every register index is a constant)
+template <int CB>
+__device__ __forceinline__ void wpi_steps(double (&a)[8][8], double (&x)[8][8], int h, int lane, double dl,
+                                          int64_t col, unsigned long long* __restrict__ fail, bool& bad) {
+  const int ti = lane & 7, tk = lane >> 3;
+  for (int cc = 0; cc < 8; ++cc) {
+    const int j = 8 * CB + cc;
+    if (j >= h) return;
+    const double p = __shfl(a[CB][CB], cc + 8 * cc);
+    const double a0 = __shfl(dl, j);
+    const double d = sqrt(p);
+    if (lane == 0 && !bad) {
+      if (!(p > PIVOT_TOL * a0) || !isfinite(p)) {
+        atomicMin(fail, (unsigned long long)(col + j));
+        bad = true;
+      }
+    }
+    const double rd = 1.0 / d;
+    if (tk == cc) {  // the owners of column j scale it
+#pragma unroll
+      for (int r = CB; r < 8; ++r) {
+        const int i = ti + 8 * r;
+        if (i > j) a[r][CB] /= d;
+        else if (i == j) a[r][CB] = d;
+      }
+    }
+    if (ti == cc) {  // the owners of row j of X divide it by L(j, j)
+#pragma unroll
+      for (int c = 0; c <= CB; ++c) x[CB][c] *= rd;
+    }
+    double li[8], lk[8], xj[8];
+#pragma unroll
+    for (int r = CB; r < 8; ++r) li[r] = __shfl(a[r][CB], ti + 8 * cc);  // L(ti + 8r, j)
+#pragma unroll
+    for (int c = CB; c < 8; ++c) lk[c] = __shfl(a[c][CB], tk + 8 * cc);  // L(tk + 8c, j)
+#pragma unroll
+    for (int c = 0; c <= CB; ++c) xj[c] = __shfl(x[CB][c], cc + 8 * tk);  // X(j, tk + 8c)
+#pragma unroll
+    for (int r = CB; r < 8; ++r) {
+      const int i = ti + 8 * r;
+#pragma unroll
+      for (int c = CB; c < 8; ++c) {
+        const int k = tk + 8 * c;
+        if (k > j && i >= k) a[r][c] -= li[r] * lk[c];
+      }
+      if (i > j) {
+#pragma unroll
+        for (int c = 0; c <= CB; ++c) x[r][c] -= li[r] * xj[c];
+      }
+    }
+  }
+}
+
+__device__ inline void wave_potrf_inv(double (*S)[NB + 1], double* xd, int o, int h, int lane,
+                                      const double* __restrict__ dg, int64_t col,
+                                      unsigned long long* __restrict__ fail, bool& bad) {
+  const int ti = lane & 7, tk = lane >> 3;
+  // the reference diagonal of every column, read once: the pivot test then costs a shuffle
+  const double dl = lane < h ? fabs(dg[lane]) : 0.0;
+  double a[8][8], x[8][8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r)
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int i = ti + 8 * r, k = tk + 8 * c;
+      a[r][c] = (i < h && k < h && i >= k) ? S[o + i][o + k] : (i == k ? 1.0 : 0.0);
+      x[r][c] = i == k ? 1.0 : 0.0;
+    }
+  wpi_steps<0>(a, x, h, lane, dl, col, fail, bad);
+  wpi_steps<1>(a, x, h, lane, dl, col, fail, bad);
+  wpi_steps<2>(a, x, h, lane, dl, col, fail, bad);
+  wpi_steps<3>(a, x, h, lane, dl, col, fail, bad);
+  wpi_steps<4>(a, x, h, lane, dl, col, fail, bad);
+  wpi_steps<5>(a, x, h, lane, dl, col, fail, bad);
+  wpi_steps<6>(a, x, h, lane, dl, col, fail, bad);
+  wpi_steps<7>(a, x, h, lane, dl, col, fail, bad);
+#pragma unroll
+  for (int r = 0; r < 8; ++r)
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int i = ti + 8 * r, k = tk + 8 * c;
+      if (i < h && k < h) {
+        if (i > k) {
+          S[o + i][o + k] = a[r][c];
+          S[o + k][o + i] = x[r][c];
+        } else if (i == k) {
+          S[o + i][o + i] = a[r][c];
+          xd[o + i] = x[r][c];
+        }
+      }
+    }
+}
+
 template <typename T>
-__global__ __launch_bounds__(256, 1) void k_potrf_inv128(const Item* __restrict__ items, const FrontDev* __restrict__ fronts,
-                                                         T* __restrict__ L, const double* __restrict__ diag,
-                                                         unsigned long long* __restrict__ fail, T* __restrict__ Inv) {
-  constexpr int G = 16, R = NB / G;
-  __shared__ double Ls[NB][NB + 1];
-  __shared__ double col[NB], row[NB], piv[NB + 1];
+__global__ __launch_bounds__(256, 1) void k_potrf_blk(const Item* __restrict__ items, const FrontDev* __restrict__ fronts,
+                                                      T* __restrict__ L, const double* __restrict__ diag,
+                                                      unsigned long long* __restrict__ fail, T* __restrict__ Inv) {
+  constexpr int H = 64;
+  __shared__ double S[NB][NB + 1];
+  __shared__ double xd[NB];
   const Item it = items[blockIdx.x];
   const FrontDev F = fronts[it.f];
   const int c0 = it.a, pw = min(NB, F.w - c0);
+  const int h0 = min(H, pw), h1 = pw - h0;
   T* base = L + F.lpos + (int64_t)c0 * F.ld + c0;
-  const int ti = threadIdx.x % G, tj = threadIdx.x / G;
-  double s[R][R];
-#pragma unroll
-  for (int a = 0; a < R; ++a)
-#pragma unroll
-    for (int b = 0; b < R; ++b) {
-      const int i = ti + G * a, j = tj + G * b;
-      s[a][b] = (i < pw && j < pw && i >= j) ? (double)base[(int64_t)j * F.ld + i] : (i == j ? 1.0 : 0.0);
+  const double* dg = diag + F.col0 + c0;
+  const int64_t col = (int64_t)F.col0 + c0;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  {  // the lower triangle, with independent loads in flight: thread (row, every other column)
+    const int i = tid & (NB - 1), j0 = tid >> 7;
+    if (i < pw) {
+#pragma unroll 16
+      for (int j = j0; j < NB; j += 2)
+        if (j <= i) S[i][j] = (double)base[(int64_t)j * F.ld + i];
     }
-  if (threadIdx.x == 0) piv[0] = s[0][0];
+  }
   __syncthreads();
   bool bad = false;
-#pragma unroll
-  for (int cb = 0; cb < R; ++cb) {
-    for (int cc = 0; cc < G; ++cc) {
-      const int c = G * cb + cc;
-      if (c >= pw) break;
-      const double p = piv[c];
-      const double d = sqrt(p);
-      if (threadIdx.x == 0 && !bad) {
-        const double a0 = diag[F.col0 + c0 + c];
-        if (!(p > PIVOT_TOL * fabs(a0)) || !isfinite(p)) {
-          atomicMin(fail, (unsigned long long)(F.col0 + c0 + c));
-          bad = true;
-        }
-      }
-      if (tj == cc) {  // the owners of column c scale it and publish it
-#pragma unroll
-        for (int a = 0; a < R; ++a) {
-          const int i = ti + G * a;
-          if (i > c) {
-            s[a][cb] /= d;
-            col[i] = s[a][cb];
-          } else if (i == c) {
-            s[a][cb] = d;
-          }
-        }
-      }
-      __syncthreads();
-      double li[R], lj[R];
-#pragma unroll
-      for (int a = 0; a < R; ++a) li[a] = (ti + G * a > c) ? col[ti + G * a] : 0.0;
-#pragma unroll
-      for (int b = 0; b < R; ++b) lj[b] = (tj + G * b > c) ? col[tj + G * b] : 0.0;
-#pragma unroll
-      for (int a = 0; a < R; ++a)
-#pragma unroll
-        for (int b = 0; b < R; ++b)
-          if (ti + G * a >= tj + G * b) s[a][b] -= li[a] * lj[b];
-      // the next pivot, from its owner
-      const int n1 = c + 1;
-      if (n1 < pw && ti == n1 % G && tj == n1 % G) piv[n1] = (cc + 1 < G) ? s[cb][cb] : s[(cb + 1) % R][(cb + 1) % R];
-      __syncthreads();
-    }
-  }
-  // L11 to LDS (and back to the front), then X = L11^-1 in registers
-#pragma unroll
-  for (int a = 0; a < R; ++a)
-#pragma unroll
-    for (int b = 0; b < R; ++b) {
-      const int i = ti + G * a, j = tj + G * b;
-      if (i >= j) {
-        Ls[i][j] = s[a][b];
-        if (i < pw && j < pw) base[(int64_t)j * F.ld + i] = (T)s[a][b];
-      }
-    }
-#pragma unroll
-  for (int a = 0; a < R; ++a)
-#pragma unroll
-    for (int b = 0; b < R; ++b) s[a][b] = (ti + G * a == tj + G * b) ? 1.0 : 0.0;  // X = I
+  if (wave == 0) wave_potrf_inv(S, xd, 0, h0, lane, dg, col, fail, bad);  // L11, X11
   __syncthreads();
+  // X11(k, t): S[t][k] below the diagonal (k > t), xd[t] on it, 0 above
+  auto x11 = [&](int k, int t) { return k > t ? S[t][k] : (k == t ? xd[t] : 0.0); };
+  if (h1 > 0) {
+    const int i = lane, t0 = 16 * wave;  // row i of the lower half, columns t0 + [0, 16)
+    {  // L21 = A21 X11^T: L21(i, t) = sum_{k <= t} A21(i, k) X11(t, k)
+      double acc[16] = {};
+      if (i < h1) {
+        for (int k = 0; k < h0; ++k) {
+          const double av = S[h0 + i][k];
 #pragma unroll
-  for (int kb = 0; kb < R; ++kb) {
-    for (int kc = 0; kc < G; ++kc) {
-      const int k = G * kb + kc;
-      if (k >= pw) break;
-      const double dk = Ls[k][k];
-      if (ti == kc) {  // the owners of row k of X finish it and publish it
-#pragma unroll
-        for (int b = 0; b < R; ++b) {
-          s[kb][b] /= dk;
-          row[tj + G * b] = s[kb][b];
+          for (int q = 0; q < 16; ++q) acc[q] += av * x11(t0 + q, k);
         }
       }
-      __syncthreads();
-      double lk[R], xk[R];
+      __syncthreads();  // every A21 read before L21 is stored over it
+      if (i < h1)
 #pragma unroll
-      for (int a = 0; a < R; ++a) lk[a] = (ti + G * a > k) ? Ls[ti + G * a][k] : 0.0;
-#pragma unroll
-      for (int b = 0; b < R; ++b) xk[b] = row[tj + G * b];
-#pragma unroll
-      for (int a = 0; a < R; ++a)
-#pragma unroll
-        for (int b = 0; b < R; ++b) s[a][b] -= lk[a] * xk[b];
-      __syncthreads();
+        for (int q = 0; q < 16; ++q) S[h0 + i][t0 + q] = acc[q];
     }
+    __syncthreads();
+    {  // A22 -= L21 L21^T, a 4 x 4 block of the lower triangle per thread
+      const int bi = tid & 15, bj = tid >> 4;
+      if (bi >= bj && 4 * bj < h1) {
+        double acc[4][4] = {};
+        for (int k = 0; k < h0; ++k) {
+          double u[4], v[4];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            u[q] = 4 * bi + q < h1 ? S[h0 + 4 * bi + q][k] : 0.0;
+            v[q] = 4 * bj + q < h1 ? S[h0 + 4 * bj + q][k] : 0.0;
+          }
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) acc[q][r] += u[q] * v[r];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int ii = 4 * bi + q, jj = 4 * bj + r;
+            if (ii < h1 && ii >= jj) S[h0 + ii][h0 + jj] -= acc[q][r];
+          }
+      }
+    }
+    __syncthreads();
+    if (wave == 0) wave_potrf_inv(S, xd, h0, h1, lane, dg + h0, col + h0, fail, bad);  // L22, X22
+    __syncthreads();
+    // L is final: write it out; T = L21 X11 (row i, columns t0 + [0, 16)) in registers
+    for (int j = 0; j < pw; ++j)
+      for (int r = j + tid; r < pw; r += blockDim.x) base[(int64_t)j * F.ld + r] = (T)S[r][j];
+    double tr[16] = {};
+    if (i < h1) {
+      for (int k = 0; k < h0; ++k) {
+        const double l = S[h0 + i][k];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) tr[q] += l * x11(k, t0 + q);
+      }
+    }
+    __syncthreads();  // L21 written out and read
+    if (i < h1)
+#pragma unroll
+      for (int q = 0; q < 16; ++q) S[h0 + i][t0 + q] = tr[q];  // T over L21
+    __syncthreads();
+    {  // X21 = -X22 T: column t = lane, rows 16 wave + [0, 16), stored at S[t][h0 + i]
+      const int t = lane, i0 = 16 * wave;
+      double acc[16] = {};
+      for (int k = 0; k < h1; ++k) {
+        const double tk = S[h0 + k][t];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+          const int ii = i0 + q;  // X22(ii, k): S[h0 + k][h0 + ii] below the diagonal, xd on it
+          const double xv = ii > k ? S[h0 + k][h0 + ii] : (ii == k ? xd[h0 + ii] : 0.0);
+          acc[q] += xv * tk;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 16; ++q)
+        if (i0 + q < h1) S[t][h0 + i0 + q] = -acc[q];
+    }
+  } else {
+    for (int j = 0; j < pw; ++j)
+      for (int r = j + tid; r < pw; r += blockDim.x) base[(int64_t)j * F.ld + r] = (T)S[r][j];
   }
+  __syncthreads();
   T* inv = Inv + F.inv0 + (int64_t)(c0 / NB) * NB * NB;  // ld = pw
-#pragma unroll
-  for (int a = 0; a < R; ++a)
-#pragma unroll
-    for (int b = 0; b < R; ++b) {
-      const int i = ti + G * a, j = tj + G * b;
-      if (i < pw && j < pw) inv[(int64_t)j * pw + i] = (T)(i >= j ? s[a][b] : 0.0);
-    }
+  for (int j = 0; j < pw; ++j)
+    for (int r = tid; r < pw; r += blockDim.x)
+      inv[(int64_t)j * pw + r] = (T)(r == j ? xd[r] : (r > j ? S[j][r] : 0.0));
 }
 
 }  // namespace fc

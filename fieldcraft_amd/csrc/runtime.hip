@@ -99,6 +99,8 @@ struct Program {
   std::vector<fc::ReduceItem> ritems;  // split-K reductions
   std::vector<fc::Launch> launches;
   std::vector<i64> f_inv0;             // per front: its diagonal-block inverses in the Inv buffer
+  std::vector<i64> f_bpos;             // per front: its L part in the BF16 copies (mixed form)
+  i64 b_size = 0;                      // entries of each BF16 copy
   i64 inv_size = 0;                    // entries of the Inv buffer (kept with the factor)
   i64 part_size = 0;                   // entries of the split-K workspace
   std::vector<fc::Item> sitems;       // solve work items
@@ -109,8 +111,9 @@ struct Program {
 };
 
 // kinds of solve launches
-enum SolveKind : int32_t { S_GATHER = 0, S_FWD_STEP = 1, S_BWD_BELOW = 2, S_BWD_STEP = 3 };
+enum SolveKind : int32_t { S_GATHER = 0, S_FWD_STEP = 1, S_BWD_GATHER = 2, S_BWD_BELOW = 3, S_BWD_FIRST = 4, S_BWD_STEP = 5 };
 static_assert(fc::SNB == fc::NB, "solve blocks are the factorization's panels");
+static_assert(fc::BCOLS == fc::SNB, "a backward column tile is one block");
 
 // lower pairs (i >= j) of rows [r0, r1) x columns [c0, c1)
 double lower_pairs(i64 r0, i64 r1, i64 c0, i64 c1) {
@@ -148,7 +151,9 @@ void add_gemm(Program& p, const std::vector<Tile>& tiles, int kind, i64 level) {
       continue;
     }
     i64 chunk = (t.g.k + split - 1) / split;
-    chunk = (chunk + fc::BK - 1) / fc::BK * fc::BK;  // whole LDS stages of either kernel
+    // whole stages of either kernel: every piece of an UPDATE or SYRK product (which all start
+    // at column 0 of their front) then starts on a chunk of the BF16 copies
+    chunk = (chunk + fc::BK - 1) / fc::BK * fc::BK;
     split = (t.g.k + chunk - 1) / chunk;
     fc::ReduceItem r{};
     r.c = t.g.c;
@@ -196,6 +201,12 @@ Program build_program(const fc::Symbolic& s) {
   };
   // the inverses of every front's diagonal blocks (NB x NB, the last one pw x pw), kept for
   // the solves
+  // the BF16 copies of the mixed form: each front's L part by chunks of BK columns
+  p.f_bpos.resize(s.nf);
+  for (i64 f = 0; f < s.nf; ++f) {
+    p.f_bpos[f] = p.b_size;
+    p.b_size += (s.f_ld[f] * ((s.f_w[f] + fc::BK - 1) / fc::BK) * fc::BK + 63) / 64 * 64;
+  }
   p.f_inv0.resize(s.nf);
   for (i64 f = 0; f < s.nf; ++f) {
     p.f_inv0[f] = p.inv_size;
@@ -262,13 +273,13 @@ Program build_program(const fc::Symbolic& s) {
         add_gemm(p, tiles, fc::K_UPDATE, l);
       }
       // diagonal blocks, each to the narrowest kernel variant that holds it
-      for (int variant : {32, 64, (int)fc::NB}) {
+      for (int variant : {32, (int)fc::NB}) {  // k_potrf_inv<T, 32>, k_potrf_blk
         i0 = (i64)p.items.size();
         for (i64 t = 0; t < nl; ++t) {
           const i64 f = fl[t];
           if (s.f_w[f] <= c0) continue;
           const i64 pw = std::min<i64>(fc::NB, s.f_w[f] - c0);
-          const int need = pw <= 32 ? 32 : (pw <= 64 ? 64 : (int)fc::NB);
+          const int need = pw <= 32 ? 32 : (int)fc::NB;
           if (need == variant) {
             p.items.push_back(Item{(int32_t)f, (int32_t)c0, 0, 0, 0, 0, 0, 0});
             flops += (double)pw * (pw + 1) * (2 * pw + 1) / 6.0;  // sum_{t <= pw} t^2
@@ -304,6 +315,18 @@ Program build_program(const fc::Symbolic& s) {
         }
       }
       add_gemm(p, tiles, fc::K_TRSM, l);
+      // the finished panel's BF16 copies (run by the mixed form only)
+      i0 = (i64)p.items.size();
+      for (i64 t = 0; t < nl; ++t) {
+        const i64 f = fl[t];
+        if (s.f_w[f] <= c0) continue;
+        const i64 c1 = std::min(c0 + fc::NB, s.f_w[f]);
+        for (i64 r = c0; r < s.f_m[f]; r += 256) {
+          p.items.push_back(Item{(int32_t)f, (int32_t)c0, (int32_t)r, 0, 0, 0, 0, 0});
+          entries += (double)(std::min<i64>(r + 256, s.f_m[f]) - r) * (double)(c1 - c0);
+        }
+      }
+      add(fc::K_SPLIT, l, i0);
     }
     // update matrices U = -L21 L21^T
     tiles.clear();
@@ -363,12 +386,9 @@ Program build_program(const fc::Symbolic& s) {
         const i64 f = fl[t];
         if (s.f_w[f] <= c0) continue;
         const i64 c1 = std::min(c0 + fc::SNB, s.f_w[f]);
-        // one item stores the block's solution even when no rows lie below it
-        i64 r = c1;
-        do {
-          p.sitems.push_back(Item{(int32_t)f, (int32_t)c0, (int32_t)r, r == c1 ? 1 : 0, 0, 0, 0, 0});
-          r += fc::SROWS;
-        } while (r < s.f_m[f]);
+        // the first row tile also forms the next block's solution
+        for (i64 r = c1; r < s.f_m[f]; r += fc::SROWS)
+          p.sitems.push_back(Item{(int32_t)f, (int32_t)c0, (int32_t)r, (r == c1 && c1 < s.f_w[f]) ? 1 : 0, 0, 0, 0, 0});
       }
       sadd(p.sfwd, S_FWD_STEP, l, i0);
     }
@@ -378,6 +398,10 @@ Program build_program(const fc::Symbolic& s) {
     const i64 nl = s.lvl_ptr[l + 1] - s.lvl_ptr[l];
     i64 wmax = 0;
     i64 i0 = (i64)p.sitems.size();
+    for (i64 t = 0; t < nl; ++t)
+      if (s.f_m[fl[t]] > s.f_w[fl[t]]) p.sitems.push_back(Item{(int32_t)fl[t], 0, 0, 0, 0, 0, 0, 0});
+    sadd(p.sbwd, S_BWD_GATHER, l, i0);
+    i0 = (i64)p.sitems.size();
     for (i64 t = 0; t < nl; ++t) {
       const i64 f = fl[t];
       wmax = std::max(wmax, s.f_w[f]);
@@ -385,19 +409,19 @@ Program build_program(const fc::Symbolic& s) {
       for (i64 j = 0; j < s.f_w[f]; j += fc::SCOLS) p.sitems.push_back(Item{(int32_t)f, (int32_t)j, 0, 0, 0, 0, 0, 0});
     }
     sadd(p.sbwd, S_BWD_BELOW, l, i0);
+    i0 = (i64)p.sitems.size();
+    for (i64 t = 0; t < nl; ++t) p.sitems.push_back(Item{(int32_t)fl[t], 0, 0, 0, 0, 0, 0, 0});
+    sadd(p.sbwd, S_BWD_FIRST, l, i0);
     const i64 nblocks = (wmax + fc::SNB - 1) / fc::SNB;
-    for (i64 b = nblocks - 1; b >= 0; --b) {
+    for (i64 b = nblocks - 1; b >= 1; --b) {
       const i64 c0 = b * fc::SNB;
       i0 = (i64)p.sitems.size();
       for (i64 t = 0; t < nl; ++t) {
         const i64 f = fl[t];
         if (s.f_w[f] <= c0) continue;
-        // one item stores the block's solution even when no columns lie before it
-        i64 j = 0;
-        do {
-          p.sitems.push_back(Item{(int32_t)f, (int32_t)c0, (int32_t)j, j == 0 ? 1 : 0, 0, 0, 0, 0});
-          j += fc::SCOLS;
-        } while (j < c0);
+        // the last column tile (the previous block's columns) also forms that block's solution
+        for (i64 j = 0; j < c0; j += fc::BCOLS)
+          p.sitems.push_back(Item{(int32_t)f, (int32_t)c0, (int32_t)j, j == c0 - fc::BCOLS ? 1 : 0, 0, 0, 0, 0});
       }
       sadd(p.sbwd, S_BWD_STEP, l, i0);
     }
@@ -461,6 +485,7 @@ struct fc_symbolic {
       d.rel_off = S.f_rel_off[f];
       d.rows_off = S.f_rptr[f];
       d.inv0 = prog.f_inv0[f];
+      d.bpos = prog.f_bpos[f];
       d.ld = (int32_t)S.f_ld[f];
       d.w = (int32_t)S.f_w[f];
       d.m = (int32_t)S.f_m[f];
@@ -505,6 +530,9 @@ struct fc_factor {
   Buffer U;                // update-matrix arena (kept between factorizations)
   Buffer inv, part;        // diagonal-block inverses, split-K pieces
   Buffer fexp;             // mixed form: per front, the exponent of its power-of-two scale
+  Buffer lb;               // mixed form: BF16 copies (hi, then lo) of the finished panels
+  hipGraph_t graph = nullptr;          // opts.use_graph: the captured factorization
+  hipGraphExec_t graph_exec = nullptr;
   Buffer vals;             // the matrix values on the device (host-value entry points)
   mutable std::mutex solve_mu;
   mutable Buffer xp, yp, usolve, refine_work;  // solve workspaces
@@ -523,6 +551,9 @@ struct fc_factor {
     inv.release();
     part.release();
     fexp.release();
+    lb.release();
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
     vals.release();
     xp.release();
     yp.release();
@@ -540,6 +571,14 @@ struct fc_matrix {
   int32_t* idx = nullptr;
   double* val = nullptr;
   double norm_inf = 0.0;
+  std::vector<double> row_abs;   // sum |a_ij| of each row (host)
+  // independent segments of equations (rows [seg[s], seg[s + 1])) and their norms; reductions
+  // run over row blocks of at most RED_ROWS rows that never straddle a segment
+  std::vector<i64> seg;
+  std::vector<double> seg_norm;
+  std::vector<int64_t> blk;      // row block starts (nblk + 1)
+  std::vector<int32_t> blk_seg;  // segment of each row block
+  int64_t* dblk = nullptr;
   ~fc_matrix() {
     int cur = 0;
     (void)hipGetDevice(&cur);
@@ -547,6 +586,7 @@ struct fc_matrix {
     dfree(ptr);
     dfree(idx);
     dfree(val);
+    dfree(dblk);
     (void)hipSetDevice(cur);
   }
 };
@@ -579,8 +619,12 @@ struct Events {
   }
 };
 
+// Every launch of one numeric factorization, on stream st. With `eager` the cancel flag is
+// polled between levels and the kinds selected by the profile mask are timed with events;
+// otherwise (stream capture into a graph) neither.
 template <typename T>
-void run_factor(fc_factor* F, const double* dval, i64 nnz, hipStream_t st) {
+void enqueue_factor(fc_factor* F, const double* dval, i64 nnz, hipStream_t st, bool eager, Events& ev,
+                    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>>& timed) {
   const fc::Symbolic& s = F->sym->s;
   DevicePlan& P = *F->plan;
   T* L = (T*)F->L;
@@ -588,29 +632,30 @@ void run_factor(fc_factor* F, const double* dval, i64 nnz, hipStream_t st) {
   const auto& pr = F->sym->prog;
   T* Inv = (T*)F->inv.get((size_t)std::max<i64>(pr.inv_size, 1) * sizeof(T));
   T* Part = (T*)F->part.get((size_t)std::max<i64>(pr.part_size, 1) * sizeof(T));
-  Events ev;
-  hipEvent_t e0 = ev.make(), e1 = ev.make();
-  HIP_CHECK(hipEventRecord(e0, st));
   fc::k_fill<T><<<blocks_for(s.l_size), 256, 0, st>>>(L, s.l_size, T(0));
   fc::k_scatter_a<T><<<blocks_for(nnz), 256, 0, st>>>(dval, P.a_pos, nnz, L);
   fc::k_gather_diag<<<blocks_for(s.n), 256, 0, st>>>(dval, P.diag_src, s.n, F->diag);
   fc::k_fill<unsigned long long><<<1, 64, 0, st>>>(F->fail, 1, ~0ull);
   const bool mixed = F->precision == FC_MIXED;
   int32_t* fexp = nullptr;
+  __bf16 *Lh = nullptr, *Ll = nullptr;
   if (mixed) {
+    const size_t nb = (size_t)std::max<i64>(F->sym->prog.b_size, 1);
+    Lh = (__bf16*)F->lb.get(2 * nb * sizeof(__bf16));
+    Ll = Lh + nb;
     fexp = (int32_t*)F->fexp.get((size_t)std::max<i64>(s.nf, 1) * sizeof(int32_t));
     fc::k_front_scale<<<(int)std::min<i64>(std::max<i64>(s.nf, 1), 1 << 16), 256, 0, st>>>(P.fronts, P.rows, F->diag,
                                                                                           s.nf, fexp);
   }
   const auto& prog = F->sym->prog;
-  const int mask = F->opts.profile_mask;
-  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> timed;
+  const int mask = eager ? F->opts.profile_mask : 0;
   int level = -1;
   for (const fc::Launch& l : prog.launches) {
     if (l.level != level) {
       level = l.level;
-      if (F->opts.cancel && *F->opts.cancel) throw fc::Failure{fc::CANCELLED, -1, 0, 0, "cancelled"};
+      if (eager && F->opts.cancel && *F->opts.cancel) throw fc::Failure{fc::CANCELLED, -1, 0, 0, "cancelled"};
     }
+    if (l.kind == fc::K_SPLIT && !mixed) continue;  // the BF16 copies serve the mixed form only
     const fc::Item* it = P.items + l.item0;
     dim3 g(l.count), b(256);
     hipEvent_t a = nullptr, z = nullptr;
@@ -628,12 +673,19 @@ void run_factor(fc_factor* F, const double* dval, i64 nnz, hipStream_t st) {
       case fc::K_SYRK:
         if constexpr (std::is_same<T, float>::value) {
           if (mixed) {
-            if (F->opts.bf16_terms == 1) fc::k_gemm_bf16<1><<<g, b, 0, st>>>(P.gitems + l.item0, L, U, Part, fexp);
-            else fc::k_gemm_bf16<3><<<g, b, 0, st>>>(P.gitems + l.item0, L, U, Part, fexp);
+            if (F->opts.bf16_terms == 1)
+              fc::k_gemm_b16<1><<<g, b, 0, st>>>(P.gitems + l.item0, P.fronts, Lh, Ll, L, U, Part, fexp);
+            else
+              fc::k_gemm_b16<3><<<g, b, 0, st>>>(P.gitems + l.item0, P.fronts, Lh, Ll, L, U, Part, fexp);
             break;
           }
         }
         fc::k_gemm<T><<<g, b, 0, st>>>(P.gitems + l.item0, L, U, Inv, Part);
+        break;
+      case fc::K_SPLIT:
+        if constexpr (std::is_same<T, float>::value) {
+          if (mixed) fc::k_split_panel<<<g, b, 0, st>>>(it, P.fronts, L, Lh, Ll, fexp);
+        }
         break;
       case fc::K_TRSM:
         fc::k_gemm<T><<<g, b, 0, st>>>(P.gitems + l.item0, L, U, Inv, Part);
@@ -643,8 +695,7 @@ void run_factor(fc_factor* F, const double* dval, i64 nnz, hipStream_t st) {
         break;
       case fc::K_POTRF:
         if (l.pad == 32) fc::k_potrf_inv<T, 32><<<g, dim3(64), 0, st>>>(it, P.fronts, L, F->diag, F->fail, Inv);
-        else if (l.pad == 64) fc::k_potrf_inv<T, 64><<<g, dim3(128), 0, st>>>(it, P.fronts, L, F->diag, F->fail, Inv);
-        else fc::k_potrf_inv128<T><<<g, dim3(256), 0, st>>>(it, P.fronts, L, F->diag, F->fail, Inv);
+        else fc::k_potrf_blk<T><<<g, dim3(256), 0, st>>>(it, P.fronts, L, F->diag, F->fail, Inv);
         break;
       default:
         throw fc::Failure{fc::INTERNAL, -1, 0, 0, "unknown launch"};
@@ -655,6 +706,52 @@ void run_factor(fc_factor* F, const double* dval, i64 nnz, hipStream_t st) {
     }
   }
   HIP_CHECK(hipGetLastError());
+}
+
+// One numeric factorization. With opts.use_graph the launches are captured into a hipGraph on
+// the first call (from the factor's own copy of the values, so replays need no new arguments)
+// and replayed on every call: the cancel flag is then polled once, before the replay, and no
+// kind is timed.
+template <typename T>
+void run_factor(fc_factor* F, const double* dval, i64 nnz, hipStream_t st) {
+  Events ev;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> timed;
+  hipEvent_t e0 = ev.make(), e1 = ev.make();
+  if (F->opts.use_graph) {
+    double* own = (double*)F->vals.get((size_t)std::max<i64>(nnz, 1) * sizeof(double));
+    if (dval != own && nnz) HIP_CHECK(hipMemcpyAsync(own, dval, nnz * sizeof(double), hipMemcpyDeviceToDevice, st));
+    if (F->opts.cancel && *F->opts.cancel) throw fc::Failure{fc::CANCELLED, -1, 0, 0, "cancelled"};
+    if (!F->graph_exec) {
+      // every workspace is allocated before the capture (enqueue_factor then finds them)
+      const fc::Symbolic& s = F->sym->s;
+      const auto& pr = F->sym->prog;
+      (void)F->U.get((size_t)std::max<i64>(s.u_size, 1) * sizeof(T));
+      (void)F->inv.get((size_t)std::max<i64>(pr.inv_size, 1) * sizeof(T));
+      (void)F->part.get((size_t)std::max<i64>(pr.part_size, 1) * sizeof(T));
+      if (F->precision == FC_MIXED) {
+        (void)F->lb.get(2 * (size_t)std::max<i64>(pr.b_size, 1) * sizeof(__bf16));
+        (void)F->fexp.get((size_t)std::max<i64>(s.nf, 1) * sizeof(int32_t));
+      }
+      HIP_CHECK(hipStreamSynchronize(st));
+      hipGraph_t g = nullptr;
+      HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      try {
+        enqueue_factor<T>(F, own, nnz, st, false, ev, timed);
+      } catch (...) {
+        (void)hipStreamEndCapture(st, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      HIP_CHECK(hipStreamEndCapture(st, &g));
+      F->graph = g;
+      HIP_CHECK(hipGraphInstantiate(&F->graph_exec, g, nullptr, nullptr, 0));
+    }
+    HIP_CHECK(hipEventRecord(e0, st));
+    HIP_CHECK(hipGraphLaunch(F->graph_exec, st));
+  } else {
+    HIP_CHECK(hipEventRecord(e0, st));
+    enqueue_factor<T>(F, dval, nnz, st, true, ev, timed);
+  }
   HIP_CHECK(hipEventRecord(e1, st));
   HIP_CHECK(hipEventSynchronize(e1));
   float ms = 0.f;
@@ -720,8 +817,8 @@ int factor_host(fc_factor* F, const double* upper_val, uint64_t nnz, fc_error* e
   }
 }
 
-template <typename T>
-void run_solve(const fc_factor* F, double* xp, double* yp, i64 ldx, int nrhs, hipStream_t st) {
+template <typename T, int R>
+void run_solve_r(const fc_factor* F, double* xp, double* yp, i64 ldx, int nrhs, hipStream_t st) {
   const Program& prog = F->sym->prog;
   DevicePlan& P = *F->plan;
   const T* L = (const T*)F->L;
@@ -732,22 +829,34 @@ void run_solve(const fc_factor* F, double* xp, double* yp, i64 ldx, int nrhs, hi
     dim3 g(l.count), b(256);
     switch (l.kind) {
       case S_GATHER:
-        fc::k_solve_gather<<<g, b, 0, st>>>(it, P.fronts, P.kids, P.rel, xp, ldx, nrhs, u, P.uoff);
+        fc::k_fwd_gather<T, R><<<g, b, 0, st>>>(it, P.fronts, P.kids, P.rel, Inv, xp, yp, ldx, nrhs, u, P.uoff);
         break;
       case S_FWD_STEP:
-        fc::k_fwd_step<T><<<g, b, 0, st>>>(it, P.fronts, L, Inv, xp, yp, ldx, nrhs, u, P.uoff);
+        fc::k_fwd_step<T, R><<<g, b, 0, st>>>(it, P.fronts, L, Inv, xp, yp, ldx, nrhs, u, P.uoff);
+        break;
+      case S_BWD_GATHER:
+        fc::k_bwd_gather<<<g, b, 0, st>>>(it, P.fronts, P.rows, xp, ldx, nrhs, u, P.uoff);
         break;
       case S_BWD_BELOW:
-        fc::k_bwd_below<T><<<g, b, 0, st>>>(it, P.fronts, P.rows, L, xp, yp, ldx, nrhs);
+        fc::k_bwd_below<T, R><<<g, b, 0, st>>>(it, P.fronts, L, u, P.uoff, yp, ldx, nrhs);
+        break;
+      case S_BWD_FIRST:
+        fc::k_bwd_first<T, R><<<g, b, 0, st>>>(it, P.fronts, Inv, yp, xp, ldx, nrhs);
         break;
       case S_BWD_STEP:
-        fc::k_bwd_step<T><<<g, b, 0, st>>>(it, P.fronts, L, Inv, yp, xp, ldx, nrhs);
+        fc::k_bwd_step<T, R><<<g, b, 0, st>>>(it, P.fronts, L, Inv, yp, xp, ldx, nrhs);
         break;
     }
   };
   for (const fc::Launch& l : prog.sfwd) run(l);
   for (const fc::Launch& l : prog.sbwd) run(l);
   HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void run_solve(const fc_factor* F, double* xp, double* yp, i64 ldx, int nrhs, hipStream_t st) {
+  if (nrhs == 1) run_solve_r<T, 1>(F, xp, yp, ldx, nrhs, st);
+  else run_solve_r<T, fc::RC>(F, xp, yp, ldx, nrhs, st);
 }
 
 // b (device, n x nrhs, ldb) solved in place, MAX_RHS right-hand sides at a time; the caller
@@ -767,88 +876,169 @@ void solve_device(const fc_factor* F, double* b, i64 n, int nrhs, i64 ldb, hipSt
   HIP_CHECK(hipGetLastError());
 }
 
+constexpr i64 RED_ROWS = 8192;  // rows of a reduction block
+
+// Segments and row blocks of a matrix (one segment: the whole matrix).
+void set_segments(fc_matrix* A, const std::vector<i64>& seg) {
+  A->seg = seg;
+  A->seg_norm.assign(seg.size() - 1, 0.0);
+  A->blk.clear();
+  A->blk_seg.clear();
+  for (size_t q = 0; q + 1 < seg.size(); ++q) {
+    for (i64 r = seg[q]; r < seg[q + 1]; ++r) A->seg_norm[q] = std::max(A->seg_norm[q], A->row_abs[r]);
+    for (i64 r = seg[q]; r < seg[q + 1]; r += RED_ROWS) {
+      A->blk.push_back(r);
+      A->blk_seg.push_back((int32_t)q);
+    }
+  }
+  A->blk.push_back(A->n);
+  A->norm_inf = 0.0;
+  for (double v : A->seg_norm) A->norm_inf = std::max(A->norm_inf, v);
+  dfree(A->dblk);
+  A->dblk = nullptr;
+  A->dblk = dupload(A->blk, nullptr);
+  HIP_CHECK(hipDeviceSynchronize());
+}
+
+// Column-wise reductions over the matrix's row blocks, combined on the host in order.
 struct Reducer {
-  double* partial = nullptr;
-  std::vector<double> host = std::vector<double>(fc::RED_BLOCKS);
-  double run(hipStream_t st, i64 n, const double* a, const double* b, int mode) {
-    fc::k_reduce<<<fc::RED_BLOCKS, 256, 0, st>>>(n, a, b, mode, partial);
-    HIP_CHECK(hipMemcpyAsync(host.data(), partial, fc::RED_BLOCKS * sizeof(double), hipMemcpyDeviceToHost, st));
+  const fc_matrix* A = nullptr;
+  double* partial = nullptr;  // k * nblk on the device
+  std::vector<double> host;
+  int nblk = 0, k = 0;
+  void run(hipStream_t st, const double* a, const double* b, int mode, int kk, i64 ld) {
+    k = kk;
+    nblk = (int)A->blk.size() - 1;
+    fc::k_reduce<<<dim3(nblk, k), 256, 0, st>>>(A->dblk, nblk, a, b, ld, mode, partial);
+    host.resize((size_t)nblk * k);
+    HIP_CHECK(hipMemcpyAsync(host.data(), partial, host.size() * sizeof(double), hipMemcpyDeviceToHost, st));
     HIP_CHECK(hipStreamSynchronize(st));
+  }
+  double sum(int c) const {
     double s = 0.0;
-    for (double v : host) s = mode == 0 ? s + v : std::max(s, v);
+    for (int q = 0; q < nblk; ++q) s += host[(size_t)c * nblk + q];
     return s;
+  }
+  // max over the blocks of each segment, for column c
+  void seg_max(int c, std::vector<double>& out) const {
+    out.assign(A->seg.size() - 1, 0.0);
+    for (int q = 0; q < nblk; ++q) out[A->blk_seg[q]] = std::max(out[A->blk_seg[q]], host[(size_t)c * nblk + q]);
   }
 };
 
-// Refinement on device vectors db (right-hand side) and dx (solution); the caller holds solve_mu.
-void refine_device(const fc_factor* f, const fc_matrix* a, const double* db, double* dx, i64 N, fc_refine_opts o,
-                   fc_refine_report& rep, Buffer& work) {
+// Refinement of k <= MAX_RHS solutions X (N x k, ld = N) of A X = B with the factor's solves:
+// plain refinement while every step at least halves the backward error, then conjugate
+// gradients preconditioned by the factor (per-column scalars). The backward error of a column
+// is the largest over the segments of ||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf);
+// iteration stops when every column meets the target. berr[c] receives each column's.
+void refine_block(const fc_factor* f, const fc_matrix* a, const double* db, double* dx, i64 N, int k, fc_refine_opts o,
+                  fc_refine_report& rep, double* berr_cols, Buffer& work) {
   hipStream_t st = f->stream;
-  double* buf = (double*)work.get(((size_t)N * 4 + fc::RED_BLOCKS) * sizeof(double));
-  double *dr = buf, *dz = buf + N, *dp = buf + 2 * N, *dq = buf + 3 * N;
+  const i64 NK = N * k;
+  const int nblk = (int)a->blk.size() - 1;
+  double* buf = (double*)work.get(((size_t)NK * 4 + (size_t)nblk * k) * sizeof(double));
+  double *dr = buf, *dz = buf + NK, *dp = buf + 2 * NK, *dq = buf + 3 * NK;
   Reducer red;
-  red.partial = buf + 4 * N;
-  const int g = blocks_for(N);
+  red.A = a;
+  red.partial = buf + 4 * NK;
+  const int g = blocks_for(NK);
   Events ev;
   hipEvent_t e0 = ev.make(), e1 = ev.make();
   HIP_CHECK(hipEventRecord(e0, st));
-  const double bnorm = red.run(st, N, db, db, 1);
-  auto spmv = [&](const double* in, double* out) {
-    fc::k_spmv<<<blocks_for(N), 256, 0, st>>>(N, a->ptr, a->idx, a->val, in, out);
+  const size_t nseg = a->seg.size() - 1;
+  std::vector<std::vector<double>> bn(k);
+  red.run(st, db, db, 1, k, N);
+  for (int c = 0; c < k; ++c) red.seg_max(c, bn[c]);
+  auto spmm = [&](const double* in, double* out) {
+    fc::k_spmm<<<blocks_for(N * 8), 256, 0, st>>>(N, a->ptr, a->idx, a->val, in, out, k, N);
   };
-  auto berr_of = [&](const double* r) {
-    const double rn = red.run(st, N, r, r, 1), xn = red.run(st, N, dx, dx, 1);
-    const double den = a->norm_inf * xn + bnorm;
-    return den > 0 ? rn / den : 0.0;
+  std::vector<double> berr(k, 0.0), rn, xn;
+  auto berr_of = [&](const double* r) {  // per column, into berr; returns the largest
+    std::vector<std::vector<double>> rs(k);
+    red.run(st, r, r, 1, k, N);
+    for (int c = 0; c < k; ++c) red.seg_max(c, rs[c]);
+    red.run(st, dx, dx, 1, k, N);
+    double worst = 0.0;
+    for (int c = 0; c < k; ++c) {
+      red.seg_max(c, xn);
+      berr[c] = 0.0;
+      for (size_t q = 0; q < nseg; ++q) {
+        const double den = a->seg_norm[q] * xn[q] + bn[c][q];
+        berr[c] = std::max(berr[c], den > 0 ? rs[c][q] / den : 0.0);
+      }
+      worst = std::max(worst, berr[c]);
+    }
+    return worst;
   };
-  auto residual = [&]() {  // dr = b - A x
-    spmv(dx, dq);
-    fc::k_residual<<<g, 256, 0, st>>>(N, db, dq, dr);
+  auto residual = [&]() {  // R = B - A X
+    spmm(dx, dq);
+    fc::k_residual<<<g, 256, 0, st>>>(NK, db, dq, dr);
     return berr_of(dr);
   };
-  HIP_CHECK(hipMemcpyAsync(dx, db, N * sizeof(double), hipMemcpyDeviceToDevice, st));
-  solve_device(f, dx, N, 1, N, st);
-  double berr = residual();
-  rep.berr_history[0] = berr;
+  HIP_CHECK(hipMemcpyAsync(dx, db, NK * sizeof(double), hipMemcpyDeviceToDevice, st));
+  solve_device(f, dx, N, k, N, st);
+  double worst = residual();
+  rep.berr_history[0] = worst;
   int it = 0;
   bool cg = o.method == 1;
-  double prev = berr;
+  std::vector<double> prev = berr;
+  fc::Coefs one{};
   // plain refinement while it contracts
-  while (!cg && berr > o.target_berr && it < o.max_iter) {
-    HIP_CHECK(hipMemcpyAsync(dz, dr, N * sizeof(double), hipMemcpyDeviceToDevice, st));
-    solve_device(f, dz, N, 1, N, st);
-    fc::k_axpy<<<g, 256, 0, st>>>(N, 1.0, dz, dx);
+  while (!cg && worst > o.target_berr && it < o.max_iter) {
+    HIP_CHECK(hipMemcpyAsync(dz, dr, NK * sizeof(double), hipMemcpyDeviceToDevice, st));
+    solve_device(f, dz, N, k, N, st);
+    for (int c = 0; c < k; ++c) one.v[c] = berr[c] > o.target_berr ? 1.0 : 0.0;
+    fc::k_axpy_cols<<<g, 256, 0, st>>>(N, k, N, one, dz, dx);
     ++it;
-    berr = residual();
-    if (it < 64) rep.berr_history[it] = berr;
-    if (o.method == 2 && berr > 0.5 * prev) cg = true;  // not contracting: conjugate gradients
+    worst = residual();
+    if (it < 64) rep.berr_history[it] = worst;
+    for (int c = 0; c < k; ++c)
+      if (o.method == 2 && berr[c] > o.target_berr && berr[c] > 0.5 * prev[c]) cg = true;  // not contracting
     prev = berr;
   }
-  if (cg && berr > o.target_berr && it < o.max_iter) {
-    // conjugate gradients on A d = r from the current x, preconditioned by the factor
-    HIP_CHECK(hipMemcpyAsync(dz, dr, N * sizeof(double), hipMemcpyDeviceToDevice, st));
-    solve_device(f, dz, N, 1, N, st);
-    HIP_CHECK(hipMemcpyAsync(dp, dz, N * sizeof(double), hipMemcpyDeviceToDevice, st));
-    double rz = red.run(st, N, dr, dz, 0);
+  if (cg && worst > o.target_berr && it < o.max_iter) {
+    // conjugate gradients on A D = R from the current X, preconditioned by the factor
+    HIP_CHECK(hipMemcpyAsync(dz, dr, NK * sizeof(double), hipMemcpyDeviceToDevice, st));
+    solve_device(f, dz, N, k, N, st);
+    HIP_CHECK(hipMemcpyAsync(dp, dz, NK * sizeof(double), hipMemcpyDeviceToDevice, st));
+    std::vector<double> rz(k), pq(k);
+    red.run(st, dr, dz, 0, k, N);
+    for (int c = 0; c < k; ++c) rz[c] = red.sum(c);
+    std::vector<bool> live(k);
+    for (int c = 0; c < k; ++c) live[c] = berr[c] > o.target_berr;
     while (it < o.max_iter) {
-      spmv(dp, dq);
-      const double pq = red.run(st, N, dp, dq, 0);
-      if (!(pq > 0)) break;
-      const double alpha = rz / pq;
-      fc::k_axpy<<<g, 256, 0, st>>>(N, alpha, dp, dx);
-      fc::k_axpy<<<g, 256, 0, st>>>(N, -alpha, dq, dr);
+      spmm(dp, dq);
+      red.run(st, dp, dq, 0, k, N);
+      fc::Coefs al{}, nal{};
+      bool any = false;
+      for (int c = 0; c < k; ++c) {
+        pq[c] = red.sum(c);
+        if (live[c] && !(pq[c] > 0)) live[c] = false;  // breakdown: keep this column as it is
+        al.v[c] = live[c] ? rz[c] / pq[c] : 0.0;
+        nal.v[c] = -al.v[c];
+        any = any || live[c];
+      }
+      if (!any) break;
+      fc::k_axpy_cols<<<g, 256, 0, st>>>(N, k, N, al, dp, dx);
+      fc::k_axpy_cols<<<g, 256, 0, st>>>(N, k, N, nal, dq, dr);
       ++it;
       // the backward error from the true residual, into dq (free until the next A p)
-      spmv(dx, dq);
-      fc::k_residual<<<g, 256, 0, st>>>(N, db, dq, dq);
-      berr = berr_of(dq);
-      if (it < 64) rep.berr_history[it] = berr;
-      if (berr <= o.target_berr) break;
-      HIP_CHECK(hipMemcpyAsync(dz, dr, N * sizeof(double), hipMemcpyDeviceToDevice, st));
-      solve_device(f, dz, N, 1, N, st);
-      const double rz2 = red.run(st, N, dr, dz, 0);
-      fc::k_xpby<<<g, 256, 0, st>>>(N, dz, rz2 / rz, dp);
-      rz = rz2;
+      spmm(dx, dq);
+      fc::k_residual<<<g, 256, 0, st>>>(NK, db, dq, dq);
+      worst = berr_of(dq);
+      if (it < 64) rep.berr_history[it] = worst;
+      if (worst <= o.target_berr) break;
+      for (int c = 0; c < k; ++c) live[c] = live[c] && berr[c] > o.target_berr;
+      HIP_CHECK(hipMemcpyAsync(dz, dr, NK * sizeof(double), hipMemcpyDeviceToDevice, st));
+      solve_device(f, dz, N, k, N, st);
+      red.run(st, dr, dz, 0, k, N);
+      fc::Coefs be{};
+      for (int c = 0; c < k; ++c) {
+        const double rz2 = red.sum(c);
+        be.v[c] = (live[c] && rz[c] != 0.0) ? rz2 / rz[c] : 0.0;
+        rz[c] = rz2;
+      }
+      fc::k_xpby_cols<<<g, 256, 0, st>>>(N, k, N, dz, be, dp);
     }
   }
   HIP_CHECK(hipEventRecord(e1, st));
@@ -856,9 +1046,32 @@ void refine_device(const fc_factor* f, const fc_matrix* a, const double* db, dou
   float ms = 0.f;
   HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
   rep.iterations = it;
-  rep.berr = berr;
-  rep.converged = berr <= o.target_berr;
+  rep.berr = worst;
+  rep.converged = worst <= o.target_berr;
   rep.seconds = ms * 1e-3;
+  if (berr_cols)
+    for (int c = 0; c < k; ++c) berr_cols[c] = berr[c];
+}
+
+// refine_block over blocks of MAX_RHS columns; the report keeps the worst block's berr, the
+// iterations of the slowest block and the summed time
+void refine_many(const fc_factor* f, const fc_matrix* a, const double* db, double* dx, i64 N, i64 nrhs,
+                 fc_refine_opts o, fc_refine_report& rep, double* berr_cols, Buffer& work) {
+  std::memset(&rep, 0, sizeof rep);
+  rep.converged = 1;
+  for (i64 c0 = 0; c0 < nrhs; c0 += fc::MAX_RHS) {
+    const int k = (int)std::min<i64>(fc::MAX_RHS, nrhs - c0);
+    fc_refine_report r;
+    std::memset(&r, 0, sizeof r);
+    refine_block(f, a, db + c0 * N, dx + c0 * N, N, k, o, r, berr_cols ? berr_cols + c0 : nullptr, work);
+    if (r.iterations >= rep.iterations) {
+      rep.iterations = r.iterations;
+      std::memcpy(rep.berr_history, r.berr_history, sizeof rep.berr_history);
+    }
+    rep.berr = std::max(rep.berr, r.berr);
+    rep.converged = rep.converged && r.converged;
+    rep.seconds += r.seconds;
+  }
 }
 
 fc_refine_opts refine_defaults(const fc_refine_opts* opts) {
@@ -1216,17 +1429,16 @@ int fc_matrix_new(uint64_t n, const uint64_t* ptr, const uint64_t* idx, const do
     std::vector<int32_t> ix(A->nnz);
     for (i64 t = 0; t < A->nnz; ++t) ix[t] = (int32_t)idx[t];
     std::vector<double> v(val, val + A->nnz);
-    double norm = 0.0;
+    A->row_abs.assign(n, 0.0);
     for (i64 r = 0; r < (i64)n; ++r) {
       double s = 0.0;
       for (i64 t = p[r]; t < p[r + 1]; ++t) s += std::fabs(v[t]);
-      norm = std::max(norm, s);
+      A->row_abs[r] = s;
     }
-    A->norm_inf = norm;
     A->ptr = dupload(p, nullptr);
     A->idx = dupload(ix, nullptr);
     A->val = dupload(v, nullptr);
-    HIP_CHECK(hipDeviceSynchronize());
+    set_segments(A.get(), std::vector<i64>{0, (i64)n});
   });
   if (st != FC_OK) return st;
   *out = A.release();
@@ -1234,6 +1446,24 @@ int fc_matrix_new(uint64_t n, const uint64_t* ptr, const uint64_t* idx, const do
 }
 
 void fc_matrix_free(fc_matrix* a) { delete a; }
+
+int fc_matrix_set_segments(fc_matrix* a, uint64_t count, const uint64_t* offsets, fc_error* err) {
+  if (!a || !offsets || count == 0) {
+    set_error(err, FC_INVALID_ARGUMENT, "null argument or no segment");
+    return FC_INVALID_ARGUMENT;
+  }
+  std::vector<i64> seg(offsets, offsets + count + 1);
+  bool ok = seg.front() == 0 && seg.back() == a->n;
+  for (size_t q = 0; ok && q + 1 < seg.size(); ++q) ok = seg[q] <= seg[q + 1];
+  if (!ok) {
+    set_error(err, FC_INVALID_ARGUMENT, "segment offsets must run from 0 to n without decreasing");
+    return FC_INVALID_ARGUMENT;
+  }
+  return guarded(err, "not enough device memory for the segments", [&] {
+    HIP_CHECK(hipSetDevice(a->device));
+    set_segments(a, seg);
+  });
+}
 
 int fc_solve_refined(const fc_factor* f, const fc_matrix* a, const double* b, double* x, uint64_t n,
                      const fc_refine_opts* opts, fc_refine_report* report, fc_error* err) {
@@ -1260,7 +1490,7 @@ int fc_solve_refined(const fc_factor* f, const fc_matrix* a, const double* b, do
     double* db = (double*)vec.get((size_t)2 * N * sizeof(double));
     double* dx = db + N;
     HIP_CHECK(hipMemcpyAsync(db, b, N * sizeof(double), hipMemcpyHostToDevice, f->stream));
-    refine_device(f, a, db, dx, N, o, rep, work);
+    refine_many(f, a, db, dx, N, 1, o, rep, nullptr, work);
     HIP_CHECK(hipMemcpyAsync(x, dx, N * sizeof(double), hipMemcpyDeviceToHost, f->stream));
     HIP_CHECK(hipStreamSynchronize(f->stream));
   });
@@ -1290,7 +1520,36 @@ int fc_solve_refined_device(const fc_factor* f, const fc_matrix* a, const double
       return;
     }
     HIP_CHECK(hipSetDevice(f->device));
-    refine_device(f, a, b, x, (i64)n, o, rep, f->refine_work);
+    refine_many(f, a, b, x, (i64)n, 1, o, rep, nullptr, f->refine_work);
+  });
+  if (report) *report = rep;
+  return st;
+}
+
+int fc_solve_refined_many_device(const fc_factor* f, const fc_matrix* a, const double* B, double* X, uint64_t n,
+                                 uint64_t nrhs, const fc_refine_opts* opts, fc_refine_report* report, double* berr,
+                                 fc_error* err) {
+  if (!f || !f->valid || !a || (nrhs && (!B || !X))) {
+    set_error(err, FC_INVALID_ARGUMENT, "null argument or invalid factor");
+    return FC_INVALID_ARGUMENT;
+  }
+  if ((i64)n != f->sym->s.n || a->n != (i64)n) {
+    set_error(err, FC_INTERNAL, "right-hand side length");
+    return FC_INTERNAL;
+  }
+  const fc_refine_opts o = refine_defaults(opts);
+  fc_refine_report rep;
+  std::memset(&rep, 0, sizeof rep);
+  rep.converged = 1;
+  std::lock_guard<std::mutex> g(f->solve_mu);
+  int st = guarded(err, "not enough device memory for refinement", [&] {
+    if (n == 0 || nrhs == 0) {
+      if (berr)
+        for (uint64_t c = 0; c < nrhs; ++c) berr[c] = 0.0;
+      return;
+    }
+    HIP_CHECK(hipSetDevice(f->device));
+    refine_many(f, a, B, X, (i64)n, (i64)nrhs, o, rep, berr, f->refine_work);
   });
   if (report) *report = rep;
   return st;

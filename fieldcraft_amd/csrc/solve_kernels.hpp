@@ -8,14 +8,15 @@
 // read from the solution. Contributions travel up the tree instead of being scattered into
 // ancestors, so no two workgroups write one entry and no atomics are used.
 //
-// A front is solved by blocks of NB columns, its factorization's panels, whose diagonal blocks'
-// inverses the factorization keeps. One launch per block: every workgroup forms the block's
-// solution with the inverse (a 128 x 128 product, cheap to repeat), the first one stores it,
-// and each updates its own share of the rows below (forward) or of the columns before
-// (backward). The forward pass reads v from x and writes y to a second vector; the backward
-// pass reads z from that vector and writes x, so no workgroup reads what another writes. The
-// right-hand sides are columns (postorder numbering, leading dimension ldx); every entry of L
-// is read once per RC right-hand sides.
+// A front is solved by blocks of SNB = 128 columns, its factorization's panels, whose diagonal
+// blocks' inverses the factorization keeps (column-major, ld = the block's width, zeros above
+// the diagonal), so a block's triangle is a small matrix-vector product. One launch per block
+// each way; the workgroup that finishes the rows (forward) or columns (backward) of the next
+// block also forms that block's solution, so the block solutions never wait for a launch of
+// their own. Forward reads v from x and writes y to a second vector; backward reads z from it
+// and writes x. The right-hand sides are columns (postorder numbering, leading dimension ldx),
+// R of them held at a time (1, or 16 for blocks of load cases); every entry of L is read once
+// per R right-hand sides, with independent loads kept in flight.
 #pragma once
 
 #include "device.hpp"
@@ -24,18 +25,61 @@ namespace fc {
 
 constexpr int SNB = 128;      // columns of a solve block (the factorization's panel width)
 constexpr int SROWS = 256;    // rows of a forward workgroup
-constexpr int SCOLS = 32;     // columns of a backward workgroup
-constexpr int RC = 16;        // right-hand sides a workgroup keeps in LDS at a time
-constexpr int MAX_RHS = 64;   // right-hand sides one launch handles (the host splits larger blocks)
+constexpr int SCOLS = 32;     // columns of a below-the-front workgroup (backward)
+constexpr int BCOLS = 128;    // columns of a backward block-step workgroup
+constexpr int RC = 16;        // right-hand sides of the multi-column kernels
+constexpr int MAX_RHS = 64;   // right-hand sides one solve pass handles (the host splits larger blocks)
 
-// Item: f. Zero the front's contribution block and add its children's into it and into v.
-__global__ __launch_bounds__(256) void k_solve_gather(const Item* __restrict__ items, const FrontDev* __restrict__ fronts,
-                                                      const int32_t* __restrict__ kids, const int32_t* __restrict__ rel,
-                                                      double* __restrict__ x, int64_t ldx, int nrhs, double* u,
-                                                      const int64_t* __restrict__ uoff) {
+// y[b] = sum_k Inv(b, k) v[k] for b < pw (Inv column-major, ld = pw, zero above the diagonal):
+// thread b of the first 128, loads coalesced along b and independent along k. V: LDS, [k][r].
+template <typename T, int R>
+__device__ __forceinline__ void inv_apply(const T* __restrict__ inv, int pw, const double (*V)[R + 1], int nr,
+                                          double* __restrict__ out, int64_t ldx) {
+  const int b = threadIdx.x;
+  if (b >= pw) return;
+  double s[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) s[r] = 0.0;
+#pragma unroll 16
+  for (int k = 0; k < pw; ++k) {
+    const double a = (double)inv[(int64_t)k * pw + b];
+#pragma unroll
+    for (int r = 0; r < R; ++r) s[r] += a * V[k][r];
+  }
+  for (int r = 0; r < nr; ++r) out[(int64_t)r * ldx + b] = s[r];
+}
+
+// x[b] = sum_k Inv(k, b) z[k] (the transpose): a wave per output b, lanes along k (column b of
+// Inv is contiguous), summed across the wave in a fixed order. Z: LDS, [k][r].
+template <typename T, int R>
+__device__ __forceinline__ void inv_apply_t(const T* __restrict__ inv, int pw, const double (*Z)[R + 1], int nr,
+                                            double* __restrict__ out, int64_t ldx) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int b = wave; b < pw; b += blockDim.x >> 6) {
+    const T* col = inv + (int64_t)b * pw;
+    const double a0 = lane < pw ? (double)col[lane] : 0.0;
+    const double a1 = lane + 64 < pw ? (double)col[lane + 64] : 0.0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if (r >= nr) break;
+      double t = (lane < pw ? a0 * Z[lane][r] : 0.0) + (lane + 64 < pw ? a1 * Z[lane + 64][r] : 0.0);
+      for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
+      if (lane == 0) out[(int64_t)r * ldx + b] = t;
+    }
+  }
+}
+
+// Forward, per front (item f): zero the front's contribution block, add its children's into it
+// and into v, then y of the front's first block = Inv_0 v_0.
+template <typename T, int R>
+__global__ __launch_bounds__(256) void k_fwd_gather(const Item* __restrict__ items, const FrontDev* __restrict__ fronts,
+                                                    const int32_t* __restrict__ kids, const int32_t* __restrict__ rel,
+                                                    const T* __restrict__ Inv, double* v, double* __restrict__ y,
+                                                    int64_t ldx, int nrhs, double* u, const int64_t* __restrict__ uoff) {
+  __shared__ double Vb[SNB][R + 1];
   const int f = items[blockIdx.x].f;
   const FrontDev F = fronts[f];
-  const int w = F.w, mu = F.m - F.w;
+  const int w = F.w, mu = F.m - F.w, pw = min(SNB, w);
   double* uf = u + uoff[f] * nrhs;
   for (int e = threadIdx.x; e < mu * nrhs; e += blockDim.x) uf[e] = 0.0;
   __syncthreads();
@@ -45,138 +89,235 @@ __global__ __launch_bounds__(256) void k_solve_gather(const Item* __restrict__ i
     const int cmu = C.m - C.w;
     const double* uc = u + uoff[c] * nrhs;
     const int32_t* rc = rel + C.rel_off;
-    for (int e = threadIdx.x; e < cmu * nrhs; e += blockDim.x) {
-      const int i = e % cmu, r = e / cmu;
-      const int p = rc[i];
-      if (p < w) x[r * ldx + F.col0 + p] += uc[e];
-      else uf[r * mu + p - w] += uc[e];
-    }
+    for (int r = 0; r < nrhs; ++r)
+      for (int i = threadIdx.x; i < cmu; i += blockDim.x) {
+        const int p = rc[i];
+        const double add = uc[(int64_t)r * cmu + i];
+        if (p < w) v[(int64_t)r * ldx + F.col0 + p] += add;
+        else uf[(int64_t)r * mu + p - w] += add;
+      }
+    __syncthreads();
+  }
+  for (int q0 = 0; q0 < nrhs; q0 += R) {
+    const int nr = min(R, nrhs - q0);
+    for (int r = 0; r < nr; ++r)
+      for (int k = threadIdx.x; k < pw; k += blockDim.x) Vb[k][r] = v[(int64_t)(q0 + r) * ldx + F.col0 + k];
+    __syncthreads();
+    inv_apply<T, R>(Inv + F.inv0, pw, Vb, nr, y + (int64_t)q0 * ldx + F.col0, ldx);
     __syncthreads();
   }
 }
 
-// Forward step of block [c0, c1): y_b = Linv v_b (stored by the item with c = 1), then rows
-// [r0, r0 + SROWS) at or below c1 lose L(r, block) y_b: rows of the front's columns in v (x),
-// rows below in the front's contribution u. Item: f, a = c0, b = r0, c = store flag.
-template <typename T>
+// Forward step of block [c0, c1) (y_b already in y): rows [r0, r0 + SROWS) at or below c1 lose
+// L(r, block) y_b, in v for the front's own columns, in its contribution u below them. The item
+// with c = 1 (r0 = c1 and a next block [c1, c2) in the front) then forms y of the next block
+// from the rows it has just finished. Item: f, a = c0, b = r0, c.
+template <typename T, int R>
 __global__ __launch_bounds__(256) void k_fwd_step(const Item* __restrict__ items, const FrontDev* __restrict__ fronts,
                                                   const T* __restrict__ L, const T* __restrict__ Inv, double* v,
                                                   double* __restrict__ y, int64_t ldx, int nrhs, double* __restrict__ u,
                                                   const int64_t* __restrict__ uoff) {
-  __shared__ double Vb[SNB][RC + 1];
-  __shared__ double Yb[SNB][RC + 1];
+  __shared__ double Yb[SNB][R + 1];
+  __shared__ double Vn[SNB][R + 1];
   const Item it = items[blockIdx.x];
   const FrontDev F = fronts[it.f];
   const int c0 = it.a, pw = min(SNB, F.w - c0), c1 = c0 + pw, r0 = it.b;
-  const T* inv = Inv + F.inv0 + (int64_t)(c0 / SNB) * SNB * SNB;  // ld = pw
-  const T* lf = L + F.lpos;
+  const int pn = min(SNB, F.w - c1);  // width of the next block (designated item)
+  const T* lf = L + F.lpos + (int64_t)c0 * F.ld;
   const int mu = F.m - F.w;
   double* uf = u + uoff[it.f] * nrhs;
   const int i = r0 + threadIdx.x;
-  const bool row = threadIdx.x < SROWS && i >= c1 && i < F.m;
-  for (int q0 = 0; q0 < nrhs; q0 += RC) {
-    const int nr = min(RC, nrhs - q0);
-    for (int e = threadIdx.x; e < pw * nr; e += blockDim.x) {
-      const int k = e % pw, r = e / pw;
-      Vb[k][r] = v[(int64_t)(q0 + r) * ldx + F.col0 + c0 + k];
-    }
-    __syncthreads();
-    for (int e = threadIdx.x; e < pw * nr; e += blockDim.x) {
-      const int b = e % pw, r = e / pw;
-      double s = 0.0;
-      for (int k = 0; k <= b; ++k) s += (double)inv[(int64_t)k * pw + b] * Vb[k][r];
-      Yb[b][r] = s;
-      if (it.c) y[(int64_t)(q0 + r) * ldx + F.col0 + c0 + b] = s;
-    }
+  const bool row = i < F.m;
+  for (int q0 = 0; q0 < nrhs; q0 += R) {
+    const int nr = min(R, nrhs - q0);
+    for (int r = 0; r < nr; ++r)
+      for (int k = threadIdx.x; k < pw; k += blockDim.x) Yb[k][r] = y[(int64_t)(q0 + r) * ldx + F.col0 + c0 + k];
     __syncthreads();
     if (row) {
-      double s[RC];
+      double s[R];
 #pragma unroll
-      for (int r = 0; r < RC; ++r) s[r] = 0.0;
+      for (int r = 0; r < R; ++r) s[r] = 0.0;
+#pragma unroll 16
       for (int j = 0; j < pw; ++j) {
-        const double l = (double)lf[(int64_t)(c0 + j) * F.ld + i];
+        const double l = (double)lf[(int64_t)j * F.ld + i];
 #pragma unroll
-        for (int r = 0; r < RC; ++r) s[r] += l * Yb[j][r];
+        for (int r = 0; r < R; ++r) s[r] += l * Yb[j][r];
       }
       if (i < F.w) {
-        for (int r = 0; r < nr; ++r) v[(int64_t)(q0 + r) * ldx + F.col0 + i] -= s[r];
+        for (int r = 0; r < nr; ++r) {
+          double* p = v + (int64_t)(q0 + r) * ldx + F.col0 + i;
+          const double nv = *p - s[r];
+          *p = nv;
+          if (it.c && i < c1 + pn) Vn[i - c1][r] = nv;
+        }
       } else {
         for (int r = 0; r < nr; ++r) uf[(int64_t)(q0 + r) * mu + i - F.w] -= s[r];
       }
     }
+    if (it.c) {
+      __syncthreads();
+      inv_apply<T, R>(Inv + F.inv0 + (int64_t)(c1 / SNB) * SNB * SNB, pn, Vn, nr,
+                      y + (int64_t)q0 * ldx + F.col0 + c1, ldx);
+    }
     __syncthreads();
   }
 }
 
-// Backward: z_j -= L21(:, j)^T x_below for the columns [a, a + SCOLS) of a front, z in y and
-// x_below in x; a wave per column at a time. Item f, a = j0.
-template <typename T>
+// Backward, per front (item f): its rows below, u_f[i] = x[below[i]], gathered from the
+// solution (their columns belong to ancestors, solved before), so the transposed products read
+// them contiguously.
+__global__ __launch_bounds__(256) void k_bwd_gather(const Item* __restrict__ items, const FrontDev* __restrict__ fronts,
+                                                    const int32_t* __restrict__ rows, const double* __restrict__ x,
+                                                    int64_t ldx, int nrhs, double* __restrict__ u,
+                                                    const int64_t* __restrict__ uoff) {
+  const int f = items[blockIdx.x].f;
+  const FrontDev F = fronts[f];
+  const int mu = F.m - F.w;
+  const int32_t* below = rows + F.rows_off + F.w;
+  double* uf = u + uoff[f] * nrhs;
+  for (int r = 0; r < nrhs; ++r)
+    for (int i = threadIdx.x; i < mu; i += blockDim.x) uf[(int64_t)r * mu + i] = x[(int64_t)r * ldx + below[i]];
+}
+
+// Backward: z_j -= L21(:, j)^T x_below for the columns [a, a + SCOLS) of a front (z in y,
+// x_below gathered in u). A wave takes four columns (two with several right-hand sides) at a
+// time with lanes along the rows, so each x_below value is loaded once for them; a column's
+// sum runs over the rows in the
+// same order (lane-strided, then across the wave) for any number of right-hand sides.
+// Item f, a = j0.
+template <typename T, int R>
 __global__ __launch_bounds__(256) void k_bwd_below(const Item* __restrict__ items, const FrontDev* __restrict__ fronts,
-                                                   const int32_t* __restrict__ rows, const T* __restrict__ L,
-                                                   const double* __restrict__ x, double* __restrict__ z, int64_t ldx,
-                                                   int nrhs) {
+                                                   const T* __restrict__ L, const double* __restrict__ u,
+                                                   const int64_t* __restrict__ uoff, double* __restrict__ z,
+                                                   int64_t ldx, int nrhs) {
+  constexpr int JC = R == 1 ? 4 : 2;  // columns a wave takes at a time
   const Item it = items[blockIdx.x];
   const FrontDev F = fronts[it.f];
   const int mu = F.m - F.w;
-  const T* lf = L + F.lpos;
-  const int32_t* below = rows + F.rows_off + F.w;
+  const T* lf = L + F.lpos + F.w;
+  const double* uf = u + uoff[it.f] * nrhs;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int j1 = min(F.w, it.a + SCOLS);
-  for (int j = it.a + wave; j < j1; j += 4) {
-    const T* col = lf + (int64_t)j * F.ld + F.w;
-    for (int r = 0; r < nrhs; ++r) {
-      const double* xr = x + (int64_t)r * ldx;
-      double s = 0.0;
-      for (int i = lane; i < mu; i += 64) s += (double)col[i] * xr[below[i]];
-      for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-      if (lane == 0) z[(int64_t)r * ldx + F.col0 + j] -= s;
+  for (int jb = it.a + JC * wave; jb < j1; jb += 4 * JC) {
+    const int nc = min(JC, j1 - jb);
+    for (int q0 = 0; q0 < nrhs; q0 += R) {
+      const int nr = min(R, nrhs - q0);
+      double s[JC][R];
+#pragma unroll
+      for (int c = 0; c < JC; ++c)
+#pragma unroll
+        for (int r = 0; r < R; ++r) s[c][r] = 0.0;
+#pragma unroll(R == 1 ? 2 : 1)
+      for (int i = lane; i < mu; i += 64) {
+        double xv[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) xv[r] = r < nr ? uf[(int64_t)(q0 + r) * mu + i] : 0.0;
+#pragma unroll
+        for (int c = 0; c < JC; ++c) {
+          const double l = c < nc ? (double)lf[(int64_t)(jb + c) * F.ld + i] : 0.0;
+#pragma unroll
+          for (int r = 0; r < R; ++r) s[c][r] += l * xv[r];
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < JC; ++c)
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          double t = s[c][r];
+#pragma unroll
+          for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
+          if (lane == 0 && c < nc && r < nr) z[(int64_t)(q0 + r) * ldx + F.col0 + jb + c] -= t;
+        }
     }
   }
 }
 
-// Backward step of block [c0, c1): x_b = Linv^T z_b (stored by the item with c = 1), then the
-// columns [j0, j0 + SCOLS) before c0 lose L(block, j)^T x_b in z. Item: f, a = c0, b = j0,
-// c = store flag.
-template <typename T>
+// Backward, per front (item f): x of the front's last block = Inv_last^T z_last.
+template <typename T, int R>
+__global__ __launch_bounds__(256) void k_bwd_first(const Item* __restrict__ items, const FrontDev* __restrict__ fronts,
+                                                   const T* __restrict__ Inv, const double* __restrict__ z,
+                                                   double* __restrict__ x, int64_t ldx, int nrhs) {
+  __shared__ double Zb[SNB][R + 1];
+  const FrontDev F = fronts[items[blockIdx.x].f];
+  const int cl = (F.w - 1) / SNB * SNB, pw = F.w - cl;
+  for (int q0 = 0; q0 < nrhs; q0 += R) {
+    const int nr = min(R, nrhs - q0);
+    for (int r = 0; r < nr; ++r)
+      for (int k = threadIdx.x; k < pw; k += blockDim.x) Zb[k][r] = z[(int64_t)(q0 + r) * ldx + F.col0 + cl + k];
+    __syncthreads();
+    inv_apply_t<T, R>(Inv + F.inv0 + (int64_t)(cl / SNB) * SNB * SNB, pw, Zb, nr, x + (int64_t)q0 * ldx + F.col0 + cl,
+                      ldx);
+    __syncthreads();
+  }
+}
+
+// Backward step of block [c0, c1) (x_b already in x): the columns [j0, j0 + BCOLS) before c0
+// lose L(block, j)^T x_b in z. The tile L(c0.., j0..) is staged through LDS 64 columns at a
+// time with every load independent (coalesced along the block's rows), then four threads share
+// a column, each summing a quarter of the rows for every right-hand side, and the quarters are
+// added in a fixed order (so each right-hand side's result does not depend on how many are
+// solved together). The item with
+// c = 1 (j0 = c0 - BCOLS: the previous block's columns) then forms x of that block from the z it
+// has just finished. Item: f, a = c0, b = j0, c.
+template <typename T, int R>
 __global__ __launch_bounds__(256) void k_bwd_step(const Item* __restrict__ items, const FrontDev* __restrict__ fronts,
                                                   const T* __restrict__ L, const T* __restrict__ Inv, double* z,
                                                   double* __restrict__ x, int64_t ldx, int nrhs) {
-  __shared__ double Zb[SNB][RC + 1];
-  __shared__ double Xb[SNB][RC + 1];
+  constexpr int HC = 64;  // columns staged at a time
+  __shared__ T Lt[HC][SNB + 1];
+  __shared__ double Xb[SNB][R + 1];
+  __shared__ double Zn[SNB][R + 1];
+  __shared__ double part[4][HC][R + 1];
   const Item it = items[blockIdx.x];
   const FrontDev F = fronts[it.f];
   const int c0 = it.a, pw = min(SNB, F.w - c0), j0 = it.b;
-  const T* inv = Inv + F.inv0 + (int64_t)(c0 / SNB) * SNB * SNB;  // ld = pw
-  const T* lf = L + F.lpos;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int j1 = min(c0, j0 + SCOLS);
-  for (int q0 = 0; q0 < nrhs; q0 += RC) {
-    const int nr = min(RC, nrhs - q0);
-    for (int e = threadIdx.x; e < pw * nr; e += blockDim.x) {
-      const int k = e % pw, r = e / pw;
-      Zb[k][r] = z[(int64_t)(q0 + r) * ldx + F.col0 + c0 + k];
-    }
-    __syncthreads();
-    for (int e = threadIdx.x; e < pw * nr; e += blockDim.x) {
-      const int b = e % pw, r = e / pw;
-      double s = 0.0;
-      const T* col = inv + (int64_t)b * pw;
-      for (int k = b; k < pw; ++k) s += (double)col[k] * Zb[k][r];
-      Xb[b][r] = s;
-      if (it.c) x[(int64_t)(q0 + r) * ldx + F.col0 + c0 + b] = s;
-    }
-    __syncthreads();
-    for (int j = j0 + wave; j < j1; j += 4) {
-      const T* col = lf + (int64_t)j * F.ld + c0;
-      const bool h0 = lane < pw, h1 = lane + 64 < pw;  // Xb holds pw rows only
-      const double l0 = h0 ? (double)col[lane] : 0.0;
-      const double l1 = h1 ? (double)col[lane + 64] : 0.0;
-      for (int r = 0; r < nr; ++r) {
-        double s = (h0 ? l0 * Xb[lane][r] : 0.0) + (h1 ? l1 * Xb[lane + 64][r] : 0.0);
-        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-        if (lane == 0) z[(int64_t)(q0 + r) * ldx + F.col0 + j] -= s;
+  const T* lf = L + F.lpos + c0;
+  const int tid = threadIdx.x, jq = tid & (HC - 1), g = tid >> 6;
+  const int nj = min(BCOLS, c0 - j0);
+  for (int q0 = 0; q0 < nrhs; q0 += R) {
+    const int nr = min(R, nrhs - q0);
+    for (int r = 0; r < nr; ++r)
+      for (int k = tid; k < pw; k += blockDim.x) Xb[k][r] = x[(int64_t)(q0 + r) * ldx + F.col0 + c0 + k];
+    for (int h0 = 0; h0 < nj; h0 += HC) {
+      const int hc = min(HC, nj - h0);
+      {  // thread (row k = tid % 128, every other column)
+        const int k = tid & (SNB - 1), jb = tid >> 7;
+        if (k < pw) {
+#pragma unroll 16
+          for (int jj = jb; jj < HC; jj += 2)
+            if (jj < hc) Lt[jj][k] = lf[(int64_t)(j0 + h0 + jj) * F.ld + k];
+        }
       }
+      __syncthreads();
+      {  // partial sums of column jq over rows [32 g, 32 g + 32), every right-hand side
+        double t[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) t[r] = 0.0;
+        if (jq < hc) {
+          const int k1 = min(pw, 32 * g + 32);
+          for (int k = 32 * g; k < k1; ++k) {
+            const double l = (double)Lt[jq][k];
+#pragma unroll
+            for (int r = 0; r < R; ++r) t[r] += l * Xb[k][r];
+          }
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) part[g][jq][r] = t[r];
+      }
+      __syncthreads();
+      // the four partial sums in a fixed order, the same for any number of right-hand sides
+      for (int e = tid; e < hc * nr; e += blockDim.x) {
+        const int jj = e % hc, r = e / hc;
+        const double d = (part[0][jj][r] + part[1][jj][r]) + (part[2][jj][r] + part[3][jj][r]);
+        double* p = z + (int64_t)(q0 + r) * ldx + F.col0 + j0 + h0 + jj;
+        const double nz = *p - d;
+        *p = nz;
+        if (it.c) Zn[h0 + jj][r] = nz;
+      }
+      __syncthreads();
     }
+    if (it.c) inv_apply_t<T, R>(Inv + F.inv0 + (int64_t)(j0 / SNB) * SNB * SNB, SNB, Zn, nr,
+                                x + (int64_t)q0 * ldx + F.col0 + j0, ldx);
     __syncthreads();
   }
 }
@@ -191,50 +332,83 @@ __global__ void k_permute(const double* __restrict__ src, double* __restrict__ d
     }
 }
 
-// y = A x by rows (Csr::mul, sparse.rs:42-50): eight lanes per row, each row's partial sums
-// combined in a fixed order.
-__global__ __launch_bounds__(256) void k_spmv(int64_t n, const int64_t* __restrict__ ptr, const int32_t* __restrict__ idx,
+// ------------------------------------------------------------------------------------------
+// Vector steps of refinement, FP64, on blocks of k <= MAX_RHS columns (column c at c * ld).
+// Reductions are two-stage: each workgroup of a launch reduces one row block of one column
+// (row blocks never straddle a segment of independent equations), and the host combines the
+// partials in a fixed order, so results are the same on every run.
+
+// Y = A X by rows (Csr::mul, sparse.rs:42-50): eight lanes per row, partial sums combined in a
+// fixed order; columns in chunks of eight so a row's entries are read once per chunk.
+__global__ __launch_bounds__(256) void k_spmm(int64_t n, const int64_t* __restrict__ ptr, const int32_t* __restrict__ idx,
                                               const double* __restrict__ val, const double* __restrict__ x,
-                                              double* __restrict__ y) {
+                                              double* __restrict__ y, int k, int64_t ld) {
   const int sub = threadIdx.x & 7;
   for (int64_t r = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 3; r < n; r += ((int64_t)gridDim.x * blockDim.x) >> 3) {
-    double s = 0.0;
-    for (int64_t t = ptr[r] + sub; t < ptr[r + 1]; t += 8) s += val[t] * x[idx[t]];
-    s += __shfl_xor(s, 1);
-    s += __shfl_xor(s, 2);
-    s += __shfl_xor(s, 4);
-    if (sub == 0) y[r] = s;
+    for (int c0 = 0; c0 < k; c0 += 8) {
+      double s[8] = {};
+      for (int64_t t = ptr[r] + sub; t < ptr[r + 1]; t += 8) {
+        const double v = val[t];
+        const int64_t j = idx[t];
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+          if (c0 + q < k) s[q] += v * x[(int64_t)(c0 + q) * ld + j];
+      }
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        s[q] += __shfl_xor(s[q], 1);
+        s[q] += __shfl_xor(s[q], 2);
+        s[q] += __shfl_xor(s[q], 4);
+      }
+      if (sub == 0)
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+          if (c0 + q < k) y[(int64_t)(c0 + q) * ld + r] = s[q];
+    }
   }
 }
 
-// ------------------------------------------------------------------------------------------
-// Vector steps of refinement, FP64. Reductions are two-stage with a fixed grid (partials of
-// RED_BLOCKS workgroups, summed on the host in order), so they are the same on every run.
-constexpr int RED_BLOCKS = 512;
+struct Coefs {
+  double v[MAX_RHS];
+};
 
+// r = b - ax over n entries
 __global__ void k_residual(int64_t n, const double* __restrict__ b, const double* __restrict__ ax,
                            double* __restrict__ r) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     r[i] = b[i] - ax[i];
 }
 
-__global__ void k_axpy(int64_t n, double alpha, const double* __restrict__ x, double* __restrict__ y) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    y[i] += alpha * x[i];
+// Y[:, c] += a_c X[:, c]
+__global__ void k_axpy_cols(int64_t n, int k, int64_t ld, Coefs a, const double* __restrict__ x, double* __restrict__ y) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n * k; i += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(i / n);
+    const int64_t e = (int64_t)c * ld + i % n;
+    y[e] += a.v[c] * x[e];
+  }
 }
 
-__global__ void k_xpby(int64_t n, const double* __restrict__ x, double beta, double* __restrict__ y) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    y[i] = x[i] + beta * y[i];
+// Y[:, c] = X[:, c] + b_c Y[:, c]
+__global__ void k_xpby_cols(int64_t n, int k, int64_t ld, const double* __restrict__ x, Coefs b, double* __restrict__ y) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n * k; i += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(i / n);
+    const int64_t e = (int64_t)c * ld + i % n;
+    y[e] = x[e] + b.v[c] * y[e];
+  }
 }
 
-// partial[block] = sum a*b over the block's stride (mode 0) or max |a| (mode 1)
-__global__ __launch_bounds__(256) void k_reduce(int64_t n, const double* __restrict__ a, const double* __restrict__ b,
-                                                int mode, double* __restrict__ partial) {
+// partial[c * nblk + blk] = sum a*b (mode 0) or max |a| (mode 1) over rows [rb[blk], rb[blk + 1])
+// of column c; grid (nblk, k)
+__global__ __launch_bounds__(256) void k_reduce(const int64_t* __restrict__ rb, int nblk, const double* __restrict__ a,
+                                                const double* __restrict__ b, int64_t ld, int mode,
+                                                double* __restrict__ partial) {
   __shared__ double red[4];
+  const int blk = blockIdx.x, c = blockIdx.y;
+  const double* ac = a + (int64_t)c * ld;
+  const double* bc = b + (int64_t)c * ld;
   double s = 0.0;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    s = mode == 0 ? s + a[i] * b[i] : fmax(s, fabs(a[i]));
+  for (int64_t i = rb[blk] + threadIdx.x; i < rb[blk + 1]; i += blockDim.x)
+    s = mode == 0 ? s + ac[i] * bc[i] : fmax(s, fabs(ac[i]));
   for (int o = 32; o > 0; o >>= 1) {
     const double t = __shfl_xor(s, o);
     s = mode == 0 ? s + t : fmax(s, t);
@@ -244,7 +418,7 @@ __global__ __launch_bounds__(256) void k_reduce(int64_t n, const double* __restr
   if (threadIdx.x == 0) {
     double t = red[0];
     for (int w = 1; w < 4; ++w) t = mode == 0 ? t + red[w] : fmax(t, red[w]);
-    partial[blockIdx.x] = t;
+    partial[(int64_t)c * nblk + blk] = t;
   }
 }
 

@@ -267,6 +267,53 @@ def random_model(seed: int, dofs_lo: float = 2e3, dofs_hi: float = 6e4) -> Probl
     return p
 
 
+@dataclass
+class Batch:
+    """Independent models as one block-diagonal matrix: model s owns equations
+    [segments[s], segments[s + 1]). Its symbolic analysis is the union of the models' own
+    (the elimination forest of a block-diagonal matrix is the models' trees side by side), so
+    one factorization runs every model's fronts of a tree level in the same launches."""
+
+    n: int
+    ptr: np.ndarray
+    idx: np.ndarray
+    kval: np.ndarray
+    segments: np.ndarray
+    models: list
+    rhs: np.ndarray | None = None  # n x nrhs, column-major
+
+    @property
+    def nnz(self) -> int:
+        return int(self.ptr[-1])
+
+    upper = Problem.upper
+    mul = Problem.mul
+
+
+def batch(models: list, nrhs: int = 64, seed: int = 20243) -> Batch:
+    """Block-diagonal concatenation of `models` with nrhs seeded N(0, 1) right-hand sides per
+    model (one generator per model, seeded by seed and the model's index)."""
+    ns = np.array([m.n for m in models], np.int64)
+    seg = np.zeros(len(models) + 1, np.int64)
+    np.cumsum(ns, out=seg[1:])
+    nz = np.array([m.nnz for m in models], np.int64)
+    zoff = np.zeros(len(models) + 1, np.int64)
+    np.cumsum(nz, out=zoff[1:])
+    n = int(seg[-1])
+    ptr = np.zeros(n + 1, np.int64)
+    idx = np.zeros(int(zoff[-1]), np.int64)
+    kval = np.zeros(int(zoff[-1]))
+    rhs = np.zeros((n, nrhs), order="F")
+    for s, m in enumerate(models):
+        ptr[seg[s] + 1:seg[s + 1] + 1] = m.ptr[1:] + zoff[s]
+        idx[zoff[s]:zoff[s + 1]] = m.idx + seg[s]
+        kval[zoff[s]:zoff[s + 1]] = m.kval
+        rng = np.random.default_rng([seed, s])
+        rhs[seg[s]:seg[s + 1]] = rng.standard_normal((m.n, nrhs))
+    meta = [dict(getattr(m, "meta", {}), n=m.n) for m in models]
+    return Batch(n, ptr, idx, kval, seg, meta, rhs)
+
+
 def block_tet10(cells=(50, 50, 50), size=(0.5, 0.5, 0.5), rho: float = 7850.0) -> Problem:
     """Tet10 Kuhn block of steel with its consistent mass, z = 0 clamped."""
     p = block("tet10", cells, size, E_STEEL, 0.3, clamp_face=4, rho=rho)

@@ -80,7 +80,8 @@ enum {
   FC_KIND_TRSM = 4,     /* rows below a panel against its diagonal block                     */
   FC_KIND_SYRK = 5,     /* update matrices U = -L21 L21^T                                    */
   FC_KIND_REDUCE = 6,   /* sums of the pieces of a GEMM split along k                        */
-  FC_KINDS = 7
+  FC_KIND_SPLIT = 7,    /* FC_MIXED: a finished panel's scaled BF16 (hi, lo) copies          */
+  FC_KINDS = 8
 };
 
 typedef struct {
@@ -178,6 +179,13 @@ int fc_matrix_new(uint64_t n, const uint64_t* ptr, const uint64_t* idx, const do
                   fc_matrix** out, fc_error* err);
 void fc_matrix_free(fc_matrix* a);
 
+/* Declares rows [offsets[s], offsets[s + 1]) (s < count, offsets[0] = 0, offsets[count] = n,
+ * nondecreasing) as independent segments: the diagonal blocks of a batch of models factored as
+ * one block-diagonal matrix. Refinement then measures each segment's backward error with its
+ * own norms and stops when every segment of every column meets the target. No counterpart in
+ * the reference, which factors one model at a time; default: one segment. */
+int fc_matrix_set_segments(fc_matrix* a, uint64_t count, const uint64_t* offsets, fc_error* err);
+
 typedef struct {
   int32_t method;        /* 0: refinement x += solve(b - A x); 1: conjugate gradients
                             preconditioned by the factor; 2: 0, switching to 1 when a step
@@ -199,6 +207,15 @@ typedef struct {
  * of n doubles (x may alias b). */
 int fc_solve_refined(const fc_factor* f, const fc_matrix* a, const double* b, double* x, uint64_t n,
                      const fc_refine_opts* opts, fc_refine_report* report, fc_error* err);
+
+/* fc_solve_refined for nrhs right-hand sides in device memory: B and X are n x nrhs,
+ * column-major (leading dimension n), distinct. Columns are refined together, MAX 64 at a
+ * time, each with its own scalars; berr (nrhs doubles on the host, or NULL) receives each
+ * column's final backward error (the largest over the segments); the report holds the worst.
+ * Solve many load cases at once (Factor::solve_many, cholesky.rs:1568-1570, refined). */
+int fc_solve_refined_many_device(const fc_factor* f, const fc_matrix* a, const double* B, double* X, uint64_t n,
+                                 uint64_t nrhs, const fc_refine_opts* opts, fc_refine_report* report, double* berr,
+                                 fc_error* err);
 
 /* fc_solve_refined with b and x in device memory (n doubles each, distinct). */
 int fc_solve_refined_device(const fc_factor* f, const fc_matrix* a, const double* b, double* x, uint64_t n,

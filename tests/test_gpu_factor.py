@@ -132,3 +132,110 @@ def test_refined_solve_reaches_the_target(gpu):
         r = p.mul(x) - p.rhs
         berr = np.abs(r).max() / (p.norm_inf() * np.abs(x).max() + np.abs(p.rhs).max())
         assert berr <= 1e-12, (prec, berr, rep)
+
+
+@pytest.mark.parametrize("terms,bound", [(3, 1e-4), (1, 3e-3)])
+def test_mixed_factor_is_close_to_the_oracle(gpu, terms, bound):
+    """The mixed form on a 16^3 cube: BF16 Schur products (three products: about 16 significant
+    bits; one: 8) keep the factor values within `bound` of the oracle's FP64 ones (measured
+    1.4e-6 and 2.8e-4 on this cube). A wrong lane layout or scale would be off by O(1)."""
+    p = P.cube_hex8(16)
+    up = p.upper()
+    ref = O.Factor(p.n, *up)
+    sym = CH.Symbolic(CH.UpperColumns(p.n, *up))
+    f = CH.Factor(sym, up[2], precision="mixed", bf16_terms=terms)
+    assert rel(f.values(), ref.values()) <= bound
+
+
+def test_mixed_rejects_other_term_counts(gpu):
+    p = P.cantilever_hex8(4, 2, 2)
+    up = p.upper()
+    sym = CH.Symbolic(CH.UpperColumns(p.n, *up))
+    with pytest.raises(CH.InvalidArgument):
+        CH.Factor(sym, up[2], precision="mixed", bf16_terms=2)
+
+
+def berr_of(p, x, b):
+    r = p.mul(x) - b
+    return np.abs(r).max() / (p.norm_inf() * np.abs(x).max() + np.abs(b).max())
+
+
+def test_mixed_refined_solve_reaches_the_target(gpu):
+    """Mixed factor + FP64 refinement reaches berr <= 1e-12 on the slender cantilever (its
+    condition number makes the single-product form lose definiteness there)."""
+    p = P.cantilever_hex8()
+    up = p.upper()
+    sym = CH.Symbolic(CH.UpperColumns(p.n, *up))
+    A = CH.Matrix(p.n, p.ptr, p.idx, p.kval)
+    f = CH.Factor(sym, up[2], precision="mixed", bf16_terms=3)
+    x, rep = f.solve_refined(A, p.rhs, target_berr=1e-12)
+    assert rep["converged"], rep
+    assert berr_of(p, x, p.rhs) <= 1e-12
+    xr = O.Factor(p.n, *up).solve(p.rhs)
+    assert rel(x, xr) <= 1e-8
+
+
+def test_refined_solve_of_many_columns(gpu):
+    """Five load cases refined together: each column reaches the target on its own."""
+    import torch
+    p = P.cube_hex8(12)
+    up = p.upper()
+    sym = CH.Symbolic(CH.UpperColumns(p.n, *up))
+    A = CH.Matrix(p.n, p.ptr, p.idx, p.kval)
+    f = CH.Factor(sym, up[2], precision="fp32")
+    rng = np.random.default_rng(7)
+    B = np.asfortranarray(rng.standard_normal((p.n, 5)))
+    dB = torch.from_numpy(B.T.copy()).cuda()  # row r of dB = column r of B
+    dX = torch.zeros_like(dB)
+    rep, berr = f.solve_refined_many_device(A, dB.data_ptr(), dX.data_ptr(), 5, target_berr=1e-12)
+    assert rep["converged"] and (berr <= 1e-12).all(), (rep, berr)
+    X = dX.cpu().numpy().T
+    for c in range(5):
+        assert berr_of(p, X[:, c], B[:, c]) <= 1e-12
+
+
+def test_batch_of_models_matches_each_model(gpu):
+    """Independent models factored as one block-diagonal matrix: in FP64 each model's block of
+    the solution matches the oracle's solve of that model alone (1e-12); in FP32 with
+    refinement of 4 load cases per model, every model reaches berr <= 1e-12 measured with its
+    own norms."""
+    import torch
+    models = [P.random_model(s, 1.5e3, 6e3) for s in range(6)]
+    bt = P.batch(models, nrhs=4)
+    up = bt.upper()
+    sym = CH.Symbolic(CH.UpperColumns(bt.n, *up))
+    f64 = CH.Factor(sym, up[2], precision="fp64")
+    x = f64.solve(bt.rhs[:, 0])
+    for s, m in enumerate(models):
+        lo, hi = bt.segments[s], bt.segments[s + 1]
+        xr = O.Factor(m.n, *m.upper()).solve(bt.rhs[lo:hi, 0])
+        assert rel(x[lo:hi], xr) <= REL
+    A = CH.Matrix(bt.n, bt.ptr, bt.idx, bt.kval)
+    A.set_segments(bt.segments)
+    f32 = CH.Factor(sym, up[2], precision="fp32")
+    dB = torch.from_numpy(np.ascontiguousarray(bt.rhs.T)).cuda()
+    dX = torch.zeros_like(dB)
+    rep, berr = f32.solve_refined_many_device(A, dB.data_ptr(), dX.data_ptr(), 4, target_berr=1e-12)
+    assert rep["converged"] and (berr <= 1e-12).all(), (rep, berr)
+    X = dX.cpu().numpy().T
+    for s, m in enumerate(models):
+        lo, hi = bt.segments[s], bt.segments[s + 1]
+        for c in range(4):
+            assert berr_of(m, X[lo:hi, c], bt.rhs[lo:hi, c]) <= 1e-12
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_graph_replays_match_eager_launches(gpu, precision):
+    """A factorization captured into a hipGraph and replayed for new values gives the same bits
+    as the eager launches, on every replay, with eager solves in between."""
+    p = P.cube_hex8(10)
+    up = p.upper()
+    sym = CH.Symbolic(CH.UpperColumns(p.n, *up))
+    vals = [up[2], up[2] * 2.0, up[2] * 0.5 + 0.0, up[2]]
+    g = CH.Factor(sym, vals[0], precision=precision, use_graph=True)
+    for q, v in enumerate(vals):
+        if q:
+            g.refactor(v)
+        e = CH.Factor(sym, v, precision=precision)
+        assert np.array_equal(g.values().view(np.uint64), e.values().view(np.uint64)), q
+        assert np.array_equal(g.solve(p.rhs).view(np.uint64), e.solve(p.rhs).view(np.uint64)), q

@@ -193,3 +193,33 @@ def test_amalgamated_fronts_cover_the_supernodes():
     assert info["front_flops"] >= info["flops"]
     assert info["stored"] >= info["entries"]
     assert abs(sum(info["kind_flops"].values()) - info["front_flops"]) <= 1e-6 * info["front_flops"]
+
+
+def test_batch_symbolic_is_the_union_of_the_models():
+    """Batching keeps the symbolic phase bit-exact: the analysis of the block-diagonal batch is
+    each model's own analysis (ordering, supernodes, row structures, update maps) side by side,
+    with column, supernode, row and update offsets."""
+    models = [P.random_model(s, 1.5e3, 5e3) for s in (3, 4, 5)]
+    b = P.batch(models, nrhs=1)
+    whole = CH.Symbolic(CH.UpperColumns(b.n, *b.upper())).export()
+    parts = [CH.Symbolic(CH.UpperColumns(m.n, *m.upper())).export() for m in models]
+    col = sn = nrow = nupd = 0
+    want = {k: [] for k in ("perm", "first", "rptr", "rows", "uptr", "updates")}
+    for e in parts:
+        ns = len(e["first"]) - 1
+        want["perm"].append(e["perm"] + col)
+        want["first"].append(e["first"][:-1] + col)
+        want["rptr"].append(e["rptr"][:-1] + nrow)
+        want["rows"].append(e["rows"] + col)
+        want["uptr"].append(e["uptr"][:-1] + nupd)
+        want["updates"].append(e["updates"] + np.array([sn, 0, 0]))
+        col += len(e["perm"])
+        sn += ns
+        nrow += len(e["rows"])
+        nupd += len(e["updates"])
+    want["first"].append(np.array([col]))
+    want["rptr"].append(np.array([nrow]))
+    want["uptr"].append(np.array([nupd]))
+    for k, v in want.items():
+        np.testing.assert_array_equal(whole[k], np.concatenate(v), err_msg=k)
+    assert whole["entries"] == sum(e["entries"] for e in parts)
