@@ -47,10 +47,13 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_leg(CH, sym, dvals, A, b_ptr, x_ptr, n, precision, steps, warmup, profile, bf16_terms=1):
+def run_leg(CH, sym, dvals, A, b_ptr, x_ptr, n, precision, steps, warmup, profile, bf16_terms=0):
     import torch
 
-    f = CH.Factor(sym, precision=precision, device_values=dvals, profile=profile, bf16_terms=bf16_terms)
+    # the solves replay from hipGraphs (use_graph bit 1); the factorization's launches stay eager
+    # so that every launch of the profiled kinds is timed with events
+    f = CH.Factor(sym, precision=precision, device_values=dvals, profile=profile, bf16_terms=bf16_terms,
+                  use_graph=2)
     rep = f.solve_refined_device(A, b_ptr, x_ptr)
     for _ in range(max(warmup - 1, 0)):
         f.refactor_device(dvals)
@@ -154,7 +157,7 @@ def main():
     x = torch.zeros_like(b)
     A = CH.Matrix(p.n, p.ptr, p.idx, p.kval)
     torch.cuda.synchronize()
-    profile = ["syrk", "update", "extend_l", "extend_u", "trsm", "potrf", "reduce"]
+    profile = ["syrk", "update", "extend_l", "extend_u", "trsm", "potrf", "reduce", "split"]
     legs = {}
     if not args.no_fp64:
         log("bench: fp64 leg")
@@ -187,6 +190,8 @@ def main():
             name, bound, peak = f"k_extend_add<{el}>", "hbm", HBM_PEAK
         elif kind == "potrf":
             name, bound, peak = f"k_potrf_blk<{el}> + k_potrf_inv<{el},32>", "latency", None
+        elif kind == "split":
+            name, bound, peak = "k_split_panel", "hbm", HBM_PEAK
         else:
             name, bound, peak = f"k_reduce_parts<{el}>", "hbm", HBM_PEAK
         g = funcs.setdefault(name, dict(kinds=[], seconds=0.0, launches=0.0, flops=0.0, entries=0.0, bound=bound,
@@ -199,7 +204,10 @@ def main():
     avg = g["seconds"] / max(g["launches"], 1)
     if g["bound"] == "hbm":
         es = 8 if form == "fp64" else 4
-        per_launch = g["entries"] * (3 * es + 4) / max(g["launches"], 1)  # child entry read, target RMW, row map
+        if dom == "k_split_panel":
+            per_launch = g["entries"] * 8 / max(g["launches"], 1)  # FP32 read, two BF16 written
+        else:
+            per_launch = g["entries"] * (3 * es + 4) / max(g["launches"], 1)  # child entry read, target RMW, row map
         achieved = per_launch / avg / 1e9
         roof = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK, unit="GB/s", frac=achieved / HBM_PEAK,
                     traffic=None, kernel=dom, kinds=g["kinds"], launches=g["launches"], bytes_per_launch=per_launch,

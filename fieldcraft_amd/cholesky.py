@@ -281,11 +281,13 @@ class Factor:
     """A Cholesky factor on the GPU (Factor, cholesky.rs:1532-1570)."""
 
     def __init__(self, symbolic: Symbolic, val=None, precision: str | int = "fp64", device: int = 0,
-                 bf16_terms: int = 1, cancel: C.c_int32 | None = None, pivot_fallback: float = 0.0,
-                 use_graph: bool = False, profile: tuple = (), device_values: int | None = None,
+                 bf16_terms: int = 0, cancel: C.c_int32 | None = None, pivot_fallback: float = 0.0,
+                 use_graph: int = 0, profile: tuple = (), device_values: int | None = None,
                  stream: int | None = None):
         """Factor the matrix with values `val` (host array) or `device_values` (a device
-        pointer to the upper columns' values, nnz doubles)."""
+        pointer to the upper columns' values, nnz doubles). bf16_terms (mixed form): 1 or 3
+        BF16 products per operand pair, 0 for the library's default (3). use_graph: bit 0
+        replays the factorization from a hipGraph, bit 1 the solves."""
         L = lib()
         self.symbolic = symbolic
         self.n = symbolic.n

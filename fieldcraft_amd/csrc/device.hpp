@@ -51,10 +51,6 @@ struct Launch {
 
 constexpr int KINDS = 8;
 
-constexpr int PANEL = 64;      // columns of a panel (the reference's PANEL, cholesky.rs:55)
-constexpr int TILE_M = 64;     // rows of a GEMM tile
-constexpr int TILE_N = 64;     // columns of a GEMM tile
 constexpr int EA_COLS = 8;     // columns of the parent one extend-add workgroup owns
-constexpr int TRSM_ROWS = 64;  // rows of a TRSM tile
 
 }  // namespace fc

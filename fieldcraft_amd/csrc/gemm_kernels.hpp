@@ -191,6 +191,7 @@ __global__ __launch_bounds__(256) void k_split_panel(const Item* __restrict__ it
   if (i >= F.m) return;
   const float sc = ldexpf(1.0f, -fexp[it.f]);
   const float* row = L + F.lpos + i;
+#pragma unroll 4
   for (int j0 = c0; j0 < cend; j0 += 8) {
     float x[8];
 #pragma unroll
@@ -329,6 +330,125 @@ __global__ __launch_bounds__(256, 2) void k_gemm_b16(const GemmItem* __restrict_
       }
 }
 
+// The same products with two waves per 128 x 128 tile, each wave a 128 x 64 half as 8 x 4
+// blocks of the 16x16x32 MFMA: every fragment read from LDS feeds twice as many products, and
+// two workgroups share a CU.
+template <int TERMS>
+__global__ __launch_bounds__(128, 2) void k_gemm_b16w(const GemmItem* __restrict__ items, const FrontDev* __restrict__ fronts,
+                                                      const __bf16* __restrict__ Lh, const __bf16* __restrict__ Ll,
+                                                      float* L, float* U, float* Part, const int32_t* __restrict__ fexp) {
+  static_assert(TERMS == 1 || TERMS == 3, "one or three BF16 products");
+  constexpr int NS = TERMS == 3 ? 2 : 1;
+  __shared__ __attribute__((aligned(16))) __bf16 As[2][NS][GB][BROW];
+  __shared__ __attribute__((aligned(16))) __bf16 Bs[2][NS][GB][BROW];
+  const GemmItem it = items[blockIdx.x];
+  const FrontDev F = fronts[it.front];
+  const int e = fexp[it.front];
+  const int64_t oa = it.a - F.lpos, ob = it.b - F.lpos;
+  const int64_t chunk = (int64_t)F.ld * BK;
+  const int64_t a0 = F.bpos + (oa / F.ld) / BK * chunk + (oa % F.ld) * BK;
+  const int64_t b0 = F.bpos + (ob / F.ld) / BK * chunk + (ob % F.ld) * BK;
+  auto base = [&](int8_t src) { return src == S_L ? L : (src == S_U ? U : Part); };
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wj = wave * 64;  // the wave's 64 columns; all 128 rows
+  // loader: 16-byte vectors v = tid + 128 q (q < 4) of a stage (row v / 4, k 8 (v % 4) + [0, 8))
+  u32x4 ra[4][NS], rb[4][NS];
+  auto load = [&](int s) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int v = tid + 128 * q, row = v >> 2, part = v & 3;
+      const int64_t oa2 = a0 + s * chunk + row * BK + part * 8, ob2 = b0 + s * chunk + row * BK + part * 8;
+      const u32x4 z = {0, 0, 0, 0};
+      ra[q][0] = row < it.m ? *(const u32x4*)(Lh + oa2) : z;
+      rb[q][0] = row < it.n ? *(const u32x4*)(Lh + ob2) : z;
+      if (TERMS == 3) {
+        ra[q][NS - 1] = row < it.m ? *(const u32x4*)(Ll + oa2) : z;
+        rb[q][NS - 1] = row < it.n ? *(const u32x4*)(Ll + ob2) : z;
+      }
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int v = tid + 128 * q, row = v >> 2, part = v & 3;
+#pragma unroll
+      for (int t = 0; t < NS; ++t) {
+        *(u32x4*)&As[buf][t][row][part * 8] = ra[q][t];
+        *(u32x4*)&Bs[buf][t][row][part * 8] = rb[q][t];
+      }
+    }
+  };
+  v4f acc[8][4];
+#pragma unroll
+  for (int bi = 0; bi < 8; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 4; ++bj) acc[bi][bj] = v4f{0, 0, 0, 0};
+  const int stages = (it.k + BK - 1) / BK;
+  const int fr = lane & 15, fk = 8 * (lane >> 4);
+  load(0);
+  store(0);
+  __syncthreads();
+  for (int s = 0; s < stages; ++s) {
+    const int cur = s & 1;
+    if (s + 1 < stages) load(s + 1);
+    v8bf fj[4], gj[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      fj[b] = *(const v8bf*)&Bs[cur][0][wj + 16 * b + fr][fk];
+      if (TERMS == 3) gj[b] = *(const v8bf*)&Bs[cur][NS - 1][wj + 16 * b + fr][fk];
+    }
+#pragma unroll
+    for (int bi = 0; bi < 8; ++bi) {
+      const v8bf fi = *(const v8bf*)&As[cur][0][16 * bi + fr][fk];
+      if (TERMS == 3) {
+        const v8bf gi = *(const v8bf*)&As[cur][NS - 1][16 * bi + fr][fk];
+#pragma unroll
+        for (int bj = 0; bj < 4; ++bj) {
+          acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(gj[bj], fi, acc[bi][bj], 0, 0, 0);
+          acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fj[bj], gi, acc[bi][bj], 0, 0, 0);
+        }
+      }
+#pragma unroll
+      for (int bj = 0; bj < 4; ++bj)
+        acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fj[bj], fi, acc[bi][bj], 0, 0, 0);
+    }
+    if (s + 1 < stages) store(cur ^ 1);
+    __syncthreads();
+  }
+  // acc[bi][bj][v] = C(16 bi + lane % 16, wj + 16 bj + 4 (lane / 16) + v) / sc^2
+  const float unscale = ldexpf(1.0f, 2 * e);
+  const int il = lane & 15;
+  if (it.mode == G_PART) {
+    float* P = Part + it.c;
+#pragma unroll
+    for (int bi = 0; bi < 8; ++bi)
+#pragma unroll
+      for (int bj = 0; bj < 4; ++bj)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int i = 16 * bi + il, j = wj + 16 * bj + Mma<float>::row(lane, v);
+          P[(int64_t)j * GB + i] = acc[bi][bj][v] * unscale;
+        }
+    return;
+  }
+  float* C = base(it.csrc) + it.c;
+#pragma unroll
+  for (int bi = 0; bi < 8; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 4; ++bj)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = 16 * bi + il, j = wj + 16 * bj + Mma<float>::row(lane, v);
+        if (i < it.m && j < it.n && i - j + it.diag >= 0) {
+          float* p = C + (int64_t)j * it.ldc + i;
+          const float x = acc[bi][bj][v] * unscale;
+          if (it.mode == G_SUB) *p -= x;
+          else if (it.mode == G_NEG) *p = -x;
+          else *p = x;
+        }
+      }
+}
+
 // Sum of the partial tiles of a split product, in order, applied to C as its mode says. Each
 // tile is shared by RSPLIT workgroups (GB * GB / RSPLIT entries each).
 constexpr int RSPLIT = 16;
@@ -445,7 +565,10 @@ __global__ __launch_bounds__(K == NB ? 256 : 64 * (K / 32), 1) void k_potrf_inv(
 // shuffles (the trailing update is then local), and takes the same step of the inverse,
 // right-looking on L X = I: row j of X is divided by L(j, j), then X(i, :) -= L(i, j) X(j, :)
 // for i > j. Lane 0 applies the pivot test and records the first failing column.
-// Steps 8 CB .. 8 CB + 7 (a template, so that every register index is a constant)
+// Steps 8 CB .. 8 CB + 7 (a template, so that every register index is a constant). Only the
+// 8 x 8 register blocks the step can change are touched: (r, c) with r >= c >= CB for L and
+// r >= CB, c <= CB for X; within them the remaining conditions are per-lane selects, not
+// branches.
 template <int CB>
 __device__ __forceinline__ void wpi_steps(double (&a)[8][8], double (&x)[8][8], int h, int lane, double dl,
                                           int64_t col, unsigned long long* __restrict__ fail, bool& bad) {
@@ -456,24 +579,23 @@ __device__ __forceinline__ void wpi_steps(double (&a)[8][8], double (&x)[8][8], 
     const double p = __shfl(a[CB][CB], cc + 8 * cc);
     const double a0 = __shfl(dl, j);
     const double d = sqrt(p);
-    if (lane == 0 && !bad) {
-      if (!(p > PIVOT_TOL * a0) || !isfinite(p)) {
-        atomicMin(fail, (unsigned long long)(col + j));
-        bad = true;
-      }
+    if (lane == 0 && !bad && (!(p > PIVOT_TOL * a0) || !isfinite(p))) {
+      atomicMin(fail, (unsigned long long)(col + j));
+      bad = true;
     }
     const double rd = 1.0 / d;
-    if (tk == cc) {  // the owners of column j scale it
+    {  // column j of L: rows below scaled, the pivot replaced by d (lanes tk == cc own it)
+      const bool own = tk == cc;
 #pragma unroll
       for (int r = CB; r < 8; ++r) {
-        const int i = ti + 8 * r;
-        if (i > j) a[r][CB] /= d;
-        else if (i == j) a[r][CB] = d;
+        const bool below = r > CB || ti > cc, diag = r == CB && ti == cc;
+        a[r][CB] = own ? (below ? a[r][CB] * rd : (diag ? d : a[r][CB])) : a[r][CB];
       }
     }
-    if (ti == cc) {  // the owners of row j of X divide it by L(j, j)
+    {  // row j of X divided by L(j, j) (lanes ti == cc own it)
+      const bool own = ti == cc;
 #pragma unroll
-      for (int c = 0; c <= CB; ++c) x[CB][c] *= rd;
+      for (int c = 0; c <= CB; ++c) x[CB][c] = own ? x[CB][c] * rd : x[CB][c];
     }
     double li[8], lk[8], xj[8];
 #pragma unroll
@@ -482,17 +604,20 @@ __device__ __forceinline__ void wpi_steps(double (&a)[8][8], double (&x)[8][8], 
     for (int c = CB; c < 8; ++c) lk[c] = __shfl(a[c][CB], tk + 8 * cc);  // L(tk + 8c, j)
 #pragma unroll
     for (int c = 0; c <= CB; ++c) xj[c] = __shfl(x[CB][c], cc + 8 * tk);  // X(j, tk + 8c)
+    const bool kj = tk > cc, ik = ti >= tk, ij = ti > cc;
 #pragma unroll
     for (int r = CB; r < 8; ++r) {
-      const int i = ti + 8 * r;
 #pragma unroll
-      for (int c = CB; c < 8; ++c) {
-        const int k = tk + 8 * c;
-        if (k > j && i >= k) a[r][c] -= li[r] * lk[c];
+      for (int c = CB; c <= r; ++c) {  // L(i, k) -= L(i, j) L(k, j) for i >= k > j
+        const bool upd = (c > CB || kj) && (r > c || ik);
+        const double t = fma(-li[r], lk[c], a[r][c]);
+        a[r][c] = upd ? t : a[r][c];
       }
-      if (i > j) {
+      const bool below = r > CB || ij;  // X(i, :) -= L(i, j) X(j, :) for i > j
 #pragma unroll
-        for (int c = 0; c <= CB; ++c) x[r][c] -= li[r] * xj[c];
+      for (int c = 0; c <= CB; ++c) {
+        const double t = fma(-li[r], xj[c], x[r][c]);
+        x[r][c] = below ? t : x[r][c];
       }
     }
   }
@@ -510,7 +635,8 @@ __device__ inline void wave_potrf_inv(double (*S)[NB + 1], double* xd, int o, in
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
       const int i = ti + 8 * r, k = tk + 8 * c;
-      a[r][c] = (i < h && k < h && i >= k) ? S[o + i][o + k] : (i == k ? 1.0 : 0.0);
+      const double v = S[o + i][o + k];  // in bounds for every lane (o + 64 <= NB)
+      a[r][c] = (i < h && k < h && i >= k) ? v : (i == k ? 1.0 : 0.0);
       x[r][c] = i == k ? 1.0 : 0.0;
     }
   wpi_steps<0>(a, x, h, lane, dl, col, fail, bad);
@@ -521,19 +647,17 @@ __device__ inline void wave_potrf_inv(double (*S)[NB + 1], double* xd, int o, in
   wpi_steps<5>(a, x, h, lane, dl, col, fail, bad);
   wpi_steps<6>(a, x, h, lane, dl, col, fail, bad);
   wpi_steps<7>(a, x, h, lane, dl, col, fail, bad);
+  // L below and on the diagonal, X above it (transposed), X's diagonal in xd; every lane
+  // stores to in-bounds addresses, the entries outside the h x h block are not written
 #pragma unroll
   for (int r = 0; r < 8; ++r)
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
       const int i = ti + 8 * r, k = tk + 8 * c;
       if (i < h && k < h) {
-        if (i > k) {
-          S[o + i][o + k] = a[r][c];
-          S[o + k][o + i] = x[r][c];
-        } else if (i == k) {
-          S[o + i][o + i] = a[r][c];
-          xd[o + i] = x[r][c];
-        }
+        if (i >= k) S[o + i][o + k] = a[r][c];
+        if (i > k) S[o + k][o + i] = x[r][c];
+        if (i == k) xd[o + i] = x[r][c];
       }
     }
 }
@@ -553,29 +677,33 @@ __global__ __launch_bounds__(256, 1) void k_potrf_blk(const Item* __restrict__ i
   const double* dg = diag + F.col0 + c0;
   const int64_t col = (int64_t)F.col0 + c0;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  {  // the lower triangle, with independent loads in flight: thread (row, every other column)
-    const int i = tid & (NB - 1), j0 = tid >> 7;
-    if (i < pw) {
-#pragma unroll 16
-      for (int j = j0; j < NB; j += 2)
-        if (j <= i) S[i][j] = (double)base[(int64_t)j * F.ld + i];
+  const int rr = tid & (NB - 1), jp = tid >> 7;  // a thread's row, and which of every two columns
+  {  // the pw x pw square (its upper triangle is zeros in L, in bounds): 64 independent loads
+    T v[NB / 2];
+#pragma unroll
+    for (int q = 0; q < NB / 2; ++q) {
+      const int j = jp + 2 * q;
+      v[q] = (rr < pw && j < pw) ? base[(int64_t)j * F.ld + rr] : T(0);
     }
+#pragma unroll
+    for (int q = 0; q < NB / 2; ++q) S[rr][jp + 2 * q] = (double)v[q];
   }
   __syncthreads();
   bool bad = false;
   if (wave == 0) wave_potrf_inv(S, xd, 0, h0, lane, dg, col, fail, bad);  // L11, X11
   __syncthreads();
-  // X11(k, t): S[t][k] below the diagonal (k > t), xd[t] on it, 0 above
-  auto x11 = [&](int k, int t) { return k > t ? S[t][k] : (k == t ? xd[t] : 0.0); };
   if (h1 > 0) {
     const int i = lane, t0 = 16 * wave;  // row i of the lower half, columns t0 + [0, 16)
-    {  // L21 = A21 X11^T: L21(i, t) = sum_{k <= t} A21(i, k) X11(t, k)
+    {  // L21 = A21 X11^T: L21(i, t) = sum_{k <= t} A21(i, k) X11(t, k), X11(t, k) = S[k][t] (t > k)
       double acc[16] = {};
-      if (i < h1) {
-        for (int k = 0; k < h0; ++k) {
-          const double av = S[h0 + i][k];
+      for (int k = 0; k < h0; ++k) {
+        const double av = S[h0 + min(i, h1 - 1)][k];
+        const double xk = xd[k];
 #pragma unroll
-          for (int q = 0; q < 16; ++q) acc[q] += av * x11(t0 + q, k);
+        for (int q = 0; q < 16; ++q) {
+          const int t = t0 + q;
+          const double sv = S[k][t];
+          acc[q] = fma(av, t > k ? sv : (t == k ? xk : 0.0), acc[q]);
         }
       }
       __syncthreads();  // every A21 read before L21 is stored over it
@@ -586,20 +714,20 @@ __global__ __launch_bounds__(256, 1) void k_potrf_blk(const Item* __restrict__ i
     __syncthreads();
     {  // A22 -= L21 L21^T, a 4 x 4 block of the lower triangle per thread
       const int bi = tid & 15, bj = tid >> 4;
-      if (bi >= bj && 4 * bj < h1) {
-        double acc[4][4] = {};
-        for (int k = 0; k < h0; ++k) {
-          double u[4], v[4];
+      double acc[4][4] = {};
+      for (int k = 0; k < h0; ++k) {
+        double u[4], v[4];
 #pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            u[q] = 4 * bi + q < h1 ? S[h0 + 4 * bi + q][k] : 0.0;
-            v[q] = 4 * bj + q < h1 ? S[h0 + 4 * bj + q][k] : 0.0;
-          }
-#pragma unroll
-          for (int q = 0; q < 4; ++q)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) acc[q][r] += u[q] * v[r];
+        for (int q = 0; q < 4; ++q) {
+          u[q] = S[h0 + min(4 * bi + q, h1 - 1)][k];
+          v[q] = S[h0 + min(4 * bj + q, h1 - 1)][k];
         }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) acc[q][r] = fma(u[q], v[r], acc[q][r]);
+      }
+      if (bi >= bj)
 #pragma unroll
         for (int q = 0; q < 4; ++q)
 #pragma unroll
@@ -607,20 +735,31 @@ __global__ __launch_bounds__(256, 1) void k_potrf_blk(const Item* __restrict__ i
             const int ii = 4 * bi + q, jj = 4 * bj + r;
             if (ii < h1 && ii >= jj) S[h0 + ii][h0 + jj] -= acc[q][r];
           }
-      }
     }
     __syncthreads();
     if (wave == 0) wave_potrf_inv(S, xd, h0, h1, lane, dg + h0, col + h0, fail, bad);  // L22, X22
     __syncthreads();
-    // L is final: write it out; T = L21 X11 (row i, columns t0 + [0, 16)) in registers
-    for (int j = 0; j < pw; ++j)
-      for (int r = j + tid; r < pw; r += blockDim.x) base[(int64_t)j * F.ld + r] = (T)S[r][j];
-    double tr[16] = {};
-    if (i < h1) {
-      for (int k = 0; k < h0; ++k) {
-        const double l = S[h0 + i][k];
+    // L is final: write it out (reads first, then stores); T = L21 X11 (row i, columns
+    // t0 + [0, 16)) in registers
+    {
+      T v[NB / 2];
 #pragma unroll
-        for (int q = 0; q < 16; ++q) tr[q] += l * x11(k, t0 + q);
+      for (int q = 0; q < NB / 2; ++q) v[q] = (T)S[rr][jp + 2 * q];
+#pragma unroll
+      for (int q = 0; q < NB / 2; ++q) {
+        const int j = jp + 2 * q;
+        if (rr < pw && j <= rr) base[(int64_t)j * F.ld + rr] = v[q];
+      }
+    }
+    double tr[16] = {};
+    for (int k = 0; k < h0; ++k) {
+      const double l = S[h0 + min(i, h1 - 1)][k];
+      const double xk = xd[k];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {  // X11(k, t) = S[t][k] for k > t
+        const int t = t0 + q;
+        const double sv = S[t][k];
+        tr[q] = fma(l, k > t ? sv : (k == t ? xk : 0.0), tr[q]);
       }
     }
     __syncthreads();  // L21 written out and read
@@ -628,16 +767,17 @@ __global__ __launch_bounds__(256, 1) void k_potrf_blk(const Item* __restrict__ i
 #pragma unroll
       for (int q = 0; q < 16; ++q) S[h0 + i][t0 + q] = tr[q];  // T over L21
     __syncthreads();
-    {  // X21 = -X22 T: column t = lane, rows 16 wave + [0, 16), stored at S[t][h0 + i]
+    {  // X21 = -X22 T: column t = lane, rows i0 + [0, 16), stored at S[t][h0 + i]
       const int t = lane, i0 = 16 * wave;
       double acc[16] = {};
       for (int k = 0; k < h1; ++k) {
         const double tk = S[h0 + k][t];
+        const double xk = xd[h0 + k];
 #pragma unroll
-        for (int q = 0; q < 16; ++q) {
-          const int ii = i0 + q;  // X22(ii, k): S[h0 + k][h0 + ii] below the diagonal, xd on it
-          const double xv = ii > k ? S[h0 + k][h0 + ii] : (ii == k ? xd[h0 + ii] : 0.0);
-          acc[q] += xv * tk;
+        for (int q = 0; q < 16; ++q) {  // X22(ii, k) = S[h0 + k][h0 + ii] for ii > k
+          const int ii = i0 + q;
+          const double sv = S[h0 + k][h0 + ii];
+          acc[q] = fma(ii > k ? sv : (ii == k ? xk : 0.0), tk, acc[q]);
         }
       }
 #pragma unroll
@@ -645,14 +785,32 @@ __global__ __launch_bounds__(256, 1) void k_potrf_blk(const Item* __restrict__ i
         if (i0 + q < h1) S[t][h0 + i0 + q] = -acc[q];
     }
   } else {
-    for (int j = 0; j < pw; ++j)
-      for (int r = j + tid; r < pw; r += blockDim.x) base[(int64_t)j * F.ld + r] = (T)S[r][j];
+    T v[NB / 2];
+#pragma unroll
+    for (int q = 0; q < NB / 2; ++q) v[q] = (T)S[rr][jp + 2 * q];
+#pragma unroll
+    for (int q = 0; q < NB / 2; ++q) {
+      const int j = jp + 2 * q;
+      if (rr < pw && j <= rr) base[(int64_t)j * F.ld + rr] = v[q];
+    }
   }
   __syncthreads();
-  T* inv = Inv + F.inv0 + (int64_t)(c0 / NB) * NB * NB;  // ld = pw
-  for (int j = 0; j < pw; ++j)
-    for (int r = tid; r < pw; r += blockDim.x)
-      inv[(int64_t)j * pw + r] = (T)(r == j ? xd[r] : (r > j ? S[j][r] : 0.0));
+  {  // the inverse, column-major with ld = pw, zeros above the diagonal: reads, then stores
+    T* inv = Inv + F.inv0 + (int64_t)(c0 / NB) * NB * NB;
+    T v[NB / 2];
+    const double xr = xd[rr];
+#pragma unroll
+    for (int q = 0; q < NB / 2; ++q) {
+      const int j = jp + 2 * q;
+      const double sv = S[j][rr];
+      v[q] = (T)(rr == j ? xr : (rr > j ? sv : 0.0));
+    }
+#pragma unroll
+    for (int q = 0; q < NB / 2; ++q) {
+      const int j = jp + 2 * q;
+      if (rr < pw && j < pw) inv[(int64_t)j * pw + rr] = v[q];
+    }
+  }
 }
 
 }  // namespace fc

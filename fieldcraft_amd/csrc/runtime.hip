@@ -531,8 +531,15 @@ struct fc_factor {
   Buffer inv, part;        // diagonal-block inverses, split-K pieces
   Buffer fexp;             // mixed form: per front, the exponent of its power-of-two scale
   Buffer lb;               // mixed form: BF16 copies (hi, then lo) of the finished panels
-  hipGraph_t graph = nullptr;          // opts.use_graph: the captured factorization
+  hipGraph_t graph = nullptr;          // opts.use_graph bit 0: the captured factorization
   hipGraphExec_t graph_exec = nullptr;
+  struct SolveGraph {                  // opts.use_graph bit 1: a captured solve
+    int nr;
+    void *xp, *yp, *u;
+    hipGraph_t g;
+    hipGraphExec_t exec;
+  };
+  mutable std::vector<SolveGraph> sgraphs;
   Buffer vals;             // the matrix values on the device (host-value entry points)
   mutable std::mutex solve_mu;
   mutable Buffer xp, yp, usolve, refine_work;  // solve workspaces
@@ -554,6 +561,10 @@ struct fc_factor {
     lb.release();
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph) (void)hipGraphDestroy(graph);
+    for (auto& sg : sgraphs) {
+      (void)hipGraphExecDestroy(sg.exec);
+      (void)hipGraphDestroy(sg.g);
+    }
     vals.release();
     xp.release();
     yp.release();
@@ -717,7 +728,7 @@ void run_factor(fc_factor* F, const double* dval, i64 nnz, hipStream_t st) {
   Events ev;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> timed;
   hipEvent_t e0 = ev.make(), e1 = ev.make();
-  if (F->opts.use_graph) {
+  if (F->opts.use_graph & 1) {
     double* own = (double*)F->vals.get((size_t)std::max<i64>(nnz, 1) * sizeof(double));
     if (dval != own && nnz) HIP_CHECK(hipMemcpyAsync(own, dval, nnz * sizeof(double), hipMemcpyDeviceToDevice, st));
     if (F->opts.cancel && *F->opts.cancel) throw fc::Failure{fc::CANCELLED, -1, 0, 0, "cancelled"};
@@ -818,12 +829,11 @@ int factor_host(fc_factor* F, const double* upper_val, uint64_t nnz, fc_error* e
 }
 
 template <typename T, int R>
-void run_solve_r(const fc_factor* F, double* xp, double* yp, i64 ldx, int nrhs, hipStream_t st) {
+void run_solve_r(const fc_factor* F, double* xp, double* yp, double* u, i64 ldx, int nrhs, hipStream_t st) {
   const Program& prog = F->sym->prog;
   DevicePlan& P = *F->plan;
   const T* L = (const T*)F->L;
   const T* Inv = (const T*)F->inv.p;
-  double* u = (double*)F->usolve.get((size_t)std::max<i64>(prog.u_solve * nrhs, 1) * sizeof(double));
   auto run = [&](const fc::Launch& l) {
     const fc::Item* it = P.sitems + l.item0;
     dim3 g(l.count), b(256);
@@ -854,9 +864,37 @@ void run_solve_r(const fc_factor* F, double* xp, double* yp, i64 ldx, int nrhs, 
 }
 
 template <typename T>
-void run_solve(const fc_factor* F, double* xp, double* yp, i64 ldx, int nrhs, hipStream_t st) {
-  if (nrhs == 1) run_solve_r<T, 1>(F, xp, yp, ldx, nrhs, st);
-  else run_solve_r<T, fc::RC>(F, xp, yp, ldx, nrhs, st);
+void run_solve(const fc_factor* F, double* xp, double* yp, double* u, i64 ldx, int nrhs, hipStream_t st) {
+  if (nrhs == 1) run_solve_r<T, 1>(F, xp, yp, u, ldx, nrhs, st);
+  else run_solve_r<T, fc::RC>(F, xp, yp, u, ldx, nrhs, st);
+}
+
+// The solve's launches for nr right-hand sides in xp/yp/u, replayed from a hipGraph when
+// opts.use_graph asks for it (bit 1): captured on first use for each (nr, workspaces).
+void solve_launches(const fc_factor* F, double* xp, double* yp, double* u, i64 n, int nr, hipStream_t st) {
+  auto eager = [&]() {
+    if (F->precision == FC_FP64) run_solve<double>(F, xp, yp, u, n, nr, st);
+    else run_solve<float>(F, xp, yp, u, n, nr, st);
+  };
+  if (!(F->opts.use_graph & 2)) return eager();
+  for (const auto& sg : F->sgraphs)
+    if (sg.nr == nr && sg.xp == xp && sg.yp == yp && sg.u == u) {
+      HIP_CHECK(hipGraphLaunch(sg.exec, st));
+      return;
+    }
+  fc_factor::SolveGraph sg{nr, xp, yp, u, nullptr, nullptr};
+  HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  try {
+    eager();
+  } catch (...) {
+    (void)hipStreamEndCapture(st, &sg.g);
+    if (sg.g) (void)hipGraphDestroy(sg.g);
+    throw;
+  }
+  HIP_CHECK(hipStreamEndCapture(st, &sg.g));
+  HIP_CHECK(hipGraphInstantiate(&sg.exec, sg.g, nullptr, nullptr, 0));
+  F->sgraphs.push_back(sg);
+  HIP_CHECK(hipGraphLaunch(sg.exec, st));
 }
 
 // b (device, n x nrhs, ldb) solved in place, MAX_RHS right-hand sides at a time; the caller
@@ -868,9 +906,9 @@ void solve_device(const fc_factor* F, double* b, i64 n, int nrhs, i64 ldb, hipSt
     double* br = b + (i64)r0 * ldb;
     double* xp = (double*)F->xp.get((size_t)n * nr * sizeof(double));
     double* yp = (double*)F->yp.get((size_t)n * nr * sizeof(double));
+    double* u = (double*)F->usolve.get((size_t)std::max<i64>(F->sym->prog.u_solve * nr, 1) * sizeof(double));
     fc::k_permute<<<blocks_for(n), 256, 0, st>>>(br, xp, F->plan->perm, n, nr, ldb, n, 0);
-    if (F->precision == FC_FP64) run_solve<double>(F, xp, yp, n, nr, st);
-    else run_solve<float>(F, xp, yp, n, nr, st);
+    solve_launches(F, xp, yp, u, n, nr, st);
     fc::k_permute<<<blocks_for(n), 256, 0, st>>>(xp, br, F->plan->perm, n, nr, n, ldb, 1);
   }
   HIP_CHECK(hipGetLastError());

@@ -30,42 +30,73 @@ constexpr int BCOLS = 128;    // columns of a backward block-step workgroup
 constexpr int RC = 16;        // right-hand sides of the multi-column kernels
 constexpr int MAX_RHS = 64;   // right-hand sides one solve pass handles (the host splits larger blocks)
 
-// y[b] = sum_k Inv(b, k) v[k] for b < pw (Inv column-major, ld = pw, zero above the diagonal):
-// thread b of the first 128, loads coalesced along b and independent along k. V: LDS, [k][r].
+// Loads in the solve kernels are made in batches from clamped (always in-bounds) addresses and
+// the out-of-range values are selected away afterwards, so a batch is issued back to back
+// without per-load branches and costs one round of memory latency.
+
+// y[b] = sum_k Inv(b, k) v[k] for b < pw (Inv column-major, ld = pw, zero above the diagonal).
+// Thread (b, h) of the 256 sums the half h of k (loads coalesced along b), then the halves are
+// added in a fixed order. V: LDS, [k][r]; H: LDS scratch for the halves.
 template <typename T, int R>
 __device__ __forceinline__ void inv_apply(const T* __restrict__ inv, int pw, const double (*V)[R + 1], int nr,
-                                          double* __restrict__ out, int64_t ldx) {
-  const int b = threadIdx.x;
-  if (b >= pw) return;
+                                          double* __restrict__ out, int64_t ldx, double (*H)[SNB][R + 1]) {
+  const int b = threadIdx.x & (SNB - 1), h = threadIdx.x >> 7, bb = min(b, pw - 1);
   double s[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) s[r] = 0.0;
-#pragma unroll 16
-  for (int k = 0; k < pw; ++k) {
-    const double a = (double)inv[(int64_t)k * pw + b];
+  constexpr int KB = R == 1 ? 16 : 4;  // loads per batch
+#pragma unroll(R == 1 ? 4 : 1)
+  for (int k0 = 0; k0 < 64; k0 += KB) {
+    double a[KB];
 #pragma unroll
-    for (int r = 0; r < R; ++r) s[r] += a * V[k][r];
+    for (int q = 0; q < KB; ++q) a[q] = (double)inv[(int64_t)min(64 * h + k0 + q, pw - 1) * pw + bb];
+#pragma unroll
+    for (int q = 0; q < KB; ++q) {
+      const int k = 64 * h + k0 + q;
+      const bool in = k < pw;
+#pragma unroll
+      for (int r = 0; r < R; ++r) s[r] = fma(in ? a[q] : 0.0, V[min(k, SNB - 1)][r], s[r]);
+    }
   }
-  for (int r = 0; r < nr; ++r) out[(int64_t)r * ldx + b] = s[r];
+#pragma unroll
+  for (int r = 0; r < R; ++r) H[h][b][r] = s[r];
+  __syncthreads();
+  for (int e = threadIdx.x; e < pw * nr; e += blockDim.x) {
+    const int c = e % pw, r = e / pw;
+    out[(int64_t)r * ldx + c] = H[0][c][r] + H[1][c][r];
+  }
 }
 
-// x[b] = sum_k Inv(k, b) z[k] (the transpose): a wave per output b, lanes along k (column b of
-// Inv is contiguous), summed across the wave in a fixed order. Z: LDS, [k][r].
+// x[b] = sum_k Inv(k, b) z[k] (the transpose): thread (b, h) sums the half h of column b of Inv
+// (contiguous), then the halves are added in a fixed order. Z: LDS, [k][r]; H: LDS scratch.
 template <typename T, int R>
 __device__ __forceinline__ void inv_apply_t(const T* __restrict__ inv, int pw, const double (*Z)[R + 1], int nr,
-                                            double* __restrict__ out, int64_t ldx) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int b = wave; b < pw; b += blockDim.x >> 6) {
-    const T* col = inv + (int64_t)b * pw;
-    const double a0 = lane < pw ? (double)col[lane] : 0.0;
-    const double a1 = lane + 64 < pw ? (double)col[lane + 64] : 0.0;
+                                            double* __restrict__ out, int64_t ldx, double (*H)[SNB][R + 1]) {
+  const int b = threadIdx.x & (SNB - 1), h = threadIdx.x >> 7, bb = min(b, pw - 1);
+  const T* col = inv + (int64_t)bb * pw;
+  double s[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      if (r >= nr) break;
-      double t = (lane < pw ? a0 * Z[lane][r] : 0.0) + (lane + 64 < pw ? a1 * Z[lane + 64][r] : 0.0);
-      for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
-      if (lane == 0) out[(int64_t)r * ldx + b] = t;
+  for (int r = 0; r < R; ++r) s[r] = 0.0;
+  constexpr int KB = R == 1 ? 16 : 4;  // loads per batch
+#pragma unroll(R == 1 ? 4 : 1)
+  for (int k0 = 0; k0 < 64; k0 += KB) {
+    double a[KB];
+#pragma unroll
+    for (int q = 0; q < KB; ++q) a[q] = (double)col[min(64 * h + k0 + q, pw - 1)];
+#pragma unroll
+    for (int q = 0; q < KB; ++q) {
+      const int k = 64 * h + k0 + q;
+      const bool in = k < pw;
+#pragma unroll
+      for (int r = 0; r < R; ++r) s[r] = fma(in ? a[q] : 0.0, Z[min(k, SNB - 1)][r], s[r]);
     }
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) H[h][b][r] = s[r];
+  __syncthreads();
+  for (int e = threadIdx.x; e < pw * nr; e += blockDim.x) {
+    const int c = e % pw, r = e / pw;
+    out[(int64_t)r * ldx + c] = H[0][c][r] + H[1][c][r];
   }
 }
 
@@ -77,6 +108,7 @@ __global__ __launch_bounds__(256) void k_fwd_gather(const Item* __restrict__ ite
                                                     const T* __restrict__ Inv, double* v, double* __restrict__ y,
                                                     int64_t ldx, int nrhs, double* u, const int64_t* __restrict__ uoff) {
   __shared__ double Vb[SNB][R + 1];
+  __shared__ double H[2][SNB][R + 1];
   const int f = items[blockIdx.x].f;
   const FrontDev F = fronts[f];
   const int w = F.w, mu = F.m - F.w, pw = min(SNB, w);
@@ -103,7 +135,7 @@ __global__ __launch_bounds__(256) void k_fwd_gather(const Item* __restrict__ ite
     for (int r = 0; r < nr; ++r)
       for (int k = threadIdx.x; k < pw; k += blockDim.x) Vb[k][r] = v[(int64_t)(q0 + r) * ldx + F.col0 + k];
     __syncthreads();
-    inv_apply<T, R>(Inv + F.inv0, pw, Vb, nr, y + (int64_t)q0 * ldx + F.col0, ldx);
+    inv_apply<T, R>(Inv + F.inv0, pw, Vb, nr, y + (int64_t)q0 * ldx + F.col0, ldx, H);
     __syncthreads();
   }
 }
@@ -119,6 +151,7 @@ __global__ __launch_bounds__(256) void k_fwd_step(const Item* __restrict__ items
                                                   const int64_t* __restrict__ uoff) {
   __shared__ double Yb[SNB][R + 1];
   __shared__ double Vn[SNB][R + 1];
+  __shared__ double H[2][SNB][R + 1];
   const Item it = items[blockIdx.x];
   const FrontDev F = fronts[it.f];
   const int c0 = it.a, pw = min(SNB, F.w - c0), c1 = c0 + pw, r0 = it.b;
@@ -137,11 +170,17 @@ __global__ __launch_bounds__(256) void k_fwd_step(const Item* __restrict__ items
       double s[R];
 #pragma unroll
       for (int r = 0; r < R; ++r) s[r] = 0.0;
-#pragma unroll 16
-      for (int j = 0; j < pw; ++j) {
-        const double l = (double)lf[(int64_t)j * F.ld + i];
+      constexpr int JB = R == 1 ? 16 : 4;  // independent loads per batch
+      for (int j0 = 0; j0 < pw; j0 += JB) {
+        double l[JB];
 #pragma unroll
-        for (int r = 0; r < R; ++r) s[r] += l * Yb[j][r];
+        for (int q = 0; q < JB; ++q) l[q] = (double)lf[(int64_t)min(j0 + q, pw - 1) * F.ld + i];
+#pragma unroll
+        for (int q = 0; q < JB; ++q) {
+          const bool in = j0 + q < pw;  // Yb holds pw rows only
+#pragma unroll
+          for (int r = 0; r < R; ++r) s[r] = fma(in ? l[q] : 0.0, Yb[min(j0 + q, SNB - 1)][r], s[r]);
+        }
       }
       if (i < F.w) {
         for (int r = 0; r < nr; ++r) {
@@ -157,7 +196,7 @@ __global__ __launch_bounds__(256) void k_fwd_step(const Item* __restrict__ items
     if (it.c) {
       __syncthreads();
       inv_apply<T, R>(Inv + F.inv0 + (int64_t)(c1 / SNB) * SNB * SNB, pn, Vn, nr,
-                      y + (int64_t)q0 * ldx + F.col0 + c1, ldx);
+                      y + (int64_t)q0 * ldx + F.col0 + c1, ldx, H);
     }
     __syncthreads();
   }
@@ -207,16 +246,24 @@ __global__ __launch_bounds__(256) void k_bwd_below(const Item* __restrict__ item
       for (int c = 0; c < JC; ++c)
 #pragma unroll
         for (int r = 0; r < R; ++r) s[c][r] = 0.0;
-#pragma unroll(R == 1 ? 2 : 1)
-      for (int i = lane; i < mu; i += 64) {
-        double xv[R];
+      constexpr int NQ = R == 1 ? 8 : 2;  // rows a lane loads per batch
+      for (int i0 = lane; i0 < mu; i0 += 64 * NQ) {
+        double xv[NQ][R], l[NQ][JC];
 #pragma unroll
-        for (int r = 0; r < R; ++r) xv[r] = r < nr ? uf[(int64_t)(q0 + r) * mu + i] : 0.0;
+        for (int q = 0; q < NQ; ++q) {
+          const int i = min(i0 + 64 * q, mu - 1);
 #pragma unroll
-        for (int c = 0; c < JC; ++c) {
-          const double l = c < nc ? (double)lf[(int64_t)(jb + c) * F.ld + i] : 0.0;
+          for (int r = 0; r < R; ++r) xv[q][r] = uf[(int64_t)(q0 + min(r, nr - 1)) * mu + i];
 #pragma unroll
-          for (int r = 0; r < R; ++r) s[c][r] += l * xv[r];
+          for (int c = 0; c < JC; ++c) l[q][c] = (double)lf[(int64_t)(jb + min(c, nc - 1)) * F.ld + i];
+        }
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+          const bool in = i0 + 64 * q < mu;
+#pragma unroll
+          for (int c = 0; c < JC; ++c)
+#pragma unroll
+            for (int r = 0; r < R; ++r) s[c][r] = fma(in ? l[q][c] : 0.0, xv[q][r], s[c][r]);
         }
       }
 #pragma unroll
@@ -238,6 +285,7 @@ __global__ __launch_bounds__(256) void k_bwd_first(const Item* __restrict__ item
                                                    const T* __restrict__ Inv, const double* __restrict__ z,
                                                    double* __restrict__ x, int64_t ldx, int nrhs) {
   __shared__ double Zb[SNB][R + 1];
+  __shared__ double H[2][SNB][R + 1];
   const FrontDev F = fronts[items[blockIdx.x].f];
   const int cl = (F.w - 1) / SNB * SNB, pw = F.w - cl;
   for (int q0 = 0; q0 < nrhs; q0 += R) {
@@ -246,7 +294,7 @@ __global__ __launch_bounds__(256) void k_bwd_first(const Item* __restrict__ item
       for (int k = threadIdx.x; k < pw; k += blockDim.x) Zb[k][r] = z[(int64_t)(q0 + r) * ldx + F.col0 + cl + k];
     __syncthreads();
     inv_apply_t<T, R>(Inv + F.inv0 + (int64_t)(cl / SNB) * SNB * SNB, pw, Zb, nr, x + (int64_t)q0 * ldx + F.col0 + cl,
-                      ldx);
+                      ldx, H);
     __syncthreads();
   }
 }
@@ -267,7 +315,8 @@ __global__ __launch_bounds__(256) void k_bwd_step(const Item* __restrict__ items
   __shared__ T Lt[HC][SNB + 1];
   __shared__ double Xb[SNB][R + 1];
   __shared__ double Zn[SNB][R + 1];
-  __shared__ double part[4][HC][R + 1];
+  __shared__ double part[4][HC][R + 1];  // also the halves of inv_apply_t (2 x SNB = 4 x HC)
+  static_assert(2 * SNB == 4 * HC, "the halves fit the partial sums");
   const Item it = items[blockIdx.x];
   const FrontDev F = fronts[it.f];
   const int c0 = it.a, pw = min(SNB, F.w - c0), j0 = it.b;
@@ -280,13 +329,14 @@ __global__ __launch_bounds__(256) void k_bwd_step(const Item* __restrict__ items
       for (int k = tid; k < pw; k += blockDim.x) Xb[k][r] = x[(int64_t)(q0 + r) * ldx + F.col0 + c0 + k];
     for (int h0 = 0; h0 < nj; h0 += HC) {
       const int hc = min(HC, nj - h0);
-      {  // thread (row k = tid % 128, every other column)
+      {  // thread (row k = tid % 128, every other column): 32 independent loads, then stores
         const int k = tid & (SNB - 1), jb = tid >> 7;
-        if (k < pw) {
-#pragma unroll 16
-          for (int jj = jb; jj < HC; jj += 2)
-            if (jj < hc) Lt[jj][k] = lf[(int64_t)(j0 + h0 + jj) * F.ld + k];
-        }
+        T v[HC / 2];
+#pragma unroll
+        for (int q = 0; q < HC / 2; ++q)
+          v[q] = lf[(int64_t)(j0 + h0 + min(jb + 2 * q, hc - 1)) * F.ld + min(k, pw - 1)];
+#pragma unroll
+        for (int q = 0; q < HC / 2; ++q) Lt[jb + 2 * q][k] = (k < pw && jb + 2 * q < hc) ? v[q] : T(0);
       }
       __syncthreads();
       {  // partial sums of column jq over rows [32 g, 32 g + 32), every right-hand side
@@ -298,7 +348,7 @@ __global__ __launch_bounds__(256) void k_bwd_step(const Item* __restrict__ items
           for (int k = 32 * g; k < k1; ++k) {
             const double l = (double)Lt[jq][k];
 #pragma unroll
-            for (int r = 0; r < R; ++r) t[r] += l * Xb[k][r];
+            for (int r = 0; r < R; ++r) t[r] = fma(l, Xb[k][r], t[r]);
           }
         }
 #pragma unroll
@@ -317,7 +367,8 @@ __global__ __launch_bounds__(256) void k_bwd_step(const Item* __restrict__ items
       __syncthreads();
     }
     if (it.c) inv_apply_t<T, R>(Inv + F.inv0 + (int64_t)(j0 / SNB) * SNB * SNB, SNB, Zn, nr,
-                                x + (int64_t)q0 * ldx + F.col0 + j0, ldx);
+                                x + (int64_t)q0 * ldx + F.col0 + j0, ldx,
+                                reinterpret_cast<double (*)[SNB][R + 1]>(&part[0][0][0]));
     __syncthreads();
   }
 }

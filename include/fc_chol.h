@@ -63,7 +63,10 @@ typedef struct {
   const volatile int32_t* cancel; /* polled between levels of the front tree; NULL: none    */
   int32_t bf16_terms;             /* FC_MIXED: 1 = one BF16 product per operand pair,
                                      3 = split FP32 operands (hi*hi + hi*lo + lo*hi)       */
-  int32_t use_graph;              /* replay the level-ordered launches from a hipGraph      */
+  int32_t use_graph;              /* bit 0: capture the factorization's launches into a hipGraph
+                                     on the first call and replay it (the cancel flag is then
+                                     polled once per call and no kind is timed); bit 1: the
+                                     same for the solves, one graph per right-hand-side count */
   double pivot_fallback;          /* FC_MIXED: relative pivot below which a diagonal block
                                      is refactored in FP64 (0: the default, 1e-4)           */
   int32_t profile_mask;           /* bit k: time every launch of kernel kind k with HIP events

@@ -226,13 +226,14 @@ def test_batch_of_models_matches_each_model(gpu):
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_graph_replays_match_eager_launches(gpu, precision):
-    """A factorization captured into a hipGraph and replayed for new values gives the same bits
-    as the eager launches, on every replay, with eager solves in between."""
+    """A factorization captured into a hipGraph and replayed for new values, and solves replayed
+    from their own graphs, give the same bits as the eager launches on every replay, with eager
+    work in between."""
     p = P.cube_hex8(10)
     up = p.upper()
     sym = CH.Symbolic(CH.UpperColumns(p.n, *up))
     vals = [up[2], up[2] * 2.0, up[2] * 0.5 + 0.0, up[2]]
-    g = CH.Factor(sym, vals[0], precision=precision, use_graph=True)
+    g = CH.Factor(sym, vals[0], precision=precision, use_graph=3)  # factor and solves
     for q, v in enumerate(vals):
         if q:
             g.refactor(v)
