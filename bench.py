@@ -183,13 +183,13 @@ def main():
     funcs = {}
     for kind, v in head["kinds"].items():
         if kind in ("update", "syrk") and form == "mixed":
-            name, bound, peak = f"k_gemm_bf16<{args.bf16_terms}>", "mfma", PEAK["bf16"]
+            name, bound, peak = f"k_gemm_b16<{args.bf16_terms}>", "mfma", PEAK["bf16"]
         elif kind in ("update", "syrk", "trsm"):
             name, bound, peak = f"k_gemm<{el}>", "mfma", PEAK["fp64" if form == "fp64" else "fp32"]
         elif kind in ("extend_l", "extend_u"):
             name, bound, peak = f"k_extend_add<{el}>", "hbm", HBM_PEAK
         elif kind == "potrf":
-            name, bound, peak = f"k_potrf_blk<{el}> + k_potrf_inv<{el},32>", "latency", None
+            name, bound, peak = f"k_potrf_blk<{el}>", "latency", None
         elif kind == "split":
             name, bound, peak = "k_split_panel", "hbm", HBM_PEAK
         else:

@@ -4,7 +4,7 @@
 //   panel update   L[r.., c0..c1) -= L[r.., 0..c0) L[c0..c1, 0..c0)^T      (G_SUB)
 //   rows below     L[r.., c0..c1)  = L[r.., c0..c1) Linv^T                   (G_SET, in place)
 //   update matrix  U[i.., j..)     = -L[W+i.., 0..W) L[W+j.., 0..W)^T        (G_NEG)
-// where Linv is the inverse of the panel's factored diagonal block (k_potrf_inv). A launch
+// where Linv is the inverse of the panel's factored diagonal block (k_potrf_blk). A launch
 // with few output tiles is split along k: each piece writes its partial tile (G_PART) and
 // k_reduce_parts adds the pieces in a fixed order, so results do not depend on scheduling.
 //
@@ -26,8 +26,8 @@ constexpr int GB = 128;  // tile edge
 constexpr int GK = 16;   // k per LDS stage
 constexpr int NB = 128;  // panel width of the factorization (columns of a diagonal block)
 
-enum GemmMode : int32_t { G_SUB = 0, G_NEG = 1, G_SET = 2, G_PART = 3 };
-enum GemmSrc : int8_t { S_L = 0, S_U = 1, S_INV = 2, S_PART = 3 };
+enum GemmMode : int32_t { G_SUB = 0, G_NEG = 1, G_SET = 2, G_PART = 3, G_ADD = 4 };
+enum GemmSrc : int8_t { S_L = 0, S_U = 1, S_INV = 2, S_PART = 3, S_W = 4 };
 
 struct GemmItem {
   int64_t a, b, c;        // element offsets: A(i, k) = A[a + k lda + i], B(j, k), C(i, j) = C[c + j ldc + i]
@@ -70,12 +70,14 @@ struct Mma<float> {
 };
 
 template <typename T>
-__global__ __launch_bounds__(256, 2) void k_gemm(const GemmItem* __restrict__ items, T* L, T* U, T* Inv, T* Part) {
+__global__ __launch_bounds__(256, 2) void k_gemm(const GemmItem* __restrict__ items, T* L, T* U, T* Inv, T* Part, T* W) {
   constexpr int PAD = 4;
   __shared__ T As[2][GK][GB + PAD];
   __shared__ T Bs[2][GK][GB + PAD];
   const GemmItem it = items[blockIdx.x];
-  auto base = [&](int8_t src) { return src == S_L ? L : (src == S_U ? U : (src == S_INV ? Inv : Part)); };
+  auto base = [&](int8_t src) {
+    return src == S_L ? L : (src == S_U ? U : (src == S_INV ? Inv : (src == S_W ? W : Part)));
+  };
   const T* __restrict__ A = base(it.asrc) + it.a;
   const T* __restrict__ B = base(it.bsrc) + it.b;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -155,9 +157,24 @@ __global__ __launch_bounds__(256, 2) void k_gemm(const GemmItem* __restrict__ it
           const T x = acc[bi][bj][v];
           if (it.mode == G_SUB) *p -= x;
           else if (it.mode == G_NEG) *p = -x;
+          else if (it.mode == G_ADD) *p += x;
           else *p = x;
         }
       }
+}
+
+// A tile's entries C(i, j) = A(i, j) (i < m, j < n): the rows below a panel saved before the
+// product that overwrites them (the FP64 TRSM refines its result against them).
+template <typename T>
+__global__ __launch_bounds__(256) void k_copy_tiles(const GemmItem* __restrict__ items, const T* __restrict__ L,
+                                                    T* __restrict__ W) {
+  const GemmItem it = items[blockIdx.x];
+  const T* A = L + it.a;
+  T* C = W + it.c;
+  for (int e = threadIdx.x; e < it.m * it.n; e += blockDim.x) {
+    const int i = e % it.m, j = e / it.m;
+    C[(int64_t)j * it.ldc + i] = A[(int64_t)j * it.lda + i];
+  }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -195,7 +212,9 @@ __global__ __launch_bounds__(256) void k_split_panel(const Item* __restrict__ it
   for (int j0 = c0; j0 < cend; j0 += 8) {
     float x[8];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) x[q] = j0 + q < c1 ? row[(int64_t)(j0 + q) * F.ld] * sc : 0.0f;
+    for (int q = 0; q < 8; ++q) x[q] = row[(int64_t)min(j0 + q, c1 - 1) * F.ld];  // in bounds
+#pragma unroll
+    for (int q = 0; q < 8; ++q) x[q] = j0 + q < c1 ? x[q] * sc : 0.0f;
     v8bf h, l;
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
@@ -227,30 +246,35 @@ __global__ __launch_bounds__(256, 2) void k_gemm_b16(const GemmItem* __restrict_
   auto base = [&](int8_t src) { return src == S_L ? L : (src == S_U ? U : Part); };
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wi = (wave >> 1) * 64, wj = (wave & 1) * 64;
-  // loader: 16-byte vectors v = tid + 256 q of a stage (row v / 4, k 8 (v % 4) + [0, 8))
-  u32x4 ra[2][NS], rb[2][NS];
-  auto load = [&](int s) {
+  // loader: 16-byte vectors v = tid + 256 q of a stage (row v / 4, k 8 (v % 4) + [0, 8)), two
+  // register sets, so a stage's loads are issued two stages before it is stored to LDS
+  u32x4 ra[2][2][NS], rb[2][2][NS];
+  auto load = [&](int set, int s) {
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
       const int v = tid + 256 * q, row = v >> 2, part = v & 3;
-      const int64_t oa2 = a0 + s * chunk + row * BK + part * 8, ob2 = b0 + s * chunk + row * BK + part * 8;
+      // rows past the tile read the last row (in bounds) and are zeroed after the load
+      const int64_t oa2 = a0 + s * chunk + min(row, it.m - 1) * BK + part * 8;
+      const int64_t ob2 = b0 + s * chunk + min(row, it.n - 1) * BK + part * 8;
       const u32x4 z = {0, 0, 0, 0};
-      ra[q][0] = row < it.m ? *(const u32x4*)(Lh + oa2) : z;
-      rb[q][0] = row < it.n ? *(const u32x4*)(Lh + ob2) : z;
+      const u32x4 ah = *(const u32x4*)(Lh + oa2), bh = *(const u32x4*)(Lh + ob2);
+      ra[set][q][0] = row < it.m ? ah : z;
+      rb[set][q][0] = row < it.n ? bh : z;
       if (TERMS == 3) {
-        ra[q][NS - 1] = row < it.m ? *(const u32x4*)(Ll + oa2) : z;
-        rb[q][NS - 1] = row < it.n ? *(const u32x4*)(Ll + ob2) : z;
+        const u32x4 al = *(const u32x4*)(Ll + oa2), bl = *(const u32x4*)(Ll + ob2);
+        ra[set][q][NS - 1] = row < it.m ? al : z;
+        rb[set][q][NS - 1] = row < it.n ? bl : z;
       }
     }
   };
-  auto store = [&](int buf) {
+  auto store = [&](int set, int buf) {
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
       const int v = tid + 256 * q, row = v >> 2, part = v & 3;
 #pragma unroll
       for (int t = 0; t < NS; ++t) {
-        *(u32x4*)&As[buf][t][row][part * 8] = ra[q][t];
-        *(u32x4*)&Bs[buf][t][row][part * 8] = rb[q][t];
+        *(u32x4*)&As[buf][t][row][part * 8] = ra[set][q][t];
+        *(u32x4*)&Bs[buf][t][row][part * 8] = rb[set][q][t];
       }
     }
   };
@@ -261,12 +285,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_b16(const GemmItem* __restrict_
     for (int bj = 0; bj < 4; ++bj) acc[bi][bj] = v4f{0, 0, 0, 0};
   const int stages = (it.k + BK - 1) / BK;
   const int fr = lane & 15, fk = 8 * (lane >> 4);  // a lane's fragment row and first k
-  load(0);
-  store(0);
-  __syncthreads();
-  for (int s = 0; s < stages; ++s) {
-    const int cur = s & 1;
-    if (s + 1 < stages) load(s + 1);
+  auto mul = [&](int cur) {
     v8bf fj[4], fi[4];
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
@@ -293,8 +312,23 @@ __global__ __launch_bounds__(256, 2) void k_gemm_b16(const GemmItem* __restrict_
 #pragma unroll
       for (int bj = 0; bj < 4; ++bj)
         acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fj[bj], fi[bi], acc[bi][bj], 0, 0, 0);
-    if (s + 1 < stages) store(cur ^ 1);
+  };
+  // stage s lives in register set s % 2 from its load until it is stored to LDS buffer s % 2
+  load(0, 0);
+  if (stages > 1) load(1, 1);
+  store(0, 0);
+  __syncthreads();
+  for (int s = 0; s < stages; s += 2) {
+    if (s + 2 < stages) load(0, s + 2);
+    mul(0);
+    if (s + 1 < stages) store(1, 1);
     __syncthreads();
+    if (s + 1 < stages) {
+      if (s + 3 < stages) load(1, s + 3);
+      mul(1);
+      if (s + 2 < stages) store(0, 0);
+      __syncthreads();
+    }
   }
   // acc[bi][bj][v] = C(wi + 16 bi + lane % 16, wj + 16 bj + 4 (lane / 16) + v) / sc^2
   const float unscale = ldexpf(1.0f, 2 * e);
@@ -357,13 +391,17 @@ __global__ __launch_bounds__(128, 2) void k_gemm_b16w(const GemmItem* __restrict
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int v = tid + 128 * q, row = v >> 2, part = v & 3;
-      const int64_t oa2 = a0 + s * chunk + row * BK + part * 8, ob2 = b0 + s * chunk + row * BK + part * 8;
+      // rows past the tile read the last row (in bounds) and are zeroed after the load
+      const int64_t oa2 = a0 + s * chunk + min(row, it.m - 1) * BK + part * 8;
+      const int64_t ob2 = b0 + s * chunk + min(row, it.n - 1) * BK + part * 8;
       const u32x4 z = {0, 0, 0, 0};
-      ra[q][0] = row < it.m ? *(const u32x4*)(Lh + oa2) : z;
-      rb[q][0] = row < it.n ? *(const u32x4*)(Lh + ob2) : z;
+      const u32x4 ah = *(const u32x4*)(Lh + oa2), bh = *(const u32x4*)(Lh + ob2);
+      ra[q][0] = row < it.m ? ah : z;
+      rb[q][0] = row < it.n ? bh : z;
       if (TERMS == 3) {
-        ra[q][NS - 1] = row < it.m ? *(const u32x4*)(Ll + oa2) : z;
-        rb[q][NS - 1] = row < it.n ? *(const u32x4*)(Ll + ob2) : z;
+        const u32x4 al = *(const u32x4*)(Ll + oa2), bl = *(const u32x4*)(Ll + ob2);
+        ra[q][NS - 1] = row < it.m ? al : z;
+        rb[q][NS - 1] = row < it.n ? bl : z;
       }
     }
   };
@@ -449,6 +487,126 @@ __global__ __launch_bounds__(128, 2) void k_gemm_b16w(const GemmItem* __restrict
       }
 }
 
+// The same products with the fragments loaded straight from the copies into registers: in the
+// k-chunked layout a 16-row x 32-k fragment (lane: row lane % 16, k 8 (lane / 16) + [0, 8)) is
+// one contiguous 1 KB block, so every load is a full 16-byte vector per lane and no LDS stage or
+// barrier is needed. WN = 8: two waves per 128 x 128 tile, each 64 rows x 128 columns as 4 x 8
+// blocks of the 16x16x32 MFMA; WN = 4: four waves of 64 x 64. The next stage's fragments are
+// loaded while the current one is multiplied.
+template <int TERMS, int WN>
+__global__ __launch_bounds__(WN == 8 ? 128 : 256, 1) void k_gemm_b16d(const GemmItem* __restrict__ items, const FrontDev* __restrict__ fronts,
+                                                      const __bf16* __restrict__ Lh, const __bf16* __restrict__ Ll,
+                                                      float* L, float* U, float* Part, const int32_t* __restrict__ fexp) {
+  static_assert(TERMS == 1 || TERMS == 3, "one or three BF16 products");
+  constexpr int NS = TERMS == 3 ? 2 : 1;
+  const GemmItem it = items[blockIdx.x];
+  const FrontDev F = fronts[it.front];
+  const int e = fexp[it.front];
+  const int64_t oa = it.a - F.lpos, ob = it.b - F.lpos;
+  const int64_t chunk = (int64_t)F.ld * BK;
+  const int64_t a0 = F.bpos + (oa / F.ld) / BK * chunk + (oa % F.ld) * BK;
+  const int64_t b0 = F.bpos + (ob / F.ld) / BK * chunk + (ob % F.ld) * BK;
+  auto base = [&](int8_t src) { return src == S_L ? L : (src == S_U ? U : Part); };
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int fr = lane & 15, fk = 8 * (lane >> 4);
+  const int wi = WN == 8 ? 64 * wave : 64 * (wave >> 1), wj = WN == 8 ? 0 : 64 * (wave & 1);
+  // a lane's fragment rows (clamped into the tile; rows past it are zeroed after the load)
+  const int ra = min(wi + fr, it.m - 1), rb = min(wj + fr, it.n - 1);
+  typedef u32x4 Frag[NS];
+  Frag fa[2][4], fb[2][WN];
+  const u32x4 z = {0, 0, 0, 0};
+  auto load = [&](int buf, int s) {
+    const __bf16* ah = Lh + a0 + (int64_t)s * chunk + fk;
+    const __bf16* bh = Lh + b0 + (int64_t)s * chunk + fk;
+    const __bf16* al = Ll + a0 + (int64_t)s * chunk + fk;
+    const __bf16* bl = Ll + b0 + (int64_t)s * chunk + fk;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int r = min(ra + 16 * b, it.m - 1);
+      const bool in = wi + 16 * b + fr < it.m;
+      const u32x4 h = *(const u32x4*)(ah + r * BK);
+      fa[buf][b][0] = in ? h : z;
+      if (TERMS == 3) {
+        const u32x4 l = *(const u32x4*)(al + r * BK);
+        fa[buf][b][NS - 1] = in ? l : z;
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < WN; ++b) {
+      const int r = min(rb + 16 * b, it.n - 1);
+      const bool in = wj + 16 * b + fr < it.n;
+      const u32x4 h = *(const u32x4*)(bh + r * BK);
+      fb[buf][b][0] = in ? h : z;
+      if (TERMS == 3) {
+        const u32x4 l = *(const u32x4*)(bl + r * BK);
+        fb[buf][b][NS - 1] = in ? l : z;
+      }
+    }
+  };
+  v4f acc[4][WN];
+#pragma unroll
+  for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < WN; ++bj) acc[bi][bj] = v4f{0, 0, 0, 0};
+  auto mul = [&](int buf) {
+#pragma unroll
+    for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+      for (int bj = 0; bj < WN; ++bj) {
+        const v8bf ah = __builtin_bit_cast(v8bf, fa[buf][bi][0]), bh = __builtin_bit_cast(v8bf, fb[buf][bj][0]);
+        if (TERMS == 3) {
+          const v8bf al = __builtin_bit_cast(v8bf, fa[buf][bi][NS - 1]);
+          const v8bf bl = __builtin_bit_cast(v8bf, fb[buf][bj][NS - 1]);
+          acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl, ah, acc[bi][bj], 0, 0, 0);
+          acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh, al, acc[bi][bj], 0, 0, 0);
+        }
+        acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh, ah, acc[bi][bj], 0, 0, 0);
+      }
+  };
+  const int stages = (it.k + BK - 1) / BK;
+  load(0, 0);
+  for (int s = 0; s < stages; s += 2) {
+    if (s + 1 < stages) load(1, s + 1);
+    mul(0);
+    if (s + 1 < stages) {
+      if (s + 2 < stages) load(0, s + 2);
+      mul(1);
+    }
+  }
+  // acc[bi][bj][v] = C(wi + 16 bi + lane % 16, wj + 16 bj + 4 (lane / 16) + v) / sc^2
+  const float unscale = ldexpf(1.0f, 2 * e);
+  const int il = lane & 15;
+  if (it.mode == G_PART) {
+    float* P = Part + it.c;
+#pragma unroll
+    for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+      for (int bj = 0; bj < WN; ++bj)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int i = wi + 16 * bi + il, j = wj + 16 * bj + Mma<float>::row(lane, v);
+          P[(int64_t)j * GB + i] = acc[bi][bj][v] * unscale;
+        }
+    return;
+  }
+  float* C = base(it.csrc) + it.c;
+#pragma unroll
+  for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < WN; ++bj)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = wi + 16 * bi + il, j = wj + 16 * bj + Mma<float>::row(lane, v);
+        if (i < it.m && j < it.n && i - j + it.diag >= 0) {
+          float* p = C + (int64_t)j * it.ldc + i;
+          const float x = acc[bi][bj][v] * unscale;
+          if (it.mode == G_SUB) *p -= x;
+          else if (it.mode == G_NEG) *p = -x;
+          else *p = x;
+        }
+      }
+}
+
 // Sum of the partial tiles of a split product, in order, applied to C as its mode says. Each
 // tile is shared by RSPLIT workgroups (GB * GB / RSPLIT entries each).
 constexpr int RSPLIT = 16;
@@ -475,79 +633,12 @@ __global__ __launch_bounds__(256) void k_reduce_parts(const ReduceItem* __restri
 // The diagonal block of a panel: factored column by column with the reference's pivot test
 // (cholesky.rs:1146-1171: a pivot not above 1e-10 |A_jj|, or not finite, fails the column),
 // then inverted, Linv = L11^-1, for the rows below (a GEMM). The arithmetic is FP64 whatever
-// the storage. Item: f, a = c0, b = slot of Linv (NB x NB, column-major) in Inv.
+// the storage. Item: f, a = c0; Linv goes to the panel's slot in Inv (F.inv0 + (c0 / NB) NB^2).
 
 constexpr double PIVOT_TOL = 1e-10;  // cholesky.rs:48
 
-// K: the widest block the variant takes (panels up to 32 wide, one wave, use it; wider ones go
-// to k_potrf_blk). One barrier per column: column c's trailing update reads
-// column c unscaled (times 1 / d_c^2) while column c - 1 is scaled in the same phase.
-template <typename T, int K>
-__global__ __launch_bounds__(K == NB ? 256 : 64 * (K / 32), 1) void k_potrf_inv(
-    const Item* __restrict__ items, const FrontDev* __restrict__ fronts, T* __restrict__ L,
-    const double* __restrict__ diag, unsigned long long* __restrict__ fail, T* __restrict__ Inv) {
-  __shared__ double S[K][K + 1];  // lower: L11; strict upper S[j][i] (j < i): Linv(i, j)
-  __shared__ double dv[K];        // the pivots' square roots, then their inverses
-  __shared__ double ad[K];        // |A_jj| of the block's columns, for the pivot test
-  const Item it = items[blockIdx.x];
-  const FrontDev F = fronts[it.f];
-  const int c0 = it.a, pw = min(K, F.w - c0);
-  T* base = L + F.lpos + (int64_t)c0 * F.ld + c0;
-  for (int j = threadIdx.x; j < pw; j += blockDim.x) ad[j] = fabs(diag[F.col0 + c0 + j]);
-  for (int e = threadIdx.x; e < K * K; e += blockDim.x) {
-    const int i = e % K, j = e / K;
-    if (i >= j) S[i][j] = (i < pw && j < pw) ? (double)base[(int64_t)j * F.ld + i] : 0.0;
-  }
-  __syncthreads();
-  bool bad = false;
-  for (int c = 0; c <= pw; ++c) {
-    if (c > 0) {  // scale the previous column
-      const double d = dv[c - 1];
-      for (int i = c + threadIdx.x; i < pw; i += blockDim.x) S[i][c - 1] /= d;
-      if (threadIdx.x == 0) S[c - 1][c - 1] = d;
-    }
-    if (c < pw) {
-      const double p = S[c][c];
-      const double d = sqrt(p);
-      if (threadIdx.x == 0) {
-        dv[c] = d;
-        if (!bad && (!(p > PIVOT_TOL * ad[c]) || !isfinite(p))) {
-          atomicMin(fail, (unsigned long long)(F.col0 + c0 + c));
-          bad = true;
-        }
-      }
-      const double q = 1.0 / (d * d);
-      const int rest = pw - c - 1;
-      for (int e = threadIdx.x; e < rest * rest; e += blockDim.x) {
-        const int i = c + 1 + e % rest, j = c + 1 + e / rest;
-        if (i >= j) S[i][j] -= S[i][c] * S[j][c] * q;
-      }
-    }
-    __syncthreads();
-  }
-  // Linv row by row: Linv(i, j) = -(sum_{k=j}^{i-1} L(i, k) Linv(k, j)) / L(i, i), j < i
-  for (int i = threadIdx.x; i < pw; i += blockDim.x) dv[i] = 1.0 / S[i][i];
-  __syncthreads();
-  for (int i = 1; i < pw; ++i) {
-    for (int j = threadIdx.x; j < i; j += blockDim.x) {
-      double s = S[i][j] * dv[j];
-      for (int k = j + 1; k < i; ++k) s += S[i][k] * S[j][k];  // S[j][k] = Linv(k, j)
-      S[j][i] = -s * dv[i];
-    }
-    __syncthreads();
-  }
-  T* inv = Inv + F.inv0 + (int64_t)(c0 / NB) * NB * NB;  // ld = pw
-  for (int e = threadIdx.x; e < K * K; e += blockDim.x) {
-    const int i = e % K, j = e / K;
-    if (i < pw && j < pw) {
-      inv[(int64_t)j * pw + i] = (T)(i == j ? dv[i] : (i > j ? S[j][i] : 0.0));
-      if (i >= j) base[(int64_t)j * F.ld + i] = (T)S[i][j];
-    }
-  }
-}
-
 // ------------------------------------------------------------------------------------------
-// Blocked variant for panels of 33..NB columns: the block splits into halves of h0 = 64 and
+// The diagonal block of a panel (any width up to NB): it splits into halves of h0 = 64 and
 // h1 = pw - 64 columns (h1 = 0 for panels up to 64 wide),
 //   [A11    ]   [L11    ] [L11^T L21^T]          [X11    ]
 //   [A21 A22] = [L21 L22] [      L22^T],  Linv = [X21 X22],  X21 = -X22 L21 X11,
@@ -557,7 +648,8 @@ __global__ __launch_bounds__(K == NB ? 256 : 64 * (K / 32), 1) void k_potrf_inv(
 // L21 = A21 X11^T, A22 -= L21 L21^T, T = L21 X11, X21 = -X22 T.
 //
 // LDS holds the block in FP64: S[i][j] = L(i, j) for i >= j and Linv(j, i) for i < j, the
-// inverse's diagonal in xd. Same outputs as k_potrf_inv; the pivot test is the reference's.
+// inverse's diagonal in xd. The pivot test is the reference's. Outputs: L11 over the block's lower
+// triangle in L, and Linv (pw x pw, column-major, zeros above the diagonal) in its slot of Inv.
 
 // Wave-local Cholesky and inverse of the h x h block at S[o.., o..] (h <= 64), the wave as an
 // 8 x 8 lane grid: lane (ti, tk) = (lane % 8, lane / 8) holds a[r][c] = A(ti + 8 r, tk + 8 c)
@@ -578,18 +670,17 @@ __device__ __forceinline__ void wpi_steps(double (&a)[8][8], double (&x)[8][8], 
     if (j >= h) return;
     const double p = __shfl(a[CB][CB], cc + 8 * cc);
     const double a0 = __shfl(dl, j);
-    const double d = sqrt(p);
     if (lane == 0 && !bad && (!(p > PIVOT_TOL * a0) || !isfinite(p))) {
       atomicMin(fail, (unsigned long long)(col + j));
       bad = true;
     }
-    const double rd = 1.0 / d;
+    const double d = sqrt(p), rd = 1.0 / d;
     {  // column j of L: rows below scaled, the pivot replaced by d (lanes tk == cc own it)
       const bool own = tk == cc;
 #pragma unroll
       for (int r = CB; r < 8; ++r) {
         const bool below = r > CB || ti > cc, diag = r == CB && ti == cc;
-        a[r][CB] = own ? (below ? a[r][CB] * rd : (diag ? d : a[r][CB])) : a[r][CB];
+        a[r][CB] = own ? (below ? a[r][CB] / d : (diag ? d : a[r][CB])) : a[r][CB];
       }
     }
     {  // row j of X divided by L(j, j) (lanes ti == cc own it)
@@ -700,10 +791,10 @@ __global__ __launch_bounds__(256, 1) void k_potrf_blk(const Item* __restrict__ i
         const double av = S[h0 + min(i, h1 - 1)][k];
         const double xk = xd[k];
 #pragma unroll
-        for (int q = 0; q < 16; ++q) {
+        for (int q = 0; q < 16; ++q) {  // 0/1 weights, not selects: the loads stay unconditional
           const int t = t0 + q;
-          const double sv = S[k][t];
-          acc[q] = fma(av, t > k ? sv : (t == k ? xk : 0.0), acc[q]);
+          const double x = S[k][t] * (t > k ? 1.0 : 0.0) + xk * (t == k ? 1.0 : 0.0);
+          acc[q] = fma(av, x, acc[q]);
         }
       }
       __syncthreads();  // every A21 read before L21 is stored over it
@@ -756,10 +847,10 @@ __global__ __launch_bounds__(256, 1) void k_potrf_blk(const Item* __restrict__ i
       const double l = S[h0 + min(i, h1 - 1)][k];
       const double xk = xd[k];
 #pragma unroll
-      for (int q = 0; q < 16; ++q) {  // X11(k, t) = S[t][k] for k > t
+      for (int q = 0; q < 16; ++q) {  // X11(k, t) = S[t][k] for k > t (0/1 weights)
         const int t = t0 + q;
-        const double sv = S[t][k];
-        tr[q] = fma(l, k > t ? sv : (k == t ? xk : 0.0), tr[q]);
+        const double x = S[t][k] * (k > t ? 1.0 : 0.0) + xk * (k == t ? 1.0 : 0.0);
+        tr[q] = fma(l, x, tr[q]);
       }
     }
     __syncthreads();  // L21 written out and read
@@ -774,10 +865,10 @@ __global__ __launch_bounds__(256, 1) void k_potrf_blk(const Item* __restrict__ i
         const double tk = S[h0 + k][t];
         const double xk = xd[h0 + k];
 #pragma unroll
-        for (int q = 0; q < 16; ++q) {  // X22(ii, k) = S[h0 + k][h0 + ii] for ii > k
+        for (int q = 0; q < 16; ++q) {  // X22(ii, k) = S[h0 + k][h0 + ii] for ii > k (0/1 weights)
           const int ii = i0 + q;
-          const double sv = S[h0 + k][h0 + ii];
-          acc[q] = fma(ii > k ? sv : (ii == k ? xk : 0.0), tk, acc[q]);
+          const double x = S[h0 + k][h0 + ii] * (ii > k ? 1.0 : 0.0) + xk * (ii == k ? 1.0 : 0.0);
+          acc[q] = fma(x, tk, acc[q]);
         }
       }
 #pragma unroll
@@ -802,8 +893,7 @@ __global__ __launch_bounds__(256, 1) void k_potrf_blk(const Item* __restrict__ i
 #pragma unroll
     for (int q = 0; q < NB / 2; ++q) {
       const int j = jp + 2 * q;
-      const double sv = S[j][rr];
-      v[q] = (T)(rr == j ? xr : (rr > j ? sv : 0.0));
+      v[q] = (T)(S[j][rr] * (rr > j ? 1.0 : 0.0) + xr * (rr == j ? 1.0 : 0.0));
     }
 #pragma unroll
     for (int q = 0; q < NB / 2; ++q) {

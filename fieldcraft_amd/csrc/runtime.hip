@@ -103,12 +103,17 @@ struct Program {
   i64 b_size = 0;                      // entries of each BF16 copy
   i64 inv_size = 0;                    // entries of the Inv buffer (kept with the factor)
   i64 part_size = 0;                   // entries of the split-K workspace
+  i64 w_size = 0;                      // entries of the FP64 TRSM workspace
   std::vector<fc::Item> sitems;       // solve work items
   std::vector<fc::Launch> sfwd, sbwd; // forward launches (leaves first), backward (root first)
   std::vector<i64> uoff;              // per front: start of its solve contribution
   i64 u_solve = 0;                    // total contribution length (one right-hand side)
   double work[fc::KINDS][2] = {};     // per kind: flops, entries
 };
+
+// TRSM launches (Launch.pad): the in-place product run by the FP32 and mixed factors, and the
+// FP64 factor's saved-copy, product, residual and correction steps
+enum TrsmVariant : int32_t { TRSM_IN_PLACE = 0, TRSM_COPY = 1, TRSM_PRODUCT = 2, TRSM_RESIDUAL = 3, TRSM_FIX = 4 };
 
 // kinds of solve launches
 enum SolveKind : int32_t { S_GATHER = 0, S_FWD_STEP = 1, S_BWD_GATHER = 2, S_BWD_BELOW = 3, S_BWD_FIRST = 4, S_BWD_STEP = 5 };
@@ -272,23 +277,16 @@ Program build_program(const fc::Symbolic& s) {
         }
         add_gemm(p, tiles, fc::K_UPDATE, l);
       }
-      // diagonal blocks, each to the narrowest kernel variant that holds it
-      for (int variant : {32, (int)fc::NB}) {  // k_potrf_inv<T, 32>, k_potrf_blk
-        i0 = (i64)p.items.size();
-        for (i64 t = 0; t < nl; ++t) {
-          const i64 f = fl[t];
-          if (s.f_w[f] <= c0) continue;
-          const i64 pw = std::min<i64>(fc::NB, s.f_w[f] - c0);
-          const int need = pw <= 32 ? 32 : (int)fc::NB;
-          if (need == variant) {
-            p.items.push_back(Item{(int32_t)f, (int32_t)c0, 0, 0, 0, 0, 0, 0});
-            flops += (double)pw * (pw + 1) * (2 * pw + 1) / 6.0;  // sum_{t <= pw} t^2
-          }
-        }
-        add(fc::K_POTRF, l, i0);
-        if (!p.launches.empty() && p.launches.back().kind == fc::K_POTRF && p.launches.back().item0 == i0)
-          p.launches.back().pad = variant;
+      // the diagonal blocks (k_potrf_blk, any panel width)
+      i0 = (i64)p.items.size();
+      for (i64 t = 0; t < nl; ++t) {
+        const i64 f = fl[t];
+        if (s.f_w[f] <= c0) continue;
+        const i64 pw = std::min<i64>(fc::NB, s.f_w[f] - c0);
+        p.items.push_back(Item{(int32_t)f, (int32_t)c0, 0, 0, 0, 0, 0, 0});
+        flops += (double)pw * (pw + 1) * (2 * pw + 1) / 6.0;  // sum_{t <= pw} t^2
       }
+      add(fc::K_POTRF, l, i0);
       // the rows below the diagonal block: X = B Linv^T, in place
       tiles.clear();
       for (i64 t = 0; t < nl; ++t) {
@@ -314,7 +312,51 @@ Program build_program(const fc::Symbolic& s) {
           tiles.push_back(Tile{g, (double)m * (double)pw * (double)pw});
         }
       }
-      add_gemm(p, tiles, fc::K_TRSM, l);
+      add_gemm(p, tiles, fc::K_TRSM, l);  // in place (FP32 and mixed factors)
+      // The FP64 factor refines the product once, which makes it as accurate as substitution
+      // (a product with an explicit inverse alone is not backward stable): B is saved in the
+      // workspace W (one GB x GB slot per tile), X = W Linv^T, W -= X L11^T, X += W Linv^T.
+      if (!tiles.empty()) {
+        std::vector<Tile> copy, prod, res, fix;
+        for (size_t q = 0; q < tiles.size(); ++q) {
+          const fc::GemmItem& t = tiles[q].g;
+          const i64 f = t.front, w0 = (i64)q * fc::GB * fc::GB;
+          fc::GemmItem g = t;  // W <- B
+          g.c = w0;
+          g.ldc = fc::GB;
+          g.csrc = fc::S_W;
+          copy.push_back(Tile{g, 0.0});
+          g = t;  // X = W Linv^T
+          g.a = w0;
+          g.lda = fc::GB;
+          g.asrc = fc::S_W;
+          prod.push_back(Tile{g, tiles[q].flops});
+          g = t;  // W -= X L11^T
+          g.a = t.c;
+          g.b = s.f_lpos[f] + (i64)((t.c - s.f_lpos[f]) / s.f_ld[f]) * s.f_ld[f] + (t.c - s.f_lpos[f]) / s.f_ld[f];
+          g.ldb = (int32_t)s.f_ld[f];
+          g.bsrc = fc::S_L;
+          g.c = w0;
+          g.ldc = fc::GB;
+          g.csrc = fc::S_W;
+          g.mode = fc::G_SUB;
+          res.push_back(Tile{g, 0.0});
+          g = t;  // X += W Linv^T
+          g.a = w0;
+          g.lda = fc::GB;
+          g.asrc = fc::S_W;
+          g.mode = fc::G_ADD;
+          fix.push_back(Tile{g, 0.0});
+        }
+        p.w_size = std::max<i64>(p.w_size, (i64)tiles.size() * fc::GB * fc::GB);
+        const double before = p.work[fc::K_TRSM][0];
+        int variant = TRSM_COPY;
+        for (auto* list : {&copy, &prod, &res, &fix}) {
+          add_gemm(p, *list, fc::K_TRSM, l);
+          p.launches.back().pad = variant++;
+        }
+        p.work[fc::K_TRSM][0] = before;  // the algorithmic flops are the in-place product's
+      }
       // the finished panel's BF16 copies (run by the mixed form only)
       i0 = (i64)p.items.size();
       for (i64 t = 0; t < nl; ++t) {
@@ -529,6 +571,7 @@ struct fc_factor {
   unsigned long long* fail = nullptr;
   Buffer U;                // update-matrix arena (kept between factorizations)
   Buffer inv, part;        // diagonal-block inverses, split-K pieces
+  Buffer trsm_w;           // FP64: the rows below a panel, saved for the TRSM's refinement step
   Buffer fexp;             // mixed form: per front, the exponent of its power-of-two scale
   Buffer lb;               // mixed form: BF16 copies (hi, then lo) of the finished panels
   hipGraph_t graph = nullptr;          // opts.use_graph bit 0: the captured factorization
@@ -557,6 +600,7 @@ struct fc_factor {
     U.release();
     inv.release();
     part.release();
+    trsm_w.release();
     fexp.release();
     lb.release();
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
@@ -643,6 +687,7 @@ void enqueue_factor(fc_factor* F, const double* dval, i64 nnz, hipStream_t st, b
   const auto& pr = F->sym->prog;
   T* Inv = (T*)F->inv.get((size_t)std::max<i64>(pr.inv_size, 1) * sizeof(T));
   T* Part = (T*)F->part.get((size_t)std::max<i64>(pr.part_size, 1) * sizeof(T));
+  T* W = F->precision == FC_FP64 ? (T*)F->trsm_w.get((size_t)std::max<i64>(pr.w_size, 1) * sizeof(T)) : nullptr;
   fc::k_fill<T><<<blocks_for(s.l_size), 256, 0, st>>>(L, s.l_size, T(0));
   fc::k_scatter_a<T><<<blocks_for(nnz), 256, 0, st>>>(dval, P.a_pos, nnz, L);
   fc::k_gather_diag<<<blocks_for(s.n), 256, 0, st>>>(dval, P.diag_src, s.n, F->diag);
@@ -667,6 +712,7 @@ void enqueue_factor(fc_factor* F, const double* dval, i64 nnz, hipStream_t st, b
       if (eager && F->opts.cancel && *F->opts.cancel) throw fc::Failure{fc::CANCELLED, -1, 0, 0, "cancelled"};
     }
     if (l.kind == fc::K_SPLIT && !mixed) continue;  // the BF16 copies serve the mixed form only
+    if (l.kind == fc::K_TRSM && (l.pad == TRSM_IN_PLACE) == (F->precision == FC_FP64)) continue;
     const fc::Item* it = P.items + l.item0;
     dim3 g(l.count), b(256);
     hipEvent_t a = nullptr, z = nullptr;
@@ -691,7 +737,7 @@ void enqueue_factor(fc_factor* F, const double* dval, i64 nnz, hipStream_t st, b
             break;
           }
         }
-        fc::k_gemm<T><<<g, b, 0, st>>>(P.gitems + l.item0, L, U, Inv, Part);
+        fc::k_gemm<T><<<g, b, 0, st>>>(P.gitems + l.item0, L, U, Inv, Part, W);
         break;
       case fc::K_SPLIT:
         if constexpr (std::is_same<T, float>::value) {
@@ -699,14 +745,14 @@ void enqueue_factor(fc_factor* F, const double* dval, i64 nnz, hipStream_t st, b
         }
         break;
       case fc::K_TRSM:
-        fc::k_gemm<T><<<g, b, 0, st>>>(P.gitems + l.item0, L, U, Inv, Part);
+        if (l.pad == TRSM_COPY) fc::k_copy_tiles<T><<<g, b, 0, st>>>(P.gitems + l.item0, L, W);
+        else fc::k_gemm<T><<<g, b, 0, st>>>(P.gitems + l.item0, L, U, Inv, Part, W);
         break;
       case fc::K_REDUCE:
         fc::k_reduce_parts<T><<<dim3(l.count * fc::RSPLIT), b, 0, st>>>(P.ritems + l.item0, L, U, Part);
         break;
       case fc::K_POTRF:
-        if (l.pad == 32) fc::k_potrf_inv<T, 32><<<g, dim3(64), 0, st>>>(it, P.fronts, L, F->diag, F->fail, Inv);
-        else fc::k_potrf_blk<T><<<g, dim3(256), 0, st>>>(it, P.fronts, L, F->diag, F->fail, Inv);
+        fc::k_potrf_blk<T><<<g, dim3(256), 0, st>>>(it, P.fronts, L, F->diag, F->fail, Inv);
         break;
       default:
         throw fc::Failure{fc::INTERNAL, -1, 0, 0, "unknown launch"};
@@ -739,6 +785,7 @@ void run_factor(fc_factor* F, const double* dval, i64 nnz, hipStream_t st) {
       (void)F->U.get((size_t)std::max<i64>(s.u_size, 1) * sizeof(T));
       (void)F->inv.get((size_t)std::max<i64>(pr.inv_size, 1) * sizeof(T));
       (void)F->part.get((size_t)std::max<i64>(pr.part_size, 1) * sizeof(T));
+      if (F->precision == FC_FP64) (void)F->trsm_w.get((size_t)std::max<i64>(pr.w_size, 1) * sizeof(T));
       if (F->precision == FC_MIXED) {
         (void)F->lb.get(2 * (size_t)std::max<i64>(pr.b_size, 1) * sizeof(__bf16));
         (void)F->fexp.get((size_t)std::max<i64>(s.nf, 1) * sizeof(int32_t));

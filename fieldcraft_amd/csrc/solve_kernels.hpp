@@ -30,41 +30,27 @@ constexpr int BCOLS = 128;    // columns of a backward block-step workgroup
 constexpr int RC = 16;        // right-hand sides of the multi-column kernels
 constexpr int MAX_RHS = 64;   // right-hand sides one solve pass handles (the host splits larger blocks)
 
-// Loads in the solve kernels are made in batches from clamped (always in-bounds) addresses and
-// the out-of-range values are selected away afterwards, so a batch is issued back to back
-// without per-load branches and costs one round of memory latency.
+// Where a thread's loads would otherwise wait on branches, they are made in batches from clamped
+// (always in-bounds) addresses and the out-of-range values are selected away afterwards, so a
+// batch is issued back to back and costs one round of memory latency.
 
-// y[b] = sum_k Inv(b, k) v[k] for b < pw (Inv column-major, ld = pw, zero above the diagonal).
-// Thread (b, h) of the 256 sums the half h of k (loads coalesced along b), then the halves are
-// added in a fixed order. V: LDS, [k][r]; H: LDS scratch for the halves.
+// y[b] = sum_k Inv(b, k) v[k] for b < pw (Inv column-major, ld = pw, zero above the diagonal):
+// thread b of the first 128, loads coalesced along b and independent along k. V: LDS, [k][r].
 template <typename T, int R>
 __device__ __forceinline__ void inv_apply(const T* __restrict__ inv, int pw, const double (*V)[R + 1], int nr,
-                                          double* __restrict__ out, int64_t ldx, double (*H)[SNB][R + 1]) {
-  const int b = threadIdx.x & (SNB - 1), h = threadIdx.x >> 7, bb = min(b, pw - 1);
+                                          double* __restrict__ out, int64_t ldx) {
+  const int b = threadIdx.x;
+  if (b >= pw) return;
   double s[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) s[r] = 0.0;
-  constexpr int KB = R == 1 ? 16 : 4;  // loads per batch
-#pragma unroll(R == 1 ? 4 : 1)
-  for (int k0 = 0; k0 < 64; k0 += KB) {
-    double a[KB];
+#pragma unroll(R == 1 ? 16 : 4)
+  for (int k = 0; k < pw; ++k) {
+    const double a = (double)inv[(int64_t)k * pw + b];
 #pragma unroll
-    for (int q = 0; q < KB; ++q) a[q] = (double)inv[(int64_t)min(64 * h + k0 + q, pw - 1) * pw + bb];
-#pragma unroll
-    for (int q = 0; q < KB; ++q) {
-      const int k = 64 * h + k0 + q;
-      const bool in = k < pw;
-#pragma unroll
-      for (int r = 0; r < R; ++r) s[r] = fma(in ? a[q] : 0.0, V[min(k, SNB - 1)][r], s[r]);
-    }
+    for (int r = 0; r < R; ++r) s[r] = fma(a, V[k][r], s[r]);
   }
-#pragma unroll
-  for (int r = 0; r < R; ++r) H[h][b][r] = s[r];
-  __syncthreads();
-  for (int e = threadIdx.x; e < pw * nr; e += blockDim.x) {
-    const int c = e % pw, r = e / pw;
-    out[(int64_t)r * ldx + c] = H[0][c][r] + H[1][c][r];
-  }
+  for (int r = 0; r < nr; ++r) out[(int64_t)r * ldx + b] = s[r];
 }
 
 // x[b] = sum_k Inv(k, b) z[k] (the transpose): thread (b, h) sums the half h of column b of Inv
@@ -108,7 +94,6 @@ __global__ __launch_bounds__(256) void k_fwd_gather(const Item* __restrict__ ite
                                                     const T* __restrict__ Inv, double* v, double* __restrict__ y,
                                                     int64_t ldx, int nrhs, double* u, const int64_t* __restrict__ uoff) {
   __shared__ double Vb[SNB][R + 1];
-  __shared__ double H[2][SNB][R + 1];
   const int f = items[blockIdx.x].f;
   const FrontDev F = fronts[f];
   const int w = F.w, mu = F.m - F.w, pw = min(SNB, w);
@@ -135,7 +120,7 @@ __global__ __launch_bounds__(256) void k_fwd_gather(const Item* __restrict__ ite
     for (int r = 0; r < nr; ++r)
       for (int k = threadIdx.x; k < pw; k += blockDim.x) Vb[k][r] = v[(int64_t)(q0 + r) * ldx + F.col0 + k];
     __syncthreads();
-    inv_apply<T, R>(Inv + F.inv0, pw, Vb, nr, y + (int64_t)q0 * ldx + F.col0, ldx, H);
+    inv_apply<T, R>(Inv + F.inv0, pw, Vb, nr, y + (int64_t)q0 * ldx + F.col0, ldx);
     __syncthreads();
   }
 }
@@ -151,7 +136,6 @@ __global__ __launch_bounds__(256) void k_fwd_step(const Item* __restrict__ items
                                                   const int64_t* __restrict__ uoff) {
   __shared__ double Yb[SNB][R + 1];
   __shared__ double Vn[SNB][R + 1];
-  __shared__ double H[2][SNB][R + 1];
   const Item it = items[blockIdx.x];
   const FrontDev F = fronts[it.f];
   const int c0 = it.a, pw = min(SNB, F.w - c0), c1 = c0 + pw, r0 = it.b;
@@ -170,17 +154,11 @@ __global__ __launch_bounds__(256) void k_fwd_step(const Item* __restrict__ items
       double s[R];
 #pragma unroll
       for (int r = 0; r < R; ++r) s[r] = 0.0;
-      constexpr int JB = R == 1 ? 16 : 4;  // independent loads per batch
-      for (int j0 = 0; j0 < pw; j0 += JB) {
-        double l[JB];
+#pragma unroll(R == 1 ? 16 : 4)
+      for (int j = 0; j < pw; ++j) {
+        const double l = (double)lf[(int64_t)j * F.ld + i];
 #pragma unroll
-        for (int q = 0; q < JB; ++q) l[q] = (double)lf[(int64_t)min(j0 + q, pw - 1) * F.ld + i];
-#pragma unroll
-        for (int q = 0; q < JB; ++q) {
-          const bool in = j0 + q < pw;  // Yb holds pw rows only
-#pragma unroll
-          for (int r = 0; r < R; ++r) s[r] = fma(in ? l[q] : 0.0, Yb[min(j0 + q, SNB - 1)][r], s[r]);
-        }
+        for (int r = 0; r < R; ++r) s[r] = fma(l, Yb[j][r], s[r]);
       }
       if (i < F.w) {
         for (int r = 0; r < nr; ++r) {
@@ -196,7 +174,7 @@ __global__ __launch_bounds__(256) void k_fwd_step(const Item* __restrict__ items
     if (it.c) {
       __syncthreads();
       inv_apply<T, R>(Inv + F.inv0 + (int64_t)(c1 / SNB) * SNB * SNB, pn, Vn, nr,
-                      y + (int64_t)q0 * ldx + F.col0 + c1, ldx, H);
+                      y + (int64_t)q0 * ldx + F.col0 + c1, ldx);
     }
     __syncthreads();
   }
