@@ -83,6 +83,36 @@ def run_leg(CH, sym, dvals, A, b_ptr, x_ptr, n, precision, steps, warmup, profil
     return out
 
 
+def traffic_from_profiles(kernel: str, workload: str):
+    """HBM bytes per launch of `kernel` on `workload`, as recorded in a committed rocprofv3 --pmc
+    summary under profiles/ (a line `traffic: <kernel> | <workload> | <bytes> bytes per launch`,
+    FETCH_SIZE + WRITE_SIZE scaled by the plain-copy calibration); None when no summary covers it."""
+    import glob
+    import re
+
+    pat = re.compile(r"^traffic: (.+?) \| (.+?) \| ([0-9.eE+]+) bytes per launch")
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        for ln in open(path):
+            m = pat.match(ln.strip())
+            if m and m.group(1) == kernel and m.group(2) == workload:
+                return float(m.group(3))
+    return None
+
+
+def calibrate_copy(gib: int = 1):
+    """A plain elementwise copy of a known size (torch.neg: reads and writes `gib` GiB once each),
+    run under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE to scale those counters to bytes."""
+    import torch
+
+    n = gib * (1 << 30) // 4
+    a = torch.ones(n, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        torch.neg(a, out=b)
+    torch.cuda.synchronize()
+    print(json.dumps(dict(calibration="torch.neg fp32", bytes_read=n * 4, bytes_written=n * 4, launches=3)))
+
+
 def cpu_baseline(cells: int, threads: int):
     from fieldcraft_amd import problems as P
     from oracle import oracle as O
@@ -123,6 +153,7 @@ def main():
     ap.add_argument("--workload", default="cube", choices=["cube", "batch", "shift"])
     ap.add_argument("--models", type=int, default=512)
     ap.add_argument("--tet-cells", type=int, default=50)
+    ap.add_argument("--calibrate-copy", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU: the factorization of one matrix is not split across GPUs here "
@@ -138,6 +169,8 @@ def main():
         print("no GPU", file=sys.stderr)
         sys.exit(1)
     torch.cuda.init()
+    if args.calibrate_copy:
+        return calibrate_copy()
     if args.workload == "batch":
         return batch_workload(args, CH, P)
     if args.workload == "shift":
@@ -176,7 +209,9 @@ def main():
         log(f"bench: {form} {legs[form]}")
     head = legs[form]
     F = info["flops"]
-    t_step = head["factor_s"] + head["solve_s"]
+    # a step's time is the wall clock around the timed steps (synchronized on both sides), which
+    # includes the host's launch work; the per-kernel figures below come from HIP events
+    t_step = head["wall_s"]
     # the dominant kernel function of the headline leg's factorization (kinds grouped by the
     # kernel that runs them, so the figure matches a rocprofv3 per-kernel summary)
     el = "double" if form == "fp64" else "float"
@@ -201,6 +236,7 @@ def main():
             g[key] += v[key]
     dom = max((k for k in funcs if funcs[k]["bound"] != "latency"), key=lambda k: funcs[k]["seconds"])
     g = funcs[dom]
+    traffic = traffic_from_profiles(dom, f"hex8 {args.cells}^3 {form}")
     avg = g["seconds"] / max(g["launches"], 1)
     if g["bound"] == "hbm":
         es = 8 if form == "fp64" else 4
@@ -210,12 +246,12 @@ def main():
             per_launch = g["entries"] * (3 * es + 4) / max(g["launches"], 1)  # child entry read, target RMW, row map
         achieved = per_launch / avg / 1e9
         roof = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK, unit="GB/s", frac=achieved / HBM_PEAK,
-                    traffic=None, kernel=dom, kinds=g["kinds"], launches=g["launches"], bytes_per_launch=per_launch,
+                    traffic=traffic, kernel=dom, kinds=g["kinds"], launches=g["launches"], bytes_per_launch=per_launch,
                     avg_launch_s=avg)
     else:
         achieved = g["flops"] / g["seconds"] / 1e12
         roof = dict(bound="mfma", achieved=achieved, peak=g["peak"], unit="TFLOP/s", frac=achieved / g["peak"],
-                    traffic=None, kernel=dom, kinds=g["kinds"], launches=g["launches"],
+                    traffic=traffic, kernel=dom, kinds=g["kinds"], launches=g["launches"],
                     flops_per_launch=g["flops"] / max(g["launches"], 1), avg_launch_s=avg,
                     mfma_products_per_flop=args.bf16_terms if "bf16" in dom else 1)
     line = dict(metric=METRIC, value=F / t_step / 1e12, unit="TFLOP/s", n_gpus=1, steps=args.steps,
@@ -230,12 +266,13 @@ def main():
                 roofline=roof)
     if "fp64" in legs:
         f64 = legs["fp64"]
-        t64 = f64["factor_s"] + f64["solve_s"]
-        line["fp64"] = dict(factor_s=f64["factor_s"], solve_s=f64["solve_s"], tflops_factor=F / f64["factor_s"] / 1e12,
+        t64 = f64["wall_s"]
+        line["fp64"] = dict(factor_s=f64["factor_s"], solve_s=f64["solve_s"], wall_s=f64["wall_s"], tflops_factor=F / f64["factor_s"] / 1e12,
                             frac_fp64_peak=F / f64["factor_s"] / 1e12 / PEAK["fp64"], berr=f64["berr"],
                             iterations=f64["iterations"])
         line["speedup_vs_fp64"] = t64 / t_step
-    line[form] = dict(factor_s=head["factor_s"], solve_s=head["solve_s"], iterations=head["iterations"],
+    line[form] = dict(factor_s=head["factor_s"], solve_s=head["solve_s"], wall_s=head["wall_s"],
+                      iterations=head["iterations"],
                       berr=head["berr"], converged=head["converged"], rel_err_vs_fp64=head.get("rel_err_vs_fp64"))
     line["kernels"] = {leg: {k: dict(seconds=v["seconds"], launches=v["launches"],
                                      tflops=(v["flops"] / v["seconds"] / 1e12) if v["seconds"] and v["flops"] else None)
