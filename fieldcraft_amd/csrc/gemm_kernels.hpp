@@ -644,7 +644,8 @@ constexpr double PIVOT_TOL = 1e-10;  // cholesky.rs:48
 //   [A21 A22] = [L21 L22] [      L22^T],  Linv = [X21 X22],  X21 = -X22 L21 X11,
 // Each half is factored and inverted inside one wave (wave_potrf_inv): the column values pass
 // between lanes by shuffles, so the 128 sequential column steps need no workgroup barrier and
-// wait on no memory. Everything else is a small product over LDS with independent loads:
+// wait on no memory. Everything else is a 64 x 64 x 64 product on the FP64 MFMA with its
+// operands read from LDS (strip64, a 64 x 16 strip per wave):
 // L21 = A21 X11^T, A22 -= L21 L21^T, T = L21 X11, X21 = -X22 T.
 //
 // LDS holds the block in FP64: S[i][j] = L(i, j) for i >= j and Linv(j, i) for i < j, the
@@ -753,6 +754,25 @@ __device__ inline void wave_potrf_inv(double (*S)[NB + 1], double* xd, int o, in
     }
 }
 
+// One wave's 64 x 16 strip of a product over k in [0, 64) on the FP64 MFMA:
+//   acc[ib][v] += sum_k col(t, k) row(i, k),  i = 16 ib + lane % 16,  t = (lane / 16) + 4 v
+// where col(r, k) and row(i, k) return in-bounds LDS values (col's r is the strip column in
+// [0, 16)). Each k-step reads one column value and four row values per lane for four MFMAs.
+template <class FC, class FR>
+__device__ __forceinline__ void strip64(v4d (&acc)[4], int lane, FC col, FR row) {
+  const int r = lane & 15, q = lane >> 4;
+#pragma unroll 4
+  for (int k0 = 0; k0 < 64; k0 += 4) {
+    const int k = k0 + q;
+    const double x = col(r, k);
+    double y[4];
+#pragma unroll
+    for (int ib = 0; ib < 4; ++ib) y[ib] = row(16 * ib + r, k);
+#pragma unroll
+    for (int ib = 0; ib < 4; ++ib) acc[ib] = Mma<double>::step(x, y[ib], acc[ib]);
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256, 1) void k_potrf_blk(const Item* __restrict__ items, const FrontDev* __restrict__ fronts,
                                                       T* __restrict__ L, const double* __restrict__ diag,
@@ -784,96 +804,87 @@ __global__ __launch_bounds__(256, 1) void k_potrf_blk(const Item* __restrict__ i
   if (wave == 0) wave_potrf_inv(S, xd, 0, h0, lane, dg, col, fail, bad);  // L11, X11
   __syncthreads();
   if (h1 > 0) {
-    const int i = lane, t0 = 16 * wave;  // row i of the lower half, columns t0 + [0, 16)
+    // each wave: strip columns t0 + [0, 16) of a 64 x 64 product, rows 16 ib + lane % 16
+    const int r = lane & 15, q = lane >> 4, t0 = 16 * wave;
     {  // L21 = A21 X11^T: L21(i, t) = sum_{k <= t} A21(i, k) X11(t, k), X11(t, k) = S[k][t] (t > k)
-      double acc[16] = {};
-      for (int k = 0; k < h0; ++k) {
-        const double av = S[h0 + min(i, h1 - 1)][k];
-        const double xk = xd[k];
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {  // 0/1 weights, not selects: the loads stay unconditional
-          const int t = t0 + q;
-          const double x = S[k][t] * (t > k ? 1.0 : 0.0) + xk * (t == k ? 1.0 : 0.0);
-          acc[q] = fma(av, x, acc[q]);
-        }
-      }
+      v4d acc[4] = {};
+      strip64(
+          acc, lane,
+          [&](int c, int k) {  // 0/1 weights, not selects: the loads stay unconditional
+            const int t = t0 + c;
+            return S[k][t] * (t > k ? 1.0 : 0.0) + xd[k] * (t == k ? 1.0 : 0.0);
+          },
+          [&](int i, int k) { return S[h0 + min(i, h1 - 1)][k]; });
       __syncthreads();  // every A21 read before L21 is stored over it
-      if (i < h1)
 #pragma unroll
-        for (int q = 0; q < 16; ++q) S[h0 + i][t0 + q] = acc[q];
+      for (int ib = 0; ib < 4; ++ib)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int i = 16 * ib + r;
+          if (i < h1) S[h0 + i][t0 + q + 4 * v] = acc[ib][v];
+        }
     }
     __syncthreads();
-    {  // A22 -= L21 L21^T, a 4 x 4 block of the lower triangle per thread
-      const int bi = tid & 15, bj = tid >> 4;
-      double acc[4][4] = {};
-      for (int k = 0; k < h0; ++k) {
-        double u[4], v[4];
+    {  // A22 -= L21 L21^T over the lower triangle (reads the left half, writes the right)
+      v4d acc[4] = {};
+      strip64(
+          acc, lane, [&](int c, int k) { return S[h0 + min(t0 + c, h1 - 1)][k]; },
+          [&](int i, int k) { return S[h0 + min(i, h1 - 1)][k]; });
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          u[q] = S[h0 + min(4 * bi + q, h1 - 1)][k];
-          v[q] = S[h0 + min(4 * bj + q, h1 - 1)][k];
+      for (int ib = 0; ib < 4; ++ib)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int i = 16 * ib + r, j = t0 + q + 4 * v;
+          if (i < h1 && j <= i) S[h0 + i][h0 + j] -= acc[ib][v];
         }
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-#pragma unroll
-          for (int r = 0; r < 4; ++r) acc[q][r] = fma(u[q], v[r], acc[q][r]);
-      }
-      if (bi >= bj)
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int ii = 4 * bi + q, jj = 4 * bj + r;
-            if (ii < h1 && ii >= jj) S[h0 + ii][h0 + jj] -= acc[q][r];
-          }
     }
     __syncthreads();
     if (wave == 0) wave_potrf_inv(S, xd, h0, h1, lane, dg + h0, col + h0, fail, bad);  // L22, X22
     __syncthreads();
-    // L is final: write it out (reads first, then stores); T = L21 X11 (row i, columns
-    // t0 + [0, 16)) in registers
+    // L is final: write it out (reads first, then stores)
     {
       T v[NB / 2];
 #pragma unroll
-      for (int q = 0; q < NB / 2; ++q) v[q] = (T)S[rr][jp + 2 * q];
+      for (int qq = 0; qq < NB / 2; ++qq) v[qq] = (T)S[rr][jp + 2 * qq];
 #pragma unroll
-      for (int q = 0; q < NB / 2; ++q) {
-        const int j = jp + 2 * q;
-        if (rr < pw && j <= rr) base[(int64_t)j * F.ld + rr] = v[q];
+      for (int qq = 0; qq < NB / 2; ++qq) {
+        const int j = jp + 2 * qq;
+        if (rr < pw && j <= rr) base[(int64_t)j * F.ld + rr] = v[qq];
       }
     }
-    double tr[16] = {};
-    for (int k = 0; k < h0; ++k) {
-      const double l = S[h0 + min(i, h1 - 1)][k];
-      const double xk = xd[k];
+    {  // T = L21 X11: T(i, t) = sum_{k >= t} L21(i, k) X11(k, t), X11(k, t) = S[t][k] (k > t)
+      v4d acc[4] = {};
+      strip64(
+          acc, lane,
+          [&](int c, int k) {
+            const int t = t0 + c;
+            return S[t][k] * (k > t ? 1.0 : 0.0) + xd[k] * (k == t ? 1.0 : 0.0);
+          },
+          [&](int i, int k) { return S[h0 + min(i, h1 - 1)][k]; });
+      __syncthreads();  // L21 written out and read
 #pragma unroll
-      for (int q = 0; q < 16; ++q) {  // X11(k, t) = S[t][k] for k > t (0/1 weights)
-        const int t = t0 + q;
-        const double x = S[t][k] * (k > t ? 1.0 : 0.0) + xk * (k == t ? 1.0 : 0.0);
-        tr[q] = fma(l, x, tr[q]);
-      }
-    }
-    __syncthreads();  // L21 written out and read
-    if (i < h1)
+      for (int ib = 0; ib < 4; ++ib)
 #pragma unroll
-      for (int q = 0; q < 16; ++q) S[h0 + i][t0 + q] = tr[q];  // T over L21
-    __syncthreads();
-    {  // X21 = -X22 T: column t = lane, rows i0 + [0, 16), stored at S[t][h0 + i]
-      const int t = lane, i0 = 16 * wave;
-      double acc[16] = {};
-      for (int k = 0; k < h1; ++k) {
-        const double tk = S[h0 + k][t];
-        const double xk = xd[h0 + k];
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {  // X22(ii, k) = S[h0 + k][h0 + ii] for ii > k (0/1 weights)
-          const int ii = i0 + q;
-          const double x = S[h0 + k][h0 + ii] * (ii > k ? 1.0 : 0.0) + xk * (ii == k ? 1.0 : 0.0);
-          acc[q] = fma(x, tk, acc[q]);
+        for (int v = 0; v < 4; ++v) {
+          const int i = 16 * ib + r;
+          if (i < h1) S[h0 + i][t0 + q + 4 * v] = acc[ib][v];  // T over L21
         }
-      }
+    }
+    __syncthreads();
+    {  // X21 = -X22 T, stored at S[t][h0 + ii]: X22(ii, k) = S[h0 + k][h0 + ii] (ii > k), T(k, t) = S[h0 + k][t]
+      v4d acc[4] = {};
+      strip64(
+          acc, lane, [&](int c, int k) { return S[h0 + k][t0 + c]; },
+          [&](int ii, int k) {
+            return S[h0 + k][h0 + ii] * (ii > k ? 1.0 : 0.0) + xd[h0 + k] * (ii == k ? 1.0 : 0.0);
+          });
 #pragma unroll
-      for (int q = 0; q < 16; ++q)
-        if (i0 + q < h1) S[t][h0 + i0 + q] = -acc[q];
+      for (int ib = 0; ib < 4; ++ib)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int ii = 16 * ib + r;
+          if (ii < h1) S[t0 + q + 4 * v][h0 + ii] = -acc[ib][v];
+        }
     }
   } else {
     T v[NB / 2];
