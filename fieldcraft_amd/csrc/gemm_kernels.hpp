@@ -187,13 +187,12 @@ __global__ __launch_bounds__(256) void k_copy_tiles(const GemmItem* __restrict__
 //
 // The BF16 copies are made once per panel, when it is finished (k_split_panel), in a k-chunked
 // layout: 32 consecutive columns of a row are contiguous, so a 128-row x 32-column stage of an
-// operand is one contiguous 8 KB block per copy, loaded as 16-byte vectors and stored to LDS as
-// it is; each lane then reads its 8 k of a fragment as one 16-byte LDS load. Tiles, modes and
-// split-k pieces are those of k_gemm (128 x 128, 4 waves of 64 x 64).
+// operand is one contiguous 8 KB block per copy. The product kernel moves it into LDS with the
+// direct global-to-LDS loads (no registers on the way), 1 KB (16 rows) per wave instruction,
+// while the previous stage is multiplied. Tiles, modes and split-k pieces are those of k_gemm
+// (128 x 128, 4 waves of 64 x 64).
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
-typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-constexpr int BK = 32;        // k per stage: one MFMA depth, one chunk of the copies
-constexpr int BROW = BK + 8;  // LDS row stride (80 bytes: 16-byte aligned, rows spread over the banks)
+constexpr int BK = 32;  // k per stage: one MFMA depth, one chunk of the copies
 
 // A finished panel [c0, c0 + NB) of a front into the BF16 copies, rows [r0, r0 + 256) (at or
 // below c0: the rows any later product reads). Columns past the front's width up to the next
@@ -227,14 +226,40 @@ __global__ __launch_bounds__(256) void k_split_panel(const Item* __restrict__ it
   }
 }
 
+// LDS address of a shared-memory pointer, and a 16-byte LDS read issued as an instruction the
+// compiler does not track: it would otherwise wait for every direct global-to-LDS load in
+// flight before any LDS read. The caller waits (lgkm_wait) before using the values.
+__device__ __forceinline__ uint32_t lds_addr(const void* p) {
+  return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void*)p;
+}
+template <int OFF>
+__device__ __forceinline__ v8bf lds_read16(uint32_t a) {
+  v8bf r;
+  asm volatile("ds_read_b128 %0, %1 offset:%2" : "=v"(r) : "v"(a), "i"(OFF));
+  return r;
+}
+
+// A product's tile with both operands staged through LDS by direct loads, two stages deep (64
+// KB for three terms: two workgroups share a CU). Stage s lives in buffer s % 2: it is loaded
+// while stage s - 1 is multiplied, then the workgroup waits for its own loads and meets at a
+// barrier (which also frees buffer (s - 1) % 2 for stage s + 1).
+//
+// Stage buffer: copy t of A (128 rows x 32 k), then copy t of B. A row's four 16-byte chunks sit
+// in the order chunk ^ (row / 4 % 4), so the 16 rows of one fragment read spread over all
+// banks; a wave's direct load fills 1 KB = 16 rows, lane l taking the chunk of LDS slot l.
+// Rows past the tile's m (n) load the last row of the tile (in bounds); their products land in
+// entries that are not stored. The copies are zero past the front's width, so the last stage
+// needs no k bound.
 template <int TERMS>
 __global__ __launch_bounds__(256, 2) void k_gemm_b16(const GemmItem* __restrict__ items, const FrontDev* __restrict__ fronts,
                                                      const __bf16* __restrict__ Lh, const __bf16* __restrict__ Ll,
                                                      float* L, float* U, float* Part, const int32_t* __restrict__ fexp) {
   static_assert(TERMS == 1 || TERMS == 3, "one or three BF16 products");
-  constexpr int NS = TERMS == 3 ? 2 : 1;  // hi (and lo) copies of each operand
-  __shared__ __attribute__((aligned(16))) __bf16 As[2][NS][GB][BROW];
-  __shared__ __attribute__((aligned(16))) __bf16 Bs[2][NS][GB][BROW];
+  constexpr int NS = TERMS == 3 ? 2 : 1;       // hi (and lo) copies of each operand
+  constexpr int CE = GB * BK;                  // elements of one copy of one operand per stage
+  constexpr int SE = 2 * NS * CE;              // elements of a stage buffer
+  constexpr int PW = 2 * NS * (GB / 16) / 4;   // 1 KB direct loads per wave per stage
+  __shared__ __attribute__((aligned(16))) __bf16 S[2 * SE];
   const GemmItem it = items[blockIdx.x];
   const FrontDev F = fronts[it.front];
   const int e = fexp[it.front];
@@ -243,39 +268,20 @@ __global__ __launch_bounds__(256, 2) void k_gemm_b16(const GemmItem* __restrict_
   const int64_t chunk = (int64_t)F.ld * BK;  // elements from one k-chunk of a row to the next
   const int64_t a0 = F.bpos + (oa / F.ld) / BK * chunk + (oa % F.ld) * BK;
   const int64_t b0 = F.bpos + (ob / F.ld) / BK * chunk + (ob % F.ld) * BK;
-  auto base = [&](int8_t src) { return src == S_L ? L : (src == S_U ? U : Part); };
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wi = (wave >> 1) * 64, wj = (wave & 1) * 64;
-  // loader: 16-byte vectors v = tid + 256 q of a stage (row v / 4, k 8 (v % 4) + [0, 8)), two
-  // register sets, so a stage's loads are issued two stages before it is stored to LDS
-  u32x4 ra[2][2][NS], rb[2][2][NS];
-  auto load = [&](int set, int s) {
+  const int lrow = lane >> 2, lch = (lane & 3) ^ (lane >> 4);  // the row and chunk slot l holds
+  auto issue = [&](int s, int buf) {
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      const int v = tid + 256 * q, row = v >> 2, part = v & 3;
-      // rows past the tile read the last row (in bounds) and are zeroed after the load
-      const int64_t oa2 = a0 + s * chunk + min(row, it.m - 1) * BK + part * 8;
-      const int64_t ob2 = b0 + s * chunk + min(row, it.n - 1) * BK + part * 8;
-      const u32x4 z = {0, 0, 0, 0};
-      const u32x4 ah = *(const u32x4*)(Lh + oa2), bh = *(const u32x4*)(Lh + ob2);
-      ra[set][q][0] = row < it.m ? ah : z;
-      rb[set][q][0] = row < it.n ? bh : z;
-      if (TERMS == 3) {
-        const u32x4 al = *(const u32x4*)(Ll + oa2), bl = *(const u32x4*)(Ll + ob2);
-        ra[set][q][NS - 1] = row < it.m ? al : z;
-        rb[set][q][NS - 1] = row < it.n ? bl : z;
-      }
-    }
-  };
-  auto store = [&](int set, int buf) {
-#pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      const int v = tid + 256 * q, row = v >> 2, part = v & 3;
-#pragma unroll
-      for (int t = 0; t < NS; ++t) {
-        *(u32x4*)&As[buf][t][row][part * 8] = ra[set][q][t];
-        *(u32x4*)&Bs[buf][t][row][part * 8] = rb[set][q][t];
-      }
+    for (int q = 0; q < PW; ++q) {
+      const int g = wave + 4 * q;            // which 1 KB block of the stage: operand, copy, rows
+      const int op = g / (NS * GB / 16), t = g / (GB / 16) % NS, rb = g % (GB / 16);
+      const int row = rb * 16 + lrow;
+      const int64_t src = (op == 0 ? a0 + (int64_t)min(row, it.m - 1) * BK
+                                   : b0 + (int64_t)min(row, it.n - 1) * BK) + s * chunk + lch * 8;
+      const __bf16* gp = (t == 0 ? Lh : Ll) + src;
+      __bf16* lp = S + buf * SE + (op * NS + t) * CE + rb * 16 * BK;
+      __builtin_amdgcn_global_load_lds((const void*)gp, (__attribute__((address_space(3))) void*)lp, 16, 0, 0);
     }
   };
   v4f acc[4][4];
@@ -284,28 +290,39 @@ __global__ __launch_bounds__(256, 2) void k_gemm_b16(const GemmItem* __restrict_
 #pragma unroll
     for (int bj = 0; bj < 4; ++bj) acc[bi][bj] = v4f{0, 0, 0, 0};
   const int stages = (it.k + BK - 1) / BK;
-  const int fr = lane & 15, fk = 8 * (lane >> 4);  // a lane's fragment row and first k
-  auto mul = [&](int cur) {
-    v8bf fj[4], fi[4];
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      fj[b] = *(const v8bf*)&Bs[cur][0][wj + 16 * b + fr][fk];
-      fi[b] = *(const v8bf*)&As[cur][0][wi + 16 * b + fr][fk];
-    }
+  // a lane's fragment: row lane % 16 of a 16-row block, k chunk lane / 16 (byte offset in a copy)
+  const int fr = lane & 15;
+  const uint32_t foff = (uint32_t)(fr * 4 + ((lane >> 4) ^ (fr >> 2))) * 16;
+  const uint32_t sbase = lds_addr(S);
+  auto mul = [&](int buf) {
+    const uint32_t ah = sbase + (uint32_t)(buf * SE + wi * BK) * 2 + foff;
+    const uint32_t bh = sbase + (uint32_t)(buf * SE + NS * CE + wj * BK) * 2 + foff;
+    v8bf fi[4], fj[4];
+    fi[0] = lds_read16<0>(ah); fi[1] = lds_read16<1024>(ah); fi[2] = lds_read16<2048>(ah); fi[3] = lds_read16<3072>(ah);
+    fj[0] = lds_read16<0>(bh); fj[1] = lds_read16<1024>(bh); fj[2] = lds_read16<2048>(bh); fj[3] = lds_read16<3072>(bh);
     if (TERMS == 3) {
-      v8bf gj[4], gi[4];
-#pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        gj[b] = *(const v8bf*)&Bs[cur][NS - 1][wj + 16 * b + fr][fk];
-        gi[b] = *(const v8bf*)&As[cur][NS - 1][wi + 16 * b + fr][fk];
-      }
+      const uint32_t al = ah + CE * 2, bl = bh + CE * 2;
+      v8bf gi[4], gj[4];
+      gi[0] = lds_read16<0>(al); gi[1] = lds_read16<1024>(al); gi[2] = lds_read16<2048>(al); gi[3] = lds_read16<3072>(al);
+      gj[0] = lds_read16<0>(bl); gj[1] = lds_read16<1024>(bl); gj[2] = lds_read16<2048>(bl); gj[3] = lds_read16<3072>(bl);
+      asm volatile("s_waitcnt lgkmcnt(0)"  // the fragments are in registers
+                   : "+v"(fi[0]), "+v"(fi[1]), "+v"(fi[2]), "+v"(fi[3]), "+v"(fj[0]), "+v"(fj[1]), "+v"(fj[2]),
+                     "+v"(fj[3]), "+v"(gi[0]), "+v"(gi[1]), "+v"(gi[2]), "+v"(gi[3]), "+v"(gj[0]), "+v"(gj[1]),
+                     "+v"(gj[2]), "+v"(gj[3]));
 #pragma unroll
       for (int bi = 0; bi < 4; ++bi)
 #pragma unroll
-        for (int bj = 0; bj < 4; ++bj) {
+        for (int bj = 0; bj < 4; ++bj)
           acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(gj[bj], fi[bi], acc[bi][bj], 0, 0, 0);
+#pragma unroll
+      for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 4; ++bj)
           acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fj[bj], gi[bi], acc[bi][bj], 0, 0, 0);
-        }
+    } else {
+      asm volatile("s_waitcnt lgkmcnt(0)"
+                   : "+v"(fi[0]), "+v"(fi[1]), "+v"(fi[2]), "+v"(fi[3]), "+v"(fj[0]), "+v"(fj[1]), "+v"(fj[2]),
+                     "+v"(fj[3]));
     }
 #pragma unroll
     for (int bi = 0; bi < 4; ++bi)
@@ -313,265 +330,13 @@ __global__ __launch_bounds__(256, 2) void k_gemm_b16(const GemmItem* __restrict_
       for (int bj = 0; bj < 4; ++bj)
         acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fj[bj], fi[bi], acc[bi][bj], 0, 0, 0);
   };
-  // stage s lives in register set s % 2 from its load until it is stored to LDS buffer s % 2
-  load(0, 0);
-  if (stages > 1) load(1, 1);
-  store(0, 0);
-  __syncthreads();
-  for (int s = 0; s < stages; s += 2) {
-    if (s + 2 < stages) load(0, s + 2);
-    mul(0);
-    if (s + 1 < stages) store(1, 1);
-    __syncthreads();
-    if (s + 1 < stages) {
-      if (s + 3 < stages) load(1, s + 3);
-      mul(1);
-      if (s + 2 < stages) store(0, 0);
-      __syncthreads();
-    }
-  }
-  // acc[bi][bj][v] = C(wi + 16 bi + lane % 16, wj + 16 bj + 4 (lane / 16) + v) / sc^2
-  const float unscale = ldexpf(1.0f, 2 * e);
-  const int il = lane & 15;
-  if (it.mode == G_PART) {
-    float* P = Part + it.c;
-#pragma unroll
-    for (int bi = 0; bi < 4; ++bi)
-#pragma unroll
-      for (int bj = 0; bj < 4; ++bj)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int i = wi + 16 * bi + il, j = wj + 16 * bj + Mma<float>::row(lane, v);
-          P[(int64_t)j * GB + i] = acc[bi][bj][v] * unscale;
-        }
-    return;
-  }
-  float* C = base(it.csrc) + it.c;
-#pragma unroll
-  for (int bi = 0; bi < 4; ++bi)
-#pragma unroll
-    for (int bj = 0; bj < 4; ++bj)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int i = wi + 16 * bi + il, j = wj + 16 * bj + Mma<float>::row(lane, v);
-        if (i < it.m && j < it.n && i - j + it.diag >= 0) {
-          float* p = C + (int64_t)j * it.ldc + i;
-          const float x = acc[bi][bj][v] * unscale;
-          if (it.mode == G_SUB) *p -= x;
-          else if (it.mode == G_NEG) *p = -x;
-          else *p = x;
-        }
-      }
-}
-
-// The same products with two waves per 128 x 128 tile, each wave a 128 x 64 half as 8 x 4
-// blocks of the 16x16x32 MFMA: every fragment read from LDS feeds twice as many products, and
-// two workgroups share a CU.
-template <int TERMS>
-__global__ __launch_bounds__(128, 2) void k_gemm_b16w(const GemmItem* __restrict__ items, const FrontDev* __restrict__ fronts,
-                                                      const __bf16* __restrict__ Lh, const __bf16* __restrict__ Ll,
-                                                      float* L, float* U, float* Part, const int32_t* __restrict__ fexp) {
-  static_assert(TERMS == 1 || TERMS == 3, "one or three BF16 products");
-  constexpr int NS = TERMS == 3 ? 2 : 1;
-  __shared__ __attribute__((aligned(16))) __bf16 As[2][NS][GB][BROW];
-  __shared__ __attribute__((aligned(16))) __bf16 Bs[2][NS][GB][BROW];
-  const GemmItem it = items[blockIdx.x];
-  const FrontDev F = fronts[it.front];
-  const int e = fexp[it.front];
-  const int64_t oa = it.a - F.lpos, ob = it.b - F.lpos;
-  const int64_t chunk = (int64_t)F.ld * BK;
-  const int64_t a0 = F.bpos + (oa / F.ld) / BK * chunk + (oa % F.ld) * BK;
-  const int64_t b0 = F.bpos + (ob / F.ld) / BK * chunk + (ob % F.ld) * BK;
-  auto base = [&](int8_t src) { return src == S_L ? L : (src == S_U ? U : Part); };
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wj = wave * 64;  // the wave's 64 columns; all 128 rows
-  // loader: 16-byte vectors v = tid + 128 q (q < 4) of a stage (row v / 4, k 8 (v % 4) + [0, 8))
-  u32x4 ra[4][NS], rb[4][NS];
-  auto load = [&](int s) {
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int v = tid + 128 * q, row = v >> 2, part = v & 3;
-      // rows past the tile read the last row (in bounds) and are zeroed after the load
-      const int64_t oa2 = a0 + s * chunk + min(row, it.m - 1) * BK + part * 8;
-      const int64_t ob2 = b0 + s * chunk + min(row, it.n - 1) * BK + part * 8;
-      const u32x4 z = {0, 0, 0, 0};
-      const u32x4 ah = *(const u32x4*)(Lh + oa2), bh = *(const u32x4*)(Lh + ob2);
-      ra[q][0] = row < it.m ? ah : z;
-      rb[q][0] = row < it.n ? bh : z;
-      if (TERMS == 3) {
-        const u32x4 al = *(const u32x4*)(Ll + oa2), bl = *(const u32x4*)(Ll + ob2);
-        ra[q][NS - 1] = row < it.m ? al : z;
-        rb[q][NS - 1] = row < it.n ? bl : z;
-      }
-    }
-  };
-  auto store = [&](int buf) {
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int v = tid + 128 * q, row = v >> 2, part = v & 3;
-#pragma unroll
-      for (int t = 0; t < NS; ++t) {
-        *(u32x4*)&As[buf][t][row][part * 8] = ra[q][t];
-        *(u32x4*)&Bs[buf][t][row][part * 8] = rb[q][t];
-      }
-    }
-  };
-  v4f acc[8][4];
-#pragma unroll
-  for (int bi = 0; bi < 8; ++bi)
-#pragma unroll
-    for (int bj = 0; bj < 4; ++bj) acc[bi][bj] = v4f{0, 0, 0, 0};
-  const int stages = (it.k + BK - 1) / BK;
-  const int fr = lane & 15, fk = 8 * (lane >> 4);
-  load(0);
-  store(0);
-  __syncthreads();
+  if (stages > 0) issue(0, 0);
   for (int s = 0; s < stages; ++s) {
-    const int cur = s & 1;
-    if (s + 1 < stages) load(s + 1);
-    v8bf fj[4], gj[4];
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      fj[b] = *(const v8bf*)&Bs[cur][0][wj + 16 * b + fr][fk];
-      if (TERMS == 3) gj[b] = *(const v8bf*)&Bs[cur][NS - 1][wj + 16 * b + fr][fk];
-    }
-#pragma unroll
-    for (int bi = 0; bi < 8; ++bi) {
-      const v8bf fi = *(const v8bf*)&As[cur][0][16 * bi + fr][fk];
-      if (TERMS == 3) {
-        const v8bf gi = *(const v8bf*)&As[cur][NS - 1][16 * bi + fr][fk];
-#pragma unroll
-        for (int bj = 0; bj < 4; ++bj) {
-          acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(gj[bj], fi, acc[bi][bj], 0, 0, 0);
-          acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fj[bj], gi, acc[bi][bj], 0, 0, 0);
-        }
-      }
-#pragma unroll
-      for (int bj = 0; bj < 4; ++bj)
-        acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fj[bj], fi, acc[bi][bj], 0, 0, 0);
-    }
-    if (s + 1 < stages) store(cur ^ 1);
-    __syncthreads();
-  }
-  // acc[bi][bj][v] = C(16 bi + lane % 16, wj + 16 bj + 4 (lane / 16) + v) / sc^2
-  const float unscale = ldexpf(1.0f, 2 * e);
-  const int il = lane & 15;
-  if (it.mode == G_PART) {
-    float* P = Part + it.c;
-#pragma unroll
-    for (int bi = 0; bi < 8; ++bi)
-#pragma unroll
-      for (int bj = 0; bj < 4; ++bj)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int i = 16 * bi + il, j = wj + 16 * bj + Mma<float>::row(lane, v);
-          P[(int64_t)j * GB + i] = acc[bi][bj][v] * unscale;
-        }
-    return;
-  }
-  float* C = base(it.csrc) + it.c;
-#pragma unroll
-  for (int bi = 0; bi < 8; ++bi)
-#pragma unroll
-    for (int bj = 0; bj < 4; ++bj)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int i = 16 * bi + il, j = wj + 16 * bj + Mma<float>::row(lane, v);
-        if (i < it.m && j < it.n && i - j + it.diag >= 0) {
-          float* p = C + (int64_t)j * it.ldc + i;
-          const float x = acc[bi][bj][v] * unscale;
-          if (it.mode == G_SUB) *p -= x;
-          else if (it.mode == G_NEG) *p = -x;
-          else *p = x;
-        }
-      }
-}
-
-// The same products with the fragments loaded straight from the copies into registers: in the
-// k-chunked layout a 16-row x 32-k fragment (lane: row lane % 16, k 8 (lane / 16) + [0, 8)) is
-// one contiguous 1 KB block, so every load is a full 16-byte vector per lane and no LDS stage or
-// barrier is needed. WN = 8: two waves per 128 x 128 tile, each 64 rows x 128 columns as 4 x 8
-// blocks of the 16x16x32 MFMA; WN = 4: four waves of 64 x 64. The next stage's fragments are
-// loaded while the current one is multiplied.
-template <int TERMS, int WN>
-__global__ __launch_bounds__(WN == 8 ? 128 : 256, 1) void k_gemm_b16d(const GemmItem* __restrict__ items, const FrontDev* __restrict__ fronts,
-                                                      const __bf16* __restrict__ Lh, const __bf16* __restrict__ Ll,
-                                                      float* L, float* U, float* Part, const int32_t* __restrict__ fexp) {
-  static_assert(TERMS == 1 || TERMS == 3, "one or three BF16 products");
-  constexpr int NS = TERMS == 3 ? 2 : 1;
-  const GemmItem it = items[blockIdx.x];
-  const FrontDev F = fronts[it.front];
-  const int e = fexp[it.front];
-  const int64_t oa = it.a - F.lpos, ob = it.b - F.lpos;
-  const int64_t chunk = (int64_t)F.ld * BK;
-  const int64_t a0 = F.bpos + (oa / F.ld) / BK * chunk + (oa % F.ld) * BK;
-  const int64_t b0 = F.bpos + (ob / F.ld) / BK * chunk + (ob % F.ld) * BK;
-  auto base = [&](int8_t src) { return src == S_L ? L : (src == S_U ? U : Part); };
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int fr = lane & 15, fk = 8 * (lane >> 4);
-  const int wi = WN == 8 ? 64 * wave : 64 * (wave >> 1), wj = WN == 8 ? 0 : 64 * (wave & 1);
-  // a lane's fragment rows (clamped into the tile; rows past it are zeroed after the load)
-  const int ra = min(wi + fr, it.m - 1), rb = min(wj + fr, it.n - 1);
-  typedef u32x4 Frag[NS];
-  Frag fa[2][4], fb[2][WN];
-  const u32x4 z = {0, 0, 0, 0};
-  auto load = [&](int buf, int s) {
-    const __bf16* ah = Lh + a0 + (int64_t)s * chunk + fk;
-    const __bf16* bh = Lh + b0 + (int64_t)s * chunk + fk;
-    const __bf16* al = Ll + a0 + (int64_t)s * chunk + fk;
-    const __bf16* bl = Ll + b0 + (int64_t)s * chunk + fk;
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const int r = min(ra + 16 * b, it.m - 1);
-      const bool in = wi + 16 * b + fr < it.m;
-      const u32x4 h = *(const u32x4*)(ah + r * BK);
-      fa[buf][b][0] = in ? h : z;
-      if (TERMS == 3) {
-        const u32x4 l = *(const u32x4*)(al + r * BK);
-        fa[buf][b][NS - 1] = in ? l : z;
-      }
-    }
-#pragma unroll
-    for (int b = 0; b < WN; ++b) {
-      const int r = min(rb + 16 * b, it.n - 1);
-      const bool in = wj + 16 * b + fr < it.n;
-      const u32x4 h = *(const u32x4*)(bh + r * BK);
-      fb[buf][b][0] = in ? h : z;
-      if (TERMS == 3) {
-        const u32x4 l = *(const u32x4*)(bl + r * BK);
-        fb[buf][b][NS - 1] = in ? l : z;
-      }
-    }
-  };
-  v4f acc[4][WN];
-#pragma unroll
-  for (int bi = 0; bi < 4; ++bi)
-#pragma unroll
-    for (int bj = 0; bj < WN; ++bj) acc[bi][bj] = v4f{0, 0, 0, 0};
-  auto mul = [&](int buf) {
-#pragma unroll
-    for (int bi = 0; bi < 4; ++bi)
-#pragma unroll
-      for (int bj = 0; bj < WN; ++bj) {
-        const v8bf ah = __builtin_bit_cast(v8bf, fa[buf][bi][0]), bh = __builtin_bit_cast(v8bf, fb[buf][bj][0]);
-        if (TERMS == 3) {
-          const v8bf al = __builtin_bit_cast(v8bf, fa[buf][bi][NS - 1]);
-          const v8bf bl = __builtin_bit_cast(v8bf, fb[buf][bj][NS - 1]);
-          acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl, ah, acc[bi][bj], 0, 0, 0);
-          acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh, al, acc[bi][bj], 0, 0, 0);
-        }
-        acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh, ah, acc[bi][bj], 0, 0, 0);
-      }
-  };
-  const int stages = (it.k + BK - 1) / BK;
-  load(0, 0);
-  for (int s = 0; s < stages; s += 2) {
-    if (s + 1 < stages) load(1, s + 1);
-    mul(0);
-    if (s + 1 < stages) {
-      if (s + 2 < stages) load(0, s + 2);
-      mul(1);
-    }
+    // this wave's loads of stage s are complete, its LDS reads of stage s - 1 too; after the
+    // barrier every wave's are
+    asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    if (s + 1 < stages) issue(s + 1, (s + 1) & 1);
+    mul(s & 1);
   }
   // acc[bi][bj][v] = C(wi + 16 bi + lane % 16, wj + 16 bj + 4 (lane / 16) + v) / sc^2
   const float unscale = ldexpf(1.0f, 2 * e);
@@ -581,7 +346,7 @@ __global__ __launch_bounds__(WN == 8 ? 128 : 256, 1) void k_gemm_b16d(const Gemm
 #pragma unroll
     for (int bi = 0; bi < 4; ++bi)
 #pragma unroll
-      for (int bj = 0; bj < WN; ++bj)
+      for (int bj = 0; bj < 4; ++bj)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int i = wi + 16 * bi + il, j = wj + 16 * bj + Mma<float>::row(lane, v);
@@ -589,11 +354,11 @@ __global__ __launch_bounds__(WN == 8 ? 128 : 256, 1) void k_gemm_b16d(const Gemm
         }
     return;
   }
-  float* C = base(it.csrc) + it.c;
+  float* C = (it.csrc == S_L ? L : (it.csrc == S_U ? U : Part)) + it.c;
 #pragma unroll
   for (int bi = 0; bi < 4; ++bi)
 #pragma unroll
-    for (int bj = 0; bj < WN; ++bj)
+    for (int bj = 0; bj < 4; ++bj)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int i = wi + 16 * bi + il, j = wj + 16 * bj + Mma<float>::row(lane, v);
