@@ -23,6 +23,24 @@ struct FrontDev {
 };
 static_assert(sizeof(FrontDev) == 80, "FrontDev layout");
 
+// Strided runs of loads through a buffer resource: the lane's part of the address is one VGPR
+// offset and each load's part a scalar, so a run of independent loads issued back to back needs
+// no 64-bit address per load. Raw resource (byte offsets, stride 0); every access made through
+// it is in bounds by construction, the record count only has to cover the lane offsets.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t raw_rsrc(const void* base) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, 0x7fffffff, 0x00020000);
+}
+template <typename T>
+__device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, uint32_t lane_off, uint32_t off);
+template <>
+__device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsrc_t r, uint32_t lane_off, uint32_t off) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, lane_off, off, 0));
+}
+template <>
+__device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t r, uint32_t lane_off, uint32_t off) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, lane_off, off, 0));
+}
+
 // A unit of work of one launch; the meaning of a, b, c, d depends on the kernel.
 struct Item {
   int32_t f, a, b, c, d, e, pad0, pad1;
@@ -51,6 +69,7 @@ struct Launch {
 
 constexpr int KINDS = 8;
 
+constexpr int EA_BATCH = 4;   // rows an extend-add thread moves at a time
 constexpr int EA_COLS = 8;     // columns of the parent one extend-add workgroup owns
 
 }  // namespace fc

@@ -95,12 +95,24 @@ __global__ __launch_bounds__(256) void k_extend_add(const Item* __restrict__ ite
     for (int jc = jc0; jc < jc1; ++jc) {
       const int jp = r[jc];
       const T* src = cu + (int64_t)jc * cmu;
-      if (jp < pw) {
-        T* dst = L + P.lpos + (int64_t)jp * P.ld;
-        for (int ic = jc + threadIdx.x; ic < cmu; ic += blockDim.x) dst[r[ic]] += src[ic];
-      } else {
-        T* dst = Uw + P.upos + (int64_t)(jp - pw) * pmu - pw;
-        for (int ic = jc + threadIdx.x; ic < cmu; ic += blockDim.x) dst[r[ic]] += src[ic];
+      T* dst = jp < pw ? L + P.lpos + (int64_t)jp * P.ld : Uw + P.upos + (int64_t)(jp - pw) * pmu - pw;
+      // EA_BATCH rows a thread at a time: the row map and the child's entries, then the targets,
+      // then the sums (rel rises, so the rows of one column land on distinct targets). Rows past
+      // the column read its last row (in bounds) and store nothing.
+      for (int i0 = jc + threadIdx.x; i0 < cmu; i0 += EA_BATCH * blockDim.x) {
+        int ri[EA_BATCH];
+        T sv[EA_BATCH], dv[EA_BATCH];
+#pragma unroll
+        for (int k = 0; k < EA_BATCH; ++k) {
+          const int ic = min(i0 + k * (int)blockDim.x, cmu - 1);
+          ri[k] = r[ic];
+          sv[k] = src[ic];
+        }
+#pragma unroll
+        for (int k = 0; k < EA_BATCH; ++k) dv[k] = dst[ri[k]];
+#pragma unroll
+        for (int k = 0; k < EA_BATCH; ++k)
+          if (i0 + k * (int)blockDim.x < cmu) dst[ri[k]] = dv[k] + sv[k];
       }
     }
     __syncthreads();

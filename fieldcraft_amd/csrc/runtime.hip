@@ -901,7 +901,8 @@ void run_solve_r(const fc_factor* F, double* xp, double* yp, double* u, i64 ldx,
         fc::k_bwd_first<T, R><<<g, b, 0, st>>>(it, P.fronts, Inv, yp, xp, ldx, nrhs);
         break;
       case S_BWD_STEP:
-        fc::k_bwd_step<T, R><<<g, b, 0, st>>>(it, P.fronts, L, Inv, yp, xp, ldx, nrhs);
+        if constexpr (R == 1) fc::k_bwd_step1<T><<<g, b, 0, st>>>(it, P.fronts, L, Inv, yp, xp);
+        else fc::k_bwd_step<T, R><<<g, b, 0, st>>>(it, P.fronts, L, Inv, yp, xp, ldx, nrhs);
         break;
     }
   };

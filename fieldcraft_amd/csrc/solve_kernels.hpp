@@ -44,11 +44,24 @@ __device__ __forceinline__ void inv_apply(const T* __restrict__ inv, int pw, con
   double s[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) s[r] = 0.0;
-#pragma unroll(R == 1 ? 16 : 4)
-  for (int k = 0; k < pw; ++k) {
-    const double a = (double)inv[(int64_t)k * pw + b];
+  if (R == 1 && pw == SNB) {  // a full block, one right-hand side: the whole row in flight at once
+    constexpr int KB = sizeof(T) == 4 ? SNB : SNB / 2;
+    const __amdgpu_buffer_rsrc_t rs = raw_rsrc(inv);
 #pragma unroll
-    for (int r = 0; r < R; ++r) s[r] = fma(a, V[k][r], s[r]);
+    for (int k0 = 0; k0 < SNB; k0 += KB) {
+      T a[KB];
+#pragma unroll
+      for (int q = 0; q < KB; ++q) a[q] = buf_load<T>(rs, b * sizeof(T), (k0 + q) * SNB * sizeof(T));
+#pragma unroll
+      for (int q = 0; q < KB; ++q) s[0] = fma((double)a[q], V[k0 + q][0], s[0]);
+    }
+  } else {
+#pragma unroll 4
+    for (int k = 0; k < pw; ++k) {
+      const double a = (double)inv[(int64_t)k * pw + b];
+#pragma unroll
+      for (int r = 0; r < R; ++r) s[r] = fma(a, V[k][r], s[r]);
+    }
   }
   for (int r = 0; r < nr; ++r) out[(int64_t)r * ldx + b] = s[r];
 }
@@ -59,16 +72,17 @@ template <typename T, int R>
 __device__ __forceinline__ void inv_apply_t(const T* __restrict__ inv, int pw, const double (*Z)[R + 1], int nr,
                                             double* __restrict__ out, int64_t ldx, double (*H)[SNB][R + 1]) {
   const int b = threadIdx.x & (SNB - 1), h = threadIdx.x >> 7, bb = min(b, pw - 1);
-  const T* col = inv + (int64_t)bb * pw;
+  const __amdgpu_buffer_rsrc_t rs = raw_rsrc(inv);
+  const uint32_t co = bb * pw * sizeof(T);  // column bb of Inv
   double s[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) s[r] = 0.0;
-  constexpr int KB = R == 1 ? 16 : 4;  // loads per batch
-#pragma unroll(R == 1 ? 4 : 1)
+  constexpr int KB = R == 1 ? 64 : 4;  // loads per batch (one batch for a single right-hand side)
+#pragma unroll 1
   for (int k0 = 0; k0 < 64; k0 += KB) {
     double a[KB];
 #pragma unroll
-    for (int q = 0; q < KB; ++q) a[q] = (double)col[min(64 * h + k0 + q, pw - 1)];
+    for (int q = 0; q < KB; ++q) a[q] = (double)buf_load<T>(rs, co, min(64 * h + k0 + q, pw - 1) * sizeof(T));
 #pragma unroll
     for (int q = 0; q < KB; ++q) {
       const int k = 64 * h + k0 + q;
@@ -89,7 +103,7 @@ __device__ __forceinline__ void inv_apply_t(const T* __restrict__ inv, int pw, c
 // Forward, per front (item f): zero the front's contribution block, add its children's into it
 // and into v, then y of the front's first block = Inv_0 v_0.
 template <typename T, int R>
-__global__ __launch_bounds__(256) void k_fwd_gather(const Item* __restrict__ items, const FrontDev* __restrict__ fronts,
+__global__ __launch_bounds__(256, 2) void k_fwd_gather(const Item* __restrict__ items, const FrontDev* __restrict__ fronts,
                                                     const int32_t* __restrict__ kids, const int32_t* __restrict__ rel,
                                                     const T* __restrict__ Inv, double* v, double* __restrict__ y,
                                                     int64_t ldx, int nrhs, double* u, const int64_t* __restrict__ uoff) {
@@ -130,7 +144,7 @@ __global__ __launch_bounds__(256) void k_fwd_gather(const Item* __restrict__ ite
 // with c = 1 (r0 = c1 and a next block [c1, c2) in the front) then forms y of the next block
 // from the rows it has just finished. Item: f, a = c0, b = r0, c.
 template <typename T, int R>
-__global__ __launch_bounds__(256) void k_fwd_step(const Item* __restrict__ items, const FrontDev* __restrict__ fronts,
+__global__ __launch_bounds__(256, 2) void k_fwd_step(const Item* __restrict__ items, const FrontDev* __restrict__ fronts,
                                                   const T* __restrict__ L, const T* __restrict__ Inv, double* v,
                                                   double* __restrict__ y, int64_t ldx, int nrhs, double* __restrict__ u,
                                                   const int64_t* __restrict__ uoff) {
@@ -150,16 +164,30 @@ __global__ __launch_bounds__(256) void k_fwd_step(const Item* __restrict__ items
     for (int r = 0; r < nr; ++r)
       for (int k = threadIdx.x; k < pw; k += blockDim.x) Yb[k][r] = y[(int64_t)(q0 + r) * ldx + F.col0 + c0 + k];
     __syncthreads();
-    if (row) {
-      double s[R];
+    double s[R];
 #pragma unroll
-      for (int r = 0; r < R; ++r) s[r] = 0.0;
-#pragma unroll(R == 1 ? 16 : 4)
+    for (int r = 0; r < R; ++r) s[r] = 0.0;
+    if (R == 1 && pw == SNB) {  // a full block, one right-hand side: the whole row in flight at once
+      constexpr int KB = sizeof(T) == 4 ? SNB : SNB / 2;
+      const __amdgpu_buffer_rsrc_t rs = raw_rsrc(lf);
+      const uint32_t lo = min(i, F.m - 1) * sizeof(T);  // rows past the front load its last row (not stored)
+#pragma unroll
+      for (int j0 = 0; j0 < SNB; j0 += KB) {
+        T l[KB];
+#pragma unroll
+        for (int q = 0; q < KB; ++q) l[q] = buf_load<T>(rs, lo, (uint32_t)(j0 + q) * F.ld * sizeof(T));
+#pragma unroll
+        for (int q = 0; q < KB; ++q) s[0] = fma((double)l[q], Yb[j0 + q][0], s[0]);
+      }
+    } else if (row) {
+#pragma unroll 4
       for (int j = 0; j < pw; ++j) {
         const double l = (double)lf[(int64_t)j * F.ld + i];
 #pragma unroll
         for (int r = 0; r < R; ++r) s[r] = fma(l, Yb[j][r], s[r]);
       }
+    }
+    if (row) {
       if (i < F.w) {
         for (int r = 0; r < nr; ++r) {
           double* p = v + (int64_t)(q0 + r) * ldx + F.col0 + i;
@@ -305,17 +333,23 @@ __global__ __launch_bounds__(256) void k_bwd_step(const Item* __restrict__ items
     const int nr = min(R, nrhs - q0);
     for (int r = 0; r < nr; ++r)
       for (int k = tid; k < pw; k += blockDim.x) Xb[k][r] = x[(int64_t)(q0 + r) * ldx + F.col0 + c0 + k];
+    // thread (row k = tid % 128, every other column): 32 independent loads per half, the next
+    // half's issued before the current one is multiplied
+    const int lk = tid & (SNB - 1), jb = tid >> 7;
+    T v[HC / 2];
+    const __amdgpu_buffer_rsrc_t rs = raw_rsrc(lf + (int64_t)j0 * F.ld);
+    const uint32_t lo = min(lk, pw - 1) * sizeof(T);
+    auto load_half = [&](int h0) {
+      const int hc = min(HC, nj - h0);
+#pragma unroll
+      for (int q = 0; q < HC / 2; ++q) v[q] = buf_load<T>(rs, lo, (uint32_t)(h0 + min(jb + 2 * q, hc - 1)) * F.ld * sizeof(T));
+    };
+    load_half(0);
     for (int h0 = 0; h0 < nj; h0 += HC) {
       const int hc = min(HC, nj - h0);
-      {  // thread (row k = tid % 128, every other column): 32 independent loads, then stores
-        const int k = tid & (SNB - 1), jb = tid >> 7;
-        T v[HC / 2];
 #pragma unroll
-        for (int q = 0; q < HC / 2; ++q)
-          v[q] = lf[(int64_t)(j0 + h0 + min(jb + 2 * q, hc - 1)) * F.ld + min(k, pw - 1)];
-#pragma unroll
-        for (int q = 0; q < HC / 2; ++q) Lt[jb + 2 * q][k] = (k < pw && jb + 2 * q < hc) ? v[q] : T(0);
-      }
+      for (int q = 0; q < HC / 2; ++q) Lt[jb + 2 * q][lk] = (lk < pw && jb + 2 * q < hc) ? v[q] : T(0);
+      if (h0 + HC < nj) load_half(h0 + HC);
       __syncthreads();
       {  // partial sums of column jq over rows [32 g, 32 g + 32), every right-hand side
         double t[R];
@@ -348,6 +382,76 @@ __global__ __launch_bounds__(256) void k_bwd_step(const Item* __restrict__ items
                                 x + (int64_t)q0 * ldx + F.col0 + j0, ldx,
                                 reinterpret_cast<double (*)[SNB][R + 1]>(&part[0][0][0]));
     __syncthreads();
+  }
+}
+
+// One right-hand side, backward: the same step with every global load a workgroup needs issued
+// at its entry (a block step then waits on memory once), and the same sums in the same order as
+// k_bwd_step, so a single column's result equals that column's result in a block.
+
+// Backward step, item as k_bwd_step (128 columns [j0, j0 + 128) before the block): the tile is
+// loaded whole at entry (thread = one row of the block, every other column: coalesced), stored
+// to LDS, and thread (jq, g) sums rows 32 g + [0, 32) of columns j0 + jq and j0 + 64 + jq; the
+// designated item's threads (b, h) hold half h of column b of the previous block's inverse.
+template <typename T>
+__global__ __launch_bounds__(256) void k_bwd_step1(const Item* __restrict__ items, const FrontDev* __restrict__ fronts,
+                                                   const T* __restrict__ L, const T* __restrict__ Inv, double* z,
+                                                   double* __restrict__ x) {
+  __shared__ T Lt[SNB][SNB + 1];  // Lt[j][k] = L(c0 + k, j0 + j)
+  __shared__ double Xb[SNB];
+  __shared__ double Zn[SNB];
+  __shared__ double part[2][4][64];
+  __shared__ double H[2][SNB];
+  const Item it = items[blockIdx.x];
+  const FrontDev F = fronts[it.f];
+  const int c0 = it.a, pw = min(SNB, F.w - c0), j0 = it.b;
+  const int tid = threadIdx.x, jq = tid & 63, g = tid >> 6;
+  const int lk = tid & (SNB - 1), jb = tid >> 7;
+  const __amdgpu_buffer_rsrc_t rs = raw_rsrc(L + F.lpos + (int64_t)j0 * F.ld + c0);
+  const uint32_t lo = min(lk, pw - 1) * sizeof(T);
+  T v[SNB / 2];
+#pragma unroll
+  for (int q = 0; q < SNB / 2; ++q) v[q] = buf_load<T>(rs, lo, (uint32_t)(jb + 2 * q) * F.ld * sizeof(T));
+  double xv = 0.0, zv = 0.0;
+  if (tid < SNB) {
+    xv = x[F.col0 + c0 + min(tid, pw - 1)];
+    zv = z[F.col0 + j0 + tid];
+  }
+  const int b = tid & (SNB - 1), hh = tid >> 7;
+  T a[64];
+  if (it.c) {
+    const T* col = Inv + F.inv0 + (int64_t)(j0 / SNB) * SNB * SNB + (int64_t)b * SNB + 64 * hh;
+#pragma unroll
+    for (int q = 0; q < 64; ++q) a[q] = col[q];
+  }
+  if (tid < SNB) Xb[tid] = tid < pw ? xv : 0.0;
+#pragma unroll
+  for (int q = 0; q < SNB / 2; ++q) Lt[jb + 2 * q][lk] = v[q];
+  __syncthreads();
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    double t = 0.0;
+#pragma unroll
+    for (int r = 0; r < 32; ++r)
+      if (32 * g + r < pw) t = fma((double)Lt[64 * h + jq][32 * g + r], Xb[32 * g + r], t);
+    part[h][g][jq] = t;
+  }
+  __syncthreads();
+  if (tid < SNB) {
+    const int h = tid >> 6, jj = tid & 63;
+    const double d = (part[h][0][jj] + part[h][1][jj]) + (part[h][2][jj] + part[h][3][jj]);
+    const double nz = zv - d;
+    z[F.col0 + j0 + tid] = nz;
+    Zn[tid] = nz;
+  }
+  if (it.c) {
+    __syncthreads();
+    double t = 0.0;
+#pragma unroll
+    for (int q = 0; q < 64; ++q) t = fma((double)a[q], Zn[64 * hh + q], t);
+    H[hh][b] = t;
+    __syncthreads();
+    if (tid < SNB) x[F.col0 + j0 + tid] = H[0][tid] + H[1][tid];
   }
 }
 
