@@ -65,6 +65,8 @@ struct Launch {
   int32_t pad;
   double flops;   // algorithmic flops of the launch (GEMM-like kinds)
   double entries; // algorithmic entries moved (extend-add: child entries added)
+  int32_t stream = 0;  // 0: the caller's stream, 1: the factor's side stream (lookahead)
+  int32_t slot = 0;    // event slot of a record / wait step
 };
 
 constexpr int KINDS = 8;

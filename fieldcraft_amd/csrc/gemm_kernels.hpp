@@ -245,8 +245,9 @@ __device__ __forceinline__ v8bf lds_read16(uint32_t a) {
 // barrier (which also frees buffer (s - 1) % 2 for stage s + 1).
 //
 // Stage buffer: copy t of A (128 rows x 32 k), then copy t of B. A row's four 16-byte chunks sit
-// in the order chunk ^ (row / 4 % 4), so the 16 rows of one fragment read spread over all
-// banks; a wave's direct load fills 1 KB = 16 rows, lane l taking the chunk of LDS slot l.
+// in the order chunk ^ (row / 2 % 4), the arrangement under which the fragment reads measure
+// free of bank conflicts on gfx950; a wave's direct load fills 1 KB = 16 rows, lane l taking
+// the chunk of LDS slot l.
 // Rows past the tile's m (n) load the last row of the tile (in bounds); their products land in
 // entries that are not stored. The copies are zero past the front's width, so the last stage
 // needs no k bound.
@@ -270,7 +271,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_b16(const GemmItem* __restrict_
   const int64_t b0 = F.bpos + (ob / F.ld) / BK * chunk + (ob % F.ld) * BK;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wi = (wave >> 1) * 64, wj = (wave & 1) * 64;
-  const int lrow = lane >> 2, lch = (lane & 3) ^ (lane >> 4);  // the row and chunk slot l holds
+  const int lrow = lane >> 2, lch = (lane & 3) ^ ((lrow >> 1) & 3);  // the row and chunk slot l holds
   auto issue = [&](int s, int buf) {
 #pragma unroll
     for (int q = 0; q < PW; ++q) {
@@ -292,7 +293,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_b16(const GemmItem* __restrict_
   const int stages = (it.k + BK - 1) / BK;
   // a lane's fragment: row lane % 16 of a 16-row block, k chunk lane / 16 (byte offset in a copy)
   const int fr = lane & 15;
-  const uint32_t foff = (uint32_t)(fr * 4 + ((lane >> 4) ^ (fr >> 2))) * 16;
+  const uint32_t foff = (uint32_t)(fr * 4 + ((lane >> 4) ^ ((fr >> 1) & 3))) * 16;
   const uint32_t sbase = lds_addr(S);
   auto mul = [&](int buf) {
     const uint32_t ah = sbase + (uint32_t)(buf * SE + wi * BK) * 2 + foff;

@@ -115,6 +115,13 @@ struct Program {
 // FP64 factor's saved-copy, product, residual and correction steps
 enum TrsmVariant : int32_t { TRSM_IN_PLACE = 0, TRSM_COPY = 1, TRSM_PRODUCT = 2, TRSM_RESIDUAL = 3, TRSM_FIX = 4 };
 
+// Ordering steps between the caller's stream and the factor's side stream (Launch.kind values
+// outside the launch kinds): record event slot `slot` on stream `stream`, or make `stream` wait
+// for that slot's last record.
+constexpr int32_t L_RECORD = 100;
+constexpr int32_t L_WAIT = 101;
+constexpr int SYNC_SLOTS = 4;
+
 // kinds of solve launches
 enum SolveKind : int32_t { S_GATHER = 0, S_FWD_STEP = 1, S_BWD_GATHER = 2, S_BWD_BELOW = 3, S_BWD_FIRST = 4, S_BWD_STEP = 5 };
 static_assert(fc::SNB == fc::NB, "solve blocks are the factorization's panels");
@@ -247,36 +254,37 @@ Program build_program(const fc::Symbolic& s) {
       }
     }
     add(fc::K_EXTEND_L, l, i0);
-    // panels of NB columns, left-looking inside each front (cholesky.rs:1138-1186)
     i64 wmax = 0;
     for (i64 t = 0; t < nl; ++t) wmax = std::max(wmax, s.f_w[fl[t]]);
     std::vector<Tile> tiles;
-    for (i64 c0 = 0; c0 < wmax; c0 += fc::NB) {
-      if (c0 > 0) {  // the panel minus its products with the front's columns before it
-        tiles.clear();
-        for (i64 t = 0; t < nl; ++t) {
-          const i64 f = fl[t];
-          if (s.f_w[f] <= c0) continue;
-          const i64 c1 = std::min(c0 + fc::NB, s.f_w[f]);
-          for (i64 r = c0; r < s.f_m[f]; r += fc::GB) {
-            const i64 m = std::min<i64>(fc::GB, s.f_m[f] - r);
-            fc::GemmItem g{};
-            g.a = s.f_lpos[f] + r;
-            g.b = s.f_lpos[f] + c0;
-            g.c = s.f_lpos[f] + c0 * s.f_ld[f] + r;
-            g.lda = g.ldb = g.ldc = (int32_t)s.f_ld[f];
-            g.m = (int32_t)m;
-            g.n = (int32_t)(c1 - c0);
-            g.k = (int32_t)c0;
-            g.mode = fc::G_SUB;
-            g.diag = (int32_t)(r - c0);
-            g.asrc = g.bsrc = g.csrc = fc::S_L;
-            g.front = (int32_t)f;
-            tiles.push_back(Tile{g, 2.0 * (double)c0 * lower_pairs(r, r + m, c0, c1)});
-          }
+    // The panel at c0 minus its products with the front's columns [k0, k1) (all before c0).
+    auto update = [&](i64 c0, i64 k0, i64 k1) {
+      tiles.clear();
+      for (i64 t = 0; t < nl; ++t) {
+        const i64 f = fl[t];
+        if (s.f_w[f] <= c0) continue;
+        const i64 c1 = std::min(c0 + fc::NB, s.f_w[f]);
+        for (i64 r = c0; r < s.f_m[f]; r += fc::GB) {
+          const i64 m = std::min<i64>(fc::GB, s.f_m[f] - r);
+          fc::GemmItem g{};
+          g.a = s.f_lpos[f] + k0 * s.f_ld[f] + r;
+          g.b = s.f_lpos[f] + k0 * s.f_ld[f] + c0;
+          g.c = s.f_lpos[f] + c0 * s.f_ld[f] + r;
+          g.lda = g.ldb = g.ldc = (int32_t)s.f_ld[f];
+          g.m = (int32_t)m;
+          g.n = (int32_t)(c1 - c0);
+          g.k = (int32_t)(k1 - k0);
+          g.mode = fc::G_SUB;
+          g.diag = (int32_t)(r - c0);
+          g.asrc = g.bsrc = g.csrc = fc::S_L;
+          g.front = (int32_t)f;
+          tiles.push_back(Tile{g, 2.0 * (double)(k1 - k0) * lower_pairs(r, r + m, c0, c1)});
         }
-        add_gemm(p, tiles, fc::K_UPDATE, l);
       }
+      add_gemm(p, tiles, fc::K_UPDATE, l);
+    };
+    // The panel at c0 once its updates are in: diagonal block, rows below, BF16 copies.
+    auto chain = [&](i64 c0) {
       // the diagonal blocks (k_potrf_blk, any panel width)
       i0 = (i64)p.items.size();
       for (i64 t = 0; t < nl; ++t) {
@@ -369,6 +377,37 @@ Program build_program(const fc::Symbolic& s) {
         }
       }
       add(fc::K_SPLIT, l, i0);
+    };
+    auto sync = [&](int kind, int stream, int slot) {
+      Launch q{kind, (int32_t)l, 0, 0, 0, 0.0, 0.0};
+      q.stream = stream;
+      q.slot = slot;
+      p.launches.push_back(q);
+    };
+    // Panels of NB columns, left-looking inside each front (cholesky.rs:1138-1186), one panel
+    // of lookahead: the side stream takes panel k's products with panels 0 .. k - 2 (as soon as
+    // panel k - 2 is finished) while the main stream finishes panel k - 1; the main stream then
+    // subtracts panel k - 1's product and factors panel k. Events: slot k % 2 marks "panel k
+    // finished" (main), slot 2 + k % 2 "panel k's side update done" (side); each is waited on
+    // before it is recorded again.
+    const i64 np = (wmax + fc::NB - 1) / fc::NB;
+    auto side = [&](i64 k) {  // panel k's products with panels [0, k - 1), after panel k - 2
+      if (k >= np) return;
+      sync(L_WAIT, 1, (int)((k - 2) % 2));
+      const i64 from = (i64)p.launches.size();
+      update(k * fc::NB, 0, (k - 1) * fc::NB);
+      for (i64 q = from; q < (i64)p.launches.size(); ++q) p.launches[q].stream = 1;
+      sync(L_RECORD, 1, 2 + (int)(k % 2));
+    };
+    for (i64 k = 0; k < np; ++k) {
+      const i64 c0 = k * fc::NB;
+      if (k >= 2) sync(L_WAIT, 0, 2 + (int)(k % 2));
+      if (k >= 1) update(c0, c0 - fc::NB, c0);  // the product with the panel just finished
+      chain(c0);
+      if (k + 2 < np) {
+        sync(L_RECORD, 0, (int)(k % 2));
+        side(k + 2);
+      }
     }
     // update matrices U = -L21 L21^T
     tiles.clear();
@@ -566,6 +605,8 @@ struct fc_factor {
   fc_factor_opts opts{};
   hipStream_t stream = nullptr;
   bool own_stream = false;
+  hipStream_t side = nullptr;                  // the lookahead's second stream (default priority)
+  hipEvent_t sync_ev[SYNC_SLOTS] = {};         // its ordering events
   void* L = nullptr;       // factor storage (double or float)
   double* diag = nullptr;  // A's diagonal (postorder)
   unsigned long long* fail = nullptr;
@@ -614,8 +655,17 @@ struct fc_factor {
     yp.release();
     usolve.release();
     refine_work.release();
+    for (auto e : sync_ev)
+      if (e) (void)hipEventDestroy(e);
+    if (side) (void)hipStreamDestroy(side);
     if (own_stream && stream) (void)hipStreamDestroy(stream);
     (void)hipSetDevice(cur);
+  }
+  // The side stream and its events, made once (before any capture) on the factor's device.
+  void make_side() {
+    if (side) return;
+    HIP_CHECK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+    for (auto& e : sync_ev) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   }
 };
 
@@ -711,54 +761,61 @@ void enqueue_factor(fc_factor* F, const double* dval, i64 nnz, hipStream_t st, b
       level = l.level;
       if (eager && F->opts.cancel && *F->opts.cancel) throw fc::Failure{fc::CANCELLED, -1, 0, 0, "cancelled"};
     }
+    if (l.kind == L_RECORD || l.kind == L_WAIT) {
+      hipStream_t on = l.stream ? F->side : st;
+      if (l.kind == L_RECORD) HIP_CHECK(hipEventRecord(F->sync_ev[l.slot], on));
+      else HIP_CHECK(hipStreamWaitEvent(on, F->sync_ev[l.slot], 0));
+      continue;
+    }
     if (l.kind == fc::K_SPLIT && !mixed) continue;  // the BF16 copies serve the mixed form only
     if (l.kind == fc::K_TRSM && (l.pad == TRSM_IN_PLACE) == (F->precision == FC_FP64)) continue;
     const fc::Item* it = P.items + l.item0;
+    hipStream_t ls = l.stream ? F->side : st;  // the launch's stream
     dim3 g(l.count), b(256);
     hipEvent_t a = nullptr, z = nullptr;
     if (mask & (1 << l.kind)) {
       a = ev.make();
       z = ev.make();
-      HIP_CHECK(hipEventRecord(a, st));
+      HIP_CHECK(hipEventRecord(a, ls));
     }
     switch (l.kind) {
       case fc::K_EXTEND_L:
       case fc::K_EXTEND_U:
-        fc::k_extend_add<T><<<g, b, 0, st>>>(it, P.fronts, P.kids, P.rel, L, U, U);
+        fc::k_extend_add<T><<<g, b, 0, ls>>>(it, P.fronts, P.kids, P.rel, L, U, U);
         break;
       case fc::K_UPDATE:
       case fc::K_SYRK:
         if constexpr (std::is_same<T, float>::value) {
           if (mixed) {
             if (F->opts.bf16_terms == 1)
-              fc::k_gemm_b16<1><<<g, b, 0, st>>>(P.gitems + l.item0, P.fronts, Lh, Ll, L, U, Part, fexp);
+              fc::k_gemm_b16<1><<<g, b, 0, ls>>>(P.gitems + l.item0, P.fronts, Lh, Ll, L, U, Part, fexp);
             else
-              fc::k_gemm_b16<3><<<g, b, 0, st>>>(P.gitems + l.item0, P.fronts, Lh, Ll, L, U, Part, fexp);
+              fc::k_gemm_b16<3><<<g, b, 0, ls>>>(P.gitems + l.item0, P.fronts, Lh, Ll, L, U, Part, fexp);
             break;
           }
         }
-        fc::k_gemm<T><<<g, b, 0, st>>>(P.gitems + l.item0, L, U, Inv, Part, W);
+        fc::k_gemm<T><<<g, b, 0, ls>>>(P.gitems + l.item0, L, U, Inv, Part, W);
         break;
       case fc::K_SPLIT:
         if constexpr (std::is_same<T, float>::value) {
-          if (mixed) fc::k_split_panel<<<g, b, 0, st>>>(it, P.fronts, L, Lh, Ll, fexp);
+          if (mixed) fc::k_split_panel<<<g, b, 0, ls>>>(it, P.fronts, L, Lh, Ll, fexp);
         }
         break;
       case fc::K_TRSM:
-        if (l.pad == TRSM_COPY) fc::k_copy_tiles<T><<<g, b, 0, st>>>(P.gitems + l.item0, L, W);
-        else fc::k_gemm<T><<<g, b, 0, st>>>(P.gitems + l.item0, L, U, Inv, Part, W);
+        if (l.pad == TRSM_COPY) fc::k_copy_tiles<T><<<g, b, 0, ls>>>(P.gitems + l.item0, L, W);
+        else fc::k_gemm<T><<<g, b, 0, ls>>>(P.gitems + l.item0, L, U, Inv, Part, W);
         break;
       case fc::K_REDUCE:
-        fc::k_reduce_parts<T><<<dim3(l.count * fc::RSPLIT), b, 0, st>>>(P.ritems + l.item0, L, U, Part);
+        fc::k_reduce_parts<T><<<dim3(l.count * fc::RSPLIT), b, 0, ls>>>(P.ritems + l.item0, L, U, Part);
         break;
       case fc::K_POTRF:
-        fc::k_potrf_blk<T><<<g, dim3(256), 0, st>>>(it, P.fronts, L, F->diag, F->fail, Inv);
+        fc::k_potrf_blk<T><<<g, dim3(256), 0, ls>>>(it, P.fronts, L, F->diag, F->fail, Inv);
         break;
       default:
         throw fc::Failure{fc::INTERNAL, -1, 0, 0, "unknown launch"};
     }
     if (a) {
-      HIP_CHECK(hipEventRecord(z, st));
+      HIP_CHECK(hipEventRecord(z, ls));
       timed.push_back({l.kind, {a, z}});
     }
   }
@@ -771,6 +828,7 @@ void enqueue_factor(fc_factor* F, const double* dval, i64 nnz, hipStream_t st, b
 // kind is timed.
 template <typename T>
 void run_factor(fc_factor* F, const double* dval, i64 nnz, hipStream_t st) {
+  F->make_side();
   Events ev;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> timed;
   hipEvent_t e0 = ev.make(), e1 = ev.make();
@@ -1251,7 +1309,8 @@ int fc_symbolic_info_get(const fc_symbolic* h, fc_symbolic_info* info) {
   info->l_bytes_fp64 = (uint64_t)s.l_size * 8;
   info->u_bytes_fp64 = (uint64_t)s.u_size * 8;
   info->by_blocks = s.by_blocks;
-  info->launches = h->prog.launches.size();
+  info->launches = 0;
+  for (const fc::Launch& l : h->prog.launches) info->launches += l.kind < fc::KINDS;  // not the ordering steps
   for (int k = 0; k < fc::KINDS && k < 8; ++k) {
     info->kind_flops[k] = h->prog.work[k][0];
     info->kind_entries[k] = h->prog.work[k][1];
