@@ -55,12 +55,17 @@ __device__ __forceinline__ void inv_apply(const T* __restrict__ inv, int pw, con
 #pragma unroll
       for (int q = 0; q < KB; ++q) s[0] = fma((double)a[q], V[k0 + q][0], s[0]);
     }
-  } else {
-#pragma unroll 4
-    for (int k = 0; k < pw; ++k) {
-      const double a = (double)inv[(int64_t)k * pw + b];
+  } else {  // batches of 32 rows of Inv, each batch's loads in flight together
+    const __amdgpu_buffer_rsrc_t rs = raw_rsrc(inv);
+    for (int k0 = 0; k0 < pw; k0 += 32) {
+      T a[32];
 #pragma unroll
-      for (int r = 0; r < R; ++r) s[r] = fma(a, V[k][r], s[r]);
+      for (int q = 0; q < 32; ++q) a[q] = buf_load<T>(rs, b * sizeof(T), (uint32_t)min(k0 + q, pw - 1) * pw * sizeof(T));
+#pragma unroll
+      for (int q = 0; q < 32; ++q)
+        if (k0 + q < pw)
+#pragma unroll
+          for (int r = 0; r < R; ++r) s[r] = fma((double)a[q], V[k0 + q][r], s[r]);
     }
   }
   for (int r = 0; r < nr; ++r) out[(int64_t)r * ldx + b] = s[r];
@@ -77,7 +82,7 @@ __device__ __forceinline__ void inv_apply_t(const T* __restrict__ inv, int pw, c
   double s[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) s[r] = 0.0;
-  constexpr int KB = R == 1 ? 64 : 4;  // loads per batch (one batch for a single right-hand side)
+  constexpr int KB = R == 1 ? 64 : 16;  // loads per batch (one batch for a single right-hand side)
 #pragma unroll 1
   for (int k0 = 0; k0 < 64; k0 += KB) {
     double a[KB];
@@ -120,19 +125,34 @@ __global__ __launch_bounds__(256, 2) void k_fwd_gather(const Item* __restrict__ 
     const int cmu = C.m - C.w;
     const double* uc = u + uoff[c] * nrhs;
     const int32_t* rc = rel + C.rel_off;
-    for (int r = 0; r < nrhs; ++r)
-      for (int i = threadIdx.x; i < cmu; i += blockDim.x) {
-        const int p = rc[i];
-        const double add = uc[(int64_t)r * cmu + i];
-        if (p < w) v[(int64_t)r * ldx + F.col0 + p] += add;
-        else uf[(int64_t)r * mu + p - w] += add;
+    // a child's rows land on distinct rows of the front (rel rises): each thread's row is read
+    // for every column before it is written
+    for (int i = threadIdx.x; i < cmu; i += blockDim.x) {
+      const int p = rc[i];
+      double* t = p < w ? v + F.col0 + p : uf + p - w;
+      const int64_t ld = p < w ? ldx : mu;
+      for (int r0 = 0; r0 < nrhs; r0 += R) {
+        double add[R], old[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int rr = min(r0 + r, nrhs - 1);
+          add[r] = uc[(int64_t)rr * cmu + i];
+          old[r] = t[(int64_t)rr * ld];
+        }
+        for (int r = 0; r < R && r0 + r < nrhs; ++r) t[(int64_t)(r0 + r) * ld] = old[r] + add[r];
       }
+    }
     __syncthreads();
   }
   for (int q0 = 0; q0 < nrhs; q0 += R) {
     const int nr = min(R, nrhs - q0);
-    for (int r = 0; r < nr; ++r)
-      for (int k = threadIdx.x; k < pw; k += blockDim.x) Vb[k][r] = v[(int64_t)(q0 + r) * ldx + F.col0 + k];
+    if (threadIdx.x < pw) {
+      double vv[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) vv[r] = v[(int64_t)(q0 + min(r, nr - 1)) * ldx + F.col0 + threadIdx.x];
+#pragma unroll
+      for (int r = 0; r < R; ++r) Vb[threadIdx.x][r] = vv[r];
+    }
     __syncthreads();
     inv_apply<T, R>(Inv + F.inv0, pw, Vb, nr, y + (int64_t)q0 * ldx + F.col0, ldx);
     __syncthreads();
@@ -161,8 +181,13 @@ __global__ __launch_bounds__(256, 2) void k_fwd_step(const Item* __restrict__ it
   const bool row = i < F.m;
   for (int q0 = 0; q0 < nrhs; q0 += R) {
     const int nr = min(R, nrhs - q0);
-    for (int r = 0; r < nr; ++r)
-      for (int k = threadIdx.x; k < pw; k += blockDim.x) Yb[k][r] = y[(int64_t)(q0 + r) * ldx + F.col0 + c0 + k];
+    if (threadIdx.x < pw) {  // y of the block: every column's load issued before the stores
+      double yv[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) yv[r] = y[(int64_t)(q0 + min(r, nr - 1)) * ldx + F.col0 + c0 + threadIdx.x];
+#pragma unroll
+      for (int r = 0; r < R; ++r) Yb[threadIdx.x][r] = yv[r];
+    }
     __syncthreads();
     double s[R];
 #pragma unroll
@@ -179,24 +204,30 @@ __global__ __launch_bounds__(256, 2) void k_fwd_step(const Item* __restrict__ it
 #pragma unroll
         for (int q = 0; q < KB; ++q) s[0] = fma((double)l[q], Yb[j0 + q][0], s[0]);
       }
-    } else if (row) {
-#pragma unroll 4
-      for (int j = 0; j < pw; ++j) {
-        const double l = (double)lf[(int64_t)j * F.ld + i];
+    } else {  // batches of 32 columns, each batch's loads in flight together
+      const __amdgpu_buffer_rsrc_t rs = raw_rsrc(lf);
+      const uint32_t lo = min(i, F.m - 1) * sizeof(T);  // rows past the front load its last row (not stored)
+      for (int j0 = 0; j0 < pw; j0 += 32) {
+        T l[32];
 #pragma unroll
-        for (int r = 0; r < R; ++r) s[r] = fma(l, Yb[j][r], s[r]);
+        for (int q = 0; q < 32; ++q) l[q] = buf_load<T>(rs, lo, (uint32_t)min(j0 + q, pw - 1) * F.ld * sizeof(T));
+#pragma unroll
+        for (int q = 0; q < 32; ++q)
+          if (j0 + q < pw)
+#pragma unroll
+            for (int r = 0; r < R; ++r) s[r] = fma((double)l[q], Yb[j0 + q][r], s[r]);
       }
     }
-    if (row) {
-      if (i < F.w) {
-        for (int r = 0; r < nr; ++r) {
-          double* p = v + (int64_t)(q0 + r) * ldx + F.col0 + i;
-          const double nv = *p - s[r];
-          *p = nv;
-          if (it.c && i < c1 + pn) Vn[i - c1][r] = nv;
-        }
-      } else {
-        for (int r = 0; r < nr; ++r) uf[(int64_t)(q0 + r) * mu + i - F.w] -= s[r];
+    if (row) {  // every column's old value read before the first store
+      double* p = i < F.w ? v + F.col0 + i : uf + i - F.w;
+      const int64_t ld = i < F.w ? ldx : mu;
+      double old[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) old[r] = p[(int64_t)(q0 + min(r, nr - 1)) * ld];
+      for (int r = 0; r < nr; ++r) {
+        const double nv = old[r] - s[r];
+        p[(int64_t)(q0 + r) * ld] = nv;
+        if (it.c && i >= c1 && i < c1 + pn) Vn[i - c1][r] = nv;
       }
     }
     if (it.c) {
@@ -220,22 +251,32 @@ __global__ __launch_bounds__(256) void k_bwd_gather(const Item* __restrict__ ite
   const int mu = F.m - F.w;
   const int32_t* below = rows + F.rows_off + F.w;
   double* uf = u + uoff[f] * nrhs;
-  for (int r = 0; r < nrhs; ++r)
-    for (int i = threadIdx.x; i < mu; i += blockDim.x) uf[(int64_t)r * mu + i] = x[(int64_t)r * ldx + below[i]];
+  for (int i = threadIdx.x; i < mu; i += blockDim.x) {
+    const int b = below[i];
+    for (int r0 = 0; r0 < nrhs; r0 += 16) {  // 16 columns' loads in flight, then the stores
+      double xv[16];
+#pragma unroll
+      for (int r = 0; r < 16; ++r) xv[r] = x[(int64_t)min(r0 + r, nrhs - 1) * ldx + b];
+      for (int r = 0; r < 16 && r0 + r < nrhs; ++r) uf[(int64_t)(r0 + r) * mu + i] = xv[r];
+    }
+  }
 }
 
 // Backward: z_j -= L21(:, j)^T x_below for the columns [a, a + SCOLS) of a front (z in y,
 // x_below gathered in u). A wave takes four columns (two with several right-hand sides) at a
-// time with lanes along the rows, so each x_below value is loaded once for them; a column's
-// sum runs over the rows in the
-// same order (lane-strided, then across the wave) for any number of right-hand sides.
-// Item f, a = j0.
+// time with lanes along the rows, so each x_below value is loaded once for them; the rows go
+// in batches of 64 NQ, the next batch's loads issued before the current one is multiplied. A
+// column's sum runs over the rows in the same order (lane-strided, then across the wave) for any
+// number of right-hand sides; every lane ends the butterfly with the same sum, and lane c R + r
+// applies the pair (column c, right-hand side r). Item f, a = j0.
 template <typename T, int R>
 __global__ __launch_bounds__(256) void k_bwd_below(const Item* __restrict__ items, const FrontDev* __restrict__ fronts,
                                                    const T* __restrict__ L, const double* __restrict__ u,
                                                    const int64_t* __restrict__ uoff, double* __restrict__ z,
                                                    int64_t ldx, int nrhs) {
   constexpr int JC = R == 1 ? 4 : 2;  // columns a wave takes at a time
+  constexpr int NQ = R == 1 ? 8 : 2;  // rows a lane loads per batch
+  static_assert(JC * R <= 64, "one lane per (column, right-hand side) pair");
   const Item it = items[blockIdx.x];
   const FrontDev F = fronts[it.f];
   const int mu = F.m - F.w;
@@ -252,26 +293,38 @@ __global__ __launch_bounds__(256) void k_bwd_below(const Item* __restrict__ item
       for (int c = 0; c < JC; ++c)
 #pragma unroll
         for (int r = 0; r < R; ++r) s[c][r] = 0.0;
-      constexpr int NQ = R == 1 ? 8 : 2;  // rows a lane loads per batch
-      for (int i0 = lane; i0 < mu; i0 += 64 * NQ) {
-        double xv[NQ][R], l[NQ][JC];
+      // rows b0 + lane + 64 q (clamped in bounds; rows past mu weigh zero)
+      auto load = [&](int b0, double (&xv)[NQ][R], double (&l)[NQ][JC]) {
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
-          const int i = min(i0 + 64 * q, mu - 1);
+          const int i = min(b0 + lane + 64 * q, mu - 1);
 #pragma unroll
           for (int r = 0; r < R; ++r) xv[q][r] = uf[(int64_t)(q0 + min(r, nr - 1)) * mu + i];
 #pragma unroll
           for (int c = 0; c < JC; ++c) l[q][c] = (double)lf[(int64_t)(jb + min(c, nc - 1)) * F.ld + i];
         }
+      };
+      auto mul = [&](int b0, const double (&xv)[NQ][R], const double (&l)[NQ][JC]) {
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
-          const bool in = i0 + 64 * q < mu;
+          const bool in = b0 + lane + 64 * q < mu;
 #pragma unroll
           for (int c = 0; c < JC; ++c)
 #pragma unroll
             for (int r = 0; r < R; ++r) s[c][r] = fma(in ? l[q][c] : 0.0, xv[q][r], s[c][r]);
         }
+      };
+      double xa[NQ][R], la[NQ][JC], xb[NQ][R], lb[NQ][JC];
+      constexpr int BS = 64 * NQ;
+      if (mu > 0) load(0, xa, la);
+      for (int b0 = 0; b0 < mu; b0 += 2 * BS) {
+        if (b0 + BS < mu) load(b0 + BS, xb, lb);
+        mul(b0, xa, la);
+        if (b0 + BS >= mu) break;
+        if (b0 + 2 * BS < mu) load(b0 + 2 * BS, xa, la);
+        mul(b0 + BS, xb, lb);
       }
+      double mine = 0.0;
 #pragma unroll
       for (int c = 0; c < JC; ++c)
 #pragma unroll
@@ -279,8 +332,10 @@ __global__ __launch_bounds__(256) void k_bwd_below(const Item* __restrict__ item
           double t = s[c][r];
 #pragma unroll
           for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
-          if (lane == 0 && c < nc && r < nr) z[(int64_t)(q0 + r) * ldx + F.col0 + jb + c] -= t;
+          mine = lane == c * R + r ? t : mine;
         }
+      const int c = lane / R, r = lane % R;
+      if (c < nc && r < nr) z[(int64_t)(q0 + r) * ldx + F.col0 + jb + c] -= mine;
     }
   }
 }
@@ -296,8 +351,13 @@ __global__ __launch_bounds__(256) void k_bwd_first(const Item* __restrict__ item
   const int cl = (F.w - 1) / SNB * SNB, pw = F.w - cl;
   for (int q0 = 0; q0 < nrhs; q0 += R) {
     const int nr = min(R, nrhs - q0);
-    for (int r = 0; r < nr; ++r)
-      for (int k = threadIdx.x; k < pw; k += blockDim.x) Zb[k][r] = z[(int64_t)(q0 + r) * ldx + F.col0 + cl + k];
+    if (threadIdx.x < pw) {
+      double zv[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) zv[r] = z[(int64_t)(q0 + min(r, nr - 1)) * ldx + F.col0 + cl + threadIdx.x];
+#pragma unroll
+      for (int r = 0; r < R; ++r) Zb[threadIdx.x][r] = zv[r];
+    }
     __syncthreads();
     inv_apply_t<T, R>(Inv + F.inv0 + (int64_t)(cl / SNB) * SNB * SNB, pw, Zb, nr, x + (int64_t)q0 * ldx + F.col0 + cl,
                       ldx, H);
@@ -331,8 +391,13 @@ __global__ __launch_bounds__(256) void k_bwd_step(const Item* __restrict__ items
   const int nj = min(BCOLS, c0 - j0);
   for (int q0 = 0; q0 < nrhs; q0 += R) {
     const int nr = min(R, nrhs - q0);
-    for (int r = 0; r < nr; ++r)
-      for (int k = tid; k < pw; k += blockDim.x) Xb[k][r] = x[(int64_t)(q0 + r) * ldx + F.col0 + c0 + k];
+    if (tid < pw) {
+      double xv[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) xv[r] = x[(int64_t)(q0 + min(r, nr - 1)) * ldx + F.col0 + c0 + tid];
+#pragma unroll
+      for (int r = 0; r < R; ++r) Xb[tid][r] = xv[r];
+    }
     // thread (row k = tid % 128, every other column): 32 independent loads per half, the next
     // half's issued before the current one is multiplied
     const int lk = tid & (SNB - 1), jb = tid >> 7;
@@ -367,14 +432,27 @@ __global__ __launch_bounds__(256) void k_bwd_step(const Item* __restrict__ items
         for (int r = 0; r < R; ++r) part[g][jq][r] = t[r];
       }
       __syncthreads();
-      // the four partial sums in a fixed order, the same for any number of right-hand sides
-      for (int e = tid; e < hc * nr; e += blockDim.x) {
-        const int jj = e % hc, r = e / hc;
-        const double d = (part[0][jj][r] + part[1][jj][r]) + (part[2][jj][r] + part[3][jj][r]);
-        double* p = z + (int64_t)(q0 + r) * ldx + F.col0 + j0 + h0 + jj;
-        const double nz = *p - d;
-        *p = nz;
-        if (it.c) Zn[h0 + jj][r] = nz;
+      // the four partial sums in a fixed order, the same for any number of right-hand sides;
+      // a thread's entries of z are all read before it writes any
+      {
+        constexpr int PE = (HC * R + 255) / 256;  // entries per thread
+        double old[PE];
+#pragma unroll
+        for (int q = 0; q < PE; ++q) {
+          const int e = min(tid + 256 * q, hc * nr - 1), jj = e % hc, r = e / hc;
+          old[q] = z[(int64_t)(q0 + r) * ldx + F.col0 + j0 + h0 + jj];
+        }
+#pragma unroll
+        for (int q = 0; q < PE; ++q) {
+          const int e = tid + 256 * q;
+          if (e < hc * nr) {
+            const int jj = e % hc, r = e / hc;
+            const double d = (part[0][jj][r] + part[1][jj][r]) + (part[2][jj][r] + part[3][jj][r]);
+            const double nz = old[q] - d;
+            z[(int64_t)(q0 + r) * ldx + F.col0 + j0 + h0 + jj] = nz;
+            if (it.c) Zn[h0 + jj][r] = nz;
+          }
+        }
       }
       __syncthreads();
     }
