@@ -190,7 +190,7 @@ def main():
     x = torch.zeros_like(b)
     A = CH.Matrix(p.n, p.ptr, p.idx, p.kval)
     torch.cuda.synchronize()
-    profile = ["syrk", "update", "extend_l", "extend_u", "trsm", "potrf", "reduce", "split"]
+    profile = ["syrk", "update", "extend_l", "extend_u", "trsm", "potrf", "reduce"]
     legs = {}
     if not args.no_fp64:
         log("bench: fp64 leg")
@@ -225,8 +225,6 @@ def main():
             name, bound, peak = f"k_extend_add<{el}>", "hbm", HBM_PEAK
         elif kind == "potrf":
             name, bound, peak = f"k_potrf_blk<{el}>", "latency", None
-        elif kind == "split":
-            name, bound, peak = "k_split_panel", "hbm", HBM_PEAK
         else:
             name, bound, peak = f"k_reduce_parts<{el}>", "hbm", HBM_PEAK
         g = funcs.setdefault(name, dict(kinds=[], seconds=0.0, launches=0.0, flops=0.0, entries=0.0, bound=bound,
@@ -240,10 +238,7 @@ def main():
     avg = g["seconds"] / max(g["launches"], 1)
     if g["bound"] == "hbm":
         es = 8 if form == "fp64" else 4
-        if dom == "k_split_panel":
-            per_launch = g["entries"] * 8 / max(g["launches"], 1)  # FP32 read, two BF16 written
-        else:
-            per_launch = g["entries"] * (3 * es + 4) / max(g["launches"], 1)  # child entry read, target RMW, row map
+        per_launch = g["entries"] * (3 * es + 4) / max(g["launches"], 1)  # child entry read, target RMW, row map
         achieved = per_launch / avg / 1e9
         roof = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK, unit="GB/s", frac=achieved / HBM_PEAK,
                     traffic=traffic, kernel=dom, kinds=g["kinds"], launches=g["launches"], bytes_per_launch=per_launch,

@@ -71,7 +71,7 @@ struct Launch {
 
 constexpr int KINDS = 8;
 
-constexpr int EA_BATCH = 4;   // rows an extend-add thread moves at a time
+constexpr int EA_BATCH = 8;   // rows an extend-add thread moves at a time
 constexpr int EA_COLS = 8;     // columns of the parent one extend-add workgroup owns
 
 }  // namespace fc

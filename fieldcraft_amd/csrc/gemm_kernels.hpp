@@ -69,8 +69,12 @@ struct Mma<float> {
   static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
-template <typename T>
-__global__ __launch_bounds__(256, 2) void k_gemm(const GemmItem* __restrict__ items, T* L, T* U, T* Inv, T* Part, T* W) {
+// SPLIT (FP32, in-place rows-below products of the mixed form): the epilogue also writes the
+// result's scaled BF16 copies (see k_gemm_b16), so a finished panel needs no pass of its own.
+template <typename T, bool SPLIT = false>
+__global__ __launch_bounds__(256, 2) void k_gemm(const GemmItem* __restrict__ items, T* L, T* U, T* Inv, T* Part, T* W,
+                                                 const FrontDev* __restrict__ fronts, __bf16* __restrict__ Lh,
+                                                 __bf16* __restrict__ Ll, const int32_t* __restrict__ fexp) {
   constexpr int PAD = 4;
   __shared__ T As[2][GK][GB + PAD];
   __shared__ T Bs[2][GK][GB + PAD];
@@ -161,6 +165,36 @@ __global__ __launch_bounds__(256, 2) void k_gemm(const GemmItem* __restrict__ it
           else *p = x;
         }
       }
+  if constexpr (SPLIT) {
+    static_assert(sizeof(T) == 4, "copies of an FP32 result");
+    // a lane's four values of a block are consecutive columns of one row: one 8-byte store a
+    // copy. Columns from n up to the next multiple of 32 (the last panel of a front) are zeros
+    // (the product's B rows past n load as zeros).
+    const FrontDev F = fronts[it.front];
+    const float sc = ldexpf(1.0f, -fexp[it.front]);
+    const int64_t oc = it.c - F.lpos;
+    const int row0 = (int)(oc % F.ld), col0 = (int)(oc / F.ld);
+    const int ncp = (col0 + it.n + 31) / 32 * 32 - col0;
+#pragma unroll
+    for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+      for (int bj = 0; bj < 4; ++bj) {
+        const int i = wi + 16 * bi + il, j = wj + 16 * bj + 4 * (lane >> 4);
+        if (i < it.m && j < ncp) {
+          __bf16 h[4], l[4];
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const float x = (float)acc[bi][bj][v] * sc;
+            h[v] = (__bf16)x;
+            l[v] = (__bf16)(x - (float)h[v]);
+          }
+          const int col = col0 + j;
+          const int64_t o = F.bpos + ((int64_t)(col / 32) * F.ld + row0 + i) * 32 + col % 32;
+          *(uint2*)(Lh + o) = *(const uint2*)h;
+          *(uint2*)(Ll + o) = *(const uint2*)l;
+        }
+      }
+  }
 }
 
 // A tile's entries C(i, j) = A(i, j) (i < m, j < n): the rows below a panel saved before the
@@ -185,46 +219,16 @@ __global__ __launch_bounds__(256) void k_copy_tiles(const GemmItem* __restrict__
 // as hi = bf16(x) and lo = bf16(x - hi) and a product is hi*hi + hi*lo + lo*hi, about 16
 // significant bits; TERMS = 1 keeps hi*hi.
 //
-// The BF16 copies are made once per panel, when it is finished (k_split_panel), in a k-chunked
-// layout: 32 consecutive columns of a row are contiguous, so a 128-row x 32-column stage of an
-// operand is one contiguous 8 KB block per copy. The product kernel moves it into LDS with the
+// The BF16 copies are written once per panel, by the kernels that finish it (the diagonal
+// block's by k_potrf_blk, the rows below by the in-place k_gemm<float, true>), in a k-chunked
+// layout: entry (i, j) at bpos + ((j / 32) ld + i) 32 + j % 32, zero past the front's width up to
+// a multiple of 32, so a 128-row x 32-column stage of an operand is one contiguous 8 KB block
+// per copy and a product's last stage needs no bounds. The product kernel moves it into LDS with the
 // direct global-to-LDS loads (no registers on the way), 1 KB (16 rows) per wave instruction,
 // while the previous stage is multiplied. Tiles, modes and split-k pieces are those of k_gemm
 // (128 x 128, 4 waves of 64 x 64).
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 constexpr int BK = 32;  // k per stage: one MFMA depth, one chunk of the copies
-
-// A finished panel [c0, c0 + NB) of a front into the BF16 copies, rows [r0, r0 + 256) (at or
-// below c0: the rows any later product reads). Columns past the front's width up to the next
-// multiple of 32 get zeros, so a product's last stage needs no bounds. Item: f, a = c0, b = r0.
-__global__ __launch_bounds__(256) void k_split_panel(const Item* __restrict__ items, const FrontDev* __restrict__ fronts,
-                                                     const float* __restrict__ L, __bf16* __restrict__ Lh,
-                                                     __bf16* __restrict__ Ll, const int32_t* __restrict__ fexp) {
-  const Item it = items[blockIdx.x];
-  const FrontDev F = fronts[it.f];
-  const int c0 = it.a, c1 = min(c0 + NB, F.w), cend = min(c0 + NB, (F.w + BK - 1) / BK * BK);
-  const int i = it.b + threadIdx.x;
-  if (i >= F.m) return;
-  const float sc = ldexpf(1.0f, -fexp[it.f]);
-  const float* row = L + F.lpos + i;
-#pragma unroll 4
-  for (int j0 = c0; j0 < cend; j0 += 8) {
-    float x[8];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) x[q] = row[(int64_t)min(j0 + q, c1 - 1) * F.ld];  // in bounds
-#pragma unroll
-    for (int q = 0; q < 8; ++q) x[q] = j0 + q < c1 ? x[q] * sc : 0.0f;
-    v8bf h, l;
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      h[q] = (__bf16)x[q];
-      l[q] = (__bf16)(x[q] - (float)h[q]);
-    }
-    const int64_t o = F.bpos + ((int64_t)(j0 / BK) * F.ld + i) * BK + j0 % BK;
-    *(v8bf*)(Lh + o) = h;
-    *(v8bf*)(Ll + o) = l;
-  }
-}
 
 // LDS address of a shared-memory pointer, and a 16-byte LDS read issued as an instruction the
 // compiler does not track: it would otherwise wait for every direct global-to-LDS load in
@@ -542,7 +546,9 @@ __device__ __forceinline__ void strip64(v4d (&acc)[4], int lane, FC col, FR row)
 template <typename T>
 __global__ __launch_bounds__(256, 1) void k_potrf_blk(const Item* __restrict__ items, const FrontDev* __restrict__ fronts,
                                                       T* __restrict__ L, const double* __restrict__ diag,
-                                                      unsigned long long* __restrict__ fail, T* __restrict__ Inv) {
+                                                      unsigned long long* __restrict__ fail, T* __restrict__ Inv,
+                                                      __bf16* __restrict__ Lh, __bf16* __restrict__ Ll,
+                                                      const int32_t* __restrict__ fexp) {
   constexpr int H = 64;
   __shared__ double S[NB][NB + 1];
   __shared__ double xd[NB];
@@ -555,6 +561,29 @@ __global__ __launch_bounds__(256, 1) void k_potrf_blk(const Item* __restrict__ i
   const int64_t col = (int64_t)F.col0 + c0;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int rr = tid & (NB - 1), jp = tid >> 7;  // a thread's row, and which of every two columns
+  // Mixed form (Lh given): the block's scaled BF16 copies, from the same FP32 values as L (zero
+  // above the diagonal and past the block up to the next multiple of 32 columns), 8 columns of a
+  // row per 16-byte store. S holds the final L here (the products that follow overwrite L21).
+  auto copies = [&]() {
+    if (!Lh) return;
+    const float sc = ldexpf(1.0f, -fexp[it.f]);
+    const int ncp = (c0 + pw + 31) / 32 * 32 - c0;
+    for (int e = tid; e < pw * (ncp / 8); e += blockDim.x) {
+      const int r = e % pw, g = e / pw;
+      v8bf h, l;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int j = 8 * g + q;
+        const float x = (j <= r ? (float)(T)S[r][min(j, NB - 1)] : 0.0f) * sc;
+        h[q] = (__bf16)x;
+        l[q] = (__bf16)(x - (float)h[q]);
+      }
+      const int col = c0 + 8 * g;
+      const int64_t o = F.bpos + ((int64_t)(col / 32) * F.ld + c0 + r) * 32 + col % 32;
+      *(v8bf*)(Lh + o) = h;
+      *(v8bf*)(Ll + o) = l;
+    }
+  };
   {  // the pw x pw square (its upper triangle is zeros in L, in bounds): 64 independent loads
     T v[NB / 2];
 #pragma unroll
@@ -607,7 +636,7 @@ __global__ __launch_bounds__(256, 1) void k_potrf_blk(const Item* __restrict__ i
     __syncthreads();
     if (wave == 0) wave_potrf_inv(S, xd, h0, h1, lane, dg + h0, col + h0, fail, bad);  // L22, X22
     __syncthreads();
-    // L is final: write it out (reads first, then stores)
+    // L is final: write it out (reads first, then stores), and its copies
     {
       T v[NB / 2];
 #pragma unroll
@@ -618,6 +647,7 @@ __global__ __launch_bounds__(256, 1) void k_potrf_blk(const Item* __restrict__ i
         if (rr < pw && j <= rr) base[(int64_t)j * F.ld + rr] = v[qq];
       }
     }
+    copies();
     {  // T = L21 X11: T(i, t) = sum_{k >= t} L21(i, k) X11(k, t), X11(k, t) = S[t][k] (k > t)
       v4d acc[4] = {};
       strip64(
@@ -661,6 +691,7 @@ __global__ __launch_bounds__(256, 1) void k_potrf_blk(const Item* __restrict__ i
       const int j = jp + 2 * q;
       if (rr < pw && j <= rr) base[(int64_t)j * F.ld + rr] = v[q];
     }
+    copies();
   }
   __syncthreads();
   {  // the inverse, column-major with ld = pw, zeros above the diagonal: reads, then stores

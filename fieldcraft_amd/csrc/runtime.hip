@@ -365,18 +365,6 @@ Program build_program(const fc::Symbolic& s) {
         }
         p.work[fc::K_TRSM][0] = before;  // the algorithmic flops are the in-place product's
       }
-      // the finished panel's BF16 copies (run by the mixed form only)
-      i0 = (i64)p.items.size();
-      for (i64 t = 0; t < nl; ++t) {
-        const i64 f = fl[t];
-        if (s.f_w[f] <= c0) continue;
-        const i64 c1 = std::min(c0 + fc::NB, s.f_w[f]);
-        for (i64 r = c0; r < s.f_m[f]; r += 256) {
-          p.items.push_back(Item{(int32_t)f, (int32_t)c0, (int32_t)r, 0, 0, 0, 0, 0});
-          entries += (double)(std::min<i64>(r + 256, s.f_m[f]) - r) * (double)(c1 - c0);
-        }
-      }
-      add(fc::K_SPLIT, l, i0);
     };
     auto sync = [&](int kind, int stream, int slot) {
       Launch q{kind, (int32_t)l, 0, 0, 0, 0.0, 0.0};
@@ -767,7 +755,6 @@ void enqueue_factor(fc_factor* F, const double* dval, i64 nnz, hipStream_t st, b
       else HIP_CHECK(hipStreamWaitEvent(on, F->sync_ev[l.slot], 0));
       continue;
     }
-    if (l.kind == fc::K_SPLIT && !mixed) continue;  // the BF16 copies serve the mixed form only
     if (l.kind == fc::K_TRSM && (l.pad == TRSM_IN_PLACE) == (F->precision == FC_FP64)) continue;
     const fc::Item* it = P.items + l.item0;
     hipStream_t ls = l.stream ? F->side : st;  // the launch's stream
@@ -794,22 +781,26 @@ void enqueue_factor(fc_factor* F, const double* dval, i64 nnz, hipStream_t st, b
             break;
           }
         }
-        fc::k_gemm<T><<<g, b, 0, ls>>>(P.gitems + l.item0, L, U, Inv, Part, W);
-        break;
-      case fc::K_SPLIT:
-        if constexpr (std::is_same<T, float>::value) {
-          if (mixed) fc::k_split_panel<<<g, b, 0, ls>>>(it, P.fronts, L, Lh, Ll, fexp);
-        }
+        fc::k_gemm<T><<<g, b, 0, ls>>>(P.gitems + l.item0, L, U, Inv, Part, W, P.fronts, nullptr, nullptr, nullptr);
         break;
       case fc::K_TRSM:
-        if (l.pad == TRSM_COPY) fc::k_copy_tiles<T><<<g, b, 0, ls>>>(P.gitems + l.item0, L, W);
-        else fc::k_gemm<T><<<g, b, 0, ls>>>(P.gitems + l.item0, L, U, Inv, Part, W);
+        if (l.pad == TRSM_COPY) {
+          fc::k_copy_tiles<T><<<g, b, 0, ls>>>(P.gitems + l.item0, L, W);
+        } else {
+          if constexpr (std::is_same<T, float>::value) {
+            if (mixed) {  // the in-place product also writes the rows' BF16 copies
+              fc::k_gemm<T, true><<<g, b, 0, ls>>>(P.gitems + l.item0, L, U, Inv, Part, W, P.fronts, Lh, Ll, fexp);
+              break;
+            }
+          }
+          fc::k_gemm<T><<<g, b, 0, ls>>>(P.gitems + l.item0, L, U, Inv, Part, W, P.fronts, nullptr, nullptr, nullptr);
+        }
         break;
       case fc::K_REDUCE:
         fc::k_reduce_parts<T><<<dim3(l.count * fc::RSPLIT), b, 0, ls>>>(P.ritems + l.item0, L, U, Part);
         break;
       case fc::K_POTRF:
-        fc::k_potrf_blk<T><<<g, dim3(256), 0, ls>>>(it, P.fronts, L, F->diag, F->fail, Inv);
+        fc::k_potrf_blk<T><<<g, dim3(256), 0, ls>>>(it, P.fronts, L, F->diag, F->fail, Inv, Lh, Ll, fexp);
         break;
       default:
         throw fc::Failure{fc::INTERNAL, -1, 0, 0, "unknown launch"};

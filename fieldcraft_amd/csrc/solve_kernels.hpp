@@ -179,8 +179,26 @@ __global__ __launch_bounds__(256, 2) void k_fwd_step(const Item* __restrict__ it
   double* uf = u + uoff[it.f] * nrhs;
   const int i = r0 + threadIdx.x;
   const bool row = i < F.m;
+  const __amdgpu_buffer_rsrc_t rs = raw_rsrc(lf);
+  const uint32_t lo = min(i, F.m - 1) * sizeof(T);  // rows past the front load its last row (not stored)
+  double* p = i < F.w ? v + F.col0 + i : uf + i - F.w;  // the row's entry of v (or of u below the front)
+  const int64_t ldp = i < F.w ? ldx : mu;
   for (int q0 = 0; q0 < nrhs; q0 += R) {
     const int nr = min(R, nrhs - q0);
+    // Loads that do not wait on y go first: the row's old values and, for a full block and one
+    // right-hand side, its first (FP32: only) batch of L.
+    double old[R];
+    if (row)
+#pragma unroll
+      for (int r = 0; r < R; ++r) old[r] = p[(int64_t)(q0 + min(r, nr - 1)) * ldp];
+    // (KP columns of L per batch: the first batch goes out before the barrier, each later one
+    // before the previous is multiplied)
+    constexpr int KP = sizeof(T) == 4 ? 64 : 32;
+    const bool full1 = R == 1 && pw == SNB;
+    T lb[2][KP];
+    if (full1)
+#pragma unroll
+      for (int q = 0; q < KP; ++q) lb[0][q] = buf_load<T>(rs, lo, (uint32_t)q * F.ld * sizeof(T));
     if (threadIdx.x < pw) {  // y of the block: every column's load issued before the stores
       double yv[R];
 #pragma unroll
@@ -192,21 +210,17 @@ __global__ __launch_bounds__(256, 2) void k_fwd_step(const Item* __restrict__ it
     double s[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) s[r] = 0.0;
-    if (R == 1 && pw == SNB) {  // a full block, one right-hand side: the whole row in flight at once
-      constexpr int KB = sizeof(T) == 4 ? SNB : SNB / 2;
-      const __amdgpu_buffer_rsrc_t rs = raw_rsrc(lf);
-      const uint32_t lo = min(i, F.m - 1) * sizeof(T);  // rows past the front load its last row (not stored)
+    if (full1) {  // a full block, one right-hand side
 #pragma unroll
-      for (int j0 = 0; j0 < SNB; j0 += KB) {
-        T l[KB];
+      for (int j0 = 0; j0 < SNB; j0 += KP) {
+        const int cur = (j0 / KP) & 1;
+        if (j0 + KP < SNB)
 #pragma unroll
-        for (int q = 0; q < KB; ++q) l[q] = buf_load<T>(rs, lo, (uint32_t)(j0 + q) * F.ld * sizeof(T));
+          for (int q = 0; q < KP; ++q) lb[cur ^ 1][q] = buf_load<T>(rs, lo, (uint32_t)(j0 + KP + q) * F.ld * sizeof(T));
 #pragma unroll
-        for (int q = 0; q < KB; ++q) s[0] = fma((double)l[q], Yb[j0 + q][0], s[0]);
+        for (int q = 0; q < KP; ++q) s[0] = fma((double)lb[cur][q], Yb[j0 + q][0], s[0]);
       }
     } else {  // batches of 32 columns, each batch's loads in flight together
-      const __amdgpu_buffer_rsrc_t rs = raw_rsrc(lf);
-      const uint32_t lo = min(i, F.m - 1) * sizeof(T);  // rows past the front load its last row (not stored)
       for (int j0 = 0; j0 < pw; j0 += 32) {
         T l[32];
 #pragma unroll
@@ -218,18 +232,12 @@ __global__ __launch_bounds__(256, 2) void k_fwd_step(const Item* __restrict__ it
             for (int r = 0; r < R; ++r) s[r] = fma((double)l[q], Yb[j0 + q][r], s[r]);
       }
     }
-    if (row) {  // every column's old value read before the first store
-      double* p = i < F.w ? v + F.col0 + i : uf + i - F.w;
-      const int64_t ld = i < F.w ? ldx : mu;
-      double old[R];
-#pragma unroll
-      for (int r = 0; r < R; ++r) old[r] = p[(int64_t)(q0 + min(r, nr - 1)) * ld];
+    if (row)
       for (int r = 0; r < nr; ++r) {
         const double nv = old[r] - s[r];
-        p[(int64_t)(q0 + r) * ld] = nv;
+        p[(int64_t)(q0 + r) * ldp] = nv;
         if (it.c && i >= c1 && i < c1 + pn) Vn[i - c1][r] = nv;
       }
-    }
     if (it.c) {
       __syncthreads();
       inv_apply<T, R>(Inv + F.inv0 + (int64_t)(c1 / SNB) * SNB * SNB, pn, Vn, nr,
