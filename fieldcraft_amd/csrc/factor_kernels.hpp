@@ -92,28 +92,54 @@ __global__ __launch_bounds__(256) void k_extend_add(const Item* __restrict__ ite
     while (lo < hi) { int mid = (lo + hi) >> 1; if (r[mid] < j1) lo = mid + 1; else hi = mid; }
     const int jc1 = lo;
     const T* cu = U + C.upos;
-    for (int jc = jc0; jc < jc1; ++jc) {
-      const int jp = r[jc];
-      const T* src = cu + (int64_t)jc * cmu;
-      T* dst = jp < pw ? L + P.lpos + (int64_t)jp * P.ld : Uw + P.upos + (int64_t)(jp - pw) * pmu - pw;
-      // EA_BATCH rows a thread at a time: the row map and the child's entries, then the targets,
-      // then the sums (rel rises, so the rows of one column land on distinct targets). Rows past
-      // the column read its last row (in bounds) and store nothing.
-      for (int i0 = jc + threadIdx.x; i0 < cmu; i0 += EA_BATCH * blockDim.x) {
-        int ri[EA_BATCH];
-        T sv[EA_BATCH], dv[EA_BATCH];
+    // The child's entries that land in [j0, j1): columns jc0 + t (t < nc <= EA_COLS, rel is
+    // injective), rows jc0 + t .. cmu - 1 of each, taken as one flat range so that small
+    // children still fill the workgroup. Entry e of the range is column t with
+    // off[t] <= e < off[t + 1], row jc0 + t + e - off[t].
+    const int nc = jc1 - jc0;
+    int off[EA_COLS + 1];
+    T* dcol[EA_COLS];
+    off[0] = 0;
 #pragma unroll
-        for (int k = 0; k < EA_BATCH; ++k) {
-          const int ic = min(i0 + k * (int)blockDim.x, cmu - 1);
-          ri[k] = r[ic];
-          sv[k] = src[ic];
-        }
+    for (int t = 0; t < EA_COLS; ++t) {
+      const int jc = min(jc0 + min(t, max(nc - 1, 0)), cmu - 1);  // in bounds
+      off[t + 1] = off[t] + (t < nc ? cmu - (jc0 + t) : 0);
+      const int jp = nc > 0 ? r[jc] : 0;
+      dcol[t] = jp < pw ? L + P.lpos + (int64_t)jp * P.ld : Uw + P.upos + (int64_t)(jp - pw) * pmu - pw;
+    }
+    const int total = off[EA_COLS];
+    // EA_BATCH entries a thread at a time: the row map and the child's entries, then the
+    // targets, then the sums (distinct targets within one child). Entries past the range read
+    // its last entry (in bounds) and store nothing.
+    for (int e0 = threadIdx.x; e0 < total; e0 += EA_BATCH * blockDim.x) {
+      int ri[EA_BATCH], tc[EA_BATCH];
+      T sv[EA_BATCH], dv[EA_BATCH];
 #pragma unroll
-        for (int k = 0; k < EA_BATCH; ++k) dv[k] = dst[ri[k]];
+      for (int k = 0; k < EA_BATCH; ++k) {
+        const int e = min(e0 + k * (int)blockDim.x, total - 1);
+        int t = 0;
 #pragma unroll
-        for (int k = 0; k < EA_BATCH; ++k)
-          if (i0 + k * (int)blockDim.x < cmu) dst[ri[k]] = dv[k] + sv[k];
+        for (int q2 = 1; q2 < EA_COLS; ++q2) t += e >= off[q2];
+        const int ic = jc0 + t + e - off[t];
+        tc[k] = t;
+        ri[k] = r[ic];
+        sv[k] = cu[(int64_t)(jc0 + t) * cmu + ic];
       }
+#pragma unroll
+      for (int k = 0; k < EA_BATCH; ++k) {
+        T* d = dcol[0];
+#pragma unroll
+        for (int q2 = 1; q2 < EA_COLS; ++q2) d = tc[k] == q2 ? dcol[q2] : d;
+        dv[k] = d[ri[k]];
+      }
+#pragma unroll
+      for (int k = 0; k < EA_BATCH; ++k)
+        if (e0 + k * (int)blockDim.x < total) {
+          T* d = dcol[0];
+#pragma unroll
+          for (int q2 = 1; q2 < EA_COLS; ++q2) d = tc[k] == q2 ? dcol[q2] : d;
+          d[ri[k]] = dv[k] + sv[k];
+        }
     }
     __syncthreads();
   }
