@@ -143,11 +143,14 @@ struct Tile {
 // Output tiles a launch should have to fill the GPU (256 CUs, two workgroups each, and some
 // slack); a launch with fewer is split along k into pieces of at least SPLIT_MIN_K.
 constexpr i64 SPLIT_TARGET = 1024;
+// The side stream's panel updates only have to keep ahead of the main stream's panel chain: they
+// are split less, so they leave CUs to the chain's kernels and need fewer partial sums.
+constexpr i64 SPLIT_TARGET_SIDE = 384;
 constexpr i64 SPLIT_MIN_K = 512;
 
 // One GEMM launch of `tiles` (kind: UPDATE, TRSM or SYRK), split along k when the tiles are
 // too few, followed by the launch that reduces the pieces.
-void add_gemm(Program& p, const std::vector<Tile>& tiles, int kind, i64 level) {
+void add_gemm(Program& p, const std::vector<Tile>& tiles, int kind, i64 level, i64 split_target = SPLIT_TARGET) {
   if (tiles.empty()) return;
   const i64 nt = (i64)tiles.size();
   const i64 g0 = (i64)p.gitems.size(), r0 = (i64)p.ritems.size();
@@ -156,8 +159,8 @@ void add_gemm(Program& p, const std::vector<Tile>& tiles, int kind, i64 level) {
   for (const Tile& t : tiles) {
     flops += t.flops;
     i64 split = 1;
-    if (nt < SPLIT_TARGET && t.g.k >= 2 * SPLIT_MIN_K)
-      split = std::min<i64>((SPLIT_TARGET + nt - 1) / nt, t.g.k / SPLIT_MIN_K);
+    if (nt < split_target && t.g.k >= 2 * SPLIT_MIN_K)
+      split = std::min<i64>((split_target + nt - 1) / nt, t.g.k / SPLIT_MIN_K);
     if (split <= 1) {
       p.gitems.push_back(t.g);
       continue;
@@ -258,7 +261,7 @@ Program build_program(const fc::Symbolic& s) {
     for (i64 t = 0; t < nl; ++t) wmax = std::max(wmax, s.f_w[fl[t]]);
     std::vector<Tile> tiles;
     // The panel at c0 minus its products with the front's columns [k0, k1) (all before c0).
-    auto update = [&](i64 c0, i64 k0, i64 k1) {
+    auto update = [&](i64 c0, i64 k0, i64 k1, i64 split_target) {
       tiles.clear();
       for (i64 t = 0; t < nl; ++t) {
         const i64 f = fl[t];
@@ -281,7 +284,7 @@ Program build_program(const fc::Symbolic& s) {
           tiles.push_back(Tile{g, 2.0 * (double)(k1 - k0) * lower_pairs(r, r + m, c0, c1)});
         }
       }
-      add_gemm(p, tiles, fc::K_UPDATE, l);
+      add_gemm(p, tiles, fc::K_UPDATE, l, split_target);
     };
     // The panel at c0 once its updates are in: diagonal block, rows below, BF16 copies.
     auto chain = [&](i64 c0) {
@@ -383,14 +386,14 @@ Program build_program(const fc::Symbolic& s) {
       if (k >= np) return;
       sync(L_WAIT, 1, (int)((k - 2) % 2));
       const i64 from = (i64)p.launches.size();
-      update(k * fc::NB, 0, (k - 1) * fc::NB);
+      update(k * fc::NB, 0, (k - 1) * fc::NB, SPLIT_TARGET_SIDE);
       for (i64 q = from; q < (i64)p.launches.size(); ++q) p.launches[q].stream = 1;
       sync(L_RECORD, 1, 2 + (int)(k % 2));
     };
     for (i64 k = 0; k < np; ++k) {
       const i64 c0 = k * fc::NB;
       if (k >= 2) sync(L_WAIT, 0, 2 + (int)(k % 2));
-      if (k >= 1) update(c0, c0 - fc::NB, c0);  // the product with the panel just finished
+      if (k >= 1) update(c0, c0 - fc::NB, c0, SPLIT_TARGET);  // the product with the panel just finished
       chain(c0);
       if (k + 2 < np) {
         sync(L_RECORD, 0, (int)(k % 2));
