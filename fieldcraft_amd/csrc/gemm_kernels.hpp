@@ -432,7 +432,11 @@ constexpr double PIVOT_TOL = 1e-10;  // cholesky.rs:48
 // 8 x 8 register blocks the step can change are touched: (r, c) with r >= c >= CB for L and
 // r >= CB, c <= CB for X; within them the remaining conditions are per-lane selects, not
 // branches.
-template <int CB>
+// EXACT (FP64 storage): d = sqrt(p) and column / d, correctly rounded like the reference.
+// Otherwise (FP32 storage) the step takes rd from the hardware reciprocal square root with two
+// Newton steps and multiplies: the values are rounded to FP32 afterwards, and the step's
+// dependent chain (the part that sets this kernel's time) is shorter.
+template <int CB, bool EXACT>
 __device__ __forceinline__ void wpi_steps(double (&a)[8][8], double (&x)[8][8], int h, int lane, double dl,
                                           int64_t col, unsigned long long* __restrict__ fail, bool& bad) {
   const int ti = lane & 7, tk = lane >> 3;
@@ -445,13 +449,23 @@ __device__ __forceinline__ void wpi_steps(double (&a)[8][8], double (&x)[8][8], 
       atomicMin(fail, (unsigned long long)(col + j));
       bad = true;
     }
-    const double d = sqrt(p), rd = 1.0 / d;
+    double d, rd;
+    if constexpr (EXACT) {
+      d = sqrt(p);
+      rd = 1.0 / d;
+    } else {
+      const double y = __builtin_amdgcn_rsq(p), hp = 0.5 * p;
+      const double y1 = y * fma(-hp * y, y, 1.5);
+      rd = y1 * fma(-hp * y1, y1, 1.5);
+      d = p * rd;
+    }
     {  // column j of L: rows below scaled, the pivot replaced by d (lanes tk == cc own it)
       const bool own = tk == cc;
 #pragma unroll
       for (int r = CB; r < 8; ++r) {
         const bool below = r > CB || ti > cc, diag = r == CB && ti == cc;
-        a[r][CB] = own ? (below ? a[r][CB] / d : (diag ? d : a[r][CB])) : a[r][CB];
+        const double sc = EXACT ? a[r][CB] / d : a[r][CB] * rd;
+        a[r][CB] = own ? (below ? sc : (diag ? d : a[r][CB])) : a[r][CB];
       }
     }
     {  // row j of X divided by L(j, j) (lanes ti == cc own it)
@@ -485,6 +499,7 @@ __device__ __forceinline__ void wpi_steps(double (&a)[8][8], double (&x)[8][8], 
   }
 }
 
+template <bool EXACT>
 __device__ inline void wave_potrf_inv(double (*S)[NB + 1], double* xd, int o, int h, int lane,
                                       const double* __restrict__ dg, int64_t col,
                                       unsigned long long* __restrict__ fail, bool& bad) {
@@ -501,14 +516,14 @@ __device__ inline void wave_potrf_inv(double (*S)[NB + 1], double* xd, int o, in
       a[r][c] = (i < h && k < h && i >= k) ? v : (i == k ? 1.0 : 0.0);
       x[r][c] = i == k ? 1.0 : 0.0;
     }
-  wpi_steps<0>(a, x, h, lane, dl, col, fail, bad);
-  wpi_steps<1>(a, x, h, lane, dl, col, fail, bad);
-  wpi_steps<2>(a, x, h, lane, dl, col, fail, bad);
-  wpi_steps<3>(a, x, h, lane, dl, col, fail, bad);
-  wpi_steps<4>(a, x, h, lane, dl, col, fail, bad);
-  wpi_steps<5>(a, x, h, lane, dl, col, fail, bad);
-  wpi_steps<6>(a, x, h, lane, dl, col, fail, bad);
-  wpi_steps<7>(a, x, h, lane, dl, col, fail, bad);
+  wpi_steps<0, EXACT>(a, x, h, lane, dl, col, fail, bad);
+  wpi_steps<1, EXACT>(a, x, h, lane, dl, col, fail, bad);
+  wpi_steps<2, EXACT>(a, x, h, lane, dl, col, fail, bad);
+  wpi_steps<3, EXACT>(a, x, h, lane, dl, col, fail, bad);
+  wpi_steps<4, EXACT>(a, x, h, lane, dl, col, fail, bad);
+  wpi_steps<5, EXACT>(a, x, h, lane, dl, col, fail, bad);
+  wpi_steps<6, EXACT>(a, x, h, lane, dl, col, fail, bad);
+  wpi_steps<7, EXACT>(a, x, h, lane, dl, col, fail, bad);
   // L below and on the diagonal, X above it (transposed), X's diagonal in xd; every lane
   // stores to in-bounds addresses, the entries outside the h x h block are not written
 #pragma unroll
@@ -596,7 +611,7 @@ __global__ __launch_bounds__(256, 1) void k_potrf_blk(const Item* __restrict__ i
   }
   __syncthreads();
   bool bad = false;
-  if (wave == 0) wave_potrf_inv(S, xd, 0, h0, lane, dg, col, fail, bad);  // L11, X11
+  if (wave == 0) wave_potrf_inv<sizeof(T) == 8>(S, xd, 0, h0, lane, dg, col, fail, bad);  // L11, X11
   __syncthreads();
   if (h1 > 0) {
     // each wave: strip columns t0 + [0, 16) of a 64 x 64 product, rows 16 ib + lane % 16
@@ -634,7 +649,7 @@ __global__ __launch_bounds__(256, 1) void k_potrf_blk(const Item* __restrict__ i
         }
     }
     __syncthreads();
-    if (wave == 0) wave_potrf_inv(S, xd, h0, h1, lane, dg + h0, col + h0, fail, bad);  // L22, X22
+    if (wave == 0) wave_potrf_inv<sizeof(T) == 8>(S, xd, h0, h1, lane, dg + h0, col + h0, fail, bad);  // L22, X22
     __syncthreads();
     // L is final: write it out (reads first, then stores), and its copies
     {
