@@ -432,6 +432,14 @@ constexpr double PIVOT_TOL = 1e-10;  // cholesky.rs:48
 // 8 x 8 register blocks the step can change are touched: (r, c) with r >= c >= CB for L and
 // r >= CB, c <= CB for X; within them the remaining conditions are per-lane selects, not
 // branches.
+// A double from one lane (uniform index) to the whole wave, by scalar reads of its halves.
+__device__ __forceinline__ double readlane_d(double v, int l) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = __builtin_amdgcn_readlane((int)(unsigned)b, l);
+  const unsigned hi = __builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
 // EXACT (FP64 storage): d = sqrt(p) and column / d, correctly rounded like the reference.
 // Otherwise (FP32 storage) the step takes rd from the hardware reciprocal square root with two
 // Newton steps and multiplies: the values are rounded to FP32 afterwards, and the step's
@@ -443,8 +451,8 @@ __device__ __forceinline__ void wpi_steps(double (&a)[8][8], double (&x)[8][8], 
   for (int cc = 0; cc < 8; ++cc) {
     const int j = 8 * CB + cc;
     if (j >= h) return;
-    const double p = __shfl(a[CB][CB], cc + 8 * cc);
-    const double a0 = __shfl(dl, j);
+    const double p = readlane_d(a[CB][CB], cc + 8 * cc);  // the pivot, on the step's dependent chain
+    const double a0 = readlane_d(dl, j);
     if (lane == 0 && !bad && (!(p > PIVOT_TOL * a0) || !isfinite(p))) {
       atomicMin(fail, (unsigned long long)(col + j));
       bad = true;
