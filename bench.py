@@ -245,10 +245,12 @@ def main():
                     avg_launch_s=avg)
     else:
         achieved = g["flops"] / g["seconds"] / 1e12
+        products = args.bf16_terms if "bf16" in dom else 1
         roof = dict(bound="mfma", achieved=achieved, peak=g["peak"], unit="TFLOP/s", frac=achieved / g["peak"],
                     traffic=traffic, kernel=dom, kinds=g["kinds"], launches=g["launches"],
                     flops_per_launch=g["flops"] / max(g["launches"], 1), avg_launch_s=avg,
-                    mfma_products_per_flop=args.bf16_terms if "bf16" in dom else 1)
+                    mfma_products_per_flop=products, mfma_products_tflops=achieved * products,
+                    mfma_products_frac=achieved * products / g["peak"])
     line = dict(metric=METRIC, value=F / t_step / 1e12, unit="TFLOP/s", n_gpus=1, steps=args.steps,
                 warmup=args.warmup, ms_per_step=t_step * 1e3, higher_is_better=True, scaling="weak", vs_baseline=None,
                 dtype={"mixed": "bf16/fp32 factor, fp64 refinement", "fp32": "fp32 factor, fp64 refinement",
@@ -259,6 +261,22 @@ def main():
                             equations=p.n, factor_entries=info["entries"], flops=F, form=form,
                             target_berr=1e-12),
                 roofline=roof)
+    # The factor's bound over its precision classes: Schur products (SYRK, UPDATE) at the MFMA
+    # peak of their class (three BF16 products per flop in the mixed form), TRSM at the FP32
+    # (FP64) peak, diagonal blocks (FP64 arithmetic) at the FP64 peak.
+    def factor_bound(leg, form_):
+        k = legs[leg]["kinds"]
+        schur = k["syrk"]["flops"] + k["update"]["flops"]
+        if form_ == "mixed":
+            t_schur = args.bf16_terms * schur / (PEAK["bf16"] * 1e12)
+        else:
+            t_schur = schur / (PEAK["fp64" if form_ == "fp64" else "fp32"] * 1e12)
+        t_trsm = k["trsm"]["flops"] / (PEAK["fp64" if form_ == "fp64" else "fp32"] * 1e12)
+        t_diag = k["potrf"]["flops"] / (PEAK["fp64"] * 1e12)
+        b = t_schur + t_trsm + t_diag
+        return dict(bound_s=b, factor_s=legs[leg]["factor_s"], frac=b / legs[leg]["factor_s"],
+                    schur_flops=schur, trsm_flops=k["trsm"]["flops"], diag_flops=k["potrf"]["flops"])
+    line["factor_roofline"] = {leg: factor_bound(leg, leg) for leg in legs}
     if "fp64" in legs:
         f64 = legs["fp64"]
         t64 = f64["wall_s"]
