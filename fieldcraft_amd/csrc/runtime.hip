@@ -115,6 +115,10 @@ struct Program {
 // FP64 factor's saved-copy, product, residual and correction steps
 enum TrsmVariant : int32_t { TRSM_IN_PLACE = 0, TRSM_COPY = 1, TRSM_PRODUCT = 2, TRSM_RESIDUAL = 3, TRSM_FIX = 4 };
 
+// Side-stream panel updates (Launch.pad of their UPDATE and REDUCE launches): one variant for the
+// FP64 factor, one for the FP32 and mixed ones; each factor runs its own.
+enum SideVariant : int32_t { SIDE_ANY = 0, SIDE_FP64 = 1, SIDE_FP32 = 2 };
+
 // Ordering steps between the caller's stream and the factor's side stream (Launch.kind values
 // outside the launch kinds): record event slot `slot` on stream `stream`, or make `stream` wait
 // for that slot's last record.
@@ -144,8 +148,11 @@ struct Tile {
 // slack); a launch with fewer is split along k into pieces of at least SPLIT_MIN_K.
 constexpr i64 SPLIT_TARGET = 1024;
 // The side stream's panel updates only have to keep ahead of the main stream's panel chain: they
-// are split less, so they leave CUs to the chain's kernels and need fewer partial sums.
-constexpr i64 SPLIT_TARGET_SIDE = 384;
+// are split less, so they leave CUs to the chain's kernels and need fewer partial sums. How much
+// less depends on the chain's length: measured best at 80^3, 384 tiles for the FP64 factor and
+// 144 for the FP32 and mixed ones (whose chain is shorter).
+constexpr i64 SPLIT_TARGET_SIDE_FP64 = 384;
+constexpr i64 SPLIT_TARGET_SIDE_FP32 = 144;
 constexpr i64 SPLIT_MIN_K = 512;
 
 // One GEMM launch of `tiles` (kind: UPDATE, TRSM or SYRK), split along k when the tiles are
@@ -385,9 +392,16 @@ Program build_program(const fc::Symbolic& s) {
     auto side = [&](i64 k) {  // panel k's products with panels [0, k - 1), after panel k - 2
       if (k >= np) return;
       sync(L_WAIT, 1, (int)((k - 2) % 2));
-      const i64 from = (i64)p.launches.size();
-      update(k * fc::NB, 0, (k - 1) * fc::NB, SPLIT_TARGET_SIDE);
-      for (i64 q = from; q < (i64)p.launches.size(); ++q) p.launches[q].stream = 1;
+      const double before = p.work[fc::K_UPDATE][0];
+      for (const int variant : {SIDE_FP64, SIDE_FP32}) {
+        const i64 from = (i64)p.launches.size();
+        update(k * fc::NB, 0, (k - 1) * fc::NB, variant == SIDE_FP64 ? SPLIT_TARGET_SIDE_FP64 : SPLIT_TARGET_SIDE_FP32);
+        for (i64 q = from; q < (i64)p.launches.size(); ++q) {
+          p.launches[q].stream = 1;
+          p.launches[q].pad = variant;
+        }
+        if (variant == SIDE_FP64) p.work[fc::K_UPDATE][0] = before;  // count the products once
+      }
       sync(L_RECORD, 1, 2 + (int)(k % 2));
     };
     for (i64 k = 0; k < np; ++k) {
@@ -759,6 +773,9 @@ void enqueue_factor(fc_factor* F, const double* dval, i64 nnz, hipStream_t st, b
       continue;
     }
     if (l.kind == fc::K_TRSM && (l.pad == TRSM_IN_PLACE) == (F->precision == FC_FP64)) continue;
+    if ((l.kind == fc::K_UPDATE || l.kind == fc::K_REDUCE) && l.pad != SIDE_ANY &&
+        (l.pad == SIDE_FP64) != (F->precision == FC_FP64))
+      continue;  // the other precision's variant of a side-stream update
     const fc::Item* it = P.items + l.item0;
     hipStream_t ls = l.stream ? F->side : st;  // the launch's stream
     dim3 g(l.count), b(256);
