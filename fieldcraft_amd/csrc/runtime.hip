@@ -383,15 +383,16 @@ Program build_program(const fc::Symbolic& s) {
       p.launches.push_back(q);
     };
     // Panels of NB columns, left-looking inside each front (cholesky.rs:1138-1186), one panel
-    // of lookahead: the side stream takes panel k's products with panels 0 .. k - 2 (as soon as
-    // panel k - 2 is finished) while the main stream finishes panel k - 1; the main stream then
-    // subtracts panel k - 1's product and factors panel k. Events: slot k % 2 marks "panel k
-    // finished" (main), slot 2 + k % 2 "panel k's side update done" (side); each is waited on
+    // of lookahead: the side stream takes panel k + 1's products with panels 0 .. k - 1 while
+    // the main stream factors panel k; the main stream then subtracts panel k's product from
+    // panel k + 1 and factors it. The side update starts only once the main stream's own short
+    // product of the step is done (it needs panel k - 1, which that product follows), so that
+    // product never competes with it for the CUs. Events: slot k % 2 marks "the main stream's
+    // product of panel k done", slot 2 + k % 2 "panel k's side update done"; each is waited on
     // before it is recorded again.
     const i64 np = (wmax + fc::NB - 1) / fc::NB;
-    auto side = [&](i64 k) {  // panel k's products with panels [0, k - 1), after panel k - 2
-      if (k >= np) return;
-      sync(L_WAIT, 1, (int)((k - 2) % 2));
+    auto side = [&](i64 k) {  // panel k's products with panels [0, k - 1)
+      sync(L_WAIT, 1, (int)((k - 1) % 2));
       const double before = p.work[fc::K_UPDATE][0];
       for (const int variant : {SIDE_FP64, SIDE_FP32}) {
         const i64 from = (i64)p.launches.size();
@@ -407,12 +408,14 @@ Program build_program(const fc::Symbolic& s) {
     for (i64 k = 0; k < np; ++k) {
       const i64 c0 = k * fc::NB;
       if (k >= 2) sync(L_WAIT, 0, 2 + (int)(k % 2));
-      if (k >= 1) update(c0, c0 - fc::NB, c0, SPLIT_TARGET);  // the product with the panel just finished
-      chain(c0);
-      if (k + 2 < np) {
-        sync(L_RECORD, 0, (int)(k % 2));
-        side(k + 2);
+      if (k >= 1) {
+        update(c0, c0 - fc::NB, c0, SPLIT_TARGET);  // the product with the panel just finished
+        if (k + 1 < np) {
+          sync(L_RECORD, 0, (int)(k % 2));
+          side(k + 1);
+        }
       }
+      chain(c0);
     }
     // update matrices U = -L21 L21^T
     tiles.clear();
