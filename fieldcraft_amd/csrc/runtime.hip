@@ -149,10 +149,10 @@ struct Tile {
 constexpr i64 SPLIT_TARGET = 1024;
 // The side stream's panel updates only have to keep ahead of the main stream's panel chain: they
 // are split less, so they leave CUs to the chain's kernels and need fewer partial sums. How much
-// less depends on the chain's length: measured best at 80^3, 384 tiles for the FP64 factor and
-// 144 for the FP32 and mixed ones (whose chain is shorter).
+// less depends on the chain's length: measured at 80^3, the FP64 factor is as fast at 384 tiles
+// as at 1024, the FP32 and mixed ones are fastest from 192 to 384 (slower at 96 and below).
 constexpr i64 SPLIT_TARGET_SIDE_FP64 = 384;
-constexpr i64 SPLIT_TARGET_SIDE_FP32 = 144;
+constexpr i64 SPLIT_TARGET_SIDE_FP32 = 256;
 constexpr i64 SPLIT_MIN_K = 512;
 
 // One GEMM launch of `tiles` (kind: UPDATE, TRSM or SYRK), split along k when the tiles are
