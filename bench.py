@@ -245,7 +245,7 @@ def main():
                     avg_launch_s=avg)
     else:
         achieved = g["flops"] / g["seconds"] / 1e12
-        products = args.bf16_terms if "bf16" in dom else 1
+        products = args.bf16_terms if dom.startswith("k_gemm_b16") else 1
         roof = dict(bound="mfma", achieved=achieved, peak=g["peak"], unit="TFLOP/s", frac=achieved / g["peak"],
                     traffic=traffic, kernel=dom, kinds=g["kinds"], launches=g["launches"],
                     flops_per_launch=g["flops"] / max(g["launches"], 1), avg_launch_s=avg,
