@@ -16,14 +16,16 @@ around its launches on the factor's stream during the timed steps; `cpu_baseline
 oracle (the C restatement of the reference's CPU factorization) on a bounded sample.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--cells 80] [--form mixed|fp32|fp64]
-                      [--contrast] [--workload cube|batch|shift]
+                      [--contrast] [--workload cube|batch|shift|assemble]
 
 --contrast adds the harder variant of the same pattern (E in {210 GPa, 21 MPa} per element,
 nu = 0.49) with the same symbolic analysis. --workload batch runs 512 independent hex8/tet10
 models (2k-60k equations) as one block-diagonal factorization in FP32 with FP64 refinement of
 64 load cases per model; --workload shift factors K - sigma M of a tet10 steel block
-(sigma = (2 pi 50 Hz)^2) in the mixed form and times 40 refined block solves of 16 vectors.
-Those two print their own metric lines.
+(sigma = (2 pi 50 Hz)^2) in the mixed form and times 40 refined block solves of 16 vectors;
+--workload assemble times the step before the factor on the device: K_ff of the hex8 cube and
+K - sigma M of the tet10 block assembled into the factor's input, the reference's values bit for
+bit. Those three print their own metric lines.
 """
 from __future__ import annotations
 
@@ -150,7 +152,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--contrast", action="store_true")
-    ap.add_argument("--workload", default="cube", choices=["cube", "batch", "shift"])
+    ap.add_argument("--workload", default="cube", choices=["cube", "batch", "shift", "assemble"])
     ap.add_argument("--models", type=int, default=512)
     ap.add_argument("--tet-cells", type=int, default=50)
     ap.add_argument("--calibrate-copy", action="store_true")
@@ -175,6 +177,8 @@ def main():
         return batch_workload(args, CH, P)
     if args.workload == "shift":
         return shift_workload(args, CH, P)
+    if args.workload == "assemble":
+        return assemble_workload(args, P)
     t0 = time.perf_counter()
     p = P.cube_hex8(args.cells)
     up = p.upper()
@@ -430,6 +434,92 @@ def shift_workload(args, CH, P):
                             factor_entries=info["entries"]),
                 factor_tflops=F / out["mixed_factor_s"] / 1e12, iterations_median=float(np.median(its)),
                 berr_max=worst, **out)
+    print(json.dumps(line), flush=True)
+
+
+def assemble_workload(args, P):
+    """The step before the factor, on the device (fc_assemble_device): the element matrices and
+    their ordered sums into the upper columns of K_ff for the hex8 cube (the step) and of
+    K - sigma M for the tet10 block, from the mesh and the plan already in HBM. The CPU baseline
+    is the oracle's assembly of the same cube (the reference's loop, element matrices in
+    parallel rounds, sums in element order) on all host threads."""
+    import torch
+
+    from fieldcraft_amd import assembly as AS
+
+    sigma = (2 * np.pi * 50.0) ** 2
+    k = args.tet_cells
+    runs = {}
+    for name, make, what in [("hex8", lambda: P.cube_hex8(args.cells), "K"),
+                             ("tet10", lambda: P.block_tet10((k, k, k), (k / 100, k / 100, k / 100)), "shifted")]:
+        t0 = time.perf_counter()
+        p = make()
+        uptr, uidx, _ = p.upper()
+        t1 = time.perf_counter()
+        a = AS.Assembly.of_problem(p, uptr, uidx, timing=True)
+        plan_s = time.perf_counter() - t1
+        log(f"bench: {name} {p.n} equations, {len(p.conn)} elements (mesh {t1 - t0:.1f} s, plan {plan_s:.2f} s)")
+        out = torch.empty(a.nnz, dtype=torch.float64, device="cuda:0")
+        for _ in range(args.warmup):
+            a.assemble_device(out.data_ptr(), what, sigma)
+        wall, el, ga = [], [], []
+        for _ in range(args.steps):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            a.assemble_device(out.data_ptr(), what, sigma)
+            wall.append(time.perf_counter() - t)
+            inf = a.info()
+            el.append(inf["elements_s"])
+            ga.append(inf["gather_s"])
+        inf = a.info()
+        npe, W = (8, 24) if p.kind == "hex8" else (10, 30)
+        E, nnz, ck, cm = len(p.conn), a.nnz, inf["contributions_k"], inf["contributions_m"]
+        # algorithmic bytes: the gather reads the ranges, positions and values of each
+        # contribution and writes each entry; the element kernels write the packed stiffness
+        # (and the 10 x 10 mass) and read the connectivity, the slot map and the coordinates
+        gbytes = (nnz + 1) * 8 + ck * 12 + nnz * 8 + ((nnz + 1) * 8 + cm * 12 if cm else 0)
+        ebytes = E * (W * (W + 1) // 2 * 8 + npe * 4 + 4 + npe * 24) + (E * (100 * 8 + npe * 28 + 4) if cm else 0)
+        runs[name] = dict(p=p, a=a, what=what, wall=float(np.mean(wall)), elements_s=float(np.mean(el)),
+                          gather_s=float(np.mean(ga)), gbytes=gbytes, ebytes=ebytes, E=E, nnz=nnz, ck=ck, cm=cm,
+                          plan_s=plan_s)
+        log(f"bench: {name} assemble {np.mean(wall) * 1e3:.2f} ms (elements {np.mean(el) * 1e3:.2f} ms, "
+            f"gather {np.mean(ga) * 1e3:.2f} ms)")
+    h = runs["hex8"]
+    kern = {"k_hex8_stiffness": (h["elements_s"], h["ebytes"]), "k_assemble_gather": (h["gather_s"], h["gbytes"])}
+    dom = max(kern, key=lambda n: kern[n][0])
+    sec, byt = kern[dom]
+    achieved = byt / sec / 1e9
+    roof = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK, unit="GB/s", frac=achieved / HBM_PEAK,
+                traffic=traffic_from_profiles(dom, f"hex8 {args.cells}^3 K"), kernel=dom, avg_launch_s=sec,
+                bytes_per_launch=byt, kernels={n: dict(seconds=v[0], bytes=v[1], gbs=v[1] / v[0] / 1e9)
+                                               for n, v in kern.items()})
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+
+        threads = args.cpu_threads
+        O.set_threads(threads)
+        p = h["p"]
+        uptr, uidx, _ = p.upper()
+        t = time.perf_counter()
+        O.assemble_upper(p.kind, p.xyz, p.conn, p.young, p.poisson, p.eq_dof, uptr, uidx)
+        cs = time.perf_counter() - t
+        cpu = dict(value=h["E"] / cs, unit="elements/s", cores=threads, kind="port",
+                   sample=f"oracle assembly of the whole hex8 {args.cells}^3 K_ff ({h['E']} elements): {cs:.2f} s "
+                          f"on {threads} threads")
+        log(f"bench: cpu baseline {cs:.2f} s")
+    t10 = runs["tet10"]
+    line = dict(metric=f"assembly of K_ff on the device (reference values bit for bit), hex8 {args.cells}^3: elements/s",
+                value=h["E"] / h["wall"], unit="elements/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
+                ms_per_step=h["wall"] * 1e3, higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f64",
+                data="synthetic",
+                config=dict(workload=f"hex8 {args.cells}^3 unit cube, z=0 clamped, K_ff upper columns",
+                            equations=h["p"].n, elements=h["E"], entries=h["nnz"], contributions=h["ck"]),
+                roofline=roof, cpu_baseline=cpu, plan_s=h["plan_s"],
+                tet10_shifted=dict(workload=f"tet10 {k}^3 Kuhn block, steel, K - (2 pi 50)^2 M", equations=t10["p"].n,
+                                   elements=t10["E"], entries=t10["nnz"], ms=t10["wall"] * 1e3,
+                                   elements_ms=t10["elements_s"] * 1e3, gather_ms=t10["gather_s"] * 1e3,
+                                   gather_gbs=t10["gbytes"] / t10["gather_s"] / 1e9, plan_s=t10["plan_s"]))
     print(json.dumps(line), flush=True)
 
 

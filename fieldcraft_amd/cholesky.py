@@ -27,7 +27,7 @@ from . import _build
 FP64, MIXED, FP32 = 0, 1, 2
 PRECISIONS = {"fp64": FP64, "mixed": MIXED, "fp32": FP32}
 
-_OK, _NPD, _SINGULAR, _TOO_LARGE, _OOM, _INTERNAL, _CANCELLED, _INVALID, _DEVICE = range(9)
+_OK, _NPD, _SINGULAR, _TOO_LARGE, _OOM, _INTERNAL, _CANCELLED, _INVALID, _DEVICE, _DEGENERATE = range(10)
 
 
 class FactorError(Exception):
@@ -73,8 +73,16 @@ class DeviceError(FactorError):
     kind = "device"
 
 
+class DegenerateElement(FactorError):
+    """An inverted or degenerate element met by the assembly (Error::Degenerate, fem
+    element.rs:326-328); ``column`` holds its index in element order."""
+
+    kind = "degenerate_element"
+
+
 _KINDS = {_NPD: NotPositiveDefinite, _SINGULAR: Singular, _TOO_LARGE: TooLarge, _OOM: OutOfMemory,
-          _INTERNAL: Internal, _CANCELLED: Cancelled, _INVALID: InvalidArgument, _DEVICE: DeviceError}
+          _INTERNAL: Internal, _CANCELLED: Cancelled, _INVALID: InvalidArgument, _DEVICE: DeviceError,
+          _DEGENERATE: DegenerateElement}
 
 
 class _Error(C.Structure):
@@ -114,7 +122,8 @@ EXPORTS = ["fc_analyze", "fc_symbolic_info_get", "fc_symbolic_export", "fc_symbo
            "fc_factor_numeric_device", "fc_factor_new", "fc_refactor", "fc_refactor_device", "fc_entries", "fc_solve",
            "fc_solve_many", "fc_solve_device", "fc_factor_export", "fc_factor_seconds", "fc_factor_profile",
            "fc_factor_free", "fc_matrix_new", "fc_matrix_free", "fc_matrix_set_segments", "fc_solve_refined",
-           "fc_solve_refined_device", "fc_solve_refined_many_device", "fc_version"]
+           "fc_solve_refined_device", "fc_solve_refined_many_device", "fc_assembly_new", "fc_assemble_device",
+           "fc_assembly_info", "fc_assembly_free", "fc_version"]
 
 #: kernel kinds of the numeric phase (FC_KIND_* in include/fc_chol.h)
 KINDS = ["extend_l", "extend_u", "update", "potrf", "trsm", "syrk", "reduce", "split"]

@@ -85,6 +85,9 @@ class Problem:
     graph_chosen: bool
     rhs: np.ndarray | None = None
     meta: dict = field(default_factory=dict)
+    young: np.ndarray | None = None   # per element, or one value for all
+    poisson: float = 0.3
+    rho: float = 0.0
 
     @property
     def nnz(self) -> int:
@@ -140,7 +143,8 @@ def block(kind: str, cells, size, young, poisson: float, clamp_face: int, jitter
         mval = np.zeros(nnz) if L.fcp_has_mass(h) else None
         L.fcp_get_csr(h, _p(ptr), _p(idx), _p(kval), _p(mval))
         return Problem(kind, (nx, ny, nz), int(n), ptr, idx, kval, mval, xyz, conn, adj_ptr, adj_idx, order, eq_dof,
-                       bool(L.fcp_graph_chosen(h)))
+                       bool(L.fcp_graph_chosen(h)), young=young.copy(), poisson=float(poisson),
+                       rho=float(rho) if mval is not None else 0.0)
     finally:
         L.fcp_free(h)
 

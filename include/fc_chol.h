@@ -31,13 +31,17 @@ typedef enum {
   FC_INTERNAL = 5,              /* FactorError::Internal(String)                            */
   FC_CANCELLED = 6,             /* FactorError::Cancelled                                   */
   FC_INVALID_ARGUMENT = 7,      /* a malformed pattern, length or option                    */
-  FC_DEVICE = 8                 /* a HIP runtime error                                      */
+  FC_DEVICE = 8,                /* a HIP runtime error                                      */
+  FC_DEGENERATE_ELEMENT = 9     /* assembly: an inverted or degenerate element (its index in
+                                   element order in fc_error.column), Error::Degenerate of
+                                   crates/fem/src/element.rs:326-328                         */
 } fc_status;
 
 typedef struct {
   int32_t kind;      /* fc_status */
   int32_t pad;
-  uint64_t column;   /* failing original column (NOT_POSITIVE_DEFINITE, SINGULAR)      */
+  uint64_t column;   /* failing original column (NOT_POSITIVE_DEFINITE, SINGULAR); the
+                        element index (DEGENERATE_ELEMENT)                              */
   uint64_t entries;  /* entries of L (TOO_LARGE, OUT_OF_MEMORY)                         */
   uint64_t limit;    /* the cap (TOO_LARGE)                                             */
   char message[256];
@@ -223,6 +227,55 @@ int fc_solve_refined_many_device(const fc_factor* f, const fc_matrix* a, const d
 /* fc_solve_refined with b and x in device memory (n doubles each, distinct). */
 int fc_solve_refined_device(const fc_factor* f, const fc_matrix* a, const double* b, double* x, uint64_t n,
                             const fc_refine_opts* opts, fc_refine_report* report, fc_error* err);
+
+/* --- assembly of the factor's input ------------------------------------------------------ */
+
+/* The step before the factorization (crates/solver/src/analysis.rs:199-233 assemble,
+ * sparse.rs:199-243 add_block / add_scalar, constraints.rs:820-906 reduce, sparse.rs:81-87
+ * add_scaled, cholesky.rs:110-144 UpperColumns::from_symmetric) on the device: the solid
+ * element matrices of a mesh added in element order into the upper triangle by columns of
+ * K_ff, M_ff or K_ff - shift M_ff, the same bit for bit as the reference's, written where
+ * fc_refactor_device / fc_factor_numeric_device read the values. */
+typedef struct fc_assembly fc_assembly;
+
+enum {
+  FC_ELEMENT_HEX8 = 0,  /* 8-node hexahedra, full 2x2x2 integration (fem element.rs:146-155)  */
+  FC_ELEMENT_TET10 = 1  /* 10-node tetrahedra, 4-point stiffness, conical 4x4x4 mass (:82-91) */
+};
+
+enum {
+  FC_ASSEMBLE_K = 0,       /* the stiffness K_ff                                            */
+  FC_ASSEMBLE_M = 1,       /* the consistent mass M_ff (10-node tetrahedra, density > 0)    */
+  FC_ASSEMBLE_SHIFTED = 2  /* K_ff + (-shift) M_ff (Csr::add_scaled)                        */
+};
+
+/* Builds the gather plan of a mesh on the host and uploads the mesh. xyz: nodes x 3; conn:
+ * elements x (8 | 10) node numbers in the model's node order (fem shape.rs:3-11); young: one
+ * value per element (n_young = elements) or one for all (n_young = 1); isotropic materials
+ * (fem material.rs:40-54); density > 0 (10-node tetrahedra) also plans the mass. eq_dof:
+ * neq free equations, equation -> mesh dof 3 node + direction (constraints.rs:755-771, a plain
+ * selection); ptr/idx: the upper columns of K_ff over those equations (what fc_analyze was
+ * given). timing != 0 times each fc_assemble_device's kernels with HIP events
+ * (fc_assembly_info). Refuses material constants that are not a stable material
+ * (material.rs:22-27) and patterns that lack an entry two elements' nodes couple. */
+int fc_assembly_new(int32_t element_kind, uint64_t nodes, const double* xyz, uint64_t elements, const int64_t* conn,
+                    const double* young, uint64_t n_young, double poisson, double density, uint64_t neq,
+                    const int64_t* eq_dof, const uint64_t* ptr, const uint64_t* idx, int32_t device, int32_t timing,
+                    fc_assembly** out, fc_error* err);
+
+/* Assembles `what` into upper_val_device (nnz doubles of device memory, the pattern's entry
+ * count) on `stream` (NULL: the assembly's own stream), and waits for it. The element matrices
+ * are recomputed on every call. FC_DEGENERATE_ELEMENT names the first degenerate element in
+ * element order, as the reference's sequential loop reports it (analysis.rs:203-233); the
+ * output is then undefined. */
+int fc_assemble_device(fc_assembly* a, int32_t what, double shift, double* upper_val_device, uint64_t nnz,
+                       void* stream, fc_error* err);
+
+/* The plan's contribution counts (stiffness, mass) and, with timing on, the device seconds of
+ * the last call's element kernels and gather kernel. */
+int fc_assembly_info(const fc_assembly* a, uint64_t* contributions_k, uint64_t* contributions_m, double* elements_s,
+                     double* gather_s);
+void fc_assembly_free(fc_assembly* a);
 
 /* Library version string. */
 const char* fc_version(void);

@@ -73,6 +73,13 @@ def lib():
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_threads.restype = C.c_int
         L.orc_set_parallel_inner.argtypes = [C.c_int]
+        L.orc_solid_stiffness.argtypes = [C.c_int, _f64p, C.c_double, C.c_double, _f64p]
+        L.orc_solid_stiffness.restype = C.c_int
+        L.orc_tet10_mass.argtypes = [_f64p, C.c_double, _f64p]
+        L.orc_tet10_mass.restype = C.c_int
+        L.orc_assemble_upper.argtypes = [C.c_int, i64, _f64p, i64, _i64p, _f64p, i64, C.c_double, C.c_double,
+                                         C.c_double, C.c_int, i64, _i64p, _i64p, _i64p, _f64p]
+        L.orc_assemble_upper.restype = i64
         _lib = L
     return _lib
 
@@ -97,6 +104,55 @@ def from_symmetric(n, rptr, ridx, rval):
     val = np.zeros(max(total, 1), np.float64)
     L.orc_from_symmetric(n, rptr, ridx, rval, ptr.ctypes.data, idx.ctypes.data, val.ctypes.data)
     return ptr, idx[:total].copy(), val[:total].copy()
+
+
+ELEMENT_KINDS = {"hex8": 0, "tet10": 1}
+
+
+def solid_stiffness(kind: str, xyz, young: float, poisson: float):
+    """Solid::stiffness (fem element.rs:416-422): 3n x 3n, node by node (x, y, z); None when the
+    element is degenerate."""
+    k = ELEMENT_KINDS[kind]
+    nn = 8 if k == 0 else 10
+    ke = np.zeros((3 * nn, 3 * nn))
+    ok = lib().orc_solid_stiffness(k, _a(xyz, np.float64).reshape(-1), float(young), float(poisson), ke.reshape(-1))
+    return ke if ok else None
+
+
+def tet10_mass(xyz, density: float):
+    """Solid::mass of a tet10 (fem element.rs:427-441): 10 x 10 per displacement component."""
+    me = np.zeros((10, 10))
+    ok = lib().orc_tet10_mass(_a(xyz, np.float64).reshape(-1), float(density), me.reshape(-1))
+    return me if ok else None
+
+
+class DegenerateElement(Exception):
+    def __init__(self, element: int):
+        self.element = element
+        super().__init__(f"element {element} is inverted or degenerate")
+
+
+def assemble_upper(kind: str, xyz, conn, young, poisson: float, eq_dof, uptr, uidx, *, density: float = 0.0,
+                   what: str = "K", shift: float = 0.0):
+    """Upper columns of K_ff, M_ff or K_ff - shift*M_ff on the caller's pattern (uptr, uidx): the
+    element matrices added in element order (analysis.rs:199-233, sparse.rs:199-243), reduced to
+    the free equations (constraints.rs:820-906), K + (-shift) M (sparse.rs:81-87), and
+    UpperColumns::from_symmetric (cholesky.rs:110-144)."""
+    young = np.atleast_1d(np.asarray(young, np.float64))
+    xyz, conn = _a(xyz, np.float64), _a(conn)
+    uptr, uidx = _a(uptr), _a(uidx)
+    out = np.zeros(max(int(uptr[-1]), 1))
+    w = {"K": 0, "M": 1, "shifted": 2}[what]
+    r = lib().orc_assemble_upper(ELEMENT_KINDS[kind], len(xyz), xyz.reshape(-1), len(conn), conn.reshape(-1), young,
+                                 len(young), float(poisson), float(density), float(shift), w, len(eq_dof), _a(eq_dof),
+                                 uptr, uidx, out)
+    if r >= 0:
+        raise DegenerateElement(int(r))
+    if r == -2:
+        raise ValueError("the assembled pattern differs from the given upper columns")
+    if r == -3:
+        raise ValueError("bad element kind, mass request or material")
+    return out[:int(uptr[-1])]
 
 
 def csr_mul(n, ptr, idx, val, x):
