@@ -27,7 +27,7 @@ from . import _build
 FP64, MIXED, FP32 = 0, 1, 2
 PRECISIONS = {"fp64": FP64, "mixed": MIXED, "fp32": FP32}
 
-_OK, _NPD, _SINGULAR, _TOO_LARGE, _OOM, _INTERNAL, _CANCELLED, _INVALID, _DEVICE, _DEGENERATE = range(10)
+_OK, _NPD, _SINGULAR, _TOO_LARGE, _OOM, _INTERNAL, _CANCELLED, _INVALID, _DEVICE, _DEGENERATE, _NOT_CONVERGED = range(11)
 
 
 class FactorError(Exception):
@@ -80,9 +80,16 @@ class DegenerateElement(FactorError):
     kind = "degenerate_element"
 
 
+class NotConverged(FactorError):
+    """The Lanczos iteration stopped without the wanted modes (Error::NotConverged,
+    crates/solver/src/eigen.rs:407-411, 463, 467)."""
+
+    kind = "not_converged"
+
+
 _KINDS = {_NPD: NotPositiveDefinite, _SINGULAR: Singular, _TOO_LARGE: TooLarge, _OOM: OutOfMemory,
           _INTERNAL: Internal, _CANCELLED: Cancelled, _INVALID: InvalidArgument, _DEVICE: DeviceError,
-          _DEGENERATE: DegenerateElement}
+          _DEGENERATE: DegenerateElement, _NOT_CONVERGED: NotConverged}
 
 
 class _Error(C.Structure):
@@ -104,6 +111,16 @@ class _Info(C.Structure):
                 ("launches", C.c_uint64), ("kind_flops", C.c_double * 8), ("kind_entries", C.c_double * 8)]
 
 
+class _LanczosOpts(C.Structure):
+    _fields_ = [("count", C.c_uint64), ("min", C.c_double), ("max", C.c_double), ("block", C.c_uint32),
+                ("pad", C.c_uint32), ("max_basis", C.c_uint64), ("tolerance", C.c_double)]
+
+
+class _LanczosReport(C.Structure):
+    _fields_ = [("found", C.c_uint64), ("basis", C.c_uint64), ("negative_count", C.c_uint64),
+                ("negative", C.c_double * 16), ("seconds", C.c_double)]
+
+
 class _RefineOpts(C.Structure):
     _fields_ = [("method", C.c_int32), ("max_iter", C.c_int32), ("target_berr", C.c_double)]
 
@@ -123,7 +140,7 @@ EXPORTS = ["fc_analyze", "fc_symbolic_info_get", "fc_symbolic_export", "fc_symbo
            "fc_solve_many", "fc_solve_device", "fc_factor_export", "fc_factor_seconds", "fc_factor_profile",
            "fc_factor_free", "fc_matrix_new", "fc_matrix_free", "fc_matrix_set_segments", "fc_solve_refined",
            "fc_solve_refined_device", "fc_solve_refined_many_device", "fc_assembly_new", "fc_assemble_device",
-           "fc_assembly_info", "fc_assembly_free", "fc_version"]
+           "fc_assembly_info", "fc_assembly_free", "fc_lanczos_device", "fc_version"]
 
 #: kernel kinds of the numeric phase (FC_KIND_* in include/fc_chol.h)
 KINDS = ["extend_l", "extend_u", "update", "potrf", "trsm", "syrk", "reduce", "split"]
@@ -165,6 +182,8 @@ def lib():
         L.fc_matrix_set_segments.argtypes = [vp, C.c_uint64, _u64p, ep]
         L.fc_solve_refined_many_device.argtypes = [vp, vp, vp, vp, C.c_uint64, C.c_uint64, C.POINTER(_RefineOpts),
                                                    C.POINTER(_RefineReport), _f64p, ep]
+        L.fc_lanczos_device.argtypes = [vp, vp, C.c_double, C.POINTER(_LanczosOpts), _f64p, vp,
+                                         C.POINTER(_LanczosReport), ep]
         L.fc_version.restype = C.c_char_p
         _lib = L
     return _lib
@@ -420,6 +439,33 @@ class Factor:
                                            C.byref(opts), C.byref(rep), C.byref(err))
         _raise(st, err)
         return _report(rep)
+
+    def lanczos_device(self, mass: "Matrix", shift: float, count: int, lo: float = -np.inf, hi: float = np.inf,
+                       block: int = 6, max_basis: int | None = None, tolerance: float = 1e-9, vectors: bool = True):
+        """The lowest `count` modes of K phi = lambda M phi in [lo, hi] by the shifted block Lanczos
+        method (lanczos, eigen.rs:281-466) on the device, this factor being that of K - shift M and
+        `mass` M by rows. Returns dict(values, vectors (a torch tensor n x found, M-orthonormal, or
+        None), negative, basis, seconds)."""
+        import torch
+
+        if max_basis is None:
+            # the reference's cap (Limits::MAX_LANCZOS_VECTORS = 3000, lib.rs:114), and the basis
+            # and its M products in at most a quarter of the free device memory
+            free, _ = torch.cuda.mem_get_info()
+            max_basis = max(min(3000, self.n, int(free / 4 / (16 * max(self.n, 1)))), min(block, self.n))
+        opts = _LanczosOpts(int(count), float(lo), float(hi), int(block), 0, int(max_basis), float(tolerance))
+        vals = np.zeros(max(int(count), 1))
+        vec = torch.empty((max(int(count), 1), max(self.n, 1)), dtype=torch.float64, device="cuda") if vectors else None
+        rep = _LanczosReport()
+        err = _Error()
+        st = lib().fc_lanczos_device(self._h, mass._h, float(shift), C.byref(opts), vals,
+                                     C.c_void_p(vec.data_ptr()) if vec is not None else None, C.byref(rep),
+                                     C.byref(err))
+        _raise(st, err)
+        found = int(rep.found)
+        return dict(values=vals[:found].copy(), vectors=vec[:found].T if vec is not None else None,
+                    negative=np.array(rep.negative[:min(int(rep.negative_count), 16)]), basis=int(rep.basis),
+                    seconds=float(rep.seconds))
 
     def solve_refined_many_device(self, matrix: Matrix, b_ptr: int, x_ptr: int, nrhs: int, method: str = "auto",
                                   max_iter: int = 50, target_berr: float = 1e-12):

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <memory>
 #include <mutex>
 #include <new>
@@ -16,6 +17,7 @@
 #include "factor_kernels.hpp"
 #include "fc_chol.h"
 #include "gemm_kernels.hpp"
+#include "lanczos_kernels.hpp"
 #include "solve_kernels.hpp"
 #include "symbolic.hpp"
 
@@ -1257,6 +1259,388 @@ int guarded(fc_error* err, const char* oom, Body body) {
   return FC_OK;
 }
 
+// ---------------------------------------------------------------------------------------------
+// Shift-invert block Lanczos (crates/solver/src/eigen.rs:281-466) on the device.
+
+// xorshift64* (eigen.rs:159-175), uniform in [-1, 1): the start block and fresh directions.
+struct XorShift {
+  uint64_t s;
+  explicit XorShift(uint64_t seed) : s(seed ? seed : 1) {}
+  double uniform() {
+    s ^= s >> 12;
+    s ^= s << 25;
+    s ^= s >> 27;
+    const uint64_t v = s * 0x2545F4914F6CDD1DULL;
+    return (double)(v >> 11) / (double)(1ULL << 52) - 1.0;
+  }
+};
+
+// Eigenvalues ascending and eigenvectors (column k of the row-major `vec`) of a dense symmetric
+// n x n matrix (row-major `a`, overwritten): Householder reduction to tridiagonal form, then
+// implicit QR steps with Wilkinson shifts on the tridiagonal matrix, the transformations
+// accumulated (Golub & Van Loan, Matrix Computations, 4th ed., algorithms 8.3.1 and 8.3.3).
+// False when the QR iteration does not converge within 30 n + 30 steps.
+bool dense_symmetric_eigen(int n, std::vector<double>& a, std::vector<double>& values, std::vector<double>& vec) {
+  std::vector<double> q((size_t)n * n, 0.0), v(n), p(n);
+  for (int i = 0; i < n; ++i) q[(size_t)i * n + i] = 1.0;
+  auto A = [&](int i, int j) -> double& { return a[(size_t)i * n + j]; };
+  for (int k = 0; k + 2 < n; ++k) {
+    double nx = 0.0;
+    for (int i = k + 1; i < n; ++i) nx += A(i, k) * A(i, k);
+    nx = std::sqrt(nx);
+    if (nx == 0.0) continue;
+    const double alpha = A(k + 1, k) >= 0.0 ? -nx : nx;
+    double nv = 0.0;
+    for (int i = k + 1; i < n; ++i) {
+      v[i] = A(i, k) - (i == k + 1 ? alpha : 0.0);
+      nv += v[i] * v[i];
+    }
+    nv = std::sqrt(nv);
+    if (nv == 0.0) continue;
+    for (int i = k + 1; i < n; ++i) v[i] /= nv;
+    double vp = 0.0;
+    for (int i = k + 1; i < n; ++i) {
+      double t = 0.0;
+      for (int j = k + 1; j < n; ++j) t += A(i, j) * v[j];
+      p[i] = t;
+      vp += v[i] * t;
+    }
+    for (int i = k + 1; i < n; ++i) p[i] -= vp * v[i];
+    for (int i = k + 1; i < n; ++i)
+      for (int j = k + 1; j < n; ++j) A(i, j) -= 2.0 * (v[i] * p[j] + p[i] * v[j]);
+    A(k + 1, k) = A(k, k + 1) = alpha;
+    for (int i = k + 2; i < n; ++i) A(i, k) = A(k, i) = 0.0;
+    for (int r = 0; r < n; ++r) {
+      double t = 0.0;
+      for (int j = k + 1; j < n; ++j) t += q[(size_t)r * n + j] * v[j];
+      for (int j = k + 1; j < n; ++j) q[(size_t)r * n + j] -= 2.0 * t * v[j];
+    }
+  }
+  std::vector<double> d(n), e(std::max(n - 1, 0));
+  for (int i = 0; i < n; ++i) d[i] = A(i, i);
+  for (int i = 0; i + 1 < n; ++i) e[i] = A(i + 1, i);
+  const double eps = std::numeric_limits<double>::epsilon();
+  auto negligible = [&](double ev, double x, double y) { return std::fabs(ev) <= eps * (std::fabs(x) + std::fabs(y)); };
+  int h = n - 1, steps = 0;
+  while (h > 0) {
+    if (negligible(e[h - 1], d[h - 1], d[h])) {
+      e[h - 1] = 0.0;
+      --h;
+      continue;
+    }
+    int l = h - 1;
+    while (l > 0 && !negligible(e[l - 1], d[l - 1], d[l])) --l;
+    if (l > 0) e[l - 1] = 0.0;
+    if (++steps > 30 * n + 30) return false;
+    const double dd = (d[h - 1] - d[h]) / 2.0, eh = e[h - 1];
+    const double mu = d[h] - eh * eh / (dd + (dd >= 0.0 ? 1.0 : -1.0) * std::hypot(dd, eh));
+    double x = d[l] - mu, z = e[l];
+    for (int k = l; k < h; ++k) {
+      const double r = std::hypot(x, z);
+      const double c = r == 0.0 ? 1.0 : x / r, sn = r == 0.0 ? 0.0 : -z / r;
+      if (k > l) e[k - 1] = c * e[k - 1] - sn * z;
+      const double a0 = d[k], b0 = e[k], d1 = d[k + 1];
+      d[k] = c * c * a0 - 2.0 * c * sn * b0 + sn * sn * d1;
+      d[k + 1] = sn * sn * a0 + 2.0 * c * sn * b0 + c * c * d1;
+      e[k] = c * sn * (a0 - d1) + (c * c - sn * sn) * b0;
+      if (k + 1 < h) {
+        z = -sn * e[k + 1];
+        e[k + 1] *= c;
+        x = e[k];
+      }
+      for (int r2 = 0; r2 < n; ++r2) {
+        const double qp = q[(size_t)r2 * n + k], qq = q[(size_t)r2 * n + k + 1];
+        q[(size_t)r2 * n + k] = c * qp - sn * qq;
+        q[(size_t)r2 * n + k + 1] = sn * qp + c * qq;
+      }
+    }
+  }
+  std::vector<int> order(n);
+  for (int i = 0; i < n; ++i) order[i] = i;
+  std::sort(order.begin(), order.end(), [&](int x, int y) { return d[x] < d[y] || (d[x] == d[y] && x < y); });
+  values.resize(n);
+  vec.assign((size_t)n * n, 0.0);
+  for (int k = 0; k < n; ++k) {
+    values[k] = d[order[k]];
+    for (int r = 0; r < n; ++r) vec[(size_t)r * n + k] = q[(size_t)r * n + order[k]];
+  }
+  return true;
+}
+
+struct LanczosError {
+  std::string msg;
+};
+
+// The device side of one Lanczos run: the basis Q and its M products MQ (n x cap), a block W and
+// MW (n x 8), reduction buffers.
+struct LanczosWork {
+  const fc_factor* F;
+  const fc_matrix* M;
+  hipStream_t st;
+  i64 n;
+  i64 cap;
+  double *Q = nullptr, *MQ = nullptr, *W = nullptr, *MW = nullptr, *part = nullptr, *C = nullptr;
+  int64_t* at = nullptr;
+  int chunks = 1;
+  std::vector<double> host;
+
+  ~LanczosWork() {
+    for (void* p : {(void*)Q, (void*)MQ, (void*)W, (void*)MW, (void*)part, (void*)C, (void*)at}) dfree(p);
+  }
+  double* q(i64 j) { return Q + j * n; }
+  double* mq(i64 j) { return MQ + j * n; }
+  // Y = M X for k columns (Csr::mul by rows, sparse.rs:42-50)
+  void mass(const double* x, double* y, int k) {
+    fc::k_spmm<<<blocks_for(n * 8), 256, 0, st>>>(n, M->ptr, M->idx, M->val, x, y, k, n);
+  }
+  // host C (m x b, row-major) = A^T B: A n x m (leading dimension n), B n x b
+  const std::vector<double>& dots(const double* A, int m, const double* B, int b) {
+    host.assign((size_t)m * b, 0.0);
+    if (m == 0 || b == 0) return host;
+    dim3 grid(chunks, (m + fc::TS_COLS - 1) / fc::TS_COLS);
+    fc::k_ts_dots<<<grid, 256, 0, st>>>(n, A, n, m, B, n, b, part);
+    fc::k_ts_dots_sum<<<(m * b + 255) / 256, 256, 0, st>>>(part, chunks, m, b, C);
+    HIP_CHECK(hipMemcpyAsync(host.data(), C, sizeof(double) * m * b, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    return host;
+  }
+  // B = beta B + alpha A c for b columns; c host m x b row-major (uploaded in pieces the
+  // update kernel's LDS holds)
+  void update(const double* A, int m, const std::vector<double>& c, double* B, int b, double alpha, double beta) {
+    const int per = std::max(1, fc::TS_MAXC / std::max(b, 1));
+    for (int i0 = 0; i0 < m || (m == 0 && i0 == 0); i0 += per) {
+      const int mm = std::min(per, m - i0);
+      if (mm <= 0) {
+        if (beta != 1.0) fc::k_ts_update<<<blocks_for(n), 256, 0, st>>>(n, A, n, 0, C, b, B, n, b, alpha, beta);
+        break;
+      }
+      HIP_CHECK(hipMemcpyAsync(C, c.data() + (size_t)i0 * b, sizeof(double) * mm * b, hipMemcpyHostToDevice, st));
+      fc::k_ts_update<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, A + (i64)i0 * n, n, mm, C, b, B, n, b, alpha,
+                                                                    i0 == 0 ? beta : 1.0);
+    }
+  }
+  void scale(double* x, double s) { fc::k_scale<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, x, s); }
+  void copy(const double* x, double* y) { fc::k_copy_col<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, x, y); }
+  double mnorm2(const double* x, const double* mx) { return std::max(dots(x, 1, mx, 1)[0], 0.0); }
+  void random_into(double* x, XorShift& rng) {
+    std::vector<double> v(n);
+    for (i64 r = 0; r < n; ++r) v[r] = rng.uniform();
+    HIP_CHECK(hipMemcpyAsync(x, v.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+  }
+};
+
+// Orthonormalize the k columns of W against the basis Q[:, 0..mb) and one another in the M
+// inner product (eigen.rs:231-279): the basis part as two passes of block classical Gram-Schmidt
+// (C = MQ^T W, W -= Q C), then column by column against the block's earlier vectors, twice,
+// recording r[j][i] (the coefficient of column i on new vector j; r[i][i] its norm). A column
+// whose norm falls below 1e-8 of its norm before is replaced by a fresh random direction (at most
+// three times) and keeps a zero coefficient. The new vectors go to Q[:, mb..), their M products
+// to MQ[:, mb..). Returns how many were kept.
+int orthonormalize_block(LanczosWork& L, int k, i64 mb, XorShift& rng, std::vector<double>& r) {
+  r.assign((size_t)k * k, 0.0);
+  L.mass(L.W, L.MW, k);
+  std::vector<double> before(k);
+  {
+    const std::vector<double>& g = L.dots(L.W, k, L.MW, k);
+    for (int i = 0; i < k; ++i) before[i] = std::sqrt(std::max(g[(size_t)i * k + i], 0.0));
+  }
+  auto project_basis = [&](double* w, int cols) {
+    for (int pass = 0; pass < 2 && mb > 0; ++pass) {
+      const std::vector<double> c = L.dots(L.MQ, (int)mb, w, cols);
+      L.update(L.Q, (int)mb, c, w, cols, -1.0, 1.0);
+    }
+  };
+  project_basis(L.W, k);
+  int kept = 0;
+  for (int i = 0; i < k; ++i) {
+    double* v = L.W + (i64)i * L.n;
+    bool fresh = false;
+    for (int attempt = 0; attempt < 4; ++attempt) {
+      if (fresh) {
+        L.random_into(v, rng);
+        L.mass(v, L.MW, 1);
+        before[i] = std::sqrt(L.mnorm2(v, L.MW));
+        project_basis(v, 1);
+      }
+      for (int pass = 0; pass < 2 && kept > 0; ++pass) {
+        const std::vector<double> c = L.dots(L.mq(mb), kept, v, 1);
+        L.update(L.q(mb), kept, c, v, 1, -1.0, 1.0);
+        if (!fresh)
+          for (int j = 0; j < kept; ++j) r[(size_t)j * k + i] += c[j];
+      }
+      double* mv = L.mq(mb + kept);
+      L.mass(v, mv, 1);
+      const double norm = std::sqrt(L.mnorm2(v, mv));
+      if (norm > 1e-8 * before[i] && norm > 0.0) {
+        if (!fresh) r[(size_t)i * k + i] = norm;
+        L.copy(v, L.q(mb + kept));
+        L.scale(L.q(mb + kept), 1.0 / norm);
+        L.scale(mv, 1.0 / norm);
+        ++kept;
+        break;
+      }
+      fresh = true;
+    }
+  }
+  return kept;
+}
+
+struct LanczosResult {
+  std::vector<double> values, negative;
+  i64 basis = 0;
+};
+
+// eigen.rs:281-466 with the operator (K - shift M)^{-1} M: the factor's block solves and the
+// mass matrix's products on the device; the projected matrix T and its eigenproblem on the host.
+LanczosResult lanczos_run(const fc_factor* F, const fc_matrix* M, double shift, const fc_lanczos_opts& o,
+                          double* vectors, hipStream_t st) {
+  const i64 n = F->sym->s.n;
+  const int b = (int)std::max<i64>(1, std::min<i64>(o.block, n));
+  const i64 max_basis = std::max<i64>(std::min<i64>((i64)o.max_basis, n), b);
+  const i64 cap = max_basis + b;
+  LanczosWork L{F, M, st, n, cap};
+  L.chunks = (int)std::max<i64>(1, (n + fc::TS_ROWS - 1) / fc::TS_ROWS);
+  L.Q = dalloc<double>((size_t)n * cap);
+  L.MQ = dalloc<double>((size_t)n * cap);
+  L.W = dalloc<double>((size_t)n * fc::TS_MAXB);
+  L.MW = dalloc<double>((size_t)n * fc::TS_MAXB);
+  L.part = dalloc<double>((size_t)L.chunks * cap * fc::TS_MAXB);
+  L.C = dalloc<double>((size_t)std::max<i64>(cap * fc::TS_MAXB, fc::TS_MAXC));
+  L.at = dalloc<int64_t>((size_t)L.chunks);
+  XorShift rng(0x9E3779B97F4A7C15ULL);
+  {
+    std::vector<double> start((size_t)n * b);
+    for (size_t i = 0; i < start.size(); ++i) start[i] = rng.uniform();  // column by column
+    HIP_CHECK(hipMemcpyAsync(L.W, start.data(), sizeof(double) * start.size(), hipMemcpyHostToDevice, st));
+  }
+  std::vector<double> r;
+  if (orthonormalize_block(L, b, 0, rng, r) < b) throw LanczosError{"could not build a Lanczos start block"};
+  std::vector<double> prev_b;
+  i64 next_check = std::max<i64>((i64)o.count + b, 2 * b);
+  std::vector<double> T;  // m x m row-major
+  i64 m = 0;
+  for (;;) {
+    if (F->opts.cancel && *F->opts.cancel) throw LanczosError{"cancelled"};
+    const i64 j0 = m;
+    // W = Op Q_j: the block's M products solved together
+    HIP_CHECK(hipMemcpyAsync(L.W, L.mq(j0), sizeof(double) * n * b, hipMemcpyDeviceToDevice, st));
+    solve_device(F, L.W, n, b, n, st);
+    // A_j = MQ_j^T W, symmetrized
+    std::vector<double> a = L.dots(L.mq(j0), b, L.W, b);
+    for (int i = 0; i < b; ++i)
+      for (int j = i + 1; j < b; ++j) {
+        const double sym = 0.5 * (a[(size_t)i * b + j] + a[(size_t)j * b + i]);
+        a[(size_t)i * b + j] = a[(size_t)j * b + i] = sym;
+      }
+    m = j0 + b;
+    {
+      std::vector<double> T2((size_t)m * m, 0.0);
+      for (i64 i = 0; i < j0; ++i)
+        for (i64 j = 0; j < j0; ++j) T2[(size_t)i * m + j] = T[(size_t)i * j0 + j];
+      T.swap(T2);
+    }
+    for (int i = 0; i < b; ++i)
+      for (int j = 0; j < b; ++j) T[(size_t)(j0 + i) * m + j0 + j] = a[(size_t)i * b + j];
+    if (!prev_b.empty())
+      for (int i = 0; i < b; ++i)
+        for (int j = 0; j < b; ++j) {
+          T[(size_t)(j0 + i) * m + j0 - b + j] = prev_b[(size_t)i * b + j];
+          T[(size_t)(j0 - b + j) * m + j0 + i] = prev_b[(size_t)i * b + j];
+        }
+    const bool full = m >= max_basis || m + b > n;
+    int kept = b;
+    std::vector<double> bnext((size_t)b * b, 0.0);
+    if (!full) kept = orthonormalize_block(L, b, m, rng, bnext);
+    if (m >= next_check || full || kept < b) {
+      next_check = m + std::max<i64>(m / 4, b);
+      std::vector<double> dense(T), theta, s;
+      if (!dense_symmetric_eigen((int)m, dense, theta, s)) throw LanczosError{"the tridiagonal QR iteration did not converge"};
+      std::vector<int> idx(m);
+      for (i64 i = 0; i < m; ++i) idx[i] = (int)i;
+      std::sort(idx.begin(), idx.end(), [&](int x, int y) { return theta[x] > theta[y] || (theta[x] == theta[y] && x < y); });
+      auto residual = [&](int i) {
+        if (full) return 0.0;
+        double res2 = 0.0;
+        for (int rr = 0; rr < b; ++rr) {
+          double v = 0.0;
+          for (int c = 0; c < b; ++c) v += bnext[(size_t)rr * b + c] * s[(size_t)(m - b + c) * m + i];
+          res2 += v * v;
+        }
+        return std::sqrt(res2);
+      };
+      bool done = false;
+      std::vector<int> accepted;
+      i64 found = 0;
+      for (int i : idx) {
+        const double th = theta[i];
+        if (!(th > 0.0)) break;
+        if (residual(i) > o.tolerance * th) break;
+        const double lambda = shift + 1.0 / th;
+        if (lambda > o.max) {
+          done = true;
+          break;
+        }
+        if (lambda >= o.min) {
+          accepted.push_back(i);
+          if (++found == (i64)o.count) {
+            done = true;
+            break;
+          }
+        }
+      }
+      if (full && !done) {
+        if (m >= n) {
+          done = true;
+        } else {
+          char msg[200];
+          std::snprintf(msg, sizeof msg, "found %lld of %llu modes after %lld Lanczos vectors (the cap)", (long long)found,
+                        (unsigned long long)o.count, (long long)m);
+          throw LanczosError{msg};
+        }
+      }
+      if (done) {
+        LanczosResult res;
+        res.basis = m;
+        for (auto it = idx.rbegin(); it != idx.rend(); ++it) {
+          const double th = theta[*it];
+          if (!(th < 0.0)) break;
+          if (residual(*it) > o.tolerance * std::fabs(th)) break;
+          res.negative.push_back(shift + 1.0 / th);
+        }
+        for (size_t a2 = 0; a2 < accepted.size(); ++a2) {
+          res.values.push_back(shift + 1.0 / theta[accepted[a2]]);
+          if (!vectors) continue;
+          // x = Q s, M-normalized, the largest component positive
+          std::vector<double> c((size_t)m);
+          for (i64 k2 = 0; k2 < m; ++k2) c[k2] = s[(size_t)k2 * m + accepted[a2]];
+          double* x = vectors + (i64)a2 * n;
+          L.update(L.Q, (int)m, c, x, 1, 1.0, 0.0);
+          L.mass(x, L.MW, 1);
+          const double norm = std::sqrt(L.mnorm2(x, L.MW));
+          if (norm > 0.0) L.scale(x, 1.0 / norm);
+          fc::k_argmax_abs<<<L.chunks, 256, 0, st>>>(n, x, L.at);
+          std::vector<int64_t> at(L.chunks);
+          HIP_CHECK(hipMemcpyAsync(at.data(), L.at, sizeof(int64_t) * L.chunks, hipMemcpyDeviceToHost, st));
+          HIP_CHECK(hipStreamSynchronize(st));
+          double big = 0.0;
+          for (int64_t q : at) {
+            double v = 0.0;
+            HIP_CHECK(hipMemcpy(&v, x + q, sizeof(double), hipMemcpyDeviceToHost));
+            if (std::fabs(v) > std::fabs(big)) big = v;
+          }
+          if (big < 0.0) L.scale(x, -1.0);
+        }
+        HIP_CHECK(hipStreamSynchronize(st));
+        return res;
+      }
+    }
+    if (full) throw LanczosError{"the Lanczos basis is exhausted"};
+    if (kept < b) throw LanczosError{"the Lanczos block lost rank"};
+    prev_b = bnext;
+  }
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------------------------------
@@ -1264,6 +1648,59 @@ int guarded(fc_error* err, const char* oom, Body body) {
 extern "C" {
 
 const char* fc_version(void) { return VERSION; }
+
+int fc_lanczos_device(const fc_factor* f, const fc_matrix* m, double shift, const fc_lanczos_opts* opts,
+                      double* values, double* vectors_device, fc_lanczos_report* report, fc_error* err) {
+  if (!f || !f->valid || !m || !opts || (opts->count && !values) || m->n != f->sym->s.n || opts->block < 1 ||
+      opts->block > 8 || !(opts->tolerance > 0.0) || !std::isfinite(shift)) {
+    set_error(err, FC_INVALID_ARGUMENT, "bad arguments to fc_lanczos_device");
+    return FC_INVALID_ARGUMENT;
+  }
+  if (f->sym->s.n == 0) {
+    set_error(err, FC_INVALID_ARGUMENT, "there are no free degrees of freedom");
+    return FC_INVALID_ARGUMENT;
+  }
+  std::lock_guard<std::mutex> g(f->solve_mu);
+  LanczosResult res;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  std::string stopped;
+  int st = guarded(err, "not enough device memory for the Lanczos basis", [&] {
+    HIP_CHECK(hipSetDevice(f->device));
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    HIP_CHECK(hipEventRecord(e0, f->stream));
+    try {
+      res = lanczos_run(f, m, shift, *opts, vectors_device, f->stream);
+    } catch (const LanczosError& le) {
+      stopped = le.msg;
+    }
+    HIP_CHECK(hipEventRecord(e1, f->stream));
+    HIP_CHECK(hipEventSynchronize(e1));
+  });
+  if (st == FC_OK && !stopped.empty()) {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    const int kind = stopped == "cancelled" ? FC_CANCELLED : FC_NOT_CONVERGED;
+    set_error(err, kind, stopped);
+    return kind;
+  }
+  float ms = 0.f;
+  if (st == FC_OK && e0 && e1) (void)hipEventElapsedTime(&ms, e0, e1);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (st != FC_OK) return st;
+  for (size_t i = 0; i < res.values.size(); ++i) values[i] = res.values[i];
+  if (report) {
+    std::memset(report, 0, sizeof *report);
+    report->found = res.values.size();
+    report->basis = (uint64_t)res.basis;
+    report->negative_count = res.negative.size();
+    for (size_t i = 0; i < res.negative.size() && i < 16; ++i) report->negative[i] = res.negative[i];
+    report->seconds = ms * 1e-3;
+  }
+  set_error(err, FC_OK, "");
+  return FC_OK;
+}
 
 int fc_analyze(uint64_t n, const uint64_t* ptr, const uint64_t* idx, uint64_t max_entries, fc_symbolic** out,
                fc_error* err) {

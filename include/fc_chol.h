@@ -32,9 +32,10 @@ typedef enum {
   FC_CANCELLED = 6,             /* FactorError::Cancelled                                   */
   FC_INVALID_ARGUMENT = 7,      /* a malformed pattern, length or option                    */
   FC_DEVICE = 8,                /* a HIP runtime error                                      */
-  FC_DEGENERATE_ELEMENT = 9     /* assembly: an inverted or degenerate element (its index in
+  FC_DEGENERATE_ELEMENT = 9,    /* assembly: an inverted or degenerate element (its index in
                                    element order in fc_error.column), Error::Degenerate of
                                    crates/fem/src/element.rs:326-328                         */
+  FC_NOT_CONVERGED = 10         /* Lanczos: Error::NotConverged (crates/solver/src/eigen.rs)  */
 } fc_status;
 
 typedef struct {
@@ -276,6 +277,39 @@ int fc_assemble_device(fc_assembly* a, int32_t what, double shift, double* upper
 int fc_assembly_info(const fc_assembly* a, uint64_t* contributions_k, uint64_t* contributions_m, double* elements_s,
                      double* gather_s);
 void fc_assembly_free(fc_assembly* a);
+
+/* --- shift-invert block Lanczos ----------------------------------------------------------- */
+
+/* What lanczos (crates/solver/src/eigen.rs:281) is given: Wanted (eigen.rs:178-185), the block
+ * size, the basis cap and the relative residual tolerance of a converged Ritz pair. */
+typedef struct {
+  uint64_t count;      /* modes to return with min <= lambda <= max                        */
+  double min, max;     /* bounds on lambda (-inf, +inf for none)                             */
+  uint32_t block;      /* vectors per block, 1..8 (the reference's BLOCK = 6)               */
+  uint32_t pad;
+  uint64_t max_basis;  /* most Lanczos vectors (lanczos_basis, analysis.rs:642-645)          */
+  double tolerance;    /* ||B_{j+1} s|| <= tolerance theta (TOLERANCE = 1e-9, analysis.rs:653) */
+} fc_lanczos_opts;
+
+typedef struct {
+  uint64_t found;           /* modes returned                                                */
+  uint64_t basis;           /* Lanczos vectors used (Eigenpairs::basis)                      */
+  uint64_t negative_count;  /* converged negative Ritz values (Eigenpairs::negative)         */
+  double negative[16];      /* their eigenvalues, largest |theta| first (the first 16)       */
+  double seconds;           /* device time of the run                                        */
+} fc_lanczos_report;
+
+/* The lowest modes of K phi = lambda M phi with lambda >= shift, by the shifted block Lanczos
+ * method with full reorthogonalization (eigen.rs:281-466; ShiftInvert, analysis.rs:617-635),
+ * entirely on the device: f is the factor of K - shift M (fc_refactor_device after
+ * fc_assemble_device's FC_ASSEMBLE_SHIFTED), m the mass matrix by full rows (fc_matrix_new).
+ * values (host, opts->count doubles) receives the eigenvalues ascending; vectors_device (n x
+ * count column-major doubles of device memory, or NULL) the M-orthonormal eigenvectors, each
+ * with its largest component positive. The start block is the reference's seeded
+ * xorshift64*. Errors: FC_NOT_CONVERGED (the basis cap, a block that loses rank), FC_CANCELLED
+ * (the factor's cancel flag, polled once per block). */
+int fc_lanczos_device(const fc_factor* f, const fc_matrix* m, double shift, const fc_lanczos_opts* opts,
+                      double* values, double* vectors_device, fc_lanczos_report* report, fc_error* err);
 
 /* Library version string. */
 const char* fc_version(void);
