@@ -16,7 +16,7 @@ around its launches on the factor's stream during the timed steps; `cpu_baseline
 oracle (the C restatement of the reference's CPU factorization) on a bounded sample.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--cells 80] [--form mixed|fp32|fp64]
-                      [--contrast] [--workload cube|batch|shift|assemble]
+                      [--contrast] [--workload cube|batch|shift|assemble|modes]
 
 --contrast adds the harder variant of the same pattern (E in {210 GPa, 21 MPa} per element,
 nu = 0.49) with the same symbolic analysis. --workload batch runs 512 independent hex8/tet10
@@ -25,7 +25,9 @@ models (2k-60k equations) as one block-diagonal factorization in FP32 with FP64 
 (sigma = (2 pi 50 Hz)^2) in the mixed form and times 40 refined block solves of 16 vectors;
 --workload assemble times the step before the factor on the device: K_ff of the hex8 cube and
 K - sigma M of the tet10 block assembled into the factor's input, the reference's values bit for
-bit. Those three print their own metric lines.
+bit; --workload modes runs the modal solve on the device: K - sigma M of the tet10 block assembled,
+factored in FP64 and its 20 lowest modes above sigma found by the shifted block Lanczos method.
+Those four print their own metric lines.
 """
 from __future__ import annotations
 
@@ -152,7 +154,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--contrast", action="store_true")
-    ap.add_argument("--workload", default="cube", choices=["cube", "batch", "shift", "assemble"])
+    ap.add_argument("--workload", default="cube", choices=["cube", "batch", "shift", "assemble", "modes"])
+    ap.add_argument("--modes", type=int, default=20)
     ap.add_argument("--models", type=int, default=512)
     ap.add_argument("--tet-cells", type=int, default=50)
     ap.add_argument("--calibrate-copy", action="store_true")
@@ -179,6 +182,8 @@ def main():
         return shift_workload(args, CH, P)
     if args.workload == "assemble":
         return assemble_workload(args, P)
+    if args.workload == "modes":
+        return modes_workload(args, CH, P)
     t0 = time.perf_counter()
     p = P.cube_hex8(args.cells)
     up = p.upper()
@@ -520,6 +525,94 @@ def assemble_workload(args, P):
                                    elements=t10["E"], entries=t10["nnz"], ms=t10["wall"] * 1e3,
                                    elements_ms=t10["elements_s"] * 1e3, gather_ms=t10["gather_s"] * 1e3,
                                    gather_gbs=t10["gbytes"] / t10["gather_s"] / 1e9, plan_s=t10["plan_s"]))
+    print(json.dumps(line), flush=True)
+
+
+def modes_workload(args, CH, P):
+    """The modal solve on the device (the reference's frequency step, analysis.rs:655-740, with
+    the shift of the tet10 workload): K - sigma M assembled from the mesh (fc_assemble_device), its
+    FP64 factor from those device values, and the lowest `--modes` modes above sigma by the shifted
+    block Lanczos method (fc_lanczos_device, block 6, tolerance 1e-9). One step = assembly +
+    refactorization + Lanczos; the first factorization (which also allocates the factor) is
+    outside the timed steps. The CPU baseline is the oracle (numpy Lanczos over the C oracle's
+    factor) on a smaller tet10 block."""
+    import torch
+
+    from fieldcraft_amd import assembly as AS
+
+    k = args.tet_cells
+    sigma = (2 * np.pi * 50.0) ** 2
+    t0 = time.perf_counter()
+    p = P.block_tet10((k, k, k), (k / 100, k / 100, k / 100))
+    uptr, uidx, _ = p.upper()
+    a = AS.Assembly.of_problem(p, uptr, uidx)
+    sym = CH.Symbolic(CH.UpperColumns(p.n, uptr, uidx, np.zeros(len(uidx))))
+    info = sym.info()
+    M = CH.Matrix(p.n, p.ptr, p.idx, p.mval)
+    log(f"bench: tet10 {k}^3, {p.n} equations, set up in {time.perf_counter() - t0:.1f} s")
+    vals = torch.empty(len(uidx), dtype=torch.float64, device="cuda:0")
+    a.assemble_device(vals.data_ptr(), "shifted", sigma)
+    f = CH.Factor(sym, precision="fp64", device_values=vals.data_ptr())
+    first_factor_s = f.seconds()
+    max_basis = 600
+    runs = []
+    for step in range(args.warmup + args.steps):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        a.assemble_device(vals.data_ptr(), "shifted", sigma)
+        ta = time.perf_counter()
+        f.refactor_device(vals.data_ptr())
+        torch.cuda.synchronize()
+        tf = time.perf_counter()
+        r = f.lanczos_device(M, sigma, args.modes, block=6, max_basis=max_basis, tolerance=1e-9)
+        torch.cuda.synchronize()
+        te = time.perf_counter()
+        runs.append(dict(total=te - t, assemble=ta - t, factor=tf - ta, lanczos=te - tf, basis=r["basis"],
+                         lanczos_device_s=r["seconds"]))
+        log(f"bench: step {step}: assemble {1e3 * (ta - t):.1f} ms, factor {tf - ta:.2f} s, "
+            f"Lanczos {te - tf:.2f} s ({r['basis']} vectors, {len(r['values'])} modes)")
+    timed = runs[args.warmup:]
+    mean = {key: float(np.mean([x[key] for x in timed])) for key in timed[0]}
+    # accuracy of the last run's first modes: ||K x - lambda M x||_inf / ||K x||_inf on the host
+    import scipy.sparse as sps
+
+    K = sps.csr_matrix((p.kval, p.idx, p.ptr), shape=(p.n, p.n))
+    Ms = sps.csr_matrix((p.mval, p.idx, p.ptr), shape=(p.n, p.n))
+    X = r["vectors"][:, :3].cpu().numpy()
+    res = [float(np.abs(K @ X[:, c] - r["values"][c] * (Ms @ X[:, c])).max() / np.abs(K @ X[:, c]).max())
+           for c in range(X.shape[1])]
+    cpu = None
+    if not args.no_cpu:
+        from oracle import eigen as EG
+        from oracle import oracle as O
+
+        O.set_threads(args.cpu_threads)
+        kc = 10
+        q = P.block_tet10((kc, kc, kc), (kc / 100, kc / 100, kc / 100))
+        qu = q.upper(q.kval - sigma * q.mval)
+        t = time.perf_counter()
+        of = O.Factor(q.n, *qu)
+        Mq = sps.csr_matrix((q.mval, q.idx, q.ptr), shape=(q.n, q.n))
+        op = EG.Operator(q.n, lambda x: Mq @ x, lambda Xb: np.stack([of.solve(Xb[:, c]) for c in range(Xb.shape[1])], 1))
+        cr = EG.lanczos(op, sigma, args.modes, block=6, max_basis=max_basis, tolerance=1e-9)
+        cs = time.perf_counter() - t
+        cpu = dict(value=cs, unit="s", cores=args.cpu_threads, kind="port",
+                   sample=f"oracle factor (C, {args.cpu_threads} threads) + numpy Lanczos of the tet10 {kc}^3 block "
+                          f"({q.n} equations, {args.modes} modes, {cr['basis']} vectors): {cs:.2f} s")
+        log(f"bench: cpu baseline {cs:.2f} s on {q.n} equations")
+    F = info["flops"]
+    line = dict(metric=f"modal solve on the device, tet10 {k}^3 (~3M dof): K - sigma M assembled, FP64 factor, "
+                       f"{args.modes} modes by shift-invert block Lanczos: s",
+                value=mean["total"], unit="s", n_gpus=1, steps=args.steps, warmup=args.warmup,
+                ms_per_step=mean["total"] * 1e3, higher_is_better=False, scaling="weak", vs_baseline=None, dtype="f64",
+                data="synthetic",
+                config=dict(workload=f"tet10 {k}^3 Kuhn block, steel, z=0 clamped, sigma = (2 pi 50)^2, {args.modes} "
+                                     f"modes, block 6, tolerance 1e-9", equations=p.n, factor_entries=info["entries"],
+                            flops=F),
+                assemble_ms=mean["assemble"] * 1e3, factor_s=mean["factor"], factor_tflops=F / mean["factor"] / 1e12,
+                lanczos_s=mean["lanczos"], lanczos_device_s=mean["lanczos_device_s"], basis=int(timed[-1]["basis"]),
+                first_factor_s=first_factor_s, frequencies_hz=(np.sqrt(r["values"]) / (2 * np.pi)).tolist(),
+                residual_first3=res, cpu_baseline=cpu)
     print(json.dumps(line), flush=True)
 
 

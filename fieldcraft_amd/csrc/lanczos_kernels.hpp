@@ -9,7 +9,7 @@
 
 namespace fc {
 
-constexpr int TS_ROWS = 65536;  // rows of one partial sum of ts_dots
+constexpr int TS_ROWS = 4096;   // rows of one partial sum of ts_dots (and of the argmax chunks)
 constexpr int TS_COLS = 8;      // basis columns one ts_dots workgroup takes
 constexpr int TS_MAXB = 8;      // block columns
 

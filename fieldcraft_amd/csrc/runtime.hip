@@ -988,6 +988,7 @@ void run_solve_r(const fc_factor* F, double* xp, double* yp, double* u, i64 ldx,
 template <typename T>
 void run_solve(const fc_factor* F, double* xp, double* yp, double* u, i64 ldx, int nrhs, hipStream_t st) {
   if (nrhs == 1) run_solve_r<T, 1>(F, xp, yp, u, ldx, nrhs, st);
+  else if (nrhs <= fc::RC / 2) run_solve_r<T, fc::RC / 2>(F, xp, yp, u, ldx, nrhs, st);  // a Lanczos block
   else run_solve_r<T, fc::RC>(F, xp, yp, u, ldx, nrhs, st);
 }
 
