@@ -552,7 +552,8 @@ def modes_workload(args, CH, P):
     log(f"bench: tet10 {k}^3, {p.n} equations, set up in {time.perf_counter() - t0:.1f} s")
     vals = torch.empty(len(uidx), dtype=torch.float64, device="cuda:0")
     a.assemble_device(vals.data_ptr(), "shifted", sigma)
-    f = CH.Factor(sym, precision="fp64", device_values=vals.data_ptr())
+    gemm_kinds = ("update", "syrk", "trsm")  # the kinds k_gemm<double> runs (HIP events per launch)
+    f = CH.Factor(sym, precision="fp64", device_values=vals.data_ptr(), profile=gemm_kinds)
     first_factor_s = f.seconds()
     max_basis = 600
     runs = []
@@ -567,8 +568,10 @@ def modes_workload(args, CH, P):
         r = f.lanczos_device(M, sigma, args.modes, block=6, max_basis=max_basis, tolerance=1e-9)
         torch.cuda.synchronize()
         te = time.perf_counter()
+        prof = [f.profile(kk) for kk in gemm_kinds]
         runs.append(dict(total=te - t, assemble=ta - t, factor=tf - ta, lanczos=te - tf, basis=r["basis"],
-                         lanczos_device_s=r["seconds"]))
+                         lanczos_device_s=r["seconds"], gemm_s=sum(x["seconds"] for x in prof),
+                         gemm_flops=sum(x["flops"] for x in prof), gemm_launches=sum(x["launches"] for x in prof)))
         log(f"bench: step {step}: assemble {1e3 * (ta - t):.1f} ms, factor {tf - ta:.2f} s, "
             f"Lanczos {te - tf:.2f} s ({r['basis']} vectors, {len(r['values'])} modes)")
     timed = runs[args.warmup:]
@@ -601,6 +604,19 @@ def modes_workload(args, CH, P):
                           f"({q.n} equations, {args.modes} modes, {cr['basis']} vectors): {cs:.2f} s")
         log(f"bench: cpu baseline {cs:.2f} s on {q.n} equations")
     F = info["flops"]
+    # the step's dominant kernel: the FP64 factor's k_gemm<double> (panel updates, update matrices,
+    # TRSM), its algorithmic flops over its event-timed seconds
+    achieved = mean["gemm_flops"] / mean["gemm_s"] / 1e12
+    roof = dict(bound="mfma", achieved=achieved, peak=PEAK["fp64"], unit="TFLOP/s", frac=achieved / PEAK["fp64"],
+                traffic=None, kernel="k_gemm<double, false>", kinds=list(gemm_kinds), launches=mean["gemm_launches"],
+                avg_launch_s=mean["gemm_s"] / max(mean["gemm_launches"], 1))
+    # the Lanczos phase against HBM: per block of b = 6 vectors the FP64 solve reads L twice, the
+    # two Gram-Schmidt passes read MQ and Q (n x m each) and write the block, and about 8 products
+    # with M read its nnz values and indices
+    b, m_final, nnzm = 6, int(timed[-1]["basis"]), int(p.ptr[-1])
+    lbytes = sum(2 * info["entries"] * 8 + 2 * 2 * p.n * m * 8 + 8 * (nnzm * 12 + p.n * 8 * 2)
+                 for m in range(b, m_final + 1, b))
+    lanczos_gbs = lbytes / mean["lanczos_device_s"] / 1e9
     line = dict(metric=f"modal solve on the device, tet10 {k}^3 (~3M dof): K - sigma M assembled, FP64 factor, "
                        f"{args.modes} modes by shift-invert block Lanczos: s",
                 value=mean["total"], unit="s", n_gpus=1, steps=args.steps, warmup=args.warmup,
@@ -612,7 +628,10 @@ def modes_workload(args, CH, P):
                 assemble_ms=mean["assemble"] * 1e3, factor_s=mean["factor"], factor_tflops=F / mean["factor"] / 1e12,
                 lanczos_s=mean["lanczos"], lanczos_device_s=mean["lanczos_device_s"], basis=int(timed[-1]["basis"]),
                 first_factor_s=first_factor_s, frequencies_hz=(np.sqrt(r["values"]) / (2 * np.pi)).tolist(),
-                residual_first3=res, cpu_baseline=cpu)
+                residual_first3=res, roofline=roof,
+                lanczos_roofline=dict(bound="hbm", achieved=lanczos_gbs, peak=HBM_PEAK, unit="GB/s",
+                                      frac=lanczos_gbs / HBM_PEAK, bytes=lbytes),
+                cpu_baseline=cpu)
     print(json.dumps(line), flush=True)
 
 
