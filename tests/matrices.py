@@ -182,3 +182,114 @@ def grid_12x12():
     # the reference keeps each row in push order (no sort); the graph algorithms only test
     # membership and the fill is sorted, so sorted rows give the same results
     return xyz, to_csr_graph(adj)
+
+
+# --- the reference's LDL^T test matrices (crates/solver/src/ldlt.rs:575-632, 981-1021) -------
+
+def rows_to_csr(rows):
+    """csr (ldlt.rs:576-590): each row sorted by column (stable), full rows."""
+    ptr, idx, val = [0], [], []
+    for row in rows:
+        for c, v in sorted(row, key=lambda e: e[0]):
+            idx.append(c)
+            val.append(v)
+        ptr.append(len(idx))
+    return len(rows), np.array(ptr, np.int64), np.array(idx, np.int64), np.array(val, np.float64)
+
+
+def shifted_grid(n3, sigma):
+    """shifted_grid (ldlt.rs:596-632): a 3D grid Laplacian on 3 coupled unknowns per point,
+    shifted by sigma, and the count of its eigenvalues below sigma (Dirichlet closed form)."""
+    nx, ny, nz = n3
+    point = lambda i, j, k: i + nx * (j + ny * k)
+    size = 3 * nx * ny * nz
+    rows = [[] for _ in range(size)]
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                p = 3 * point(i, j, k)
+                for a in range(3):
+                    rows[p + a].append((p + a, 6.0 - sigma))
+                nxt = [point(i + 1, j, k) if i + 1 < nx else None, point(i, j + 1, k) if j + 1 < ny else None,
+                       point(i, j, k + 1) if k + 1 < nz else None]
+                for q in nxt:
+                    if q is None:
+                        continue
+                    for a in range(3):
+                        rows[p + a].append((3 * q + a, -1.0))
+                        rows[3 * q + a].append((p + a, -1.0))
+    axis = lambda n: [2.0 - 2.0 * np.cos(np.pi * m / (n + 1.0)) for m in range(1, n + 1)]
+    below = sum(3 for a in axis(nx) for b in axis(ny) for c in axis(nz) if a + b + c < sigma)
+    return rows_to_csr(rows), below
+
+
+def saddle(nk, nb):
+    """saddle (ldlt.rs:652-688): [[K, B], [B^T, 0]] with the zero-diagonal columns first."""
+    n = nk + nb
+    rows = [[] for _ in range(n)]
+    for i in range(nk):
+        for j in range(nk):
+            rows[i].append((j, 0.5 if i == j else 0.05 / (1.0 + (i + j))))
+    for c in range(nb):
+        for i in range(nk):
+            v = float((i * 7 + c * 3) % 5) - 2.0 + (3.0 if i == c else 0.0)
+            rows[i].append((nk + c, v))
+            rows[nk + c].append((i, v))
+        for d in range(nb):
+            rows[nk + c].append((nk + d, 0.0))
+    order = list(range(nk, n)) + list(range(nk))
+    inverse = [order.index(i) for i in range(n)]
+    permuted = [None] * n
+    for i, row in enumerate(rows):
+        permuted[inverse[i]] = [(inverse[c], v) for c, v in row]
+    return rows_to_csr(permuted)
+
+
+def hex_pattern(n, sigma, nested_dissection):
+    """hex_pattern (ldlt.rs:981-1021): a 27-point hex8-like block pattern on n^3 nodes, numbered
+    by the reference's coordinate nested dissection (passed in: the oracle's restatement)."""
+    ident = lambda i, j, k: i + n * (j + n * k)
+    nodes = n ** 3
+    xyz = np.array([[v % n, (v // n) % n, v // (n * n)] for v in range(nodes)], np.float64)
+    adjacency = [[] for _ in range(nodes)]
+    for k in range(n):
+        for j in range(n):
+            for i in range(n):
+                for t in range(27):
+                    a, b, c = t % 3, (t // 3) % 3, t // 9
+                    x, y, z = i + a, j + b, k + c
+                    if 1 <= x <= n and 1 <= y <= n and 1 <= z <= n:
+                        adjacency[ident(i, j, k)].append(ident(x - 1, y - 1, z - 1))
+    adj_ptr = np.zeros(nodes + 1, np.int64)
+    adj_ptr[1:] = np.cumsum([len(a) for a in adjacency])
+    adj_idx = np.array([w for a in adjacency for w in a], np.int64)
+    order = nested_dissection(xyz, adj_ptr, adj_idx)
+    place = np.zeros(nodes, np.int64)
+    place[order] = np.arange(nodes)
+    rows = [[] for _ in range(3 * nodes)]
+    for v, adj in enumerate(adjacency):
+        for w in adj:
+            for a in range(3):
+                for b in range(3):
+                    if v == w:
+                        value = 30.0 - sigma if a == b else 0.5
+                    elif a == b:
+                        value = -1.0
+                    else:
+                        value = 0.01 * ((v + w) % 3)
+                    rows[3 * place[v] + a].append((3 * place[w] + b, value))
+    return rows_to_csr(rows)
+
+
+def fingerprint(values):
+    """FNV-1a over the bit patterns of f64 values (ldlt.rs:756-758)."""
+    h = 0xCBF29CE484222325
+    for v in np.asarray(values, np.float64).view(np.uint64):
+        h = ((h ^ int(v)) * 0x100000001B3) & 0xFFFFFFFFFFFFFFFF
+    return h
+
+
+def frozen_rhs(n, mul):
+    """The exact right-hand side of the frozen cases (ldlt.rs:785-788): x_i = ((37 i mod 101) - 50) / 50, b = A x."""
+    x = np.array([((i * 37 % 101) - 50.0) / 50.0 for i in range(n)])
+    return mul(x)

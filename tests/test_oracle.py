@@ -113,3 +113,15 @@ def test_csr_mul_closed_form():
     n, ptr, idx, val = dense_to_csr(k)
     y = O.csr_mul(n, ptr, idx, val, np.ones(9))
     assert y.tolist() == [1.0] * 3 + [2.0] * 3 + [1.0] * 3
+
+
+def test_cholesky_solution_bits_match_the_reference_fingerprint():
+    """ldlt.rs:809-816 freezes the Cholesky solution of shifted_grid([5, 5, 5], -0.5) with the exact
+    right-hand side A x, x_i = ((37 i mod 101) - 50) / 50, as an FNV-1a fingerprint of its bits:
+    the oracle's analysis, factorization and solve reproduce the reference bit for bit."""
+    from matrices import fingerprint, frozen_rhs, shifted_grid
+
+    (n, ptr, idx, val), _ = shifted_grid([5, 5, 5], -0.5)
+    f = O.Factor(n, *O.from_symmetric(n, ptr, idx, val))
+    b = frozen_rhs(n, lambda x: O.csr_mul(n, ptr, idx, val, x))
+    assert fingerprint(f.solve(b)) == 15069623423640047316
