@@ -298,6 +298,13 @@ typedef struct orc_factor {
   i64* voff;    /* ns+1 offsets of each supernode's trapezoid in val */
   double* val;
   int have_values;
+  /* LDL^T (ldlt.rs): pivot kinds per postorder column (0 one, 1 first, 2 second of a 2 x 2),
+     each supernode's local pivot order (order[first[s] + c] = the symbolic local column at
+     pivot position c), the largest magnitude of each row of A, the count of 2 x 2 pivots */
+  signed char* piv;
+  i64* order;
+  double* scale;
+  i64 two_by_two;
 } orc_factor;
 
 static i64 s_first(const orc_factor* f, i64 s) { return f->first[s]; }
@@ -320,6 +327,7 @@ void orc_free(orc_factor* f) {
   free(f->pat.ptr); free(f->pat.idx); free(f->pat.slot); free(f->pat.crow); free(f->pat.ccol);
   free(f->upper_ptr); free(f->upper_idx);
   free(f->lval); free(f->diag); free(f->voff); free(f->val);
+  free(f->piv); free(f->order); free(f->scale);
   free(f);
 }
 
@@ -1080,4 +1088,425 @@ int orc_fill_reducing_order(i64 n, const double* xyz, const i64* aptr, const i64
   if (graph) memcpy(order, by_graph, sizeof(i64) * n);
   free(by_graph);
   return graph;
+}
+
+
+/* ------------------------------------------------------------------------------------------ */
+/* LDL^T with bounded Bunch-Kaufman (rook) pivoting inside supernodes (crates/solver/src/ldlt.rs),
+ * on the Cholesky factor's symbolic analysis, and its pivoted solves (cholesky.rs:904-1037).   */
+
+#define ALPHA 0.6403882032022076 /* (1 + sqrt 17) / 8, ldlt.rs:78 */
+#define ROOK_STEPS 64            /* ldlt.rs:82 */
+enum { PIV_ONE = 0, PIV_FIRST = 1, PIV_SECOND = 2 };
+
+/* times_d (cholesky.rs:460-484): row (entries for columns k0.. of the trapezoid t of m rows)
+ * <- row D, with the pivots from column k0 */
+static void times_d(double* row, i64 len, const double* t, i64 m, i64 k0, const signed char* piv) {
+  i64 k = 0;
+  while (k < len) {
+    if (piv[k] == PIV_FIRST) {
+      double a = row[k], b = k + 1 < len ? row[k + 1] : 0.0;
+      double d11 = AT(t, m, k0 + k, k0 + k), d21 = AT(t, m, k0 + k + 1, k0 + k), d22 = AT(t, m, k0 + k + 1, k0 + k + 1);
+      row[k] = a * d11 + b * d21;
+      if (k + 1 < len) row[k + 1] = a * d21 + b * d22;
+      k += 2;
+    } else {
+      row[k] *= AT(t, m, k0 + k, k0 + k);
+      k += 1;
+    }
+  }
+}
+
+/* products + subtract with Y = L D (cholesky.rs:531-686, pivots given): for the columns
+ * [k0, k0 + kw) of trapezoid lk (mk rows, pivots pk from k0), rows p.. :
+ * own(rel(i), rel(c)) -= sum_k L(p + i, k) Y(p + c, k), i >= c, each sum in k order from 0.0 */
+static void ldlt_update(double* own, i64 m, const double* lk, i64 mk, i64 k0, i64 kw, const signed char* pk, i64 p,
+                        i64 mu, i64 wu, const i64* rel, int inner) {
+  i64 cols = wu < mu ? wu : mu;
+  double* y = (double*)malloc(sizeof(double) * (size_t)(cols * kw > 0 ? cols * kw : 1));
+  for (i64 c = 0; c < cols; ++c) {
+    for (i64 k = 0; k < kw; ++k) y[c * kw + k] = AT(lk, mk, p + c, k0 + k);
+    times_d(y + c * kw, kw, lk, mk, k0, pk);
+  }
+#pragma omp parallel for schedule(dynamic, 1) if (inner && mu * wu * kw >= PARALLEL_WORK)
+  for (i64 c = 0; c < cols; ++c) {
+    i64 cc = rel[c];
+    double* target = own + offset(m, cc);
+    const double* yc = y + c * kw;
+    for (i64 i = c; i < mu; ++i) {
+      double s = 0.0;
+      for (i64 k = 0; k < kw; ++k) s += AT(lk, mk, p + i, k0 + k) * yc[k];
+      target[rel[i] - cc] -= s;
+    }
+  }
+  free(y);
+}
+
+/* interchange (ldlt.rs:131-145): local columns p < q of an m-row trapezoid */
+static void interchange(double* v, i64 m, i64 p, i64 q) {
+#define SWP(x, y) do { double t_ = (x); (x) = (y); (y) = t_; } while (0)
+  for (i64 c = 0; c < p; ++c) SWP(AT(v, m, p, c), AT(v, m, q, c));
+  SWP(AT(v, m, p, p), AT(v, m, q, q));
+  for (i64 c = p + 1; c < q; ++c) SWP(AT(v, m, c, p), AT(v, m, q, c));
+  for (i64 r = q + 1; r < m; ++r) SWP(AT(v, m, r, p), AT(v, m, r, q));
+#undef SWP
+}
+
+/* largest (ldlt.rs:149-164): the largest |v[i]| over [lo, hi) except skip (first index of the
+ * largest; NaN returned at once) */
+static void largest(const double* v, i64 lo, i64 hi, i64 skip, i64* at, double* value) {
+  i64 bi = lo;
+  double bv = 0.0;
+  for (i64 i = lo; i < hi; ++i) {
+    if (i == skip) continue;
+    double x = fabs(v[i]);
+    if (isnan(x)) { *at = i; *value = NAN; return; }
+    if (x > bv) { bi = i; bv = x; }
+  }
+  *at = bi;
+  *value = bv;
+}
+
+typedef struct {
+  double* values;
+  signed char* piv;
+  i64* order;
+  i64 m, w;
+  double *a, *b, *wrow;
+  i64 two_by_two;
+} block_t;
+
+/* Block::current (ldlt.rs:176-193): column c (local, >= k) rows k..m, less the products with the
+ * panel's factored columns c0..k */
+static void current(block_t* B, i64 c0, i64 k, i64 c, double* out) {
+  i64 m = B->m;
+  for (i64 r = k; r < m; ++r) out[r] = r >= c ? AT(B->values, m, r, c) : AT(B->values, m, c, r);
+  i64 len = k - c0;
+  for (i64 q = c0; q < k; ++q) B->wrow[q - c0] = AT(B->values, m, c, q);
+  times_d(B->wrow, len, B->values, m, c0, B->piv + c0);
+  for (i64 q = c0; q < k; ++q) {
+    double wq = B->wrow[q - c0];
+    for (i64 r = k; r < m; ++r) out[r] -= AT(B->values, m, r, q) * wq;
+  }
+}
+
+/* Block::swap (ldlt.rs:196-206) */
+static void bswap(block_t* B, i64 p, i64 q) {
+  if (p == q) return;
+  if (p > q) { i64 t = p; p = q; q = t; }
+  interchange(B->values, B->m, p, q);
+  i64 t = B->order[p]; B->order[p] = B->order[q]; B->order[q] = t;
+  double x = B->a[p]; B->a[p] = B->a[q]; B->a[q] = x;
+  x = B->b[p]; B->b[p] = B->b[q]; B->b[q] = x;
+}
+
+/* Block::one, Block::two (ldlt.rs:207-253) */
+static void store_one(block_t* B, i64 k, const double* col) {
+  i64 m = B->m, start = offset(m, k);
+  double d = col[k];
+  B->values[start] = d;
+  for (i64 r = k + 1; r < m; ++r) B->values[start + r - k] = col[r] / d;
+  B->piv[k] = PIV_ONE;
+}
+static void store_two(block_t* B, i64 k, const double* a, const double* b) {
+  i64 m = B->m;
+  double d11 = a[k], d21 = a[k + 1], d22 = b[k + 1];
+  double det = d11 * d22 - d21 * d21;
+  i64 s0 = offset(m, k), s1 = offset(m, k + 1);
+  B->values[s0] = d11;
+  B->values[s0 + 1] = d21;
+  B->values[s1] = d22;
+  for (i64 r = k + 2; r < m; ++r) {
+    double x = a[r], y = b[r];
+    B->values[s0 + r - k] = (x * d22 - y * d21) / det;
+    B->values[s1 + r - k - 1] = (y * d11 - x * d21) / det;
+  }
+  B->piv[k] = PIV_FIRST;
+  B->piv[k + 1] = PIV_SECOND;
+  B->two_by_two += 1;
+}
+
+/* Block::panel (ldlt.rs:256-335): factors the panel from c0; returns where it ended, or -1 with
+ * *bad the local pivot position of a singular column */
+static i64 ldlt_panel(block_t* B, i64 c0, const double* scale_local, i64* bad) {
+  i64 w = B->w;
+  i64 end = c0 + PANEL < w ? c0 + PANEL : w;
+  i64 k = c0;
+  while (k < end) {
+    current(B, c0, k, k, B->a);
+    double akk = fabs(B->a[k]);
+    i64 r0;
+    double omega_k;
+    largest(B->a, k + 1, w, -1, &r0, &omega_k);
+    if (!isfinite(akk) || !isfinite(omega_k)) { *bad = k; return -1; }
+    int two = 0;
+    i64 one_at = k, ti = k, tr = k;
+    if (akk >= ALPHA * omega_k) {
+      one_at = k;
+    } else {
+      i64 i = k, r = r0;
+      double omega_i = omega_k;
+      int step = 0;
+      for (;;) {
+        current(B, c0, k, r, B->b);
+        i64 s_at;
+        double omega_r;
+        largest(B->b, k, w, r, &s_at, &omega_r);
+        double arr = fabs(B->b[r]);
+        if (!isfinite(arr) || !isfinite(omega_r)) { *bad = k; return -1; }
+        step += 1;
+        if (arr >= ALPHA * omega_r) {
+          double* t = B->a; B->a = B->b; B->b = t;
+          one_at = r;
+          break;
+        }
+        if (omega_r <= omega_i || step >= ROOK_STEPS) { two = 1; ti = i; tr = r; break; }
+        double* t = B->a; B->a = B->b; B->b = t;
+        i = r; omega_i = omega_r; r = s_at;
+      }
+    }
+    if (!two) {
+      bswap(B, k, one_at);
+      if (!(fabs(B->a[k]) > PIVOT_TOLERANCE * scale_local[B->order[k]])) { *bad = k; return -1; }
+      store_one(B, k, B->a);
+      k += 1;
+    } else {
+      bswap(B, k, ti);
+      i64 r = tr == k ? ti : tr;
+      bswap(B, k + 1, r);
+      double off = fabs(B->a[k + 1]);
+      double s0 = scale_local[B->order[k]], s1 = scale_local[B->order[k + 1]];
+      double size = s0 > s1 ? s0 : s1;
+      if (!(off > PIVOT_TOLERANCE * size)) { *bad = k; return -1; }
+      store_two(B, k, B->a, B->b);
+      k += 2;
+    }
+  }
+  return k;
+}
+
+/* supernode (ldlt.rs:338-358): assemble (A's columns, then the descendants' L D L^T), then the
+ * panels right-looking. Returns ORC_OK or ORC_SINGULAR with *failed_column (postorder). */
+static int ldlt_supernode(orc_factor* f, i64 j, int inner, i64* failed_column) {
+  const i64* rows = s_rows(f, j);
+  i64 m = s_m(f, j), w = s_w(f, j), fc = s_first(f, j);
+  double* own = f->val + f->voff[j];
+  memset(own, 0, sizeof(double) * trapezoid(m, w));
+  for (i64 c = 0; c < w; ++c) {
+    for (i64 t = f->pat.ptr[fc + c]; t < f->pat.ptr[fc + c + 1]; ++t) {
+      i64 r = f->pat.idx[t];
+      i64 lo = 0, hi = m;
+      while (lo < hi) { i64 mid = (lo + hi) / 2; if (rows[mid] < r) lo = mid + 1; else hi = mid; }
+      if (lo >= m || rows[lo] != r || lo < c) return ORC_INTERNAL;
+      own[offset(m, c) + lo - c] = f->lval[t];
+    }
+  }
+  i64* rel = (i64*)malloc(sizeof(i64) * (m ? m : 1));
+  for (i64 q = f->uptr[j]; q < f->uptr[j + 1]; ++q) {
+    update_t u = f->upd[q];
+    const i64* kr = s_rows(f, u.k);
+    i64 km = s_m(f, u.k), kw = s_w(f, u.k);
+    i64 mu = km - u.p1, wu = u.p2 - u.p1;
+    i64 at = 0;
+    for (i64 i = 0; i < mu; ++i) {
+      i64 r = kr[u.p1 + i];
+      while (at < m && rows[at] < r) ++at;
+      if (at >= m || rows[at] != r) { free(rel); return ORC_INTERNAL; }
+      rel[i] = at;
+    }
+    ldlt_update(own, m, f->val + f->voff[u.k], km, 0, kw, f->piv + s_first(f, u.k), u.p1, mu, wu, rel, inner);
+  }
+  block_t B;
+  B.values = own;
+  B.piv = f->piv + fc;
+  B.order = f->order + fc;
+  B.m = m;
+  B.w = w;
+  B.a = (double*)calloc((size_t)(m ? m : 1), sizeof(double));
+  B.b = (double*)calloc((size_t)(m ? m : 1), sizeof(double));
+  B.wrow = (double*)malloc(sizeof(double) * (PANEL + 2));
+  B.two_by_two = 0;
+  for (i64 c = 0; c < w; ++c) { B.piv[c] = PIV_ONE; B.order[c] = c; }
+  double* scale_local = f->scale + fc;
+  for (i64 i = 0; i < m; ++i) rel[i] = i;
+  int status = ORC_OK;
+  i64 c0 = 0;
+  while (c0 < w) {
+    i64 bad = -1;
+    i64 c1 = ldlt_panel(&B, c0, scale_local, &bad);
+    if (c1 < 0) { *failed_column = fc + B.order[bad]; status = ORC_SINGULAR; break; }
+    if (c1 < w) ldlt_update(own, m, own, m, c0, c1 - c0, B.piv + c0, c1, m - c1, w - c1, rel + c1, inner);
+    c0 = c1;
+  }
+  double* pa = B.a < B.b ? B.a : B.b; /* the two work columns, whichever order the swaps left */
+  double* pb = B.a < B.b ? B.b : B.a;
+  free(pa); free(pb); free(B.wrow); free(rel);
+#pragma omp atomic
+  f->two_by_two += B.two_by_two;
+  return status;
+}
+
+/* LdltFactor::from_symbolic (ldlt.rs:377-391) with Symbolic::numeric's scale
+ * (cholesky.rs:1390-1416) and the scheduler of Symbolic::factor (cholesky.rs:1443-1529) */
+int orc_ldlt_numeric(orc_factor* f, const double* aval, i64* column_out) {
+  numeric(f, aval);
+  i64 ns = f->ns, n = f->n;
+  free(f->scale);
+  f->scale = (double*)calloc(n ? n : 1, sizeof(double));
+  for (i64 e = 0; e < f->pat.nnz; ++e) {
+    double x = fabs(aval[e]);
+    i64 r = f->pat.crow[e], c = f->pat.ccol[e];
+    /* f64::max: a NaN entry leaves the scale as it is */
+    if (x > f->scale[r]) f->scale[r] = x;
+    if (x > f->scale[c]) f->scale[c] = x;
+  }
+  free(f->voff); free(f->val); free(f->piv); free(f->order);
+  f->voff = (i64*)malloc(sizeof(i64) * (ns + 1));
+  f->voff[0] = 0;
+  for (i64 s2 = 0; s2 < ns; ++s2) f->voff[s2 + 1] = f->voff[s2] + trapezoid(s_m(f, s2), s_w(f, s2));
+  f->val = (double*)calloc(f->voff[ns] ? f->voff[ns] : 1, sizeof(double));
+  f->piv = (signed char*)calloc(n ? n : 1, 1);
+  f->order = (i64*)calloc(n ? n : 1, sizeof(i64));
+  f->two_by_two = 0;
+  f->have_values = 0;
+  i64 nt = f->ntasks;
+  i64* fail_col = (i64*)malloc(sizeof(i64) * (nt ? nt : 1));
+  int* fail_kind = (int*)calloc(nt ? nt : 1, sizeof(int));
+  int broken = ORC_OK;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (i64 t = 0; t < nt; ++t) {
+    for (i64 j = f->tasks[2 * t]; j < f->tasks[2 * t + 1]; ++j) {
+      i64 col = NONE;
+      int e = ldlt_supernode(f, j, 0, &col);
+      if (e == ORC_SINGULAR) { fail_kind[t] = e; fail_col[t] = col; break; }
+      if (e != ORC_OK) {
+#pragma omp critical
+        { if (broken == ORC_OK) broken = e; }
+        break;
+      }
+    }
+  }
+  if (broken != ORC_OK) { free(fail_col); free(fail_kind); return broken; }
+  i64 earliest = NONE;
+  for (i64 t = 0; t < nt; ++t)
+    if (fail_kind[t] && (earliest == NONE || fail_col[t] < earliest)) earliest = fail_col[t];
+  free(fail_col); free(fail_kind);
+  for (i64 q = 0; q < f->ntop; ++q) {
+    i64 j = f->top[q];
+    if (earliest != NONE && f->first[j] > earliest) break;
+    i64 col = NONE;
+    int e = ldlt_supernode(f, j, g_parallel_inner, &col);
+    if (e == ORC_SINGULAR) { *column_out = f->perm[col]; return e; }
+    if (e != ORC_OK) return e;
+  }
+  if (earliest != NONE) { *column_out = f->perm[earliest]; return ORC_SINGULAR; }
+  f->have_values = 1;
+  return ORC_OK;
+}
+
+int orc_ldlt_new(i64 n, const i64* ptr, const i64* idx, const double* val, i64 max_entries, orc_factor** out,
+                 i64* column_out, i64* entries_out) {
+  orc_factor* f = NULL;
+  int e = orc_analyze(n, ptr, idx, max_entries, &f, entries_out);
+  if (e != ORC_OK) return e;
+  e = orc_ldlt_numeric(f, val, column_out);
+  if (e != ORC_OK) { orc_free(f); return e; }
+  *out = f;
+  return ORC_OK;
+}
+
+/* solve_group (cholesky.rs:904-1037) with LDL^T's unit L, block diagonal D and pivot order, one
+ * right-hand side */
+static void ldlt_solve_one(const orc_factor* f, double* b) {
+  i64 n = f->n;
+  double* x = (double*)malloc(sizeof(double) * (n ? n : 1));
+  double* y = (double*)malloc(sizeof(double) * (n ? n : 1));
+  for (i64 i = 0; i < n; ++i) x[i] = b[f->perm[i]];
+  for (i64 k = 0; k < f->ns; ++k) {
+    const i64* rows = s_rows(f, k);
+    i64 m = s_m(f, k), fc = s_first(f, k), w = s_w(f, k);
+    const double* t = f->val + f->voff[k];
+    const signed char* pv = f->piv + fc;
+    const i64* od = f->order + fc;
+    double* own = x + fc;
+    for (i64 c = 0; c < w; ++c) y[c] = own[od[c]];
+    for (i64 q = 0; q < w; ++q) {
+      i64 end = q + (pv[q] == PIV_FIRST ? 2 : 1);
+      double yq = y[q];
+      for (i64 r = end; r < w; ++r) y[r] -= AT(t, m, r, q) * yq;
+      for (i64 r = w; r < m; ++r) x[rows[r]] -= AT(t, m, r, q) * yq;
+    }
+    i64 c = 0;
+    while (c < w) {
+      if (pv[c] == PIV_FIRST) {
+        double d11 = AT(t, m, c, c), d21 = AT(t, m, c + 1, c), d22 = AT(t, m, c + 1, c + 1);
+        double det = d11 * d22 - d21 * d21;
+        double p = y[c], q2 = y[c + 1];
+        y[c] = (d22 * p - d21 * q2) / det;
+        y[c + 1] = (d11 * q2 - d21 * p) / det;
+        c += 2;
+      } else {
+        y[c] /= AT(t, m, c, c);
+        c += 1;
+      }
+    }
+    for (i64 c2 = 0; c2 < w; ++c2) own[c2] = y[c2]; /* left in pivot order */
+  }
+  for (i64 k = f->ns - 1; k >= 0; --k) {
+    const i64* rows = s_rows(f, k);
+    i64 m = s_m(f, k), fc = s_first(f, k), w = s_w(f, k);
+    const double* t = f->val + f->voff[k];
+    const signed char* pv = f->piv + fc;
+    const i64* od = f->order + fc;
+    double* own = x + fc;
+    for (i64 c = 0; c < w; ++c) y[c] = own[c];
+    for (i64 c = w - 1; c >= 0; --c) {
+      i64 q, end;
+      if (pv[c] == PIV_SECOND && c > 0) { q = c - 1; end = c + 1; }
+      else if (pv[c] == PIV_FIRST) { q = c + 1; end = c + 2; }
+      else { q = c; end = c + 1; }
+      double sum = y[q];
+      for (i64 r = end; r < w; ++r) sum -= AT(t, m, r, q) * y[r];
+      for (i64 r = w; r < m; ++r) sum -= AT(t, m, r, q) * x[rows[r]];
+      y[q] = sum;
+    }
+    for (i64 pos = 0; pos < w; ++pos) own[od[pos]] = y[pos];
+  }
+  for (i64 i = 0; i < n; ++i) b[f->perm[i]] = x[i];
+  free(x);
+  free(y);
+}
+
+int orc_ldlt_solve(const orc_factor* f, double* bs, i64 nrhs) {
+  if (!f->have_values || !f->piv) return ORC_INTERNAL;
+#pragma omp parallel for schedule(dynamic, 1) if (nrhs > 1)
+  for (i64 r = 0; r < nrhs; ++r) ldlt_solve_one(f, bs + r * f->n);
+  return ORC_OK;
+}
+
+/* LdltFactor::inertia (ldlt.rs:412-445) and two_by_two */
+void orc_ldlt_inertia(const orc_factor* f, i64* positive, i64* negative, i64* two_by_two) {
+  i64 pos = 0, neg = 0;
+  for (i64 k = 0; k < f->ns; ++k) {
+    i64 m = s_m(f, k), fc = s_first(f, k), w = s_w(f, k);
+    const double* t = f->val + f->voff[k];
+    const signed char* pv = f->piv + fc;
+    i64 c = 0;
+    while (c < w) {
+      if (pv[c] == PIV_FIRST) {
+        double d11 = AT(t, m, c, c), d21 = AT(t, m, c + 1, c), d22 = AT(t, m, c + 1, c + 1);
+        double det = d11 * d22 - d21 * d21;
+        if (det < 0.0) { pos += 1; neg += 1; }
+        else if (d11 + d22 > 0.0) pos += 2;
+        else neg += 2;
+        c += 2;
+      } else {
+        if (AT(t, m, c, c) > 0.0) pos += 1; else neg += 1;
+        c += 1;
+      }
+    }
+  }
+  *positive = pos;
+  *negative = neg;
+  *two_by_two = f->two_by_two;
 }

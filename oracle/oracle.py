@@ -73,6 +73,12 @@ def lib():
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_threads.restype = C.c_int
         L.orc_set_parallel_inner.argtypes = [C.c_int]
+        L.orc_ldlt_new.argtypes = [i64, _i64p, _i64p, _f64p, i64, C.POINTER(C.c_void_p), C.POINTER(i64),
+                                   C.POINTER(i64)]
+        L.orc_ldlt_new.restype = C.c_int
+        L.orc_ldlt_solve.argtypes = [C.c_void_p, _f64p, i64]
+        L.orc_ldlt_solve.restype = C.c_int
+        L.orc_ldlt_inertia.argtypes = [C.c_void_p, C.POINTER(i64), C.POINTER(i64), C.POINTER(i64)]
         L.orc_solid_stiffness.argtypes = [C.c_int, _f64p, C.c_double, C.c_double, _f64p]
         L.orc_solid_stiffness.restype = C.c_int
         L.orc_tet10_mass.argtypes = [_f64p, C.c_double, _f64p]
@@ -302,3 +308,49 @@ class Factor:
         if lib().orc_solve(self._h, xt, nrhs) != OK:
             raise OracleError(INTERNAL)
         return xt.T.reshape(x.shape).copy()
+
+
+class LdltFactor:
+    """LdltFactor::new (ldlt.rs:362-391): P A P^T = L D L^T with bounded Bunch-Kaufman pivoting
+    inside supernodes, on the Cholesky symbolic analysis; solve (cholesky.rs:904-1037 with
+    pivots), inertia and two_by_two (ldlt.rs:393-445). Raises OracleError(SINGULAR, column)."""
+
+    def __init__(self, n, ptr, idx, val, max_entries=2**62):
+        L = lib()
+        self.n = n
+        self._h = C.c_void_p()
+        col, ent = C.c_int64(-1), C.c_int64(-1)
+        val = _a(val, np.float64)
+        st = L.orc_ldlt_new(n, _a(ptr), _a(idx), val if val.size else np.zeros(1), int(max_entries),
+                            C.byref(self._h), C.byref(col), C.byref(ent))
+        if st != OK:
+            self._h = None
+            raise OracleError(st, col.value, ent.value)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().orc_free(self._h)
+            self._h = None
+
+    def solve(self, b):
+        x = np.array(b, dtype=np.float64, copy=True)
+        if self.n:
+            lib().orc_ldlt_solve(self._h, x.reshape(-1, order="F") if x.ndim > 1 else x, 1 if x.ndim == 1 else x.shape[1])
+        return x
+
+    def inertia(self):
+        p, q, t = C.c_int64(), C.c_int64(), C.c_int64()
+        lib().orc_ldlt_inertia(self._h, C.byref(p), C.byref(q), C.byref(t))
+        return int(p.value), int(q.value)
+
+    def two_by_two(self) -> int:
+        p, q, t = C.c_int64(), C.c_int64(), C.c_int64()
+        lib().orc_ldlt_inertia(self._h, C.byref(p), C.byref(q), C.byref(t))
+        return int(t.value)
+
+    def values(self) -> np.ndarray:
+        """Every supernode's trapezoid (L below the pivots, D on them), in supernode order."""
+        L = lib()
+        out = np.zeros(max(L.orc_entries(self._h), 1))
+        L.orc_export_values(self._h, out)
+        return out[:L.orc_entries(self._h)]
