@@ -140,7 +140,8 @@ EXPORTS = ["fc_analyze", "fc_symbolic_info_get", "fc_symbolic_export", "fc_symbo
            "fc_solve_many", "fc_solve_device", "fc_factor_export", "fc_factor_seconds", "fc_factor_profile",
            "fc_factor_free", "fc_matrix_new", "fc_matrix_free", "fc_matrix_set_segments", "fc_solve_refined",
            "fc_solve_refined_device", "fc_solve_refined_many_device", "fc_assembly_new", "fc_assemble_device",
-           "fc_assembly_info", "fc_assembly_free", "fc_lanczos_device", "fc_version"]
+           "fc_assembly_info", "fc_assembly_free", "fc_lanczos_device", "fc_ldlt_numeric", "fc_ldlt_solve",
+           "fc_ldlt_inertia", "fc_ldlt_export", "fc_ldlt_free", "fc_version"]
 
 #: kernel kinds of the numeric phase (FC_KIND_* in include/fc_chol.h)
 KINDS = ["extend_l", "extend_u", "update", "potrf", "trsm", "syrk", "reduce", "split"]
@@ -184,6 +185,12 @@ def lib():
                                                    C.POINTER(_RefineReport), _f64p, ep]
         L.fc_lanczos_device.argtypes = [vp, vp, C.c_double, C.POINTER(_LanczosOpts), _f64p, vp,
                                          C.POINTER(_LanczosReport), ep]
+        L.fc_ldlt_numeric.argtypes = [vp, _f64p, C.c_uint64, C.c_int32, C.POINTER(vp), ep]
+        L.fc_ldlt_solve.argtypes = [vp, _f64p, C.c_uint64, ep]
+        L.fc_ldlt_inertia.argtypes = [vp] + [C.POINTER(C.c_uint64)] * 3
+        L.fc_ldlt_export.argtypes = [vp, _f64p, C.c_uint64, ep]
+        L.fc_ldlt_free.argtypes = [vp]
+        L.fc_ldlt_free.restype = None
         L.fc_version.restype = C.c_char_p
         _lib = L
     return _lib
@@ -489,3 +496,62 @@ def _report(rep) -> dict:
 
 def version() -> str:
     return lib().fc_version().decode()
+
+
+class LdltFactor:
+    """LdltFactor (crates/solver/src/ldlt.rs:362-445) on the GPU: P A P^T = L D L^T with bounded
+    Bunch-Kaufman pivoting inside supernodes, on a Cholesky analysis (``Symbolic``): ``solve``,
+    ``inertia``, ``two_by_two``, ``values`` (the reference's trapezoids). The factor and the
+    solutions are the reference's bit for bit. Raises ``Singular`` naming the original column."""
+
+    def __init__(self, symbolic: Symbolic, val, device: int = 0):
+        L = lib()
+        self.symbolic = symbolic
+        self.n = symbolic.n
+        self._L = L
+        self._h = C.c_void_p()
+        err = _Error()
+        val = np.ascontiguousarray(val, np.float64)
+        st = L.fc_ldlt_numeric(symbolic._h, val if val.size else np.zeros(1), val.size, int(device), C.byref(self._h),
+                               C.byref(err))
+        if st != _OK:
+            self._h = None
+            _raise(st, err)
+
+    @staticmethod
+    def new(a: UpperColumns, max_entries: int = 2 ** 63 - 1, **kw) -> "LdltFactor":
+        """LdltFactor::new (ldlt.rs:363-365)."""
+        return LdltFactor(Symbolic(a, max_entries), a.val, **kw)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._L.fc_ldlt_free(self._h)
+            self._h = None
+
+    def solve(self, b) -> np.ndarray:
+        x = np.array(b, dtype=np.float64, copy=True)
+        err = _Error()
+        st = lib().fc_ldlt_solve(self._h, x if x.size else np.zeros(1), x.shape[0], C.byref(err))
+        _raise(st, err)
+        return x
+
+    def _counts(self):
+        v = [C.c_uint64() for _ in range(3)]
+        lib().fc_ldlt_inertia(self._h, *[C.byref(x) for x in v])
+        return [int(x.value) for x in v]
+
+    def inertia(self) -> tuple:
+        """(positive, negative) eigenvalue counts (Sylvester's law on D, ldlt.rs:412-445)."""
+        p, q, _ = self._counts()
+        return p, q
+
+    def two_by_two(self) -> int:
+        return self._counts()[2]
+
+    def values(self) -> np.ndarray:
+        e = self.symbolic.info()["entries"]
+        out = np.zeros(max(e, 1))
+        err = _Error()
+        st = lib().fc_ldlt_export(self._h, out, e, C.byref(err))
+        _raise(st, err)
+        return out[:e]

@@ -1,0 +1,429 @@
+// LDL^T with bounded Bunch-Kaufman (rook) pivoting inside supernodes (crates/solver/src/ldlt.rs)
+// on the device: one workgroup per supernode, the supernodes of one level of the supernodal tree
+// per launch, and the pivoted solves (cholesky.rs:904-1037). Each entry's sums run in the
+// reference's order, one thread per sum, so the factor and the solutions are the reference's bit
+// for bit (each function here turns floating-point contraction off).
+//
+// Storage: the reference's lower trapezoids by columns (supernode s at voff[s], m_s rows,
+// column c holding rows c..m_s: offset(m, c) = c m - c (c - 1) / 2), pivot kinds per column
+// (0 one, 1 first, 2 second of a 2 x 2), each supernode's pivot order (symbolic local column at
+// each pivot position).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace fc {
+
+constexpr double LDL_ALPHA = 0.6403882032022076;  // (1 + sqrt 17) / 8, ldlt.rs:78
+constexpr int LDL_ROOK_STEPS = 64;                // ldlt.rs:82
+constexpr int LDL_PANEL = 64;                     // cholesky.rs:54
+constexpr double LDL_PIVOT_TOLERANCE = 1e-10;     // cholesky.rs:48
+enum : int8_t { PIV_ONE = 0, PIV_FIRST = 1, PIV_SECOND = 2 };
+
+struct LdlSupernode {
+  int64_t voff;      // trapezoid
+  int64_t rows;      // row structure (postorder columns) in the rows array
+  int64_t first;     // first column (postorder)
+  int64_t upd0, upd1;  // updates it receives
+  int64_t scratch;   // two work columns of m doubles
+  int32_t m, w;
+};
+
+struct LdlUpdate {
+  int64_t k;     // descendant supernode
+  int64_t rel;   // relative map (mu entries: rows p1.. of k -> rows of the target) in the rel array
+  int32_t p1, p2;
+};
+
+__device__ __forceinline__ int64_t ldl_offset(int64_t m, int64_t c) { return c * m - c * (c > 0 ? c - 1 : 0) / 2; }
+#define LDL_AT(t, m, r, c) ((t)[ldl_offset((m), (c)) + (r) - (c)])
+
+// Entry k of times_d(row) (cholesky.rs:460-484) for the row `row(j)` of a trapezoid t (m rows)
+// over its columns k0..: a 1 x 1 pivot scales, a 2 x 2 pivot mixes the pair with D's block.
+template <typename Row>
+__device__ __forceinline__ double times_d_at(const Row& row, const double* t, int64_t m, int64_t k0,
+                                             const int8_t* piv, int k, int len) {
+#pragma clang fp contract(off)
+  if (piv[k] == PIV_FIRST) {
+    const double a = row(k), b = k + 1 < len ? row(k + 1) : 0.0;
+    return a * LDL_AT(t, m, k0 + k, k0 + k) + b * LDL_AT(t, m, k0 + k + 1, k0 + k);
+  }
+  if (piv[k] == PIV_SECOND && k > 0 && piv[k - 1] == PIV_FIRST) {
+    const double a = row(k - 1), b = row(k);
+    return a * LDL_AT(t, m, k0 + k, k0 + k - 1) + b * LDL_AT(t, m, k0 + k, k0 + k);
+  }
+  return row(k) * LDL_AT(t, m, k0 + k, k0 + k);
+}
+
+// own(rel(i), rel(c)) -= sum_{k < kw} L(p + i, k0 + k) Y(p + c, k), Y = L D (products and
+// subtract, cholesky.rs:531-686 with pivots), for c < min(wu, mu), c <= i < mu: one thread per
+// (i, c), its sum in k order from 0.0.
+__device__ inline void ldl_update(double* own, int64_t m, const double* lk, int64_t mk, int64_t k0, int kw,
+                                  const int8_t* pk, int64_t p, int64_t mu, int64_t wu, const int32_t* rel, bool rel_id,
+                                  int64_t rel0) {
+#pragma clang fp contract(off)
+  const int64_t cols = wu < mu ? wu : mu;
+  // pairs (c, i >= c) enumerated column by column
+  const int64_t pairs = cols * mu - cols * (cols - 1) / 2;
+  for (int64_t q = threadIdx.x; q < pairs; q += blockDim.x) {
+    // column c with offset(c) <= q < offset(c + 1), offset(c) = c mu - c (c - 1) / 2
+    int64_t lo = 0, hi = cols - 1;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi + 1) / 2;
+      if (mid * mu - mid * (mid - 1) / 2 <= q) lo = mid;
+      else hi = mid - 1;
+    }
+    const int64_t c = lo, i = c + (q - (c * mu - c * (c - 1) / 2));
+    auto row = [&](int k) { return LDL_AT(lk, mk, p + c, k0 + k); };
+    double s = 0.0;
+    for (int k = 0; k < kw; ++k) s += LDL_AT(lk, mk, p + i, k0 + k) * times_d_at(row, lk, mk, k0, pk, k, kw);
+    const int64_t ri = rel_id ? rel0 + i : rel[i], rc = rel_id ? rel0 + c : rel[c];
+    LDL_AT(own, m, ri, rc) -= s;
+  }
+}
+
+// A's values into the trapezoids (the columns of A, cholesky.rs:1054-1068): every upper entry
+// has one slot; the trapezoids were zeroed.
+__global__ void k_ldlt_scatter(int64_t nnz, const double* __restrict__ aval, const int64_t* __restrict__ pos,
+                               double* __restrict__ val) {
+#pragma clang fp contract(off)
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < nnz) val[pos[e]] = aval[e];
+}
+
+__global__ void k_ldlt_zero(int64_t n, double* __restrict__ v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    v[i] = 0.0;
+}
+
+// One supernode per workgroup (supernode list sn[blockIdx.x]): the descendants' updates in
+// order, then the panels (ldlt.rs:256-358). fail[s] receives the failing pivot's postorder
+// column (or stays -1); two[s] the supernode's 2 x 2 pivots.
+__global__ __launch_bounds__(256) void k_ldlt_supernode(const int64_t* __restrict__ sn, const LdlSupernode* __restrict__ S,
+                                                        const LdlUpdate* __restrict__ U, const int32_t* __restrict__ rel,
+                                                        const double* __restrict__ scale, double* __restrict__ val,
+                                                        int8_t* __restrict__ piv, int32_t* __restrict__ order,
+                                                        double* __restrict__ work, int64_t* __restrict__ fail,
+                                                        int32_t* __restrict__ two) {
+#pragma clang fp contract(off)
+  const int64_t j = sn[blockIdx.x];
+  const LdlSupernode J = S[j];
+  const int64_t m = J.m, w = J.w;
+  double* own = val + J.voff;
+  int8_t* pv = piv + J.first;
+  int32_t* od = order + J.first;
+  for (int64_t u = J.upd0; u < J.upd1; ++u) {
+    const LdlUpdate up = U[u];
+    const LdlSupernode K = S[up.k];
+    ldl_update(own, m, val + K.voff, K.m, 0, K.w, piv + K.first, up.p1, K.m - up.p1, up.p2 - up.p1, rel + up.rel, false,
+               0);
+    __syncthreads();
+  }
+  for (int64_t c = threadIdx.x; c < w; c += blockDim.x) {
+    pv[c] = PIV_ONE;
+    od[c] = (int32_t)c;
+  }
+  __syncthreads();
+  double* A = work + J.scratch;      // two work columns (swapped by pointer, as the reference's)
+  double* B = A + m;
+  const double* sc = scale + J.first;
+  __shared__ double red_v[256];
+  __shared__ int64_t red_i[256];
+  __shared__ int64_t sh[6];          // decisions broadcast by thread 0
+  __shared__ double shd[2];
+  int twos = 0;
+  // current(c) into out: column c rows k..m less the panel's factored columns c0..k (ldlt.rs:176-193)
+  auto current = [&](int64_t c0, int64_t k, int64_t c, double* out) {
+    auto row = [&](int q) { return LDL_AT(own, m, c, c0 + q); };
+    for (int64_t r = k + threadIdx.x; r < m; r += blockDim.x) {
+      double v = r >= c ? LDL_AT(own, m, r, c) : LDL_AT(own, m, c, r);
+      for (int64_t q = c0; q < k; ++q) v -= LDL_AT(own, m, r, q) * times_d_at(row, own, m, c0, pv + c0, (int)(q - c0), (int)(k - c0));
+      out[r] = v;
+    }
+    __syncthreads();
+  };
+  // largest |v[i]| over [lo, hi) except skip (ldlt.rs:149-164): the first index of the largest,
+  // or the first NaN's index with NaN
+  auto largest = [&](const double* v, int64_t lo, int64_t hi, int64_t skip, int64_t* at, double* value) {
+    double bv = 0.0;
+    int64_t bi = -1, nan_i = -1;
+    for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+      if (i == skip) continue;
+      const double x = fabs(v[i]);
+      if (isnan(x)) {
+        if (nan_i < 0) nan_i = i;
+      } else if (x > bv) {
+        bv = x;
+        bi = i;
+      }
+    }
+    red_v[threadIdx.x] = bv;
+    red_i[threadIdx.x] = bi;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int64_t best_i = -1;
+      double best = 0.0;
+      for (int t = 0; t < (int)blockDim.x; ++t) {
+        if (red_v[t] > best || (red_v[t] == best && red_v[t] > 0.0 && red_i[t] < best_i)) {
+          best = red_v[t];
+          best_i = red_i[t];
+        }
+      }
+      sh[4] = best_i < 0 ? lo : best_i;
+      shd[0] = best;
+    }
+    __syncthreads();
+    // NaN: the reference stops at the first NaN in index order
+    red_i[threadIdx.x] = nan_i;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int64_t first_nan = -1;
+      for (int t = 0; t < (int)blockDim.x; ++t)
+        if (red_i[t] >= 0 && (first_nan < 0 || red_i[t] < first_nan)) first_nan = red_i[t];
+      if (first_nan >= 0) {  // the reference returns at the first NaN it meets, whatever came before
+        sh[4] = first_nan;
+        shd[0] = __builtin_nan("");
+      }
+    }
+    __syncthreads();
+    *at = sh[4];
+    *value = shd[0];
+    __syncthreads();
+  };
+  auto bswap = [&](int64_t p, int64_t q) {
+    if (p == q) return;
+    if (p > q) {
+      const int64_t t = p;
+      p = q;
+      q = t;
+    }
+    // interchange (ldlt.rs:131-145): rows p, q of the columns before p, the diagonals, the strip
+    // between them, the rows below q
+    const int64_t n1 = p, n2 = q - p - 1, n3 = m - q - 1;
+    for (int64_t e = threadIdx.x; e < n1 + 1 + n2 + n3; e += blockDim.x) {
+      double *x, *y;
+      if (e < n1) {
+        x = &LDL_AT(own, m, p, e);
+        y = &LDL_AT(own, m, q, e);
+      } else if (e == n1) {
+        x = &LDL_AT(own, m, p, p);
+        y = &LDL_AT(own, m, q, q);
+      } else if (e < n1 + 1 + n2) {
+        const int64_t c = p + 1 + (e - n1 - 1);
+        x = &LDL_AT(own, m, c, p);
+        y = &LDL_AT(own, m, q, c);
+      } else {
+        const int64_t r = q + 1 + (e - n1 - 1 - n2);
+        x = &LDL_AT(own, m, r, p);
+        y = &LDL_AT(own, m, r, q);
+      }
+      const double t = *x;
+      *x = *y;
+      *y = t;
+    }
+    if (threadIdx.x == 0) {
+      const int32_t t = od[p];
+      od[p] = od[q];
+      od[q] = t;
+      double x = A[p];
+      A[p] = A[q];
+      A[q] = x;
+      x = B[p];
+      B[p] = B[q];
+      B[q] = x;
+    }
+    __syncthreads();
+  };
+  int64_t c0 = 0;
+  int64_t failed = -1;
+  while (c0 < w && failed < 0) {
+    const int64_t end = c0 + LDL_PANEL < w ? c0 + LDL_PANEL : w;
+    int64_t k = c0;
+    while (k < end) {
+      current(c0, k, k, A);
+      const double akk = fabs(A[k]);
+      int64_t r0;
+      double omega_k;
+      largest(A, k + 1, w, -1, &r0, &omega_k);
+      if (!isfinite(akk) || !isfinite(omega_k)) {
+        failed = k;
+        break;
+      }
+      bool is_two = false;
+      int64_t one_at = k, ti = k, tr = k;
+      if (!(akk >= LDL_ALPHA * omega_k)) {
+        int64_t i = k, r = r0;
+        double omega_i = omega_k;
+        int step = 0;
+        for (;;) {
+          current(c0, k, r, B);
+          int64_t s_at;
+          double omega_r;
+          largest(B, k, w, r, &s_at, &omega_r);
+          const double arr = fabs(B[r]);
+          if (!isfinite(arr) || !isfinite(omega_r)) {
+            failed = k;
+            break;
+          }
+          step += 1;
+          if (arr >= LDL_ALPHA * omega_r) {
+            double* t = A;
+            A = B;
+            B = t;
+            one_at = r;
+            break;
+          }
+          if (omega_r <= omega_i || step >= LDL_ROOK_STEPS) {
+            is_two = true;
+            ti = i;
+            tr = r;
+            break;
+          }
+          double* t = A;
+          A = B;
+          B = t;
+          i = r;
+          omega_i = omega_r;
+          r = s_at;
+        }
+        if (failed >= 0) break;
+      }
+      if (!is_two) {
+        bswap(k, one_at);
+        if (!(fabs(A[k]) > LDL_PIVOT_TOLERANCE * sc[od[k]])) {
+          failed = k;
+          break;
+        }
+        // one (ldlt.rs:207-221)
+        const double d = A[k];
+        const int64_t start = ldl_offset(m, k);
+        for (int64_t r = k + threadIdx.x; r < m; r += blockDim.x) own[start + r - k] = r == k ? d : A[r] / d;
+        if (threadIdx.x == 0) pv[k] = PIV_ONE;
+        __syncthreads();
+        k += 1;
+      } else {
+        bswap(k, ti);
+        const int64_t r = tr == k ? ti : tr;
+        bswap(k + 1, r);
+        const double off = fabs(A[k + 1]);
+        const double s0 = sc[od[k]], s1 = sc[od[k + 1]];
+        if (!(off > LDL_PIVOT_TOLERANCE * (s0 > s1 ? s0 : s1))) {
+          failed = k;
+          break;
+        }
+        // two (ldlt.rs:224-253)
+        const double d11 = A[k], d21 = A[k + 1], d22 = B[k + 1];
+        const double det = d11 * d22 - d21 * d21;
+        const int64_t q0 = ldl_offset(m, k), q1 = ldl_offset(m, k + 1);
+        for (int64_t rr = k + 2 + threadIdx.x; rr < m; rr += blockDim.x) {
+          const double x = A[rr], y = B[rr];
+          own[q0 + rr - k] = (x * d22 - y * d21) / det;
+          own[q1 + rr - k - 1] = (y * d11 - x * d21) / det;
+        }
+        if (threadIdx.x == 0) {
+          own[q0] = d11;
+          own[q0 + 1] = d21;
+          own[q1] = d22;
+          pv[k] = PIV_FIRST;
+          pv[k + 1] = PIV_SECOND;
+        }
+        twos += 1;
+        __syncthreads();
+        k += 2;
+      }
+    }
+    if (failed >= 0) break;
+    const int64_t c1 = k;
+    if (c1 < w)
+      ldl_update(own, m, own, m, c0, (int)(c1 - c0), pv + c0, c1, m - c1, w - c1, nullptr, true, c1);
+    __syncthreads();
+    c0 = c1;
+  }
+  if (threadIdx.x == 0) {
+    fail[j] = failed >= 0 ? J.first + od[failed] : -1;
+    two[j] = twos;
+  }
+}
+
+// The solve (solve_group, cholesky.rs:904-1037) with LDL^T's unit L, block diagonal D and pivot
+// orders, for one right-hand side x (postorder, in place), by one workgroup walking the
+// supernodes in order (forward) and in reverse (backward), so every entry of x receives its
+// operations in the reference's order. y: work of the widest supernode's width.
+__global__ __launch_bounds__(256) void k_ldlt_solve(int64_t ns, const LdlSupernode* __restrict__ S,
+                                                    const int64_t* __restrict__ rows_all, const double* __restrict__ val,
+                                                    const int8_t* __restrict__ piv, const int32_t* __restrict__ order,
+                                                    double* __restrict__ x, double* __restrict__ y) {
+#pragma clang fp contract(off)
+  for (int64_t k = 0; k < ns; ++k) {
+    const LdlSupernode K = S[k];
+    const int64_t m = K.m, w = K.w;
+    const double* t = val + K.voff;
+    const int8_t* pv = piv + K.first;
+    const int32_t* od = order + K.first;
+    const int64_t* rows = rows_all + K.rows;
+    double* own = x + K.first;
+    for (int64_t c = threadIdx.x; c < w; c += blockDim.x) y[c] = own[od[c]];
+    __syncthreads();
+    for (int64_t q = 0; q < w; ++q) {
+      const int64_t end = q + (pv[q] == PIV_FIRST ? 2 : 1);
+      const double yq = y[q];
+      for (int64_t r = end + threadIdx.x; r < m; r += blockDim.x) {
+        const double l = LDL_AT(t, m, r, q);
+        if (r < w) y[r] -= l * yq;
+        else x[rows[r]] -= l * yq;
+      }
+      __syncthreads();
+    }
+    for (int64_t c = threadIdx.x; c < w; c += blockDim.x) {
+      if (pv[c] == PIV_FIRST) {
+        const double d11 = LDL_AT(t, m, c, c), d21 = LDL_AT(t, m, c + 1, c), d22 = LDL_AT(t, m, c + 1, c + 1);
+        const double det = d11 * d22 - d21 * d21;
+        const double p = y[c], q2 = y[c + 1];
+        y[c] = (d22 * p - d21 * q2) / det;
+        y[c + 1] = (d11 * q2 - d21 * p) / det;
+      } else if (pv[c] == PIV_ONE) {
+        y[c] /= LDL_AT(t, m, c, c);
+      }
+    }
+    __syncthreads();
+    for (int64_t c = threadIdx.x; c < w; c += blockDim.x) own[c] = y[c];
+    __syncthreads();
+  }
+  for (int64_t k = ns - 1; k >= 0; --k) {
+    const LdlSupernode K = S[k];
+    const int64_t m = K.m, w = K.w;
+    const double* t = val + K.voff;
+    const int8_t* pv = piv + K.first;
+    const int32_t* od = order + K.first;
+    const int64_t* rows = rows_all + K.rows;
+    double* own = x + K.first;
+    for (int64_t c = threadIdx.x; c < w; c += blockDim.x) y[c] = own[c];
+    __syncthreads();
+    for (int64_t c = w - 1; c >= 0; --c) {
+      int64_t q, end;
+      if (pv[c] == PIV_SECOND && c > 0) {
+        q = c - 1;
+        end = c + 1;
+      } else if (pv[c] == PIV_FIRST) {
+        q = c + 1;
+        end = c + 2;
+      } else {
+        q = c;
+        end = c + 1;
+      }
+      if (threadIdx.x == 0) {
+        double sum = y[q];
+        for (int64_t r = end; r < w; ++r) sum -= LDL_AT(t, m, r, q) * y[r];
+        for (int64_t r = w; r < m; ++r) sum -= LDL_AT(t, m, r, q) * x[rows[r]];
+        y[q] = sum;
+      }
+      __syncthreads();
+    }
+    for (int64_t c = threadIdx.x; c < w; c += blockDim.x) own[od[c]] = y[c];
+    __syncthreads();
+  }
+}
+
+}  // namespace fc

@@ -10,6 +10,7 @@
 #include <memory>
 #include <mutex>
 #include <new>
+#include <stdexcept>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -18,6 +19,7 @@
 #include "fc_chol.h"
 #include "gemm_kernels.hpp"
 #include "lanczos_kernels.hpp"
+#include "ldlt_kernels.hpp"
 #include "solve_kernels.hpp"
 #include "symbolic.hpp"
 
@@ -706,6 +708,28 @@ struct fc_matrix {
   }
 };
 
+struct fc_ldlt {
+  int device = 0;
+  std::shared_ptr<fc_symbolic> sym_keep;
+  const fc_symbolic* sym = nullptr;
+  hipStream_t stream = nullptr;
+  int64_t n = 0, ns = 0, vals = 0, max_w = 0, positive = 0, negative = 0, two_by_two = 0;
+  fc::LdlSupernode* S = nullptr;
+  int64_t* rows = nullptr;
+  double* val = nullptr;
+  int8_t* piv = nullptr;
+  int32_t* order = nullptr;
+  ~fc_ldlt() {
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(device);
+    for (void* p : {(void*)S, (void*)rows, (void*)val, (void*)piv, (void*)order})
+      if (p) (void)hipFree(p);
+    if (stream) (void)hipStreamDestroy(stream);
+    (void)hipSetDevice(cur);
+  }
+};
+
 namespace {
 
 // Registry so a factor can keep its analysis alive after fc_symbolic_free.
@@ -1368,6 +1392,94 @@ bool dense_symmetric_eigen(int n, std::vector<double>& a, std::vector<double>& v
   return true;
 }
 
+// ---------------------------------------------------------------------------------------------
+// LDL^T (crates/solver/src/ldlt.rs) on the device: the plan from the analysis, built on the host.
+
+struct LdltPlan {
+  std::vector<fc::LdlSupernode> sn;
+  std::vector<fc::LdlUpdate> up;
+  std::vector<int32_t> rel;
+  std::vector<int64_t> pos;         // upper entry -> slot in the trapezoids
+  std::vector<int64_t> coord_r, coord_c;  // upper entry -> postorder (row, column)
+  std::vector<int64_t> lvl_ptr, lvl_sn;   // supernodes by level of the supernodal tree
+  int64_t vals = 0, work = 0, max_w = 0;
+};
+
+LdltPlan ldlt_plan(const fc::Symbolic& s) {
+  LdltPlan P;
+  const i64 ns = s.ns(), n = s.n;
+  std::vector<i64> snode(n);
+  for (i64 k = 0; k < ns; ++k)
+    for (i64 c = s.first[k]; c < s.first[k + 1]; ++c) snode[c] = k;
+  P.sn.resize(ns);
+  for (i64 k = 0; k < ns; ++k) {
+    fc::LdlSupernode& J = P.sn[k];
+    J.m = (int32_t)(s.rptr[k + 1] - s.rptr[k]);
+    J.w = (int32_t)(s.first[k + 1] - s.first[k]);
+    J.voff = P.vals;
+    J.rows = s.rptr[k];
+    J.first = s.first[k];
+    J.scratch = P.work;
+    P.vals += (i64)J.w * J.m - (i64)J.w * (J.w - 1) / 2;
+    P.work += 2 * (i64)J.m;
+    P.max_w = std::max<i64>(P.max_w, J.w);
+    J.upd0 = (i64)P.up.size();
+    for (i64 q = s.uptr[k]; q < s.uptr[k + 1]; ++q) {
+      const fc::Update& u = s.upd[q];
+      fc::LdlUpdate U;
+      U.k = u.k;
+      U.p1 = (int32_t)u.p1;
+      U.p2 = (int32_t)u.p2;
+      U.rel = (i64)P.rel.size();
+      const i64 km = s.rptr[u.k + 1] - s.rptr[u.k];
+      i64 at = 0;
+      for (i64 i = u.p1; i < km; ++i) {
+        const i64 r = s.rows[s.rptr[u.k] + i];
+        while (at < J.m && s.rows[s.rptr[k] + at] < r) ++at;
+        if (at >= J.m || s.rows[s.rptr[k] + at] != r) throw std::runtime_error("an update row outside the structure");
+        // the update's columns must be columns of the target (subtract, cholesky.rs:668-671)
+        if (i < u.p2 && at >= J.w) throw std::runtime_error("an update column outside the supernode");
+        P.rel.push_back((int32_t)at);
+      }
+      P.up.push_back(U);
+    }
+    J.upd1 = (i64)P.up.size();
+  }
+  std::vector<i64> inv(n);
+  for (i64 i = 0; i < n; ++i) inv[s.perm[i]] = i;
+  const i64 nnz = (i64)s.upper_idx.size();
+  P.pos.resize(nnz);
+  P.coord_r.resize(nnz);
+  P.coord_c.resize(nnz);
+  for (i64 k = 0; k < n; ++k)
+    for (i64 e = s.upper_ptr[k]; e < s.upper_ptr[k + 1]; ++e) {
+      const i64 a = inv[s.upper_idx[e]], b = inv[k];
+      const i64 r = std::max(a, b), c = std::min(a, b);
+      const i64 sj = snode[c];
+      const fc::LdlSupernode& J = P.sn[sj];
+      const i64* rows = s.rows.data() + s.rptr[sj];
+      const i64 lr = std::lower_bound(rows, rows + J.m, r) - rows, lc = c - J.first;
+      if (lr >= J.m || rows[lr] != r || lr < lc) throw std::runtime_error("an entry of A outside the structure");
+      P.pos[e] = J.voff + lc * J.m - lc * (lc - 1 > 0 ? lc - 1 : 0) / 2 + lr - lc;
+      P.coord_r[e] = r;
+      P.coord_c[e] = c;
+    }
+  std::vector<i64> level(ns, 0);
+  i64 levels = 0;
+  for (i64 k = 0; k < ns; ++k) {  // children come before parents in postorder
+    const i64 pa = s.sparent[k];
+    if (pa >= 0 && pa < ns) level[pa] = std::max(level[pa], level[k] + 1);
+    levels = std::max(levels, level[k] + 1);
+  }
+  P.lvl_ptr.assign(levels + 1, 0);
+  for (i64 k = 0; k < ns; ++k) ++P.lvl_ptr[level[k] + 1];
+  for (i64 l = 0; l < levels; ++l) P.lvl_ptr[l + 1] += P.lvl_ptr[l];
+  P.lvl_sn.resize(ns);
+  std::vector<i64> cur(P.lvl_ptr.begin(), P.lvl_ptr.end() - 1);
+  for (i64 k = 0; k < ns; ++k) P.lvl_sn[cur[level[k]]++] = k;
+  return P;
+}
+
 struct LanczosError {
   std::string msg;
 };
@@ -1649,6 +1761,192 @@ LanczosResult lanczos_run(const fc_factor* F, const fc_matrix* M, double shift, 
 extern "C" {
 
 const char* fc_version(void) { return VERSION; }
+
+int fc_ldlt_numeric(const fc_symbolic* h, const double* upper_val, uint64_t nnz, int32_t device, fc_ldlt** out,
+                    fc_error* err) {
+  if (out) *out = nullptr;
+  if (!h || !out || (nnz && !upper_val)) {
+    set_error(err, FC_INVALID_ARGUMENT, "bad arguments to fc_ldlt_numeric");
+    return FC_INVALID_ARGUMENT;
+  }
+  const fc::Symbolic& s = h->s;
+  if ((i64)nnz != (i64)s.upper_idx.size()) {
+    set_error(err, FC_INVALID_ARGUMENT, "the matrix's pattern differs from the analysed one (entry count)");
+    return FC_INVALID_ARGUMENT;
+  }
+  std::unique_ptr<fc_ldlt> F(new fc_ldlt());
+  F->device = device;
+  F->sym_keep = keep(h);
+  F->sym = h;
+  F->n = s.n;
+  F->ns = s.ns();
+  int cur = 0;
+  (void)hipGetDevice(&cur);
+  int64_t failed = -1;
+  std::string plan_error;
+  int st = guarded(err, "not enough device memory for the LDL^T factor", [&] {
+    HIP_CHECK(hipSetDevice(device));
+    HIP_CHECK(hipStreamCreateWithFlags(&F->stream, hipStreamNonBlocking));
+    hipStream_t stq = F->stream;
+    LdltPlan P;
+    try {
+      P = ldlt_plan(s);
+    } catch (const std::runtime_error& e) {
+      plan_error = e.what();
+      return;
+    }
+    F->vals = P.vals;
+    F->max_w = P.max_w;
+    // the largest magnitude of each row (= column) of A, in postorder (cholesky.rs:1399-1414; a
+    // NaN leaves it as it is, as f64::max does)
+    std::vector<double> scale(std::max<i64>(s.n, 1), 0.0);
+    for (i64 e = 0; e < (i64)nnz; ++e) {
+      const double x = std::fabs(upper_val[e]);
+      if (x > scale[P.coord_r[e]]) scale[P.coord_r[e]] = x;
+      if (x > scale[P.coord_c[e]]) scale[P.coord_c[e]] = x;
+    }
+    F->S = dupload(P.sn, stq);
+    F->rows = dupload(s.rows, stq);
+    F->val = dalloc<double>((size_t)std::max<i64>(P.vals, 1));
+    F->piv = dalloc<int8_t>((size_t)std::max<i64>(s.n, 1));
+    F->order = dalloc<int32_t>((size_t)std::max<i64>(s.n, 1));
+    fc::LdlUpdate* dU = dupload(P.up, stq);
+    int32_t* drel = dupload(P.rel, stq);
+    int64_t* dpos = dupload(P.pos, stq);
+    int64_t* dsn = dupload(P.lvl_sn, stq);
+    double* dscale = dupload(scale, stq);
+    double* daval = dalloc<double>((size_t)std::max<uint64_t>(nnz, 1));
+    double* dwork = dalloc<double>((size_t)std::max<i64>(P.work, 1));
+    int64_t* dfail = dalloc<int64_t>((size_t)std::max<i64>(F->ns, 1));
+    int32_t* dtwo = dalloc<int32_t>((size_t)std::max<i64>(F->ns, 1));
+    struct Free {
+      std::vector<void*> p;
+      ~Free() {
+        for (void* q : p) dfree(q);
+      }
+    } tmp{{dU, drel, dpos, dsn, dscale, daval, dwork, dfail, dtwo}};
+    if (nnz) HIP_CHECK(hipMemcpyAsync(daval, upper_val, sizeof(double) * nnz, hipMemcpyHostToDevice, stq));
+    if (P.vals) fc::k_ldlt_zero<<<blocks_for(P.vals), 256, 0, stq>>>(P.vals, F->val);
+    if (nnz) fc::k_ldlt_scatter<<<(unsigned)((nnz + 255) / 256), 256, 0, stq>>>((i64)nnz, daval, dpos, F->val);
+    for (size_t l = 0; l + 1 < P.lvl_ptr.size(); ++l) {
+      const i64 cnt = P.lvl_ptr[l + 1] - P.lvl_ptr[l];
+      if (cnt > 0)
+        fc::k_ldlt_supernode<<<(unsigned)cnt, 256, 0, stq>>>(dsn + P.lvl_ptr[l], F->S, dU, drel, dscale, F->val, F->piv,
+                                                             F->order, dwork, dfail, dtwo);
+    }
+    HIP_CHECK(hipGetLastError());
+    std::vector<int64_t> fails(std::max<i64>(F->ns, 1));
+    std::vector<int32_t> twos(std::max<i64>(F->ns, 1));
+    if (F->ns) {
+      HIP_CHECK(hipMemcpyAsync(fails.data(), dfail, sizeof(int64_t) * F->ns, hipMemcpyDeviceToHost, stq));
+      HIP_CHECK(hipMemcpyAsync(twos.data(), dtwo, sizeof(int32_t) * F->ns, hipMemcpyDeviceToHost, stq));
+    }
+    HIP_CHECK(hipStreamSynchronize(stq));
+    for (i64 k = 0; k < F->ns; ++k) {
+      if (fails[k] >= 0 && (failed < 0 || fails[k] < failed)) failed = fails[k];
+      F->two_by_two += twos[k];
+    }
+    if (failed < 0) {
+      // inertia from D (ldlt.rs:412-445)
+      std::vector<double> v((size_t)std::max<i64>(P.vals, 1));
+      std::vector<int8_t> pv((size_t)std::max<i64>(s.n, 1));
+      if (P.vals) HIP_CHECK(hipMemcpyAsync(v.data(), F->val, sizeof(double) * P.vals, hipMemcpyDeviceToHost, stq));
+      if (s.n) HIP_CHECK(hipMemcpyAsync(pv.data(), F->piv, s.n, hipMemcpyDeviceToHost, stq));
+      HIP_CHECK(hipStreamSynchronize(stq));
+      for (i64 k = 0; k < F->ns; ++k) {
+        const fc::LdlSupernode& J = P.sn[k];
+        auto d = [&](i64 r, i64 c) { return v[J.voff + c * J.m - c * (c > 0 ? c - 1 : 0) / 2 + r - c]; };
+        i64 c = 0;
+        while (c < J.w) {
+          if (pv[J.first + c] == fc::PIV_FIRST) {
+            const double d11 = d(c, c), d21 = d(c + 1, c), d22 = d(c + 1, c + 1);
+            const double det = d11 * d22 - d21 * d21;
+            if (det < 0.0) {
+              F->positive += 1;
+              F->negative += 1;
+            } else if (d11 + d22 > 0.0) {
+              F->positive += 2;
+            } else {
+              F->negative += 2;
+            }
+            c += 2;
+          } else {
+            if (d(c, c) > 0.0) F->positive += 1;
+            else F->negative += 1;
+            c += 1;
+          }
+        }
+      }
+    }
+  });
+  (void)hipSetDevice(cur);
+  if (st != FC_OK) return st;
+  if (!plan_error.empty()) {
+    set_error(err, FC_INTERNAL, plan_error);
+    return FC_INTERNAL;
+  }
+  if (failed >= 0) {
+    const uint64_t col = (uint64_t)s.perm[failed];
+    set_error(err, FC_SINGULAR, "the matrix is singular (no acceptable pivot in its supernode)", col, s.entries);
+    return FC_SINGULAR;
+  }
+  *out = F.release();
+  set_error(err, FC_OK, "");
+  return FC_OK;
+}
+
+int fc_ldlt_solve(const fc_ldlt* f, double* b, uint64_t n, fc_error* err) {
+  if (!f || (n && !b)) {
+    set_error(err, FC_INVALID_ARGUMENT, "bad arguments to fc_ldlt_solve");
+    return FC_INVALID_ARGUMENT;
+  }
+  if ((i64)n != f->n) {
+    set_error(err, FC_INTERNAL, "right-hand side length");
+    return FC_INTERNAL;
+  }
+  if (n == 0) {
+    set_error(err, FC_OK, "");
+    return FC_OK;
+  }
+  const std::vector<i64>& perm = f->sym->s.perm;
+  return guarded(err, "not enough device memory for the solve", [&] {
+    HIP_CHECK(hipSetDevice(f->device));
+    std::vector<double> x(n);
+    for (uint64_t i = 0; i < n; ++i) x[i] = b[perm[i]];
+    double* dx = dalloc<double>(n);
+    double* dy = dalloc<double>((size_t)std::max<i64>(f->max_w, 1));
+    HIP_CHECK(hipMemcpyAsync(dx, x.data(), sizeof(double) * n, hipMemcpyHostToDevice, f->stream));
+    fc::k_ldlt_solve<<<1, 256, 0, f->stream>>>(f->ns, f->S, f->rows, f->val, f->piv, f->order, dx, dy);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpyAsync(x.data(), dx, sizeof(double) * n, hipMemcpyDeviceToHost, f->stream));
+    HIP_CHECK(hipStreamSynchronize(f->stream));
+    dfree(dx);
+    dfree(dy);
+    for (uint64_t i = 0; i < n; ++i) b[perm[i]] = x[i];
+  });
+}
+
+int fc_ldlt_inertia(const fc_ldlt* f, uint64_t* positive, uint64_t* negative, uint64_t* two_by_two) {
+  if (!f) return FC_INVALID_ARGUMENT;
+  if (positive) *positive = (uint64_t)f->positive;
+  if (negative) *negative = (uint64_t)f->negative;
+  if (two_by_two) *two_by_two = (uint64_t)f->two_by_two;
+  return FC_OK;
+}
+
+int fc_ldlt_export(const fc_ldlt* f, double* out, uint64_t len, fc_error* err) {
+  if (!f || (len && !out) || (i64)len != f->vals) {
+    set_error(err, FC_INVALID_ARGUMENT, "the output needs the factor's entry count");
+    return FC_INVALID_ARGUMENT;
+  }
+  return guarded(err, "device", [&] {
+    HIP_CHECK(hipSetDevice(f->device));
+    if (len) HIP_CHECK(hipMemcpyAsync(out, f->val, sizeof(double) * len, hipMemcpyDeviceToHost, f->stream));
+    HIP_CHECK(hipStreamSynchronize(f->stream));
+  });
+}
+
+void fc_ldlt_free(fc_ldlt* f) { delete f; }
 
 int fc_lanczos_device(const fc_factor* f, const fc_matrix* m, double shift, const fc_lanczos_opts* opts,
                       double* values, double* vectors_device, fc_lanczos_report* report, fc_error* err) {

@@ -278,6 +278,24 @@ int fc_assembly_info(const fc_assembly* a, uint64_t* contributions_k, uint64_t* 
                      double* gather_s);
 void fc_assembly_free(fc_assembly* a);
 
+/* --- LDL^T ------------------------------------------------------------------------------ */
+
+/* LdltFactor (crates/solver/src/ldlt.rs:362-445): P A P^T = L D L^T with bounded Bunch-Kaufman
+ * (rook) pivoting inside supernodes, on the Cholesky analysis `s` (fc_analyze), on the device.
+ * The factor, its pivots and the solutions are the reference's bit for bit (one workgroup per
+ * supernode, each entry's sums in the reference's order). FC_SINGULAR names the first column
+ * in postorder with no acceptable pivot, as an original column (ldlt.rs:300-329). */
+typedef struct fc_ldlt fc_ldlt;
+int fc_ldlt_numeric(const fc_symbolic* s, const double* upper_val, uint64_t nnz, int32_t device, fc_ldlt** out,
+                    fc_error* err);
+/* LdltFactor::solve (ldlt.rs:401-404): b (host, n doubles) in place. */
+int fc_ldlt_solve(const fc_ldlt* f, double* b, uint64_t n, fc_error* err);
+/* LdltFactor::inertia and two_by_two (ldlt.rs:393-445). */
+int fc_ldlt_inertia(const fc_ldlt* f, uint64_t* positive, uint64_t* negative, uint64_t* two_by_two);
+/* The supernodes' trapezoids (L below the pivots, D on them) in the reference's layout. */
+int fc_ldlt_export(const fc_ldlt* f, double* out, uint64_t len, fc_error* err);
+void fc_ldlt_free(fc_ldlt* f);
+
 /* --- shift-invert block Lanczos ----------------------------------------------------------- */
 
 /* What lanczos (crates/solver/src/eigen.rs:281) is given: Wanted (eigen.rs:178-185), the block

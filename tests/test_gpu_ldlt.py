@@ -1,0 +1,96 @@
+"""GPU parity of the LDL^T factor (fc_ldlt_numeric / fc_ldlt_solve through the C ABI) with the
+reference: its frozen fingerprints (ldlt.rs:760-846) bit for bit on the device (the factor's
+values, the solution of an exact right-hand side, the negative-pivot and 2 x 2 counts), which the
+oracle reproduces too (tests/test_ldlt_oracle.py); inertia against the closed-form eigenvalue
+count of the shifted grid Laplacian, the residual bounds, the saddle point's 2 x 2 pivots,
+located singular and NaN failures, the 1 x 1 and empty matrices (ldlt.rs:634-734)."""
+import numpy as np
+import pytest
+
+from fieldcraft_amd import cholesky as CH
+from oracle import oracle as O
+
+from matrices import fingerprint, frozen_rhs, hex_pattern, rows_to_csr, saddle, shifted_grid
+from test_ldlt_oracle import FROZEN
+
+pytestmark = pytest.mark.gpu
+
+
+def device_factor(a):
+    n, ptr, idx, val = a
+    up = O.from_symmetric(n, ptr, idx, val)
+    return CH.LdltFactor(CH.Symbolic(CH.UpperColumns(n, *up)), up[2])
+
+
+def residual(a, f, seed):
+    n, ptr, idx, val = a
+    x = np.sin((np.arange(n) + seed) * 0.37)
+    return np.abs(f.solve(O.csr_mul(n, ptr, idx, val, x)) - x).max()
+
+
+@pytest.mark.parametrize("name,make,negative,two,factor_bits,solution_bits", FROZEN, ids=[c[0] for c in FROZEN])
+def test_frozen_fingerprints_on_the_device(gpu, name, make, negative, two, factor_bits, solution_bits):
+    a = make()
+    n, ptr, idx, val = a
+    print(f"LDL^T {name}, n = {n}", flush=True)
+    f = device_factor(a)
+    x = f.solve(frozen_rhs(n, lambda v: O.csr_mul(n, ptr, idx, val, v)))
+    assert (f.inertia()[1], f.two_by_two()) == (negative, two)
+    assert fingerprint(f.values()) == factor_bits
+    assert fingerprint(x) == solution_bits
+
+
+@pytest.mark.parametrize("sigma", [0.0, 0.9, 3.3, 7.1])
+def test_inertia_and_residual(gpu, sigma):
+    a, below = shifted_grid([7, 6, 5], sigma)
+    f = device_factor(a)
+    assert f.inertia() == (a[0] - below, below)
+    assert residual(a, f, sigma) < 1e-10
+
+
+def test_saddle_point(gpu):
+    a = saddle(8, 3)
+    f = device_factor(a)
+    assert f.inertia() == (8, 3) and f.two_by_two() > 0
+    assert residual(a, f, 0.5) < 1e-12
+
+
+def test_located_failures_and_small_cases(gpu):
+    n = 12
+    rows = []
+    for i in range(n):
+        r, d = [], 0.0
+        for j in (i - 1, i + 1):
+            if 0 <= j < n:
+                r.append((j, -1.0))
+                d += 1.0
+        r.append((i, d))
+        rows.append(r)
+    a = rows_to_csr(rows)
+    with pytest.raises(CH.Singular) as e:
+        device_factor(a)
+    with pytest.raises(O.OracleError) as o:
+        O.LdltFactor(a[0], *O.from_symmetric(*a))
+    assert e.value.column == o.value.column
+    bad = [list(r) for r in rows]
+    bad[5][0] = (bad[5][0][0], float("nan"))
+    bad[4] = [(c, float("nan") if c == 5 else v) for c, v in bad[4]]
+    with pytest.raises(CH.Singular):
+        device_factor(rows_to_csr(bad))
+    f = device_factor(rows_to_csr([[(0, -4.0)]]))
+    assert f.solve(np.array([2.0])).tolist() == [-0.5] and f.inertia() == (0, 1)
+    empty = CH.LdltFactor(CH.Symbolic(CH.UpperColumns(0, np.zeros(1, np.int64), np.zeros(0, np.int64), np.zeros(0))),
+                          np.zeros(0))
+    assert empty.solve(np.zeros(0)).shape == (0,)
+
+
+def test_bigger_grid_matches_the_oracle_bit_for_bit(gpu):
+    """ldlt.rs:736-756's grid (wide supernodes spanning panels, many 2 x 2 pivots)."""
+    a, below = shifted_grid([14, 12, 10], 2.7)
+    n, ptr, idx, val = a
+    f = device_factor(a)
+    o = O.LdltFactor(n, *O.from_symmetric(n, ptr, idx, val))
+    assert f.inertia()[1] == below
+    assert np.array_equal(f.values().view(np.uint64), o.values().view(np.uint64))
+    b = np.cos(np.arange(n) * 0.3)
+    assert np.array_equal(f.solve(b).view(np.uint64), o.solve(b).view(np.uint64))
