@@ -16,7 +16,7 @@ around its launches on the factor's stream during the timed steps; `cpu_baseline
 oracle (the C restatement of the reference's CPU factorization) on a bounded sample.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--cells 80] [--form mixed|fp32|fp64]
-                      [--contrast] [--workload cube|batch|shift|assemble|modes]
+                      [--contrast] [--workload cube|batch|shift|assemble|modes|ldlt]
 
 --contrast adds the harder variant of the same pattern (E in {210 GPa, 21 MPa} per element,
 nu = 0.49) with the same symbolic analysis. --workload batch runs 512 independent hex8/tet10
@@ -26,8 +26,10 @@ models (2k-60k equations) as one block-diagonal factorization in FP32 with FP64 
 --workload assemble times the step before the factor on the device: K_ff of the hex8 cube and
 K - sigma M of the tet10 block assembled into the factor's input, the reference's values bit for
 bit; --workload modes runs the modal solve on the device: K - sigma M of the tet10 block assembled,
-factored in FP64 and its 20 lowest modes above sigma found by the shifted block Lanczos method.
-Those four print their own metric lines.
+factored in FP64 and its 20 lowest modes above sigma found by the shifted block Lanczos method;
+--workload ldlt factors an indefinite K - sigma M (sigma between the tet10 block's 5th and 6th
+modes) by LDL^T with pivoting and checks its inertia against the modes. Those five print their
+own metric lines.
 """
 from __future__ import annotations
 
@@ -163,7 +165,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--contrast", action="store_true")
-    ap.add_argument("--workload", default="cube", choices=["cube", "batch", "shift", "assemble", "modes"])
+    ap.add_argument("--workload", default="cube", choices=["cube", "batch", "shift", "assemble", "modes", "ldlt"])
+    ap.add_argument("--ldlt-cells", type=int, default=12)
     ap.add_argument("--modes", type=int, default=20)
     ap.add_argument("--models", type=int, default=512)
     ap.add_argument("--tet-cells", type=int, default=50)
@@ -193,6 +196,8 @@ def main():
         return assemble_workload(args, P)
     if args.workload == "modes":
         return modes_workload(args, CH, P)
+    if args.workload == "ldlt":
+        return ldlt_workload(args, CH, P)
     t0 = time.perf_counter()
     p = P.cube_hex8(args.cells)
     up = p.upper()
@@ -642,6 +647,82 @@ def modes_workload(args, CH, P):
                 residual_first3=res, roofline=roof,
                 lanczos_roofline=dict(bound="hbm", achieved=lanczos_gbs, peak=HBM_PEAK, unit="GB/s",
                                       frac=lanczos_gbs / HBM_PEAK, bytes=lbytes),
+                cpu_baseline=cpu)
+    print(json.dumps(line), flush=True)
+
+
+def ldlt_workload(args, CH, P):
+    """LDL^T with pivoting (LdltFactor, ldlt.rs) of an indefinite shifted stiffness: the tet10 block's
+    first modes found on the device (assembly, FP64 Cholesky, Lanczos), sigma put between the 5th
+    and 6th, K - sigma M factored by LDL^T on the device (its inertia must show the 5 modes below
+    sigma) and solved. One step = the LDL^T factorization (host plan included) + one solve. The
+    CPU baseline is the oracle's LDL^T of the same matrix."""
+    import torch
+
+    from fieldcraft_amd import assembly as AS
+
+    k = args.ldlt_cells
+    t0 = time.perf_counter()
+    p = P.block_tet10((k, k, k), (k / 100, k / 100, k / 100))
+    uptr, uidx, _ = p.upper()
+    sym = CH.Symbolic(CH.UpperColumns(p.n, uptr, uidx, np.zeros(len(uidx))))
+    info = sym.info()
+    a = AS.Assembly.of_problem(p, uptr, uidx)
+    sigma0 = (2 * np.pi * 50.0) ** 2
+    vals = a.assemble("shifted", sigma0)
+    chol = CH.Factor(sym, device_values=vals.data_ptr())
+    M = CH.Matrix(p.n, p.ptr, p.idx, p.mval)
+    modes = chol.lanczos_device(M, sigma0, 12, block=6, max_basis=min(p.n, 600), vectors=False)["values"]
+    # sigma between two well separated modes from the 5th on: that many eigenvalues lie below it
+    below = next(i + 1 for i in range(4, len(modes) - 1) if modes[i + 1] > 1.05 * modes[i])
+    sigma = 0.5 * (modes[below - 1] + modes[below])
+    del chol
+    up = p.upper(p.kval - sigma * p.mval)
+    log(f"bench: tet10 {k}^3, {p.n} equations, sigma between {np.sqrt(modes[below - 1]) / 2 / np.pi:.1f} and "
+        f"{np.sqrt(modes[below]) / 2 / np.pi:.1f} Hz (set up in {time.perf_counter() - t0:.1f} s)")
+    b = np.cos(np.arange(p.n) * 0.3)
+    runs = []
+    for step in range(args.warmup + args.steps):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        f = CH.LdltFactor(sym, up[2])
+        tf = time.perf_counter()
+        x = f.solve(b)
+        te = time.perf_counter()
+        runs.append(dict(total=te - t, factor=tf - t, solve=te - tf))
+        log(f"bench: step {step}: factor {tf - t:.3f} s, solve {te - tf:.3f} s")
+    timed = runs[args.warmup:]
+    mean = {key: float(np.mean([r[key] for r in timed])) for key in timed[0]}
+    inertia, two = f.inertia(), f.two_by_two()
+    res = float(np.abs(p.mul(x, p.kval - sigma * p.mval) - b).max() / np.abs(b).max())
+    F = info["flops"]
+    achieved = F / mean["factor"] / 1e12
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+
+        O.set_threads(args.cpu_threads)
+        t = time.perf_counter()
+        of = O.LdltFactor(p.n, *up)
+        of.solve(b)
+        cs = time.perf_counter() - t
+        same = bool(np.array_equal(of.values().view(np.uint64), f.values().view(np.uint64)))
+        cpu = dict(value=cs, unit="s", cores=args.cpu_threads, kind="port",
+                   sample=f"oracle LDL^T factor + solve of the same matrix ({p.n} equations) on {args.cpu_threads} "
+                          f"threads: {cs:.2f} s; factor bits equal to the device's: {same}")
+        log(f"bench: cpu baseline {cs:.2f} s, same bits {same}")
+    line = dict(metric=f"LDL^T with pivoting of an indefinite K - sigma M, tet10 {k}^3: factor + solve s",
+                value=mean["total"], unit="s", n_gpus=1, steps=args.steps, warmup=args.warmup,
+                ms_per_step=mean["total"] * 1e3, higher_is_better=False, scaling="weak", vs_baseline=None, dtype="f64",
+                data="synthetic",
+                config=dict(workload=f"tet10 {k}^3 Kuhn block, steel, z=0 clamped, sigma between modes {below} and "
+                                     f"{below + 1}",
+                            equations=p.n, factor_entries=info["entries"], flops=F, sigma=float(sigma)),
+                factor_s=mean["factor"], solve_s=mean["solve"], inertia=list(inertia), negative_expected=below,
+                two_by_two=two, residual=res,
+                roofline=dict(bound="mfma", achieved=achieved, peak=PEAK["fp64"], unit="TFLOP/s",
+                              frac=achieved / PEAK["fp64"], traffic=None, kernel="k_ldlt_supernode",
+                              note="factor time includes the host plan"),
                 cpu_baseline=cpu)
     print(json.dumps(line), flush=True)
 

@@ -58,27 +58,66 @@ __device__ __forceinline__ double times_d_at(const Row& row, const double* t, in
 }
 
 // own(rel(i), rel(c)) -= sum_{k < kw} L(p + i, k0 + k) Y(p + c, k), Y = L D (products and
-// subtract, cholesky.rs:531-686 with pivots), for c < min(wu, mu), c <= i < mu: one thread per
-// (i, c), its sum in k order from 0.0.
+// subtract, cholesky.rs:531-686 with pivots), for c < min(wu, mu), c <= i < mu, restricted to the
+// target columns rel(c) in [cb, ce): one thread per (i, c), its sum in k order from 0.0. Entries
+// of column k0 + k lie mk - k0 - k - 1 apart from those of the column before (offset()).
 __device__ inline void ldl_update(double* own, int64_t m, const double* lk, int64_t mk, int64_t k0, int kw,
                                   const int8_t* pk, int64_t p, int64_t mu, int64_t wu, const int32_t* rel, bool rel_id,
-                                  int64_t rel0) {
+                                  int64_t rel0, int64_t cb, int64_t ce) {
 #pragma clang fp contract(off)
   const int64_t cols = wu < mu ? wu : mu;
-  // pairs (c, i >= c) enumerated column by column
-  const int64_t pairs = cols * mu - cols * (cols - 1) / 2;
+  // the columns c whose target lies in [cb, ce): rel is increasing, so they form a range
+  int64_t ca = 0, cz = cols;
+  if (rel_id) {
+    ca = cb - rel0 > 0 ? cb - rel0 : 0;
+    cz = ce - rel0 < cols ? ce - rel0 : cols;
+  } else {
+    int64_t lo = 0, hi = cols;
+    while (lo < hi) { const int64_t mid = (lo + hi) / 2; if (rel[mid] < cb) lo = mid + 1; else hi = mid; }
+    ca = lo;
+    lo = ca;
+    hi = cols;
+    while (lo < hi) { const int64_t mid = (lo + hi) / 2; if (rel[mid] < ce) lo = mid + 1; else hi = mid; }
+    cz = lo;
+  }
+  if (ca >= cz) return;
+  // pairs (c, i >= c), c in [ca, cz), enumerated column by column; base(c) = pairs before c
+  auto base = [&](int64_t c) { return (c - ca) * mu - ((c * (c - 1)) / 2 - (ca * (ca - 1)) / 2); };
+  const int64_t pairs = base(cz);
+  const int64_t d0 = ldl_offset(mk, k0);  // diagonal entry of column k0
   for (int64_t q = threadIdx.x; q < pairs; q += blockDim.x) {
-    // column c with offset(c) <= q < offset(c + 1), offset(c) = c mu - c (c - 1) / 2
-    int64_t lo = 0, hi = cols - 1;
+    int64_t lo = ca, hi = cz - 1;
     while (lo < hi) {
       const int64_t mid = (lo + hi + 1) / 2;
-      if (mid * mu - mid * (mid - 1) / 2 <= q) lo = mid;
+      if (base(mid) <= q) lo = mid;
       else hi = mid - 1;
     }
-    const int64_t c = lo, i = c + (q - (c * mu - c * (c - 1) / 2));
-    auto row = [&](int k) { return LDL_AT(lk, mk, p + c, k0 + k); };
+    const int64_t c = lo, i = c + (q - base(c));
+    int64_t li = d0 + p + i - k0, lc = d0 + p + c - k0, dd = d0;  // L(p+i, k0), L(p+c, k0), D(k0, k0)
     double s = 0.0;
-    for (int k = 0; k < kw; ++k) s += LDL_AT(lk, mk, p + i, k0 + k) * times_d_at(row, lk, mk, k0, pk, k, kw);
+    double prev_a = 0.0;  // L(p + c, k0 + k - 1): the first column of a 2 x 2 pivot
+    for (int k = 0; k < kw; ++k) {
+      const double a = lk[lc];
+      double y;
+      const int8_t pv = pk[k];
+      if (pv == PIV_FIRST) {
+        // row(k) d11 + row(k + 1) d21: the next column's entry of row p + c sits mk - k0 - k - 1 on
+        const double b = k + 1 < kw ? lk[lc + (mk - k0 - k - 1)] : 0.0;
+        y = a * lk[dd] + b * lk[dd + 1];
+      } else if (pv == PIV_SECOND && k > 0 && pk[k - 1] == PIV_FIRST) {
+        // row(k - 1) d21 + row(k) d22; d21 sits just below the previous diagonal, which lies
+        // mk - k0 - k + 1 before this one
+        y = prev_a * lk[dd - (mk - k0 - k)] + a * lk[dd];
+      } else {
+        y = a * lk[dd];
+      }
+      s += lk[li] * y;
+      prev_a = a;
+      const int64_t step = mk - k0 - k - 1;
+      li += step;
+      lc += step;
+      dd += step + 1;
+    }
     const int64_t ri = rel_id ? rel0 + i : rel[i], rc = rel_id ? rel0 + c : rel[c];
     LDL_AT(own, m, ri, rc) -= s;
   }
@@ -98,15 +137,39 @@ __global__ void k_ldlt_zero(int64_t n, double* __restrict__ v) {
     v[i] = 0.0;
 }
 
-// One supernode per workgroup (supernode list sn[blockIdx.x]): the descendants' updates in
-// order, then the panels (ldlt.rs:256-358). fail[s] receives the failing pivot's postorder
+// The descendants' updates of the supernodes of one level (the assemble of cholesky.rs:1069-1124
+// with Y = L D): workgroup t takes the target columns [tiles[t].b, tiles[t].e) of supernode
+// tiles[t].j and applies every update to them in order, so each entry receives its updates in
+// the reference's order whatever the tiling.
+struct LdlTile {
+  int64_t j, b, e;
+};
+__global__ __launch_bounds__(256) void k_ldlt_assemble(const LdlTile* __restrict__ tiles, const LdlSupernode* __restrict__ S,
+                                                       const LdlUpdate* __restrict__ U, const int32_t* __restrict__ rel,
+                                                       double* __restrict__ val, const int8_t* __restrict__ piv) {
+#pragma clang fp contract(off)
+  const LdlTile T = tiles[blockIdx.x];
+  const LdlSupernode J = S[T.j];
+  double* own = val + J.voff;
+  for (int64_t u = J.upd0; u < J.upd1; ++u) {
+    const LdlUpdate up = U[u];
+    const LdlSupernode K = S[up.k];
+    ldl_update(own, J.m, val + K.voff, K.m, 0, K.w, piv + K.first, up.p1, K.m - up.p1, up.p2 - up.p1, rel + up.rel, false,
+               0, T.b, T.e);
+    __syncthreads();
+  }
+}
+
+// One supernode per workgroup (supernode list sn[blockIdx.x]), assembled: the panels
+// (ldlt.rs:256-358). fail[s] receives the failing pivot's postorder
 // column (or stays -1); two[s] the supernode's 2 x 2 pivots.
+template <bool ONE>
 __global__ __launch_bounds__(256) void k_ldlt_supernode(const int64_t* __restrict__ sn, const LdlSupernode* __restrict__ S,
                                                         const LdlUpdate* __restrict__ U, const int32_t* __restrict__ rel,
                                                         const double* __restrict__ scale, double* __restrict__ val,
                                                         int8_t* __restrict__ piv, int32_t* __restrict__ order,
                                                         double* __restrict__ work, int64_t* __restrict__ fail,
-                                                        int32_t* __restrict__ two) {
+                                                        int32_t* __restrict__ two, int64_t* __restrict__ state) {
 #pragma clang fp contract(off)
   const int64_t j = sn[blockIdx.x];
   const LdlSupernode J = S[j];
@@ -114,18 +177,20 @@ __global__ __launch_bounds__(256) void k_ldlt_supernode(const int64_t* __restric
   double* own = val + J.voff;
   int8_t* pv = piv + J.first;
   int32_t* od = order + J.first;
-  for (int64_t u = J.upd0; u < J.upd1; ++u) {
-    const LdlUpdate up = U[u];
-    const LdlSupernode K = S[up.k];
-    ldl_update(own, m, val + K.voff, K.m, 0, K.w, piv + K.first, up.p1, K.m - up.p1, up.p2 - up.p1, rel + up.rel, false,
-               0);
+  (void)U;
+  (void)rel;
+  int64_t* st = state + 4 * j;  // ONE: start and end of the last panel, failure, 2 x 2 pivots
+  if (!ONE) {
+    for (int64_t c = threadIdx.x; c < w; c += blockDim.x) {
+      pv[c] = PIV_ONE;
+      od[c] = (int32_t)c;
+    }
     __syncthreads();
+  } else if (st[2] >= 0 || st[1] >= w) {
+    __syncthreads();
+    if (threadIdx.x == 0) st[0] = st[1] = w;  // nothing left: the trailing update stays idle
+    return;
   }
-  for (int64_t c = threadIdx.x; c < w; c += blockDim.x) {
-    pv[c] = PIV_ONE;
-    od[c] = (int32_t)c;
-  }
-  __syncthreads();
   double* A = work + J.scratch;      // two work columns (swapped by pointer, as the reference's)
   double* B = A + m;
   const double* sc = scale + J.first;
@@ -236,7 +301,7 @@ __global__ __launch_bounds__(256) void k_ldlt_supernode(const int64_t* __restric
     }
     __syncthreads();
   };
-  int64_t c0 = 0;
+  int64_t c0 = ONE ? st[1] : 0;
   int64_t failed = -1;
   while (c0 < w && failed < 0) {
     const int64_t end = c0 + LDL_PANEL < w ? c0 + LDL_PANEL : w;
@@ -336,21 +401,76 @@ __global__ __launch_bounds__(256) void k_ldlt_supernode(const int64_t* __restric
     }
     if (failed >= 0) break;
     const int64_t c1 = k;
+    if (ONE) {  // the trailing update is k_ldlt_trailing's, over many workgroups
+      if (threadIdx.x == 0) {
+        st[0] = c0;
+        st[1] = c1;
+      }
+      break;
+    }
     if (c1 < w)
-      ldl_update(own, m, own, m, c0, (int)(c1 - c0), pv + c0, c1, m - c1, w - c1, nullptr, true, c1);
+      ldl_update(own, m, own, m, c0, (int)(c1 - c0), pv + c0, c1, m - c1, w - c1, nullptr, true, c1, c1, w);
     __syncthreads();
     c0 = c1;
   }
   if (threadIdx.x == 0) {
-    fail[j] = failed >= 0 ? J.first + od[failed] : -1;
-    two[j] = twos;
+    if (!ONE) {
+      fail[j] = failed >= 0 ? J.first + od[failed] : -1;
+      two[j] = twos;
+    } else {
+      st[3] += twos;
+      two[j] = (int32_t)st[3];
+      if (failed >= 0) {
+        st[2] = J.first + od[failed];
+        fail[j] = st[2];
+        st[0] = st[1] = w;
+      }
+    }
   }
 }
 
-// The solve (solve_group, cholesky.rs:904-1037) with LDL^T's unit L, block diagonal D and pivot
-// orders, for one right-hand side x (postorder, in place), by one workgroup walking the
-// supernodes in order (forward) and in reverse (backward), so every entry of x receives its
-// operations in the reference's order. y: work of the widest supernode's width.
+// State of the supernodes factored one panel per launch: pivots and order reset, no panel yet.
+__global__ void k_ldlt_state_init(const int64_t* __restrict__ sn, const LdlSupernode* __restrict__ S,
+                                  int8_t* __restrict__ piv, int32_t* __restrict__ order, int64_t* __restrict__ fail,
+                                  int32_t* __restrict__ two, int64_t* __restrict__ state) {
+  const int64_t j = sn[blockIdx.x];
+  const LdlSupernode J = S[j];
+  for (int64_t c = threadIdx.x; c < J.w; c += blockDim.x) {
+    piv[J.first + c] = PIV_ONE;
+    order[J.first + c] = (int32_t)c;
+  }
+  if (threadIdx.x == 0) {
+    state[4 * j] = 0;
+    state[4 * j + 1] = 0;
+    state[4 * j + 2] = -1;
+    state[4 * j + 3] = 0;
+    fail[j] = -1;
+    two[j] = 0;
+  }
+}
+
+// The trailing update after one panel (ldlt.rs:347-352) of a supernode factored one panel per
+// launch: workgroup t takes the target columns [tiles[t].b, tiles[t].e) that lie at or after the
+// panel's end.
+__global__ __launch_bounds__(256) void k_ldlt_trailing(const LdlTile* __restrict__ tiles, const LdlSupernode* __restrict__ S,
+                                                       double* __restrict__ val, const int8_t* __restrict__ piv,
+                                                       const int64_t* __restrict__ state) {
+#pragma clang fp contract(off)
+  const LdlTile T = tiles[blockIdx.x];
+  const LdlSupernode J = S[T.j];
+  const int64_t c0 = state[4 * T.j], c1 = state[4 * T.j + 1];
+  if (state[4 * T.j + 2] >= 0 || c1 >= J.w || c1 <= c0 || T.e <= c1) return;
+  const int64_t m = J.m, w = J.w;
+  double* own = val + J.voff;
+  ldl_update(own, m, own, m, c0, (int)(c1 - c0), piv + J.first + c0, c1, m - c1, w - c1, nullptr, true, c1,
+             T.b > c1 ? T.b : c1, T.e);
+}
+
+// The forward pass and D^-1 of the solve (solve_group, cholesky.rs:904-981) with LDL^T's unit L,
+// block diagonal D and pivot orders, for one right-hand side x (postorder, in place), by one
+// workgroup walking the supernodes in order, so every entry of x receives its operations in the
+// reference's order (siblings update the same ancestors' entries). y: work of the widest
+// supernode's width. The unknowns are left in pivot order for the backward pass.
 __global__ __launch_bounds__(256) void k_ldlt_solve(int64_t ns, const LdlSupernode* __restrict__ S,
                                                     const int64_t* __restrict__ rows_all, const double* __restrict__ val,
                                                     const int8_t* __restrict__ piv, const int32_t* __restrict__ order,
@@ -391,39 +511,62 @@ __global__ __launch_bounds__(256) void k_ldlt_solve(int64_t ns, const LdlSuperno
     for (int64_t c = threadIdx.x; c < w; c += blockDim.x) own[c] = y[c];
     __syncthreads();
   }
-  for (int64_t k = ns - 1; k >= 0; --k) {
-    const LdlSupernode K = S[k];
-    const int64_t m = K.m, w = K.w;
-    const double* t = val + K.voff;
-    const int8_t* pv = piv + K.first;
-    const int32_t* od = order + K.first;
-    const int64_t* rows = rows_all + K.rows;
-    double* own = x + K.first;
-    for (int64_t c = threadIdx.x; c < w; c += blockDim.x) y[c] = own[c];
-    __syncthreads();
-    for (int64_t c = w - 1; c >= 0; --c) {
-      int64_t q, end;
-      if (pv[c] == PIV_SECOND && c > 0) {
-        q = c - 1;
-        end = c + 1;
-      } else if (pv[c] == PIV_FIRST) {
-        q = c + 1;
-        end = c + 2;
-      } else {
-        q = c;
-        end = c + 1;
+}
+
+// The backward pass (cholesky.rs:982-1030) for the supernodes of one level (sn[blockIdx.x]),
+// levels from the root down: a supernode's unknowns are its own and its sums read only its
+// ancestors' (final) unknowns, so the supernodes of a level are independent. Each sum runs in
+// the reference's order on one thread (its loads issued ahead).
+__global__ __launch_bounds__(64) void k_ldlt_solve_bwd(const int64_t* __restrict__ sn, const LdlSupernode* __restrict__ S,
+                                                       const int64_t* __restrict__ rows_all, const double* __restrict__ val,
+                                                       const int8_t* __restrict__ piv, const int32_t* __restrict__ order,
+                                                       double* __restrict__ x, double* __restrict__ ywork) {
+#pragma clang fp contract(off)
+  const LdlSupernode K = S[sn[blockIdx.x]];
+  const int64_t m = K.m, w = K.w;
+  const double* t = val + K.voff;
+  const int8_t* pv = piv + K.first;
+  const int32_t* od = order + K.first;
+  const int64_t* rows = rows_all + K.rows;
+  double* own = x + K.first;
+  double* y = ywork + K.first;
+  for (int64_t c = threadIdx.x; c < w; c += blockDim.x) y[c] = own[c];
+  __syncthreads();
+  __shared__ double sl[1024], sv[1024];
+  __shared__ double acc;
+  for (int64_t c = w - 1; c >= 0; --c) {
+    int64_t q, end;
+    if (pv[c] == PIV_SECOND && c > 0) {
+      q = c - 1;
+      end = c + 1;
+    } else if (pv[c] == PIV_FIRST) {
+      q = c + 1;
+      end = c + 2;
+    } else {
+      q = c;
+      end = c + 1;
+    }
+    const double* col = t + ldl_offset(m, q) - q;  // col[r] = L(r, q)
+    if (threadIdx.x == 0) acc = y[q];
+    // rows end..m in chunks: the products' operands staged by all lanes, the sum by one, in order
+    for (int64_t r0 = end; r0 < m; r0 += 1024) {
+      const int64_t r1 = r0 + 1024 < m ? r0 + 1024 : m;
+      for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+        sl[r - r0] = col[r];
+        sv[r - r0] = r < w ? y[r] : x[rows[r]];
       }
+      __syncthreads();
       if (threadIdx.x == 0) {
-        double sum = y[q];
-        for (int64_t r = end; r < w; ++r) sum -= LDL_AT(t, m, r, q) * y[r];
-        for (int64_t r = w; r < m; ++r) sum -= LDL_AT(t, m, r, q) * x[rows[r]];
-        y[q] = sum;
+        double sum = acc;
+        for (int64_t r = 0; r < r1 - r0; ++r) sum -= sl[r] * sv[r];
+        acc = sum;
       }
       __syncthreads();
     }
-    for (int64_t c = threadIdx.x; c < w; c += blockDim.x) own[od[c]] = y[c];
+    if (threadIdx.x == 0) y[q] = acc;
     __syncthreads();
   }
+  for (int64_t c = threadIdx.x; c < w; c += blockDim.x) own[od[c]] = y[c];
 }
 
 }  // namespace fc

@@ -719,11 +719,13 @@ struct fc_ldlt {
   double* val = nullptr;
   int8_t* piv = nullptr;
   int32_t* order = nullptr;
+  int64_t* sn = nullptr;           // supernodes by level
+  std::vector<int64_t> lvl_ptr;
   ~fc_ldlt() {
     int cur = 0;
     (void)hipGetDevice(&cur);
     (void)hipSetDevice(device);
-    for (void* p : {(void*)S, (void*)rows, (void*)val, (void*)piv, (void*)order})
+    for (void* p : {(void*)S, (void*)rows, (void*)val, (void*)piv, (void*)order, (void*)sn})
       if (p) (void)hipFree(p);
     if (stream) (void)hipStreamDestroy(stream);
     (void)hipSetDevice(cur);
@@ -1402,8 +1404,20 @@ struct LdltPlan {
   std::vector<int64_t> pos;         // upper entry -> slot in the trapezoids
   std::vector<int64_t> coord_r, coord_c;  // upper entry -> postorder (row, column)
   std::vector<int64_t> lvl_ptr, lvl_sn;   // supernodes by level of the supernodal tree
+  std::vector<int64_t> tile_ptr;          // tiles of each level's assemble
+  std::vector<fc::LdlTile> tiles;
+  // Each level lists its narrow supernodes first (lvl_narrow of them, one workgroup each for the
+  // whole factorization) and then its wide ones, factored one panel per launch: panel launch i
+  // of the level's wide supernodes is followed by the trailing update over the tiles
+  // pan_tiles[pan_tile_ptr[i] .. pan_tile_ptr[i + 1]], panel launches pan_ptr[l] .. pan_ptr[l + 1].
+  std::vector<int64_t> lvl_narrow;
+  std::vector<int64_t> pan_ptr, pan_tile_ptr;
+  std::vector<fc::LdlTile> pan_tiles;
   int64_t vals = 0, work = 0, max_w = 0;
 };
+
+constexpr int64_t LDL_TILE_COLS = 16;  // target columns of one assemble or trailing workgroup
+constexpr int64_t LDL_WIDE = 192;      // wider supernodes are factored one panel per launch
 
 LdltPlan ldlt_plan(const fc::Symbolic& s) {
   LdltPlan P;
@@ -1476,7 +1490,43 @@ LdltPlan ldlt_plan(const fc::Symbolic& s) {
   for (i64 l = 0; l < levels; ++l) P.lvl_ptr[l + 1] += P.lvl_ptr[l];
   P.lvl_sn.resize(ns);
   std::vector<i64> cur(P.lvl_ptr.begin(), P.lvl_ptr.end() - 1);
-  for (i64 k = 0; k < ns; ++k) P.lvl_sn[cur[level[k]]++] = k;
+  for (i64 k = 0; k < ns; ++k)
+    if (P.sn[k].w <= LDL_WIDE) P.lvl_sn[cur[level[k]]++] = k;
+  P.lvl_narrow.resize(levels);
+  for (i64 l = 0; l < levels; ++l) P.lvl_narrow[l] = cur[l] - P.lvl_ptr[l];
+  for (i64 k = 0; k < ns; ++k)
+    if (P.sn[k].w > LDL_WIDE) P.lvl_sn[cur[level[k]]++] = k;
+  P.pan_ptr.assign(levels + 1, 0);
+  P.pan_tile_ptr.assign(1, 0);
+  for (i64 l = 0; l < levels; ++l) {
+    i64 wmax = 0;
+    for (i64 q = P.lvl_ptr[l] + P.lvl_narrow[l]; q < P.lvl_ptr[l + 1]; ++q) wmax = std::max<i64>(wmax, P.sn[P.lvl_sn[q]].w);
+    // each panel ends at or after min(start + LDL_PANEL, w), so panel p ends at or after
+    // min((p + 1) LDL_PANEL, w) and a tile ending there or before has no column left to update
+    const i64 panels = (wmax + fc::LDL_PANEL - 1) / fc::LDL_PANEL;
+    for (i64 p = 0; p < panels; ++p) {
+      const i64 done = (p + 1) * fc::LDL_PANEL;
+      for (i64 q = P.lvl_ptr[l] + P.lvl_narrow[l]; q < P.lvl_ptr[l + 1]; ++q) {
+        const i64 k = P.lvl_sn[q], w = P.sn[k].w;
+        for (i64 b = 0; b < w; b += LDL_TILE_COLS) {
+          const i64 e = std::min<i64>(b + LDL_TILE_COLS, w);
+          if (e > done) P.pan_tiles.push_back(fc::LdlTile{k, b, e});
+        }
+      }
+      P.pan_tile_ptr.push_back((i64)P.pan_tiles.size());
+    }
+    P.pan_ptr[l + 1] = P.pan_ptr[l] + panels;
+  }
+  P.tile_ptr.assign(levels + 1, 0);
+  for (i64 l = 0; l < levels; ++l) {
+    for (i64 q = P.lvl_ptr[l]; q < P.lvl_ptr[l + 1]; ++q) {
+      const i64 k = P.lvl_sn[q];
+      if (P.sn[k].upd0 == P.sn[k].upd1) continue;  // no update to apply
+      for (i64 b = 0; b < P.sn[k].w; b += LDL_TILE_COLS)
+        P.tiles.push_back(fc::LdlTile{k, b, std::min<i64>(b + LDL_TILE_COLS, P.sn[k].w)});
+    }
+    P.tile_ptr[l + 1] = (i64)P.tiles.size();
+  }
   return P;
 }
 
@@ -1814,25 +1864,46 @@ int fc_ldlt_numeric(const fc_symbolic* h, const double* upper_val, uint64_t nnz,
     int32_t* drel = dupload(P.rel, stq);
     int64_t* dpos = dupload(P.pos, stq);
     int64_t* dsn = dupload(P.lvl_sn, stq);
+    F->sn = dsn;
+    F->lvl_ptr = P.lvl_ptr;
     double* dscale = dupload(scale, stq);
     double* daval = dalloc<double>((size_t)std::max<uint64_t>(nnz, 1));
     double* dwork = dalloc<double>((size_t)std::max<i64>(P.work, 1));
     int64_t* dfail = dalloc<int64_t>((size_t)std::max<i64>(F->ns, 1));
     int32_t* dtwo = dalloc<int32_t>((size_t)std::max<i64>(F->ns, 1));
+    int64_t* dstate = dalloc<int64_t>((size_t)std::max<i64>(4 * F->ns, 1));
     struct Free {
       std::vector<void*> p;
       ~Free() {
         for (void* q : p) dfree(q);
       }
-    } tmp{{dU, drel, dpos, dsn, dscale, daval, dwork, dfail, dtwo}};
+    } tmp{{dU, drel, dpos, dscale, daval, dwork, dfail, dtwo, dstate}};
     if (nnz) HIP_CHECK(hipMemcpyAsync(daval, upper_val, sizeof(double) * nnz, hipMemcpyHostToDevice, stq));
     if (P.vals) fc::k_ldlt_zero<<<blocks_for(P.vals), 256, 0, stq>>>(P.vals, F->val);
     if (nnz) fc::k_ldlt_scatter<<<(unsigned)((nnz + 255) / 256), 256, 0, stq>>>((i64)nnz, daval, dpos, F->val);
+    fc::LdlTile* dtiles = dupload(P.tiles, stq);
+    tmp.p.push_back(dtiles);
+    fc::LdlTile* dptiles = dupload(P.pan_tiles, stq);
+    tmp.p.push_back(dptiles);
     for (size_t l = 0; l + 1 < P.lvl_ptr.size(); ++l) {
-      const i64 cnt = P.lvl_ptr[l + 1] - P.lvl_ptr[l];
-      if (cnt > 0)
-        fc::k_ldlt_supernode<<<(unsigned)cnt, 256, 0, stq>>>(dsn + P.lvl_ptr[l], F->S, dU, drel, dscale, F->val, F->piv,
-                                                             F->order, dwork, dfail, dtwo);
+      const i64 narrow = P.lvl_narrow[l], wide = P.lvl_ptr[l + 1] - P.lvl_ptr[l] - narrow;
+      const i64 nt = P.tile_ptr[l + 1] - P.tile_ptr[l];
+      const int64_t* lsn = dsn + P.lvl_ptr[l];
+      if (nt > 0)
+        fc::k_ldlt_assemble<<<(unsigned)nt, 256, 0, stq>>>(dtiles + P.tile_ptr[l], F->S, dU, drel, F->val, F->piv);
+      if (narrow > 0)
+        fc::k_ldlt_supernode<false><<<(unsigned)narrow, 256, 0, stq>>>(lsn, F->S, dU, drel, dscale, F->val, F->piv,
+                                                                       F->order, dwork, dfail, dtwo, dstate);
+      if (wide > 0) {
+        fc::k_ldlt_state_init<<<(unsigned)wide, 256, 0, stq>>>(lsn + narrow, F->S, F->piv, F->order, dfail, dtwo, dstate);
+        for (i64 i = P.pan_ptr[l]; i < P.pan_ptr[l + 1]; ++i) {
+          fc::k_ldlt_supernode<true><<<(unsigned)wide, 256, 0, stq>>>(lsn + narrow, F->S, dU, drel, dscale, F->val,
+                                                                      F->piv, F->order, dwork, dfail, dtwo, dstate);
+          const i64 pt = P.pan_tile_ptr[i + 1] - P.pan_tile_ptr[i];
+          if (pt > 0)
+            fc::k_ldlt_trailing<<<(unsigned)pt, 256, 0, stq>>>(dptiles + P.pan_tile_ptr[i], F->S, F->val, F->piv, dstate);
+        }
+      }
     }
     HIP_CHECK(hipGetLastError());
     std::vector<int64_t> fails(std::max<i64>(F->ns, 1));
@@ -1916,12 +1987,20 @@ int fc_ldlt_solve(const fc_ldlt* f, double* b, uint64_t n, fc_error* err) {
     double* dx = dalloc<double>(n);
     double* dy = dalloc<double>((size_t)std::max<i64>(f->max_w, 1));
     HIP_CHECK(hipMemcpyAsync(dx, x.data(), sizeof(double) * n, hipMemcpyHostToDevice, f->stream));
+    double* dyw = dalloc<double>(n);
     fc::k_ldlt_solve<<<1, 256, 0, f->stream>>>(f->ns, f->S, f->rows, f->val, f->piv, f->order, dx, dy);
+    for (i64 l = (i64)f->lvl_ptr.size() - 2; l >= 0; --l) {
+      const i64 cnt = f->lvl_ptr[l + 1] - f->lvl_ptr[l];
+      if (cnt > 0)
+        fc::k_ldlt_solve_bwd<<<(unsigned)cnt, 64, 0, f->stream>>>(f->sn + f->lvl_ptr[l], f->S, f->rows, f->val, f->piv,
+                                                                   f->order, dx, dyw);
+    }
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipMemcpyAsync(x.data(), dx, sizeof(double) * n, hipMemcpyDeviceToHost, f->stream));
     HIP_CHECK(hipStreamSynchronize(f->stream));
     dfree(dx);
     dfree(dy);
+    dfree(dyw);
     for (uint64_t i = 0; i < n; ++i) b[perm[i]] = x[i];
   });
 }

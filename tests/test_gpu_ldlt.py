@@ -94,3 +94,27 @@ def test_bigger_grid_matches_the_oracle_bit_for_bit(gpu):
     assert np.array_equal(f.values().view(np.uint64), o.values().view(np.uint64))
     b = np.cos(np.arange(n) * 0.3)
     assert np.array_equal(f.solve(b).view(np.uint64), o.solve(b).view(np.uint64))
+
+
+@pytest.mark.parametrize("case", ["grid 16 x 16 x 14", "hex block 12^3"])
+def test_wide_supernodes_match_the_oracle_bit_for_bit(gpu, case):
+    """Supernodes wider than 192 columns are factored one panel per launch with the trailing
+    update spread over many workgroups: the same bits as the oracle's one-pass order."""
+    if case.startswith("grid"):
+        a, below = shifted_grid([16, 16, 14], 4.1)
+    else:
+        a, below = hex_pattern(12, 22.0, O.nested_dissection), None
+    n, ptr, idx, val = a
+    up = O.from_symmetric(n, ptr, idx, val)
+    sym = CH.Symbolic(CH.UpperColumns(n, *up))
+    assert np.diff(sym.export()["first"]).max() > 192
+    f = CH.LdltFactor(sym, up[2])
+    o = O.LdltFactor(n, *up)
+    assert f.inertia() == o.inertia() and f.two_by_two() == o.two_by_two()
+    if below is not None:
+        assert f.inertia()[1] == below
+    assert np.array_equal(f.values().view(np.uint64), o.values().view(np.uint64))
+    b = np.cos(np.arange(n) * 0.3)
+    x = f.solve(b)
+    assert np.array_equal(x.view(np.uint64), o.solve(b).view(np.uint64))
+    assert np.abs(O.csr_mul(n, ptr, idx, val, x) - b).max() < 1e-8 * np.abs(b).max() * max(1.0, np.abs(val).max())
