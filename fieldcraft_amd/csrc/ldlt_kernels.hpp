@@ -57,70 +57,120 @@ __device__ __forceinline__ double times_d_at(const Row& row, const double* t, in
   return row(k) * LDL_AT(t, m, k0 + k, k0 + k);
 }
 
+constexpr int LDL_YC = 16;   // target columns of one update window (and one tile)
+constexpr int LDL_KC = 128;  // source columns of Y staged in LDS at a time
+constexpr int LDL_PP = 4;    // pairs per thread per batch
+
 // own(rel(i), rel(c)) -= sum_{k < kw} L(p + i, k0 + k) Y(p + c, k), Y = L D (products and
 // subtract, cholesky.rs:531-686 with pivots), for c < min(wu, mu), c <= i < mu, restricted to the
-// target columns rel(c) in [cb, ce): one thread per (i, c), its sum in k order from 0.0. Entries
-// of column k0 + k lie mk - k0 - k - 1 apart from those of the column before (offset()).
+// target columns rel(c) in [cb, ce) (at most LDL_YC of them) and the target rows rel(i) in
+// [rb, re): each (i, c) summed by one thread in k order from 0.0. The window's Y entries are
+// staged in LDS LDL_KC columns at a time; each thread carries LDL_PP sums across the stages.
+// All threads of the workgroup call it together.
 __device__ inline void ldl_update(double* own, int64_t m, const double* lk, int64_t mk, int64_t k0, int kw,
                                   const int8_t* pk, int64_t p, int64_t mu, int64_t wu, const int32_t* rel, bool rel_id,
-                                  int64_t rel0, int64_t cb, int64_t ce) {
+                                  int64_t rel0, int64_t cb, int64_t ce, int64_t rb, int64_t re) {
 #pragma clang fp contract(off)
   const int64_t cols = wu < mu ? wu : mu;
-  // the columns c whose target lies in [cb, ce): rel is increasing, so they form a range
-  int64_t ca = 0, cz = cols;
-  if (rel_id) {
-    ca = cb - rel0 > 0 ? cb - rel0 : 0;
-    cz = ce - rel0 < cols ? ce - rel0 : cols;
-  } else {
-    int64_t lo = 0, hi = cols;
-    while (lo < hi) { const int64_t mid = (lo + hi) / 2; if (rel[mid] < cb) lo = mid + 1; else hi = mid; }
-    ca = lo;
-    lo = ca;
-    hi = cols;
-    while (lo < hi) { const int64_t mid = (lo + hi) / 2; if (rel[mid] < ce) lo = mid + 1; else hi = mid; }
-    cz = lo;
-  }
-  if (ca >= cz) return;
-  // pairs (c, i >= c), c in [ca, cz), enumerated column by column; base(c) = pairs before c
-  auto base = [&](int64_t c) { return (c - ca) * mu - ((c * (c - 1)) / 2 - (ca * (ca - 1)) / 2); };
-  const int64_t pairs = base(cz);
-  const int64_t d0 = ldl_offset(mk, k0);  // diagonal entry of column k0
-  for (int64_t q = threadIdx.x; q < pairs; q += blockDim.x) {
-    int64_t lo = ca, hi = cz - 1;
+  // the columns c whose target lies in [cb, ce) and the rows i whose target lies in [rb, re):
+  // rel is increasing, so both are ranges
+  auto first_at_least = [&](int64_t v, int64_t hi) {
+    if (rel_id) {
+      const int64_t t = v - rel0;
+      return t < 0 ? (int64_t)0 : (t > hi ? hi : t);
+    }
+    int64_t lo = 0;
     while (lo < hi) {
-      const int64_t mid = (lo + hi + 1) / 2;
-      if (base(mid) <= q) lo = mid;
-      else hi = mid - 1;
+      const int64_t mid = (lo + hi) / 2;
+      if (rel[mid] < v) lo = mid + 1;
+      else hi = mid;
     }
-    const int64_t c = lo, i = c + (q - base(c));
-    int64_t li = d0 + p + i - k0, lc = d0 + p + c - k0, dd = d0;  // L(p+i, k0), L(p+c, k0), D(k0, k0)
-    double s = 0.0;
-    double prev_a = 0.0;  // L(p + c, k0 + k - 1): the first column of a 2 x 2 pivot
-    for (int k = 0; k < kw; ++k) {
-      const double a = lk[lc];
-      double y;
-      const int8_t pv = pk[k];
-      if (pv == PIV_FIRST) {
-        // row(k) d11 + row(k + 1) d21: the next column's entry of row p + c sits mk - k0 - k - 1 on
-        const double b = k + 1 < kw ? lk[lc + (mk - k0 - k - 1)] : 0.0;
-        y = a * lk[dd] + b * lk[dd + 1];
-      } else if (pv == PIV_SECOND && k > 0 && pk[k - 1] == PIV_FIRST) {
-        // row(k - 1) d21 + row(k) d22; d21 sits just below the previous diagonal, which lies
-        // mk - k0 - k + 1 before this one
-        y = prev_a * lk[dd - (mk - k0 - k)] + a * lk[dd];
-      } else {
-        y = a * lk[dd];
-      }
-      s += lk[li] * y;
-      prev_a = a;
-      const int64_t step = mk - k0 - k - 1;
-      li += step;
-      lc += step;
-      dd += step + 1;
+    return lo;
+  };
+  const int64_t ca = first_at_least(cb, cols), cz = first_at_least(ce, cols);
+  const int64_t ia = first_at_least(rb, mu), iz = first_at_least(re, mu);
+  if (ca >= cz || ia >= iz) return;
+  const int ncol = (int)(cz - ca);
+  // pairs of column ca + t: rows max(ca + t, ia) .. iz; pre[t] = pairs before it
+  __shared__ int64_t pre[LDL_YC + 1];
+  __shared__ double ys[LDL_YC * LDL_KC];
+  if (threadIdx.x == 0) {
+    int64_t acc = 0;
+    for (int t = 0; t < ncol; ++t) {
+      pre[t] = acc;
+      const int64_t lo = ca + t > ia ? ca + t : ia;
+      acc += iz > lo ? iz - lo : 0;
     }
-    const int64_t ri = rel_id ? rel0 + i : rel[i], rc = rel_id ? rel0 + c : rel[c];
-    LDL_AT(own, m, ri, rc) -= s;
+    pre[ncol] = acc;
   }
+  __syncthreads();
+  const int64_t pairs = pre[ncol];
+  const int64_t d0 = ldl_offset(mk, k0);  // diagonal entry of column k0
+  for (int64_t q0 = 0; q0 < pairs; q0 += (int64_t)blockDim.x * LDL_PP) {
+    int64_t li[LDL_PP], tgt[LDL_PP];
+    int yrow[LDL_PP];
+    double sum[LDL_PP];
+#pragma unroll
+    for (int t = 0; t < LDL_PP; ++t) {
+      const int64_t q = q0 + threadIdx.x + (int64_t)t * blockDim.x;
+      sum[t] = 0.0;
+      yrow[t] = -1;
+      li[t] = 0;
+      tgt[t] = 0;
+      if (q < pairs) {
+        int c = 0;
+        while (c + 1 < ncol && pre[c + 1] <= q) ++c;
+        const int64_t lo = ca + c > ia ? ca + c : ia;
+        const int64_t i = lo + (q - pre[c]);
+        yrow[t] = c;
+        li[t] = d0 + p + i - k0;  // L(p + i, k0)
+        const int64_t ri = rel_id ? rel0 + i : rel[i], rc = rel_id ? rel0 + ca + c : rel[ca + c];
+        tgt[t] = ldl_offset(m, rc) + ri - rc;
+      }
+    }
+    for (int kb = 0; kb < kw; kb += LDL_KC) {
+      const int kn = kw - kb < LDL_KC ? kw - kb : LDL_KC;
+      __syncthreads();  // the previous stage is consumed
+      for (int e = threadIdx.x; e < ncol * kn; e += blockDim.x) {
+        const int t = e / kn, kk = e - t * kn, k = kb + kk;
+        const int64_t r = p + ca + t;  // row of Y
+        const int64_t dg = ldl_offset(mk, k0 + k);  // D(k0 + k, k0 + k)
+        const double a = lk[dg + r - (k0 + k)];
+        double y;
+        const int8_t pv = pk[k];
+        if (pv == PIV_FIRST) {  // row(k) d11 + row(k + 1) d21
+          const double bb = k + 1 < kw ? LDL_AT(lk, mk, r, k0 + k + 1) : 0.0;
+          y = a * lk[dg] + bb * lk[dg + 1];
+        } else if (pv == PIV_SECOND && k > 0 && pk[k - 1] == PIV_FIRST) {  // row(k - 1) d21 + row(k) d22
+          y = LDL_AT(lk, mk, r, k0 + k - 1) * LDL_AT(lk, mk, k0 + k, k0 + k - 1) + a * lk[dg];
+        } else {
+          y = a * lk[dg];
+        }
+        ys[t * LDL_KC + kk] = y;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int t = 0; t < LDL_PP; ++t) {
+        if (yrow[t] < 0) continue;
+        const double* yv = ys + yrow[t] * LDL_KC;
+        int64_t at = li[t];
+        double acc = sum[t];
+        int64_t step = mk - k0 - kb - 1;  // entries of column k0 + k + 1 lie mk - k0 - k - 1 on
+#pragma unroll 4
+        for (int kk = 0; kk < kn; ++kk) {
+          acc += lk[at] * yv[kk];
+          at += step;
+          step -= 1;
+        }
+        sum[t] = acc;
+        li[t] = at;
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < LDL_PP; ++t)
+      if (yrow[t] >= 0) own[tgt[t]] -= sum[t];
+  }
+  __syncthreads();
 }
 
 // A's values into the trapezoids (the columns of A, cholesky.rs:1054-1068): every upper entry
@@ -142,7 +192,8 @@ __global__ void k_ldlt_zero(int64_t n, double* __restrict__ v) {
 // tiles[t].j and applies every update to them in order, so each entry receives its updates in
 // the reference's order whatever the tiling.
 struct LdlTile {
-  int64_t j, b, e;
+  int64_t j, b, e;  // supernode, target columns [b, e)
+  int64_t rb, re;   // target rows [rb, re)
 };
 __global__ __launch_bounds__(256) void k_ldlt_assemble(const LdlTile* __restrict__ tiles, const LdlSupernode* __restrict__ S,
                                                        const LdlUpdate* __restrict__ U, const int32_t* __restrict__ rel,
@@ -155,8 +206,7 @@ __global__ __launch_bounds__(256) void k_ldlt_assemble(const LdlTile* __restrict
     const LdlUpdate up = U[u];
     const LdlSupernode K = S[up.k];
     ldl_update(own, J.m, val + K.voff, K.m, 0, K.w, piv + K.first, up.p1, K.m - up.p1, up.p2 - up.p1, rel + up.rel, false,
-               0, T.b, T.e);
-    __syncthreads();
+               0, T.b, T.e, T.rb, T.re);
   }
 }
 
@@ -200,11 +250,20 @@ __global__ __launch_bounds__(256) void k_ldlt_supernode(const int64_t* __restric
   __shared__ double shd[2];
   int twos = 0;
   // current(c) into out: column c rows k..m less the panel's factored columns c0..k (ldlt.rs:176-193)
+  __shared__ double ydc[LDL_PANEL + 2];
   auto current = [&](int64_t c0, int64_t k, int64_t c, double* out) {
     auto row = [&](int q) { return LDL_AT(own, m, c, c0 + q); };
+    for (int64_t q = threadIdx.x; q < k - c0; q += blockDim.x)
+      ydc[q] = times_d_at(row, own, m, c0, pv + c0, (int)q, (int)(k - c0));
+    __syncthreads();
+    const int64_t o0 = ldl_offset(m, c0) - c0;
     for (int64_t r = k + threadIdx.x; r < m; r += blockDim.x) {
       double v = r >= c ? LDL_AT(own, m, r, c) : LDL_AT(own, m, c, r);
-      for (int64_t q = c0; q < k; ++q) v -= LDL_AT(own, m, r, q) * times_d_at(row, own, m, c0, pv + c0, (int)(q - c0), (int)(k - c0));
+      int64_t o = o0 + r;  // (r, q): the next column's entry lies m - q - 1 on
+      for (int64_t q = c0; q < k; ++q) {
+        v -= own[o] * ydc[q - c0];
+        o += m - q - 1;
+      }
       out[r] = v;
     }
     __syncthreads();
@@ -408,8 +467,9 @@ __global__ __launch_bounds__(256) void k_ldlt_supernode(const int64_t* __restric
       }
       break;
     }
-    if (c1 < w)
-      ldl_update(own, m, own, m, c0, (int)(c1 - c0), pv + c0, c1, m - c1, w - c1, nullptr, true, c1, c1, w);
+    for (int64_t cb = c1; cb < w; cb += LDL_YC)
+      ldl_update(own, m, own, m, c0, (int)(c1 - c0), pv + c0, c1, m - c1, w - c1, nullptr, true, c1, cb,
+                 cb + LDL_YC < w ? cb + LDL_YC : w, cb, m);
     __syncthreads();
     c0 = c1;
   }
@@ -463,7 +523,7 @@ __global__ __launch_bounds__(256) void k_ldlt_trailing(const LdlTile* __restrict
   const int64_t m = J.m, w = J.w;
   double* own = val + J.voff;
   ldl_update(own, m, own, m, c0, (int)(c1 - c0), piv + J.first + c0, c1, m - c1, w - c1, nullptr, true, c1,
-             T.b > c1 ? T.b : c1, T.e);
+             T.b > c1 ? T.b : c1, T.e, T.rb, T.re);
 }
 
 // The forward pass and D^-1 of the solve (solve_group, cholesky.rs:904-981) with LDL^T's unit L,
@@ -532,7 +592,7 @@ __global__ __launch_bounds__(64) void k_ldlt_solve_bwd(const int64_t* __restrict
   double* y = ywork + K.first;
   for (int64_t c = threadIdx.x; c < w; c += blockDim.x) y[c] = own[c];
   __syncthreads();
-  __shared__ double sl[1024], sv[1024];
+  __shared__ double sp[2048];
   __shared__ double acc;
   for (int64_t c = w - 1; c >= 0; --c) {
     int64_t q, end;
@@ -548,17 +608,23 @@ __global__ __launch_bounds__(64) void k_ldlt_solve_bwd(const int64_t* __restrict
     }
     const double* col = t + ldl_offset(m, q) - q;  // col[r] = L(r, q)
     if (threadIdx.x == 0) acc = y[q];
-    // rows end..m in chunks: the products' operands staged by all lanes, the sum by one, in order
-    for (int64_t r0 = end; r0 < m; r0 += 1024) {
-      const int64_t r1 = r0 + 1024 < m ? r0 + 1024 : m;
-      for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
-        sl[r - r0] = col[r];
-        sv[r - r0] = r < w ? y[r] : x[rows[r]];
-      }
+    // rows end..m in chunks: the products by all lanes, their subtraction by one, in row order
+    for (int64_t r0 = end; r0 < m; r0 += 2048) {
+      const int64_t r1 = r0 + 2048 < m ? r0 + 2048 : m;
+      for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) sp[r - r0] = col[r] * (r < w ? y[r] : x[rows[r]]);
       __syncthreads();
       if (threadIdx.x == 0) {
         double sum = acc;
-        for (int64_t r = 0; r < r1 - r0; ++r) sum -= sl[r] * sv[r];
+        const int cnt = (int)(r1 - r0);
+        int r = 0;
+        for (; r + 8 <= cnt; r += 8) {
+          double v[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) v[u] = sp[r + u];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) sum -= v[u];
+        }
+        for (; r < cnt; ++r) sum -= sp[r];
         acc = sum;
       }
       __syncthreads();

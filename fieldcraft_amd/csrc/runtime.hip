@@ -1416,7 +1416,13 @@ struct LdltPlan {
   int64_t vals = 0, work = 0, max_w = 0;
 };
 
-constexpr int64_t LDL_TILE_COLS = 16;  // target columns of one assemble or trailing workgroup
+constexpr int64_t LDL_TILE_COLS = fc::LDL_YC;  // target columns of one assemble or trailing workgroup
+constexpr int64_t LDL_TILE_ROWS = 512;         // and its target rows
+
+// tiles of LDL_TILE_COLS target columns [b, e) of supernode k by bands of LDL_TILE_ROWS rows from b
+void ldl_tiles(std::vector<fc::LdlTile>& out, i64 k, i64 b, i64 e, i64 m) {
+  for (i64 rb = b; rb < m; rb += LDL_TILE_ROWS) out.push_back(fc::LdlTile{k, b, e, rb, std::min<i64>(rb + LDL_TILE_ROWS, m)});
+}
 constexpr int64_t LDL_WIDE = 192;      // wider supernodes are factored one panel per launch
 
 LdltPlan ldlt_plan(const fc::Symbolic& s) {
@@ -1510,7 +1516,7 @@ LdltPlan ldlt_plan(const fc::Symbolic& s) {
         const i64 k = P.lvl_sn[q], w = P.sn[k].w;
         for (i64 b = 0; b < w; b += LDL_TILE_COLS) {
           const i64 e = std::min<i64>(b + LDL_TILE_COLS, w);
-          if (e > done) P.pan_tiles.push_back(fc::LdlTile{k, b, e});
+          if (e > done) ldl_tiles(P.pan_tiles, k, b, e, P.sn[k].m);
         }
       }
       P.pan_tile_ptr.push_back((i64)P.pan_tiles.size());
@@ -1523,7 +1529,7 @@ LdltPlan ldlt_plan(const fc::Symbolic& s) {
       const i64 k = P.lvl_sn[q];
       if (P.sn[k].upd0 == P.sn[k].upd1) continue;  // no update to apply
       for (i64 b = 0; b < P.sn[k].w; b += LDL_TILE_COLS)
-        P.tiles.push_back(fc::LdlTile{k, b, std::min<i64>(b + LDL_TILE_COLS, P.sn[k].w)});
+        ldl_tiles(P.tiles, k, b, std::min<i64>(b + LDL_TILE_COLS, P.sn[k].w), P.sn[k].m);
     }
     P.tile_ptr[l + 1] = (i64)P.tiles.size();
   }
