@@ -214,7 +214,7 @@ __global__ __launch_bounds__(256) void k_ldlt_assemble(const LdlTile* __restrict
 // (ldlt.rs:256-358). fail[s] receives the failing pivot's postorder
 // column (or stays -1); two[s] the supernode's 2 x 2 pivots.
 template <bool ONE>
-__global__ __launch_bounds__(256) void k_ldlt_supernode(const int64_t* __restrict__ sn, const LdlSupernode* __restrict__ S,
+__global__ __launch_bounds__(1024) void k_ldlt_supernode(const int64_t* __restrict__ sn, const LdlSupernode* __restrict__ S,
                                                         const LdlUpdate* __restrict__ U, const int32_t* __restrict__ rel,
                                                         const double* __restrict__ scale, double* __restrict__ val,
                                                         int8_t* __restrict__ piv, int32_t* __restrict__ order,
@@ -244,8 +244,8 @@ __global__ __launch_bounds__(256) void k_ldlt_supernode(const int64_t* __restric
   double* A = work + J.scratch;      // two work columns (swapped by pointer, as the reference's)
   double* B = A + m;
   const double* sc = scale + J.first;
-  __shared__ double red_v[256];
-  __shared__ int64_t red_i[256];
+  __shared__ double red_v[16];   // one entry per wave (up to 1024 threads)
+  __shared__ int64_t red_i[16], red_n[16];
   __shared__ int64_t sh[6];          // decisions broadcast by thread 0
   __shared__ double shd[2];
   int twos = 0;
@@ -260,6 +260,7 @@ __global__ __launch_bounds__(256) void k_ldlt_supernode(const int64_t* __restric
     for (int64_t r = k + threadIdx.x; r < m; r += blockDim.x) {
       double v = r >= c ? LDL_AT(own, m, r, c) : LDL_AT(own, m, c, r);
       int64_t o = o0 + r;  // (r, q): the next column's entry lies m - q - 1 on
+#pragma unroll 8
       for (int64_t q = c0; q < k; ++q) {
         v -= own[o] * ydc[q - c0];
         o += m - q - 1;
@@ -283,29 +284,37 @@ __global__ __launch_bounds__(256) void k_ldlt_supernode(const int64_t* __restric
         bi = i;
       }
     }
-    red_v[threadIdx.x] = bv;
-    red_i[threadIdx.x] = bi;
+    // the wave's best (largest value, then the lowest index; zeros carry no index) and first
+    // NaN, then thread 0 over the waves in order
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const double ov = __shfl_xor(bv, off, 64);
+      const int64_t oi = __shfl_xor(bi, off, 64), on = __shfl_xor(nan_i, off, 64);
+      if (ov > bv || (ov == bv && ov > 0.0 && oi < bi)) {
+        bv = ov;
+        bi = oi;
+      }
+      if (on >= 0 && (nan_i < 0 || on < nan_i)) nan_i = on;
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) {
+      red_v[wv] = bv;
+      red_i[wv] = bi;
+      red_n[wv] = nan_i;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
-      int64_t best_i = -1;
+      int64_t best_i = -1, first_nan = -1;
       double best = 0.0;
-      for (int t = 0; t < (int)blockDim.x; ++t) {
+      for (int t = 0; t < (int)(blockDim.x >> 6); ++t) {
         if (red_v[t] > best || (red_v[t] == best && red_v[t] > 0.0 && red_i[t] < best_i)) {
           best = red_v[t];
           best_i = red_i[t];
         }
+        if (red_n[t] >= 0 && (first_nan < 0 || red_n[t] < first_nan)) first_nan = red_n[t];
       }
       sh[4] = best_i < 0 ? lo : best_i;
       shd[0] = best;
-    }
-    __syncthreads();
-    // NaN: the reference stops at the first NaN in index order
-    red_i[threadIdx.x] = nan_i;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      int64_t first_nan = -1;
-      for (int t = 0; t < (int)blockDim.x; ++t)
-        if (red_i[t] >= 0 && (first_nan < 0 || red_i[t] < first_nan)) first_nan = red_i[t];
       if (first_nan >= 0) {  // the reference returns at the first NaN it meets, whatever came before
         sh[4] = first_nan;
         shd[0] = __builtin_nan("");

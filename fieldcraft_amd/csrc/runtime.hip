@@ -1897,13 +1897,14 @@ int fc_ldlt_numeric(const fc_symbolic* h, const double* upper_val, uint64_t nnz,
       const int64_t* lsn = dsn + P.lvl_ptr[l];
       if (nt > 0)
         fc::k_ldlt_assemble<<<(unsigned)nt, 256, 0, stq>>>(dtiles + P.tile_ptr[l], F->S, dU, drel, F->val, F->piv);
+      // a level of few supernodes gives each 1024 threads; a crowded one 256
       if (narrow > 0)
-        fc::k_ldlt_supernode<false><<<(unsigned)narrow, 256, 0, stq>>>(lsn, F->S, dU, drel, dscale, F->val, F->piv,
+        fc::k_ldlt_supernode<false><<<(unsigned)narrow, narrow < 256 ? 1024 : 256, 0, stq>>>(lsn, F->S, dU, drel, dscale, F->val, F->piv,
                                                                        F->order, dwork, dfail, dtwo, dstate);
       if (wide > 0) {
         fc::k_ldlt_state_init<<<(unsigned)wide, 256, 0, stq>>>(lsn + narrow, F->S, F->piv, F->order, dfail, dtwo, dstate);
         for (i64 i = P.pan_ptr[l]; i < P.pan_ptr[l + 1]; ++i) {
-          fc::k_ldlt_supernode<true><<<(unsigned)wide, 256, 0, stq>>>(lsn + narrow, F->S, dU, drel, dscale, F->val,
+          fc::k_ldlt_supernode<true><<<(unsigned)wide, 1024, 0, stq>>>(lsn + narrow, F->S, dU, drel, dscale, F->val,
                                                                       F->piv, F->order, dwork, dfail, dtwo, dstate);
           const i64 pt = P.pan_tile_ptr[i + 1] - P.pan_tile_ptr[i];
           if (pt > 0)
