@@ -535,58 +535,91 @@ __global__ __launch_bounds__(256) void k_ldlt_trailing(const LdlTile* __restrict
              T.b > c1 ? T.b : c1, T.e, T.rb, T.re);
 }
 
-// The forward pass and D^-1 of the solve (solve_group, cholesky.rs:904-981) with LDL^T's unit L,
-// block diagonal D and pivot orders, for one right-hand side x (postorder, in place), by one
-// workgroup walking the supernodes in order, so every entry of x receives its operations in the
-// reference's order (siblings update the same ancestors' entries). y: work of the widest
-// supernode's width. The unknowns are left in pivot order for the backward pass.
-__global__ __launch_bounds__(256) void k_ldlt_solve(int64_t ns, const LdlSupernode* __restrict__ S,
-                                                    const int64_t* __restrict__ rows_all, const double* __restrict__ val,
-                                                    const int8_t* __restrict__ piv, const int32_t* __restrict__ order,
-                                                    double* __restrict__ x, double* __restrict__ y) {
+// The forward pass of the solve (solve_group, cholesky.rs:904-981, with LDL^T's unit L, block
+// diagonal D and pivot orders) level by level on one right-hand side x (postorder, in place).
+// First the descendants' contributions to one level's unknowns, read as a gather: workgroup t
+// takes the columns [b, e) of supernode j and, update by update in increasing descendant order
+// (the order the one-pass sweep applies them), subtracts L_k(p1 + i, q) z_k(q) for each column q
+// of descendant k in order; z holds the descendants' forward values before D^-1.
+__global__ __launch_bounds__(256) void k_ldlt_fwd_gather(const LdlTile* __restrict__ tiles, const LdlSupernode* __restrict__ S,
+                                                         const LdlUpdate* __restrict__ U, const int32_t* __restrict__ rel,
+                                                         const double* __restrict__ val, const double* __restrict__ z,
+                                                         double* __restrict__ x) {
 #pragma clang fp contract(off)
-  for (int64_t k = 0; k < ns; ++k) {
-    const LdlSupernode K = S[k];
-    const int64_t m = K.m, w = K.w;
-    const double* t = val + K.voff;
-    const int8_t* pv = piv + K.first;
-    const int32_t* od = order + K.first;
-    const int64_t* rows = rows_all + K.rows;
-    double* own = x + K.first;
-    for (int64_t c = threadIdx.x; c < w; c += blockDim.x) y[c] = own[od[c]];
-    __syncthreads();
-    for (int64_t q = 0; q < w; ++q) {
-      const int64_t end = q + (pv[q] == PIV_FIRST ? 2 : 1);
-      const double yq = y[q];
-      for (int64_t r = end + threadIdx.x; r < m; r += blockDim.x) {
-        const double l = LDL_AT(t, m, r, q);
-        if (r < w) y[r] -= l * yq;
-        else x[rows[r]] -= l * yq;
+  const LdlTile T = tiles[blockIdx.x];
+  const LdlSupernode J = S[T.j];
+  for (int64_t u = J.upd0; u < J.upd1; ++u) {
+    const LdlUpdate up = U[u];
+    const LdlSupernode K = S[up.k];
+    const int32_t* rl = rel + up.rel;  // rl[i]: the column of J that row p1 + i of K lands on
+    const int64_t cnt = up.p2 - up.p1;
+    int64_t ia = 0, iz = cnt, hi = cnt;
+    while (ia < hi) { const int64_t mid = (ia + hi) / 2; if (rl[mid] < T.b) ia = mid + 1; else hi = mid; }
+    hi = cnt;
+    iz = ia;
+    while (iz < hi) { const int64_t mid = (iz + hi) / 2; if (rl[mid] < T.e) iz = mid + 1; else hi = mid; }
+    const double* lk = val + K.voff;
+    const double* zk = z + K.first;
+    for (int64_t i = ia + threadIdx.x; i < iz; i += blockDim.x) {
+      double* xi = x + J.first + rl[i];
+      double v = *xi;
+      int64_t at = up.p1 + i;  // L_k(p1 + i, 0); column q + 1's entry lies K.m - q - 1 on
+      for (int64_t q = 0; q < K.w; ++q) {
+        v -= lk[at] * zk[q];
+        at += K.m - q - 1;
       }
-      __syncthreads();
+      *xi = v;
     }
-    for (int64_t c = threadIdx.x; c < w; c += blockDim.x) {
-      if (pv[c] == PIV_FIRST) {
-        const double d11 = LDL_AT(t, m, c, c), d21 = LDL_AT(t, m, c + 1, c), d22 = LDL_AT(t, m, c + 1, c + 1);
-        const double det = d11 * d22 - d21 * d21;
-        const double p = y[c], q2 = y[c + 1];
-        y[c] = (d22 * p - d21 * q2) / det;
-        y[c + 1] = (d11 * q2 - d21 * p) / det;
-      } else if (pv[c] == PIV_ONE) {
-        y[c] /= LDL_AT(t, m, c, c);
-      }
-    }
-    __syncthreads();
-    for (int64_t c = threadIdx.x; c < w; c += blockDim.x) own[c] = y[c];
     __syncthreads();
   }
+}
+
+// Then each supernode of the level (one workgroup each): its own unknowns in pivot order, the
+// unit lower triangle's forward substitution, z = the values before D^-1, D^-1 (1 x 1 and 2 x 2
+// blocks), the result left in pivot order in x. y is a work vector of n entries.
+__global__ __launch_bounds__(256) void k_ldlt_fwd_own(const int64_t* __restrict__ sn, const LdlSupernode* __restrict__ S,
+                                                      const double* __restrict__ val, const int8_t* __restrict__ piv,
+                                                      const int32_t* __restrict__ order, double* __restrict__ x,
+                                                      double* __restrict__ ywork, double* __restrict__ z) {
+#pragma clang fp contract(off)
+  const LdlSupernode K = S[sn[blockIdx.x]];
+  const int64_t m = K.m, w = K.w;
+  const double* t = val + K.voff;
+  const int8_t* pv = piv + K.first;
+  const int32_t* od = order + K.first;
+  double* own = x + K.first;
+  double* y = ywork + K.first;
+  double* zk = z + K.first;
+  for (int64_t c = threadIdx.x; c < w; c += blockDim.x) y[c] = own[od[c]];
+  __syncthreads();
+  for (int64_t q = 0; q < w; ++q) {
+    const int64_t end = q + (pv[q] == PIV_FIRST ? 2 : 1);
+    const double yq = y[q];
+    for (int64_t r = end + threadIdx.x; r < w; r += blockDim.x) y[r] -= LDL_AT(t, m, r, q) * yq;
+    __syncthreads();
+  }
+  for (int64_t c = threadIdx.x; c < w; c += blockDim.x) zk[c] = y[c];
+  __syncthreads();
+  for (int64_t c = threadIdx.x; c < w; c += blockDim.x) {
+    if (pv[c] == PIV_FIRST) {
+      const double d11 = LDL_AT(t, m, c, c), d21 = LDL_AT(t, m, c + 1, c), d22 = LDL_AT(t, m, c + 1, c + 1);
+      const double det = d11 * d22 - d21 * d21;
+      const double p = y[c], q2 = y[c + 1];
+      y[c] = (d22 * p - d21 * q2) / det;
+      y[c + 1] = (d11 * q2 - d21 * p) / det;
+    } else if (pv[c] == PIV_ONE) {
+      y[c] /= LDL_AT(t, m, c, c);
+    }
+  }
+  __syncthreads();
+  for (int64_t c = threadIdx.x; c < w; c += blockDim.x) own[c] = y[c];
 }
 
 // The backward pass (cholesky.rs:982-1030) for the supernodes of one level (sn[blockIdx.x]),
 // levels from the root down: a supernode's unknowns are its own and its sums read only its
 // ancestors' (final) unknowns, so the supernodes of a level are independent. Each sum runs in
 // the reference's order on one thread (its loads issued ahead).
-__global__ __launch_bounds__(64) void k_ldlt_solve_bwd(const int64_t* __restrict__ sn, const LdlSupernode* __restrict__ S,
+__global__ __launch_bounds__(256) void k_ldlt_solve_bwd(const int64_t* __restrict__ sn, const LdlSupernode* __restrict__ S,
                                                        const int64_t* __restrict__ rows_all, const double* __restrict__ val,
                                                        const int8_t* __restrict__ piv, const int32_t* __restrict__ order,
                                                        double* __restrict__ x, double* __restrict__ ywork) {
@@ -601,7 +634,7 @@ __global__ __launch_bounds__(64) void k_ldlt_solve_bwd(const int64_t* __restrict
   double* y = ywork + K.first;
   for (int64_t c = threadIdx.x; c < w; c += blockDim.x) y[c] = own[c];
   __syncthreads();
-  __shared__ double sp[2048];
+  __shared__ __align__(16) double sp[2048 + 16];  // 16 past the end: the pipelined reads' last batch
   __shared__ double acc;
   for (int64_t c = w - 1; c >= 0; --c) {
     int64_t q, end;
@@ -623,17 +656,25 @@ __global__ __launch_bounds__(64) void k_ldlt_solve_bwd(const int64_t* __restrict
       for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) sp[r - r0] = col[r] * (r < w ? y[r] : x[rows[r]]);
       __syncthreads();
       if (threadIdx.x == 0) {
+        // the chain of subtractions, its next 16 operands read while the current 16 are used
         double sum = acc;
-        const int cnt = (int)(r1 - r0);
-        int r = 0;
-        for (; r + 8 <= cnt; r += 8) {
-          double v[8];
+        const int cnt = (int)(r1 - r0), nb = cnt / 16;
+        const double2* p2 = reinterpret_cast<const double2*>(sp);
+        double2 cur[8], nxt[8];
 #pragma unroll
-          for (int u = 0; u < 8; ++u) v[u] = sp[r + u];
+        for (int u = 0; u < 8; ++u) cur[u] = p2[u];
+        for (int bt = 0; bt < nb; ++bt) {
 #pragma unroll
-          for (int u = 0; u < 8; ++u) sum -= v[u];
+          for (int u = 0; u < 8; ++u) nxt[u] = p2[(bt + 1) * 8 + u];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            sum -= cur[u].x;
+            sum -= cur[u].y;
+          }
+#pragma unroll
+          for (int u = 0; u < 8; ++u) cur[u] = nxt[u];
         }
-        for (; r < cnt; ++r) sum -= sp[r];
+        for (int r = nb * 16; r < cnt; ++r) sum -= sp[r];
         acc = sum;
       }
       __syncthreads();

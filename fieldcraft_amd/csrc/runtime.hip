@@ -721,11 +721,16 @@ struct fc_ldlt {
   int32_t* order = nullptr;
   int64_t* sn = nullptr;           // supernodes by level
   std::vector<int64_t> lvl_ptr;
+  fc::LdlUpdate* U = nullptr;      // updates and their relative maps, for the forward gather
+  int32_t* rel = nullptr;
+  fc::LdlTile* ftiles = nullptr;   // the forward gather's tiles, ftile_ptr[l] .. ftile_ptr[l + 1] per level
+  std::vector<int64_t> ftile_ptr;
   ~fc_ldlt() {
     int cur = 0;
     (void)hipGetDevice(&cur);
     (void)hipSetDevice(device);
-    for (void* p : {(void*)S, (void*)rows, (void*)val, (void*)piv, (void*)order, (void*)sn})
+    for (void* p : {(void*)S, (void*)rows, (void*)val, (void*)piv, (void*)order, (void*)sn, (void*)U, (void*)rel,
+                    (void*)ftiles})
       if (p) (void)hipFree(p);
     if (stream) (void)hipStreamDestroy(stream);
     (void)hipSetDevice(cur);
@@ -1413,11 +1418,14 @@ struct LdltPlan {
   std::vector<int64_t> lvl_narrow;
   std::vector<int64_t> pan_ptr, pan_tile_ptr;
   std::vector<fc::LdlTile> pan_tiles;
+  std::vector<int64_t> ftile_ptr;  // the solve's forward gather: tiles of each level
+  std::vector<fc::LdlTile> ftiles;
   int64_t vals = 0, work = 0, max_w = 0;
 };
 
 constexpr int64_t LDL_TILE_COLS = fc::LDL_YC;  // target columns of one assemble or trailing workgroup
 constexpr int64_t LDL_TILE_ROWS = 512;         // and its target rows
+constexpr int64_t LDL_GATHER_COLS = 256;       // unknowns of one forward-gather workgroup
 
 // tiles of LDL_TILE_COLS target columns [b, e) of supernode k by bands of LDL_TILE_ROWS rows from b
 void ldl_tiles(std::vector<fc::LdlTile>& out, i64 k, i64 b, i64 e, i64 m) {
@@ -1532,6 +1540,16 @@ LdltPlan ldlt_plan(const fc::Symbolic& s) {
         ldl_tiles(P.tiles, k, b, std::min<i64>(b + LDL_TILE_COLS, P.sn[k].w), P.sn[k].m);
     }
     P.tile_ptr[l + 1] = (i64)P.tiles.size();
+  }
+  P.ftile_ptr.assign(levels + 1, 0);
+  for (i64 l = 0; l < levels; ++l) {
+    for (i64 q = P.lvl_ptr[l]; q < P.lvl_ptr[l + 1]; ++q) {
+      const i64 k = P.lvl_sn[q];
+      if (P.sn[k].upd0 == P.sn[k].upd1) continue;
+      for (i64 b = 0; b < P.sn[k].w; b += LDL_GATHER_COLS)
+        P.ftiles.push_back(fc::LdlTile{k, b, std::min<i64>(b + LDL_GATHER_COLS, P.sn[k].w), 0, 0});
+    }
+    P.ftile_ptr[l + 1] = (i64)P.ftiles.size();
   }
   return P;
 }
@@ -1868,6 +1886,10 @@ int fc_ldlt_numeric(const fc_symbolic* h, const double* upper_val, uint64_t nnz,
     F->order = dalloc<int32_t>((size_t)std::max<i64>(s.n, 1));
     fc::LdlUpdate* dU = dupload(P.up, stq);
     int32_t* drel = dupload(P.rel, stq);
+    F->U = dU;
+    F->rel = drel;
+    F->ftiles = dupload(P.ftiles, stq);
+    F->ftile_ptr = P.ftile_ptr;
     int64_t* dpos = dupload(P.pos, stq);
     int64_t* dsn = dupload(P.lvl_sn, stq);
     F->sn = dsn;
@@ -1883,7 +1905,7 @@ int fc_ldlt_numeric(const fc_symbolic* h, const double* upper_val, uint64_t nnz,
       ~Free() {
         for (void* q : p) dfree(q);
       }
-    } tmp{{dU, drel, dpos, dscale, daval, dwork, dfail, dtwo, dstate}};
+    } tmp{{dpos, dscale, daval, dwork, dfail, dtwo, dstate}};
     if (nnz) HIP_CHECK(hipMemcpyAsync(daval, upper_val, sizeof(double) * nnz, hipMemcpyHostToDevice, stq));
     if (P.vals) fc::k_ldlt_zero<<<blocks_for(P.vals), 256, 0, stq>>>(P.vals, F->val);
     if (nnz) fc::k_ldlt_scatter<<<(unsigned)((nnz + 255) / 256), 256, 0, stq>>>((i64)nnz, daval, dpos, F->val);
@@ -1992,21 +2014,29 @@ int fc_ldlt_solve(const fc_ldlt* f, double* b, uint64_t n, fc_error* err) {
     std::vector<double> x(n);
     for (uint64_t i = 0; i < n; ++i) x[i] = b[perm[i]];
     double* dx = dalloc<double>(n);
-    double* dy = dalloc<double>((size_t)std::max<i64>(f->max_w, 1));
+    double* dz = dalloc<double>(n);
     HIP_CHECK(hipMemcpyAsync(dx, x.data(), sizeof(double) * n, hipMemcpyHostToDevice, f->stream));
     double* dyw = dalloc<double>(n);
-    fc::k_ldlt_solve<<<1, 256, 0, f->stream>>>(f->ns, f->S, f->rows, f->val, f->piv, f->order, dx, dy);
+    for (size_t l = 0; l + 1 < f->lvl_ptr.size(); ++l) {
+      const i64 nt = f->ftile_ptr[l + 1] - f->ftile_ptr[l], cnt = f->lvl_ptr[l + 1] - f->lvl_ptr[l];
+      if (nt > 0)
+        fc::k_ldlt_fwd_gather<<<(unsigned)nt, 256, 0, f->stream>>>(f->ftiles + f->ftile_ptr[l], f->S, f->U, f->rel, f->val,
+                                                                   dz, dx);
+      if (cnt > 0)
+        fc::k_ldlt_fwd_own<<<(unsigned)cnt, 256, 0, f->stream>>>(f->sn + f->lvl_ptr[l], f->S, f->val, f->piv, f->order, dx,
+                                                                 dyw, dz);
+    }
     for (i64 l = (i64)f->lvl_ptr.size() - 2; l >= 0; --l) {
       const i64 cnt = f->lvl_ptr[l + 1] - f->lvl_ptr[l];
       if (cnt > 0)
-        fc::k_ldlt_solve_bwd<<<(unsigned)cnt, 64, 0, f->stream>>>(f->sn + f->lvl_ptr[l], f->S, f->rows, f->val, f->piv,
+        fc::k_ldlt_solve_bwd<<<(unsigned)cnt, 256, 0, f->stream>>>(f->sn + f->lvl_ptr[l], f->S, f->rows, f->val, f->piv,
                                                                    f->order, dx, dyw);
     }
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipMemcpyAsync(x.data(), dx, sizeof(double) * n, hipMemcpyDeviceToHost, f->stream));
     HIP_CHECK(hipStreamSynchronize(f->stream));
     dfree(dx);
-    dfree(dy);
+    dfree(dz);
     dfree(dyw);
     for (uint64_t i = 0; i < n; ++i) b[perm[i]] = x[i];
   });
