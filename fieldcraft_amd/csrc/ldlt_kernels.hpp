@@ -535,6 +535,18 @@ __global__ __launch_bounds__(256) void k_ldlt_trailing(const LdlTile* __restrict
              T.b > c1 ? T.b : c1, T.e, T.rb, T.re);
 }
 
+// D's blocks for the inertia: out[2 c] = D(c, c), out[2 c + 1] = D(c + 1, c) at the first column
+// of a 2 x 2 pivot (else 0), one workgroup per supernode.
+__global__ void k_ldlt_diag(const LdlSupernode* __restrict__ S, const double* __restrict__ val,
+                            const int8_t* __restrict__ piv, double* __restrict__ out) {
+  const LdlSupernode J = S[blockIdx.x];
+  const double* t = val + J.voff;
+  for (int64_t c = threadIdx.x; c < J.w; c += blockDim.x) {
+    out[2 * (J.first + c)] = LDL_AT(t, J.m, c, c);
+    out[2 * (J.first + c) + 1] = piv[J.first + c] == PIV_FIRST ? LDL_AT(t, J.m, c + 1, c) : 0.0;
+  }
+}
+
 // The forward pass of the solve (solve_group, cholesky.rs:904-981, with LDL^T's unit L, block
 // diagonal D and pivot orders) level by level on one right-hand side x (postorder, in place).
 // First the descendants' contributions to one level's unknowns, read as a gather: workgroup t
@@ -615,10 +627,54 @@ __global__ __launch_bounds__(256) void k_ldlt_fwd_own(const int64_t* __restrict_
   for (int64_t c = threadIdx.x; c < w; c += blockDim.x) own[c] = y[c];
 }
 
+// The column of the backward pass at pivot position c: it finishes unknown q from the rows
+// end.. (cholesky.rs:982-1030 with 2 x 2 pivots).
+__device__ __forceinline__ void ldl_bwd_col(const int8_t* pv, int64_t c, int64_t& q, int64_t& end) {
+  if (pv[c] == PIV_SECOND && c > 0) {
+    q = c - 1;
+    end = c + 1;
+  } else if (pv[c] == PIV_FIRST) {
+    q = c + 1;
+    end = c + 2;
+  } else {
+    q = c;
+    end = c + 1;
+  }
+}
+
+// sum - b[0] - b[1] - ... - b[cnt - 1] in that order, the next 16 operands read from LDS while the
+// current 16 are used; b is 16-byte aligned and readable 16 entries past cnt.
+__device__ __forceinline__ double ldl_chain(double sum, const double* b, int cnt) {
+#pragma clang fp contract(off)
+  const int nb = cnt / 16;
+  const double2* p2 = reinterpret_cast<const double2*>(b);
+  double2 cur[8], nxt[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) cur[u] = p2[u];
+  for (int bt = 0; bt < nb; ++bt) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) nxt[u] = p2[(bt + 1) * 8 + u];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      sum -= cur[u].x;
+      sum -= cur[u].y;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) cur[u] = nxt[u];
+  }
+  for (int r = nb * 16; r < cnt; ++r) sum -= b[r];
+  return sum;
+}
+
+constexpr int LDL_BW_CAP = 3840;  // products of one column held in LDS (two such buffers)
+
 // The backward pass (cholesky.rs:982-1030) for the supernodes of one level (sn[blockIdx.x]),
 // levels from the root down: a supernode's unknowns are its own and its sums read only its
 // ancestors' (final) unknowns, so the supernodes of a level are independent. Each sum runs in
-// the reference's order on one thread (its loads issued ahead).
+// the reference's order on one lane. With m <= LDL_BW_CAP, waves 1.. stage the next column's
+// products while lane 0 runs the current column's sum: the one product that needs the unknown
+// being finished (its row is the current column's q) is left to lane 0, which forms it first.
+// Larger supernodes stage each column in chunks before its sum.
 __global__ __launch_bounds__(256) void k_ldlt_solve_bwd(const int64_t* __restrict__ sn, const LdlSupernode* __restrict__ S,
                                                        const int64_t* __restrict__ rows_all, const double* __restrict__ val,
                                                        const int8_t* __restrict__ piv, const int32_t* __restrict__ order,
@@ -632,55 +688,61 @@ __global__ __launch_bounds__(256) void k_ldlt_solve_bwd(const int64_t* __restric
   const int64_t* rows = rows_all + K.rows;
   double* own = x + K.first;
   double* y = ywork + K.first;
+  __shared__ __align__(16) double P[2][LDL_BW_CAP + 16];
+  __shared__ double Lr[2];      // L(late row, q) of the staged column
+  __shared__ int64_t late[2];   // position of its late product, or -1
+  __shared__ double acc;
   for (int64_t c = threadIdx.x; c < w; c += blockDim.x) y[c] = own[c];
   __syncthreads();
-  __shared__ __align__(16) double sp[2048 + 16];  // 16 past the end: the pipelined reads' last batch
-  __shared__ double acc;
-  for (int64_t c = w - 1; c >= 0; --c) {
-    int64_t q, end;
-    if (pv[c] == PIV_SECOND && c > 0) {
-      q = c - 1;
-      end = c + 1;
-    } else if (pv[c] == PIV_FIRST) {
-      q = c + 1;
-      end = c + 2;
-    } else {
-      q = c;
-      end = c + 1;
+  if (m <= LDL_BW_CAP && blockDim.x > 64) {
+    {
+      int64_t q, end;
+      ldl_bwd_col(pv, w - 1, q, end);  // end = w: every product reads an ancestor's unknown
+      const double* col = t + ldl_offset(m, q) - q;
+      for (int64_t r = end + threadIdx.x; r < m; r += blockDim.x) P[0][r - end] = col[r] * (r < w ? y[r] : x[rows[r]]);
+      if (threadIdx.x == 0) late[0] = -1;
     }
-    const double* col = t + ldl_offset(m, q) - q;  // col[r] = L(r, q)
-    if (threadIdx.x == 0) acc = y[q];
-    // rows end..m in chunks: the products by all lanes, their subtraction by one, in row order
-    for (int64_t r0 = end; r0 < m; r0 += 2048) {
-      const int64_t r1 = r0 + 2048 < m ? r0 + 2048 : m;
-      for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) sp[r - r0] = col[r] * (r < w ? y[r] : x[rows[r]]);
-      __syncthreads();
+    __syncthreads();
+    for (int64_t c = w - 1; c >= 0; --c) {
+      const int cur = (int)((w - 1 - c) & 1), nx = cur ^ 1;
+      int64_t q, end;
+      ldl_bwd_col(pv, c, q, end);
       if (threadIdx.x == 0) {
-        // the chain of subtractions, its next 16 operands read while the current 16 are used
-        double sum = acc;
-        const int cnt = (int)(r1 - r0), nb = cnt / 16;
-        const double2* p2 = reinterpret_cast<const double2*>(sp);
-        double2 cur[8], nxt[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) cur[u] = p2[u];
-        for (int bt = 0; bt < nb; ++bt) {
-#pragma unroll
-          for (int u = 0; u < 8; ++u) nxt[u] = p2[(bt + 1) * 8 + u];
-#pragma unroll
-          for (int u = 0; u < 8; ++u) {
-            sum -= cur[u].x;
-            sum -= cur[u].y;
+        if (late[cur] >= 0) P[cur][late[cur]] = Lr[cur] * y[end + late[cur]];
+        y[q] = ldl_chain(y[q], P[cur], (int)(m - end));
+      } else if (threadIdx.x >= 64 && c > 0) {
+        int64_t qn, en;
+        ldl_bwd_col(pv, c - 1, qn, en);
+        const double* col = t + ldl_offset(m, qn) - qn;
+        for (int64_t r = en + (threadIdx.x - 64); r < m; r += blockDim.x - 64)
+          if (r != q) P[nx][r - en] = col[r] * (r < w ? y[r] : x[rows[r]]);
+        if (threadIdx.x == 64) {
+          if (q >= en) {
+            late[nx] = q - en;
+            Lr[nx] = col[q];
+          } else {
+            late[nx] = -1;
           }
-#pragma unroll
-          for (int u = 0; u < 8; ++u) cur[u] = nxt[u];
         }
-        for (int r = nb * 16; r < cnt; ++r) sum -= sp[r];
-        acc = sum;
       }
       __syncthreads();
     }
-    if (threadIdx.x == 0) y[q] = acc;
-    __syncthreads();
+  } else {
+    for (int64_t c = w - 1; c >= 0; --c) {
+      int64_t q, end;
+      ldl_bwd_col(pv, c, q, end);
+      const double* col = t + ldl_offset(m, q) - q;  // col[r] = L(r, q)
+      if (threadIdx.x == 0) acc = y[q];
+      for (int64_t r0 = end; r0 < m; r0 += LDL_BW_CAP) {
+        const int64_t r1 = r0 + LDL_BW_CAP < m ? r0 + LDL_BW_CAP : m;
+        for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) P[0][r - r0] = col[r] * (r < w ? y[r] : x[rows[r]]);
+        __syncthreads();
+        if (threadIdx.x == 0) acc = ldl_chain(acc, P[0], (int)(r1 - r0));
+        __syncthreads();
+      }
+      if (threadIdx.x == 0) y[q] = acc;
+      __syncthreads();
+    }
   }
   for (int64_t c = threadIdx.x; c < w; c += blockDim.x) own[od[c]] = y[c];
 }

@@ -556,8 +556,15 @@ struct DevicePlan {
 
 }  // namespace
 
+namespace {
+struct LdltPlan;
+}
+
 struct fc_symbolic {
   fc::Symbolic s;
+  // LDL^T's host plan, built on the first factorization and shared by the later ones
+  mutable std::mutex ldlt_mu;
+  mutable std::shared_ptr<const LdltPlan> ldlt_cache;
   Program prog;
   std::vector<int64_t> diag_src;  // per postorder column: the upper entry of its diagonal
   std::mutex mu;
@@ -1862,13 +1869,20 @@ int fc_ldlt_numeric(const fc_symbolic* h, const double* upper_val, uint64_t nnz,
     HIP_CHECK(hipSetDevice(device));
     HIP_CHECK(hipStreamCreateWithFlags(&F->stream, hipStreamNonBlocking));
     hipStream_t stq = F->stream;
-    LdltPlan P;
-    try {
-      P = ldlt_plan(s);
-    } catch (const std::runtime_error& e) {
-      plan_error = e.what();
-      return;
+    std::shared_ptr<const LdltPlan> Pp;
+    {
+      std::lock_guard<std::mutex> g(h->ldlt_mu);
+      if (!h->ldlt_cache) {
+        try {
+          h->ldlt_cache = std::make_shared<const LdltPlan>(ldlt_plan(s));
+        } catch (const std::runtime_error& e) {
+          plan_error = e.what();
+          return;
+        }
+      }
+      Pp = h->ldlt_cache;
     }
+    const LdltPlan& P = *Pp;
     F->vals = P.vals;
     F->max_w = P.max_w;
     // the largest magnitude of each row (= column) of A, in postorder (cholesky.rs:1399-1414; a
@@ -1948,14 +1962,19 @@ int fc_ldlt_numeric(const fc_symbolic* h, const double* upper_val, uint64_t nnz,
     }
     if (failed < 0) {
       // inertia from D (ldlt.rs:412-445)
-      std::vector<double> v((size_t)std::max<i64>(P.vals, 1));
+      std::vector<double> v((size_t)std::max<i64>(2 * s.n, 1));
       std::vector<int8_t> pv((size_t)std::max<i64>(s.n, 1));
-      if (P.vals) HIP_CHECK(hipMemcpyAsync(v.data(), F->val, sizeof(double) * P.vals, hipMemcpyDeviceToHost, stq));
+      double* ddiag = dalloc<double>((size_t)std::max<i64>(2 * s.n, 1));
+      tmp.p.push_back(ddiag);
+      if (F->ns) fc::k_ldlt_diag<<<(unsigned)F->ns, 64, 0, stq>>>(F->S, F->val, F->piv, ddiag);
+      HIP_CHECK(hipGetLastError());
+      if (s.n) HIP_CHECK(hipMemcpyAsync(v.data(), ddiag, sizeof(double) * 2 * s.n, hipMemcpyDeviceToHost, stq));
       if (s.n) HIP_CHECK(hipMemcpyAsync(pv.data(), F->piv, s.n, hipMemcpyDeviceToHost, stq));
       HIP_CHECK(hipStreamSynchronize(stq));
       for (i64 k = 0; k < F->ns; ++k) {
         const fc::LdlSupernode& J = P.sn[k];
-        auto d = [&](i64 r, i64 c) { return v[J.voff + c * J.m - c * (c > 0 ? c - 1 : 0) / 2 + r - c]; };
+        // D(r, c) for r = c or c + 1 from the extracted blocks
+        auto d = [&](i64 r, i64 c) { return r == c ? v[2 * (J.first + c)] : v[2 * (J.first + c) + 1]; };
         i64 c = 0;
         while (c < J.w) {
           if (pv[J.first + c] == fc::PIV_FIRST) {
