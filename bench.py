@@ -721,8 +721,10 @@ def ldlt_workload(args, CH, P):
                 factor_s=mean["factor"], solve_s=mean["solve"], inertia=list(inertia), negative_expected=below,
                 two_by_two=two, residual=res,
                 roofline=dict(bound="mfma", achieved=achieved, peak=PEAK["fp64"], unit="TFLOP/s",
-                              frac=achieved / PEAK["fp64"], traffic=None, kernel="k_ldlt_supernode",
-                              note="factor time includes the host plan"),
+                              frac=achieved / PEAK["fp64"], traffic=None, kernel="all LDL^T factor kernels",
+                              note="the Cholesky flop count of the same structure over the factor's wall time (host "
+                                   "plan cached by the warmup); LDL^T is bound by chains of dependent steps (pivot "
+                                   "searches, per-entry sums in a fixed order), not by flops or bytes"),
                 cpu_baseline=cpu)
     print(json.dumps(line), flush=True)
 
