@@ -280,11 +280,13 @@ void fc_assembly_free(fc_assembly* a);
 
 /* --- LDL^T ------------------------------------------------------------------------------ */
 
-/* LdltFactor (crates/solver/src/ldlt.rs:362-445): P A P^T = L D L^T with bounded Bunch-Kaufman
- * (rook) pivoting inside supernodes, on the Cholesky analysis `s` (fc_analyze), on the device.
- * The factor, its pivots and the solutions are the reference's bit for bit (one workgroup per
- * supernode, each entry's sums in the reference's order). FC_SINGULAR names the first column
- * in postorder with no acceptable pivot, as an original column (ldlt.rs:300-329). */
+/* LdltFactor::from_symbolic (crates/solver/src/ldlt.rs:378-391): P A P^T = L D L^T with bounded
+ * Bunch-Kaufman (rook) pivoting inside supernodes, on the Cholesky analysis `s` (fc_analyze), on
+ * the device; LdltFactor::new (ldlt.rs:374-376) is fc_analyze followed by this call. The factor,
+ * its pivots and the solutions are the reference's bit for bit (each entry's sums in the
+ * reference's order). FC_SINGULAR names the first column in postorder with no acceptable pivot,
+ * as an original column (ldlt.rs:300-329). The analysis keeps a host plan built by its first
+ * LDL^T factorization for the later ones. */
 typedef struct fc_ldlt fc_ldlt;
 int fc_ldlt_numeric(const fc_symbolic* s, const double* upper_val, uint64_t nnz, int32_t device, fc_ldlt** out,
                     fc_error* err);
