@@ -535,6 +535,14 @@ class LdltFactor:
         _raise(st, err)
         return x
 
+    def solve_many(self, bs) -> list:
+        """LdltFactor::solve_many (ldlt.rs:410-412): each the same bit for bit as solve; a wrong
+        length anywhere fails before any is solved."""
+        xs = [np.array(b, dtype=np.float64, copy=True) for b in bs]
+        if any(x.shape[0] != self.n for x in xs):
+            raise Internal("right-hand side length")
+        return [self.solve(x) for x in xs]
+
     def _counts(self):
         v = [C.c_uint64() for _ in range(3)]
         lib().fc_ldlt_inertia(self._h, *[C.byref(x) for x in v])
@@ -555,3 +563,79 @@ class LdltFactor:
         st = lib().fc_ldlt_export(self._h, out, e, C.byref(err))
         _raise(st, err)
         return out[:e]
+
+
+class Symmetric:
+    """Symmetric (crates/solver/src/ldlt.rs:453-560): a factored symmetric matrix, Cholesky or
+    LDL^T. ``kind`` is "definite" (Cholesky, NotPositiveDefinite otherwise), "indefinite" (LDL^T)
+    or "auto" (one analysis; Cholesky, and LDL^T on the same analysis when the matrix is not
+    positive definite). Both factors run on the device."""
+
+    def __init__(self, a: UpperColumns, max_entries: int = 2 ** 63 - 1, kind: str = "auto", device: int = 0):
+        if kind not in ("definite", "indefinite", "auto"):
+            raise InvalidArgument(f"unknown kind {kind!r}")
+        self.n = a.n
+        if kind == "indefinite":
+            self.factor = LdltFactor.new(a, max_entries, device=device)
+        elif kind == "definite":
+            self.factor = Factor.new(a, max_entries, device=device)
+        else:
+            symbolic = Symbolic(a, max_entries)
+            try:
+                self.factor = Factor(symbolic, a.val, device=device)
+            except NotPositiveDefinite:
+                self.factor = LdltFactor(symbolic, a.val, device=device)
+
+    @property
+    def indefinite(self) -> bool:
+        return isinstance(self.factor, LdltFactor)
+
+    def inertia(self) -> tuple:
+        """A Cholesky factor's matrix is positive definite (ldlt.rs:502-508)."""
+        return self.factor.inertia() if self.indefinite else (self.n, 0)
+
+    def solve(self, b) -> np.ndarray:
+        return self.factor.solve(b)
+
+    def solve_many(self, bs) -> list:
+        return self.factor.solve_many(bs)
+
+    def solve_refined(self, n: int, ptr, idx, val, b, steps: int) -> np.ndarray:
+        """solve_refined (ldlt.rs:526-558): solve, then while the residual's largest magnitude
+        shrinks, at most ``steps`` times, x += solve(b - A x) (iterative refinement in working
+        precision, Higham 2002, sec. 12.2). A is the factored matrix by full rows; its product
+        sums each row in column order, as the reference's Csr::mul. Cholesky solutions are
+        returned as solved."""
+        rhs = np.array(b, dtype=np.float64, copy=True)
+        x = self.solve(rhs)
+        if not self.indefinite:
+            return x
+        ptr = np.asarray(ptr, np.int64)
+        idx = np.asarray(idx, np.int64)
+        val = np.asarray(val, np.float64)
+
+        def norm(v):  # the largest magnitude, NaN ignored (f64::max)
+            return float(np.fmax.reduce(np.abs(v), initial=0.0))
+
+        def residual(v):
+            ax = np.zeros(n)
+            prod = val * v[idx]
+            # row sums in column order: one term per pass over the rows' k-th entries
+            lens = np.diff(ptr)
+            for k in range(int(lens.max()) if n else 0):
+                sel = ptr[:-1] + k < ptr[1:]
+                ax[sel] = ax[sel] + prod[ptr[:-1][sel] + k]
+            return rhs - ax
+
+        r = residual(x)
+        size = norm(r)
+        for _ in range(steps):
+            if not size > 0.0:
+                break
+            trial = x + self.solve(r)
+            nxt = residual(trial)
+            nsize = norm(nxt)
+            if not nsize < size:
+                break
+            x, r, size = trial, nxt, nsize
+        return x

@@ -118,3 +118,78 @@ def test_wide_supernodes_match_the_oracle_bit_for_bit(gpu, case):
     x = f.solve(b)
     assert np.array_equal(x.view(np.uint64), o.solve(b).view(np.uint64))
     assert np.abs(O.csr_mul(n, ptr, idx, val, x) - b).max() < 1e-8 * np.abs(b).max() * max(1.0, np.abs(val).max())
+
+
+def test_several_right_hand_sides_solve_as_each_alone(gpu):
+    """ldlt.rs:852-884: solve_many gives each right-hand side the bits of solving it alone; a
+    wrong length fails before anything is solved."""
+    cases = [rows_to_csr([[(0, 4.0)]]), shifted_grid([4, 3, 3], -0.5)[0], shifted_grid([4, 3, 3], 2.7)[0],
+             shifted_grid([5, 4, 3], 7.1)[0]]
+    for a in cases:
+        n = a[0]
+        f = device_factor(a)
+        for k in (1, 2, 6):
+            rhs = [np.cos((np.arange(n) * 5 + s * 3) * 0.37) for s in range(k)]
+            alone = [f.solve(b) for b in rhs]
+            together = f.solve_many(rhs)
+            assert all(np.array_equal(x.view(np.uint64), y.view(np.uint64)) for x, y in zip(together, alone))
+        good, short = np.ones(n), np.full(max(n - 1, 0), 0.25)
+        with pytest.raises(CH.Internal):
+            f.solve_many([good, short])
+
+
+def test_solve_across_pivoted_supernodes_and_rhs_errors(gpu):
+    """ldlt.rs:886-958: two dense blocks sharing a separator, interleaved so that the postorder
+    differs from the input order, the first block needing an interchange and a 2 x 2 pivot; the
+    solves of several cases to 1e-10, repeated solves bit-identical, wrong lengths refused."""
+    rows = [[] for _ in range(7)]
+    for i, row in enumerate([[0.0, 0.1, 3.0], [0.1, 5.0, 0.2], [3.0, 0.2, 0.5]]):
+        rows[i] += [(j, v) for j, v in enumerate(row)]
+    for i in range(3, 6):
+        rows[i] += [(j, 4.0 if i == j else 0.25) for j in range(3, 6)]
+    for i in range(6):
+        rows[i].append((6, 0.15))
+        rows[6].append((i, 0.15))
+    rows[6].append((6, 8.0))
+    order = [3, 0, 4, 1, 5, 2, 6]
+    inverse = [order.index(i) for i in range(7)]
+    mixed = rows_to_csr([[(inverse[j], v) for j, v in rows[i]] for i in order])
+    cases = [rows_to_csr([[(0, 4.0)]]), rows_to_csr([[(0, -4.0)]]), mixed, shifted_grid([4, 3, 3], -0.5)[0],
+             shifted_grid([4, 3, 3], 2.7)[0], shifted_grid([4, 3, 3], 7.1)[0]]
+    for a in cases:
+        n, ptr, idx, val = a
+        f = device_factor(a)
+        o = O.LdltFactor(n, *O.from_symmetric(n, ptr, idx, val))
+        assert f.two_by_two() == o.two_by_two()
+        for seed in (0.0, 0.5, 3.25):
+            expected = np.sin((np.arange(n) + seed) * 0.37)
+            rhs = O.csr_mul(n, ptr, idx, val, expected)
+            x = f.solve(rhs)
+            assert np.abs(x - expected).max() < 1e-10
+            assert np.array_equal(x.view(np.uint64), f.solve(rhs).view(np.uint64))
+            assert np.array_equal(x.view(np.uint64), o.solve(rhs).view(np.uint64))
+        for length in (n + 1, n - 1):
+            before = np.full(length, 0.25)
+            b = before.copy()
+            with pytest.raises(CH.FactorError):
+                f.solve(b)
+            assert np.array_equal(b, before)
+    assert device_factor(mixed).two_by_two() >= 1
+
+
+def test_auto_selection_and_refinement(gpu):
+    """ldlt.rs:960-982: Kind::Auto picks Cholesky for a definite grid and LDL^T (on the same
+    analysis) for an indefinite one, whose inertia it reports; solve_refined to 1e-12; Definite
+    refuses the indefinite matrix."""
+    spd, _ = shifted_grid([4, 4, 4], -1.0)
+    s = CH.Symmetric(CH.UpperColumns.from_symmetric(*spd), kind="auto")
+    assert not s.indefinite and s.inertia() == (spd[0], 0)
+    ind, below = shifted_grid([4, 4, 4], 4.4)
+    n, ptr, idx, val = ind
+    s = CH.Symmetric(CH.UpperColumns.from_symmetric(*ind), kind="auto")
+    assert s.indefinite and s.inertia()[1] == below
+    x = np.cos(np.arange(n, dtype=float))
+    b = O.csr_mul(n, ptr, idx, val, x)
+    assert np.abs(s.solve_refined(n, ptr, idx, val, b, 3) - x).max() < 1e-12
+    with pytest.raises(CH.NotPositiveDefinite):
+        CH.Symmetric(CH.UpperColumns.from_symmetric(*ind), kind="definite")
