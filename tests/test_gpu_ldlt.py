@@ -193,3 +193,22 @@ def test_auto_selection_and_refinement(gpu):
     assert np.abs(s.solve_refined(n, ptr, idx, val, b, 3) - x).max() < 1e-12
     with pytest.raises(CH.NotPositiveDefinite):
         CH.Symmetric(CH.UpperColumns.from_symmetric(*ind), kind="definite")
+
+
+def test_strongly_indefinite_stiffness_matches_the_oracle_bit_for_bit(gpu):
+    """The 9 720-equation hex8 cantilever stiffness less 0.618 of its median diagonal: about half
+    the pivots negative and hundreds of 2 x 2 pivots across panels and supernodes."""
+    from fieldcraft_amd import problems as P
+
+    p = P.cantilever_hex8()
+    diag = p.idx == np.repeat(np.arange(p.n), np.diff(p.ptr))
+    vals = p.kval - 0.6180339887 * float(np.median(p.kval[diag])) * diag
+    up = p.upper(vals)
+    f = CH.LdltFactor.new(CH.UpperColumns(p.n, *up))
+    o = O.LdltFactor(p.n, *up)
+    assert f.inertia() == o.inertia() and f.inertia()[1] > p.n // 3
+    assert f.two_by_two() == o.two_by_two() > 100
+    assert np.array_equal(f.values().view(np.uint64), o.values().view(np.uint64))
+    x = f.solve(p.rhs)
+    assert np.array_equal(x.view(np.uint64), o.solve(p.rhs).view(np.uint64))
+    assert np.abs(p.mul(x, vals) - p.rhs).max() < 1e-10 * np.abs(p.rhs).max()
