@@ -59,7 +59,7 @@ __device__ __forceinline__ double times_d_at(const Row& row, const double* t, in
 
 constexpr int LDL_YC = 16;   // target columns of one update window (and one tile)
 constexpr int LDL_KC = 128;  // source columns of Y staged in LDS at a time
-constexpr int LDL_PP = 4;    // pairs per thread per batch
+constexpr int LDL_PP = 8;    // pairs per thread per batch
 
 // own(rel(i), rel(c)) -= sum_{k < kw} L(p + i, k0 + k) Y(p + c, k), Y = L D (products and
 // subtract, cholesky.rs:531-686 with pivots), for c < min(wu, mu), c <= i < mu, restricted to the
