@@ -671,14 +671,14 @@ constexpr int LDL_BW_CAP = 3840;  // products of one column held in LDS (two suc
 // The backward pass (cholesky.rs:982-1030) for the supernodes of one level (sn[blockIdx.x]),
 // levels from the root down: a supernode's unknowns are its own and its sums read only its
 // ancestors' (final) unknowns, so the supernodes of a level are independent. Each sum runs in
-// the reference's order on one lane. With m <= LDL_BW_CAP, waves 1.. stage the next column's
+// the reference's order on one lane. With m <= cap (<= LDL_BW_CAP), waves 1.. stage the next column's
 // products while lane 0 runs the current column's sum: the one product that needs the unknown
 // being finished (its row is the current column's q) is left to lane 0, which forms it first.
 // Larger supernodes stage each column in chunks before its sum.
 __global__ __launch_bounds__(256) void k_ldlt_solve_bwd(const int64_t* __restrict__ sn, const LdlSupernode* __restrict__ S,
                                                        const int64_t* __restrict__ rows_all, const double* __restrict__ val,
                                                        const int8_t* __restrict__ piv, const int32_t* __restrict__ order,
-                                                       double* __restrict__ x, double* __restrict__ ywork) {
+                                                       double* __restrict__ x, double* __restrict__ ywork, int cap) {
 #pragma clang fp contract(off)
   const LdlSupernode K = S[sn[blockIdx.x]];
   const int64_t m = K.m, w = K.w;
@@ -688,13 +688,14 @@ __global__ __launch_bounds__(256) void k_ldlt_solve_bwd(const int64_t* __restric
   const int64_t* rows = rows_all + K.rows;
   double* own = x + K.first;
   double* y = ywork + K.first;
+  if (cap < 16 || cap > LDL_BW_CAP) cap = LDL_BW_CAP;
   __shared__ __align__(16) double P[2][LDL_BW_CAP + 16];
   __shared__ double Lr[2];      // L(late row, q) of the staged column
   __shared__ int64_t late[2];   // position of its late product, or -1
   __shared__ double acc;
   for (int64_t c = threadIdx.x; c < w; c += blockDim.x) y[c] = own[c];
   __syncthreads();
-  if (m <= LDL_BW_CAP && blockDim.x > 64) {
+  if (m <= cap && blockDim.x > 64) {
     {
       int64_t q, end;
       ldl_bwd_col(pv, w - 1, q, end);  // end = w: every product reads an ancestor's unknown
@@ -733,8 +734,8 @@ __global__ __launch_bounds__(256) void k_ldlt_solve_bwd(const int64_t* __restric
       ldl_bwd_col(pv, c, q, end);
       const double* col = t + ldl_offset(m, q) - q;  // col[r] = L(r, q)
       if (threadIdx.x == 0) acc = y[q];
-      for (int64_t r0 = end; r0 < m; r0 += LDL_BW_CAP) {
-        const int64_t r1 = r0 + LDL_BW_CAP < m ? r0 + LDL_BW_CAP : m;
+      for (int64_t r0 = end; r0 < m; r0 += cap) {
+        const int64_t r1 = r0 + cap < m ? r0 + cap : m;
         for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) P[0][r - r0] = col[r] * (r < w ? y[r] : x[rows[r]]);
         __syncthreads();
         if (threadIdx.x == 0) acc = ldl_chain(acc, P[0], (int)(r1 - r0));

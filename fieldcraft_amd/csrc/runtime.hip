@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <memory>
@@ -2036,6 +2037,13 @@ int fc_ldlt_solve(const fc_ldlt* f, double* b, uint64_t n, fc_error* err) {
     double* dz = dalloc<double>(n);
     HIP_CHECK(hipMemcpyAsync(dx, x.data(), sizeof(double) * n, hipMemcpyHostToDevice, f->stream));
     double* dyw = dalloc<double>(n);
+    // rows of one backward column staged in LDS at a time (FC_LDLT_SOLVE_ROWS, 16..3840: a tuning
+    // knob; larger supernodes take the chunked branch)
+    int cap = fc::LDL_BW_CAP;
+    if (const char* e = std::getenv("FC_LDLT_SOLVE_ROWS")) {
+      const long v = std::strtol(e, nullptr, 10);
+      if (v >= 16 && v <= fc::LDL_BW_CAP) cap = (int)v;
+    }
     for (size_t l = 0; l + 1 < f->lvl_ptr.size(); ++l) {
       const i64 nt = f->ftile_ptr[l + 1] - f->ftile_ptr[l], cnt = f->lvl_ptr[l + 1] - f->lvl_ptr[l];
       if (nt > 0)
@@ -2049,7 +2057,7 @@ int fc_ldlt_solve(const fc_ldlt* f, double* b, uint64_t n, fc_error* err) {
       const i64 cnt = f->lvl_ptr[l + 1] - f->lvl_ptr[l];
       if (cnt > 0)
         fc::k_ldlt_solve_bwd<<<(unsigned)cnt, 256, 0, f->stream>>>(f->sn + f->lvl_ptr[l], f->S, f->rows, f->val, f->piv,
-                                                                   f->order, dx, dyw);
+                                                                   f->order, dx, dyw, cap);
     }
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipMemcpyAsync(x.data(), dx, sizeof(double) * n, hipMemcpyDeviceToHost, f->stream));

@@ -212,3 +212,19 @@ def test_strongly_indefinite_stiffness_matches_the_oracle_bit_for_bit(gpu):
     x = f.solve(p.rhs)
     assert np.array_equal(x.view(np.uint64), o.solve(p.rhs).view(np.uint64))
     assert np.abs(p.mul(x, vals) - p.rhs).max() < 1e-10 * np.abs(p.rhs).max()
+
+
+@pytest.mark.parametrize("rows", ["16", "100"])
+def test_backward_solve_in_chunks_gives_the_same_bits(gpu, monkeypatch, rows):
+    """The backward pass stages at most FC_LDLT_SOLVE_ROWS rows of a column in LDS: supernodes
+    with more rows take the chunked branch, whose sums must be the same bits."""
+    a, _ = shifted_grid([14, 12, 10], 2.7)
+    n, ptr, idx, val = a
+    f = device_factor(a)
+    b = np.cos(np.arange(n) * 0.3)
+    whole = f.solve(b)
+    monkeypatch.setenv("FC_LDLT_SOLVE_ROWS", rows)
+    chunked = f.solve(b)
+    assert np.array_equal(whole.view(np.uint64), chunked.view(np.uint64))
+    o = O.LdltFactor(n, *O.from_symmetric(n, ptr, idx, val))
+    assert np.array_equal(chunked.view(np.uint64), o.solve(b).view(np.uint64))
