@@ -1510,3 +1510,393 @@ void orc_ldlt_inertia(const orc_factor* f, i64* positive, i64* negative, i64* tw
   *negative = neg;
   *two_by_two = f->two_by_two;
 }
+
+/* ------------------------------------------------------------------------------------------ */
+/* Complex-symmetric LDL^T (ldlt.rs over Complex64, scalar.rs:115-131): the same steps with
+ * num-complex 0.4's arithmetic (products (ac - bd, ad + bc), quotients by the squared norm,
+ * componentwise sums), the modulus hypot(re, im) for the pivot tests, transposes and never
+ * conjugates. The factor's values live in f->val as re, im pairs; no inertia. */
+
+typedef struct { double re, im; } zc;
+static inline zc zmk(double re, double im) { zc z = {re, im}; return z; }
+static inline zc zadd(zc a, zc b) { return zmk(a.re + b.re, a.im + b.im); }
+static inline zc zsub(zc a, zc b) { return zmk(a.re - b.re, a.im - b.im); }
+static inline zc zmul(zc a, zc b) { return zmk(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re); }
+static inline zc zdiv(zc a, zc b) {
+  double nrm = b.re * b.re + b.im * b.im;
+  return zmk((a.re * b.re + a.im * b.im) / nrm, (a.im * b.re - a.re * b.im) / nrm);
+}
+static inline double zmod(zc a) { return hypot(a.re, a.im); }
+
+static void ztimes_d(zc* row, i64 len, const zc* t, i64 m, i64 k0, const signed char* piv) {
+  i64 k = 0;
+  while (k < len) {
+    if (piv[k] == PIV_FIRST) {
+      zc a = row[k], b = k + 1 < len ? row[k + 1] : zmk(0.0, 0.0);
+      zc d11 = AT(t, m, k0 + k, k0 + k), d21 = AT(t, m, k0 + k + 1, k0 + k), d22 = AT(t, m, k0 + k + 1, k0 + k + 1);
+      row[k] = zadd(zmul(a, d11), zmul(b, d21));
+      if (k + 1 < len) row[k + 1] = zadd(zmul(a, d21), zmul(b, d22));
+      k += 2;
+    } else {
+      row[k] = zmul(row[k], AT(t, m, k0 + k, k0 + k));
+      k += 1;
+    }
+  }
+}
+
+static void zldlt_update(zc* own, i64 m, const zc* lk, i64 mk, i64 k0, i64 kw, const signed char* pk, i64 p, i64 mu,
+                         i64 wu, const i64* rel, int inner) {
+  i64 cols = wu < mu ? wu : mu;
+  zc* y = (zc*)malloc(sizeof(zc) * (size_t)(cols * kw > 0 ? cols * kw : 1));
+  for (i64 c = 0; c < cols; ++c) {
+    for (i64 k = 0; k < kw; ++k) y[c * kw + k] = AT(lk, mk, p + c, k0 + k);
+    ztimes_d(y + c * kw, kw, lk, mk, k0, pk);
+  }
+#pragma omp parallel for schedule(dynamic, 1) if (inner && mu * wu * kw >= PARALLEL_WORK)
+  for (i64 c = 0; c < cols; ++c) {
+    i64 cc = rel[c];
+    zc* target = own + offset(m, cc);
+    const zc* yc = y + c * kw;
+    for (i64 i = c; i < mu; ++i) {
+      zc s = zmk(0.0, 0.0);
+      for (i64 k = 0; k < kw; ++k) s = zadd(s, zmul(AT(lk, mk, p + i, k0 + k), yc[k]));
+      target[rel[i] - cc] = zsub(target[rel[i] - cc], s);
+    }
+  }
+  free(y);
+}
+
+static void zinterchange(zc* v, i64 m, i64 p, i64 q) {
+#define ZSWP(x, y) do { zc t_ = (x); (x) = (y); (y) = t_; } while (0)
+  for (i64 c = 0; c < p; ++c) ZSWP(AT(v, m, p, c), AT(v, m, q, c));
+  ZSWP(AT(v, m, p, p), AT(v, m, q, q));
+  for (i64 c = p + 1; c < q; ++c) ZSWP(AT(v, m, c, p), AT(v, m, q, c));
+  for (i64 r = q + 1; r < m; ++r) ZSWP(AT(v, m, r, p), AT(v, m, r, q));
+#undef ZSWP
+}
+
+static void zlargest(const zc* v, i64 lo, i64 hi, i64 skip, i64* at, double* value) {
+  i64 bi = lo;
+  double bv = 0.0;
+  for (i64 i = lo; i < hi; ++i) {
+    if (i == skip) continue;
+    double x = zmod(v[i]);
+    if (isnan(x)) { *at = i; *value = NAN; return; }
+    if (x > bv) { bi = i; bv = x; }
+  }
+  *at = bi;
+  *value = bv;
+}
+
+typedef struct {
+  zc* values;
+  signed char* piv;
+  i64* order;
+  i64 m, w;
+  zc *a, *b, *wrow;
+  i64 two_by_two;
+} zblock_t;
+
+static void zcurrent(zblock_t* B, i64 c0, i64 k, i64 c, zc* out) {
+  i64 m = B->m;
+  for (i64 r = k; r < m; ++r) out[r] = r >= c ? AT(B->values, m, r, c) : AT(B->values, m, c, r);
+  i64 len = k - c0;
+  for (i64 q = c0; q < k; ++q) B->wrow[q - c0] = AT(B->values, m, c, q);
+  ztimes_d(B->wrow, len, B->values, m, c0, B->piv + c0);
+  for (i64 q = c0; q < k; ++q) {
+    zc wq = B->wrow[q - c0];
+    for (i64 r = k; r < m; ++r) out[r] = zsub(out[r], zmul(AT(B->values, m, r, q), wq));
+  }
+}
+
+static void zbswap(zblock_t* B, i64 p, i64 q) {
+  if (p == q) return;
+  if (p > q) { i64 t = p; p = q; q = t; }
+  zinterchange(B->values, B->m, p, q);
+  i64 t = B->order[p]; B->order[p] = B->order[q]; B->order[q] = t;
+  zc x = B->a[p]; B->a[p] = B->a[q]; B->a[q] = x;
+  x = B->b[p]; B->b[p] = B->b[q]; B->b[q] = x;
+}
+
+static void zstore_one(zblock_t* B, i64 k, const zc* col) {
+  i64 m = B->m, start = offset(m, k);
+  zc d = col[k];
+  B->values[start] = d;
+  for (i64 r = k + 1; r < m; ++r) B->values[start + r - k] = zdiv(col[r], d);
+  B->piv[k] = PIV_ONE;
+}
+static void zstore_two(zblock_t* B, i64 k, const zc* a, const zc* b) {
+  i64 m = B->m;
+  zc d11 = a[k], d21 = a[k + 1], d22 = b[k + 1];
+  zc det = zsub(zmul(d11, d22), zmul(d21, d21));
+  i64 s0 = offset(m, k), s1 = offset(m, k + 1);
+  B->values[s0] = d11;
+  B->values[s0 + 1] = d21;
+  B->values[s1] = d22;
+  for (i64 r = k + 2; r < m; ++r) {
+    zc x = a[r], y = b[r];
+    B->values[s0 + r - k] = zdiv(zsub(zmul(x, d22), zmul(y, d21)), det);
+    B->values[s1 + r - k - 1] = zdiv(zsub(zmul(y, d11), zmul(x, d21)), det);
+  }
+  B->piv[k] = PIV_FIRST;
+  B->piv[k + 1] = PIV_SECOND;
+  B->two_by_two += 1;
+}
+
+static i64 zldlt_panel(zblock_t* B, i64 c0, const double* scale_local, i64* bad) {
+  i64 w = B->w;
+  i64 end = c0 + PANEL < w ? c0 + PANEL : w;
+  i64 k = c0;
+  while (k < end) {
+    zcurrent(B, c0, k, k, B->a);
+    double akk = zmod(B->a[k]);
+    i64 r0;
+    double omega_k;
+    zlargest(B->a, k + 1, w, -1, &r0, &omega_k);
+    if (!isfinite(akk) || !isfinite(omega_k)) { *bad = k; return -1; }
+    int two = 0;
+    i64 one_at = k, ti = k, tr = k;
+    if (!(akk >= ALPHA * omega_k)) {
+      i64 i = k, r = r0;
+      double omega_i = omega_k;
+      int step = 0;
+      for (;;) {
+        zcurrent(B, c0, k, r, B->b);
+        i64 s_at;
+        double omega_r;
+        zlargest(B->b, k, w, r, &s_at, &omega_r);
+        double arr = zmod(B->b[r]);
+        if (!isfinite(arr) || !isfinite(omega_r)) { *bad = k; return -1; }
+        step += 1;
+        if (arr >= ALPHA * omega_r) {
+          zc* t = B->a; B->a = B->b; B->b = t;
+          one_at = r;
+          break;
+        }
+        if (omega_r <= omega_i || step >= ROOK_STEPS) { two = 1; ti = i; tr = r; break; }
+        zc* t = B->a; B->a = B->b; B->b = t;
+        i = r; omega_i = omega_r; r = s_at;
+      }
+    }
+    if (!two) {
+      zbswap(B, k, one_at);
+      if (!(zmod(B->a[k]) > PIVOT_TOLERANCE * scale_local[B->order[k]])) { *bad = k; return -1; }
+      zstore_one(B, k, B->a);
+      k += 1;
+    } else {
+      zbswap(B, k, ti);
+      i64 r = tr == k ? ti : tr;
+      zbswap(B, k + 1, r);
+      double off = zmod(B->a[k + 1]);
+      double s0 = scale_local[B->order[k]], s1 = scale_local[B->order[k + 1]];
+      double size = s0 > s1 ? s0 : s1;
+      if (!(off > PIVOT_TOLERANCE * size)) { *bad = k; return -1; }
+      zstore_two(B, k, B->a, B->b);
+      k += 2;
+    }
+  }
+  return k;
+}
+
+static int zldlt_supernode(orc_factor* f, const zc* lval, i64 j, int inner, i64* failed_column) {
+  const i64* rows = s_rows(f, j);
+  i64 m = s_m(f, j), w = s_w(f, j), fc = s_first(f, j);
+  zc* val = (zc*)f->val;
+  zc* own = val + f->voff[j];
+  memset(own, 0, sizeof(zc) * trapezoid(m, w));
+  for (i64 c = 0; c < w; ++c) {
+    for (i64 t = f->pat.ptr[fc + c]; t < f->pat.ptr[fc + c + 1]; ++t) {
+      i64 r = f->pat.idx[t];
+      i64 lo = 0, hi = m;
+      while (lo < hi) { i64 mid = (lo + hi) / 2; if (rows[mid] < r) lo = mid + 1; else hi = mid; }
+      if (lo >= m || rows[lo] != r || lo < c) return ORC_INTERNAL;
+      own[offset(m, c) + lo - c] = lval[t];
+    }
+  }
+  i64* rel = (i64*)malloc(sizeof(i64) * (m ? m : 1));
+  for (i64 q = f->uptr[j]; q < f->uptr[j + 1]; ++q) {
+    update_t u = f->upd[q];
+    const i64* kr = s_rows(f, u.k);
+    i64 km = s_m(f, u.k), kw = s_w(f, u.k);
+    i64 mu = km - u.p1, wu = u.p2 - u.p1;
+    i64 at = 0;
+    for (i64 i = 0; i < mu; ++i) {
+      i64 r = kr[u.p1 + i];
+      while (at < m && rows[at] < r) ++at;
+      if (at >= m || rows[at] != r) { free(rel); return ORC_INTERNAL; }
+      rel[i] = at;
+    }
+    zldlt_update(own, m, val + f->voff[u.k], km, 0, kw, f->piv + s_first(f, u.k), u.p1, mu, wu, rel, inner);
+  }
+  zblock_t B;
+  B.values = own;
+  B.piv = f->piv + fc;
+  B.order = f->order + fc;
+  B.m = m;
+  B.w = w;
+  B.a = (zc*)calloc((size_t)(m ? m : 1), sizeof(zc));
+  B.b = (zc*)calloc((size_t)(m ? m : 1), sizeof(zc));
+  B.wrow = (zc*)malloc(sizeof(zc) * (PANEL + 2));
+  B.two_by_two = 0;
+  for (i64 c = 0; c < w; ++c) { B.piv[c] = PIV_ONE; B.order[c] = c; }
+  double* scale_local = f->scale + fc;
+  for (i64 i = 0; i < m; ++i) rel[i] = i;
+  int status = ORC_OK;
+  i64 c0 = 0;
+  while (c0 < w) {
+    i64 bad = -1;
+    i64 c1 = zldlt_panel(&B, c0, scale_local, &bad);
+    if (c1 < 0) { *failed_column = fc + B.order[bad]; status = ORC_SINGULAR; break; }
+    if (c1 < w) zldlt_update(own, m, own, m, c0, c1 - c0, B.piv + c0, c1, m - c1, w - c1, rel + c1, inner);
+    c0 = c1;
+  }
+  zc* pa = B.a < B.b ? B.a : B.b;
+  zc* pb = B.a < B.b ? B.b : B.a;
+  free(pa); free(pb); free(B.wrow); free(rel);
+#pragma omp atomic
+  f->two_by_two += B.two_by_two;
+  return status;
+}
+
+/* LdltFactor::<Complex64>::from_symbolic: aval holds nnz (re, im) pairs of the upper columns */
+int orc_zldlt_numeric(orc_factor* f, const double* aval_pairs, i64* column_out) {
+  const zc* aval = (const zc*)aval_pairs;
+  i64 ns = f->ns, n = f->n, nnz = f->pat.nnz;
+  zc* lval = (zc*)malloc(sizeof(zc) * (nnz ? nnz : 1));
+  for (i64 e = 0; e < nnz; ++e) lval[f->pat.slot[e]] = aval[e];
+  free(f->scale);
+  f->scale = (double*)calloc(n ? n : 1, sizeof(double));
+  for (i64 e = 0; e < nnz; ++e) {
+    double x = zmod(aval[e]);
+    i64 r = f->pat.crow[e], c = f->pat.ccol[e];
+    if (x > f->scale[r]) f->scale[r] = x;
+    if (x > f->scale[c]) f->scale[c] = x;
+  }
+  free(f->voff); free(f->val); free(f->piv); free(f->order);
+  f->voff = (i64*)malloc(sizeof(i64) * (ns + 1));
+  f->voff[0] = 0;
+  for (i64 s2 = 0; s2 < ns; ++s2) f->voff[s2 + 1] = f->voff[s2] + trapezoid(s_m(f, s2), s_w(f, s2));
+  f->val = (double*)calloc(2 * (f->voff[ns] ? f->voff[ns] : 1), sizeof(double));
+  f->piv = (signed char*)calloc(n ? n : 1, 1);
+  f->order = (i64*)calloc(n ? n : 1, sizeof(i64));
+  f->two_by_two = 0;
+  f->have_values = 0;
+  i64 nt = f->ntasks;
+  i64* fail_col = (i64*)malloc(sizeof(i64) * (nt ? nt : 1));
+  int* fail_kind = (int*)calloc(nt ? nt : 1, sizeof(int));
+  int broken = ORC_OK;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (i64 t = 0; t < nt; ++t) {
+    for (i64 j = f->tasks[2 * t]; j < f->tasks[2 * t + 1]; ++j) {
+      i64 col = NONE;
+      int e = zldlt_supernode(f, lval, j, 0, &col);
+      if (e == ORC_SINGULAR) { fail_kind[t] = e; fail_col[t] = col; break; }
+      if (e != ORC_OK) {
+#pragma omp critical
+        { if (broken == ORC_OK) broken = e; }
+        break;
+      }
+    }
+  }
+  if (broken != ORC_OK) { free(fail_col); free(fail_kind); free(lval); return broken; }
+  i64 earliest = NONE;
+  for (i64 t = 0; t < nt; ++t)
+    if (fail_kind[t] && (earliest == NONE || fail_col[t] < earliest)) earliest = fail_col[t];
+  free(fail_col); free(fail_kind);
+  for (i64 q = 0; q < f->ntop; ++q) {
+    i64 j = f->top[q];
+    if (earliest != NONE && f->first[j] > earliest) break;
+    i64 col = NONE;
+    int e = zldlt_supernode(f, lval, j, g_parallel_inner, &col);
+    if (e == ORC_SINGULAR) { free(lval); *column_out = f->perm[col]; return e; }
+    if (e != ORC_OK) { free(lval); return e; }
+  }
+  free(lval);
+  if (earliest != NONE) { *column_out = f->perm[earliest]; return ORC_SINGULAR; }
+  f->have_values = 1;
+  return ORC_OK;
+}
+
+int orc_zldlt_new(i64 n, const i64* ptr, const i64* idx, const double* val_pairs, i64 max_entries, orc_factor** out,
+                  i64* column_out, i64* entries_out) {
+  orc_factor* f = NULL;
+  int e = orc_analyze(n, ptr, idx, max_entries, &f, entries_out);
+  if (e != ORC_OK) return e;
+  e = orc_zldlt_numeric(f, val_pairs, column_out);
+  if (e != ORC_OK) { orc_free(f); return e; }
+  *out = f;
+  return ORC_OK;
+}
+
+static void zldlt_solve_one(const orc_factor* f, zc* b) {
+  i64 n = f->n;
+  const zc* val = (const zc*)f->val;
+  zc* x = (zc*)malloc(sizeof(zc) * (n ? n : 1));
+  zc* y = (zc*)malloc(sizeof(zc) * (n ? n : 1));
+  for (i64 i = 0; i < n; ++i) x[i] = b[f->perm[i]];
+  for (i64 k = 0; k < f->ns; ++k) {
+    const i64* rows = s_rows(f, k);
+    i64 m = s_m(f, k), fc = s_first(f, k), w = s_w(f, k);
+    const zc* t = val + f->voff[k];
+    const signed char* pv = f->piv + fc;
+    const i64* od = f->order + fc;
+    zc* own = x + fc;
+    for (i64 c = 0; c < w; ++c) y[c] = own[od[c]];
+    for (i64 q = 0; q < w; ++q) {
+      i64 end = q + (pv[q] == PIV_FIRST ? 2 : 1);
+      zc yq = y[q];
+      for (i64 r = end; r < w; ++r) y[r] = zsub(y[r], zmul(AT(t, m, r, q), yq));
+      for (i64 r = w; r < m; ++r) x[rows[r]] = zsub(x[rows[r]], zmul(AT(t, m, r, q), yq));
+    }
+    i64 c = 0;
+    while (c < w) {
+      if (pv[c] == PIV_FIRST) {
+        zc d11 = AT(t, m, c, c), d21 = AT(t, m, c + 1, c), d22 = AT(t, m, c + 1, c + 1);
+        zc det = zsub(zmul(d11, d22), zmul(d21, d21));
+        zc p = y[c], q2 = y[c + 1];
+        y[c] = zdiv(zsub(zmul(d22, p), zmul(d21, q2)), det);
+        y[c + 1] = zdiv(zsub(zmul(d11, q2), zmul(d21, p)), det);
+        c += 2;
+      } else {
+        y[c] = zdiv(y[c], AT(t, m, c, c));
+        c += 1;
+      }
+    }
+    for (i64 c2 = 0; c2 < w; ++c2) own[c2] = y[c2];
+  }
+  for (i64 k = f->ns - 1; k >= 0; --k) {
+    const i64* rows = s_rows(f, k);
+    i64 m = s_m(f, k), fc = s_first(f, k), w = s_w(f, k);
+    const zc* t = val + f->voff[k];
+    const signed char* pv = f->piv + fc;
+    const i64* od = f->order + fc;
+    zc* own = x + fc;
+    for (i64 c = 0; c < w; ++c) y[c] = own[c];
+    for (i64 c = w - 1; c >= 0; --c) {
+      i64 q, end;
+      if (pv[c] == PIV_SECOND && c > 0) { q = c - 1; end = c + 1; }
+      else if (pv[c] == PIV_FIRST) { q = c + 1; end = c + 2; }
+      else { q = c; end = c + 1; }
+      zc sum = y[q];
+      for (i64 r = end; r < w; ++r) sum = zsub(sum, zmul(AT(t, m, r, q), y[r]));
+      for (i64 r = w; r < m; ++r) sum = zsub(sum, zmul(AT(t, m, r, q), x[rows[r]]));
+      y[q] = sum;
+    }
+    for (i64 pos = 0; pos < w; ++pos) own[od[pos]] = y[pos];
+  }
+  for (i64 i = 0; i < n; ++i) b[f->perm[i]] = x[i];
+  free(x);
+  free(y);
+}
+
+int orc_zldlt_solve(const orc_factor* f, double* bs_pairs, i64 nrhs) {
+  if (!f->have_values || !f->piv) return ORC_INTERNAL;
+#pragma omp parallel for schedule(dynamic, 1) if (nrhs > 1)
+  for (i64 r = 0; r < nrhs; ++r) zldlt_solve_one(f, (zc*)bs_pairs + r * f->n);
+  return ORC_OK;
+}
+
+/* the complex factor's trapezoids as (re, im) pairs, 2 * orc_entries(f) doubles */
+void orc_zexport_values(const orc_factor* f, double* out) {
+  if (f->val) memcpy(out, f->val, sizeof(double) * 2 * (size_t)f->voff[f->ns]);
+}

@@ -78,6 +78,13 @@ def lib():
         L.orc_ldlt_new.restype = C.c_int
         L.orc_ldlt_solve.argtypes = [C.c_void_p, _f64p, i64]
         L.orc_ldlt_solve.restype = C.c_int
+        L.orc_zldlt_new.argtypes = [i64, _i64p, _i64p, _f64p, i64, C.POINTER(C.c_void_p), C.POINTER(i64),
+                                    C.POINTER(i64)]
+        L.orc_zldlt_new.restype = C.c_int
+        L.orc_zldlt_solve.argtypes = [C.c_void_p, _f64p, i64]
+        L.orc_zldlt_solve.restype = C.c_int
+        L.orc_zexport_values.argtypes = [C.c_void_p, _f64p]
+        L.orc_zexport_values.restype = None
         L.orc_ldlt_inertia.argtypes = [C.c_void_p, C.POINTER(i64), C.POINTER(i64), C.POINTER(i64)]
         L.orc_solid_stiffness.argtypes = [C.c_int, _f64p, C.c_double, C.c_double, _f64p]
         L.orc_solid_stiffness.restype = C.c_int
@@ -354,3 +361,44 @@ class LdltFactor:
         out = np.zeros(max(L.orc_entries(self._h), 1))
         L.orc_export_values(self._h, out)
         return out[:L.orc_entries(self._h)]
+
+
+class ZLdltFactor:
+    """LdltFactor::<Complex64>::new (ldlt.rs:362-412 over scalar.rs's Complex64): the
+    complex-symmetric P A P^T = L D L^T (transposes, never conjugates) with the modulus for the
+    pivot tests; solve and two_by_two. ``val`` is complex128. Raises OracleError(SINGULAR, column)."""
+
+    def __init__(self, n, ptr, idx, val, max_entries=2**62):
+        L = lib()
+        self.n = n
+        self._h = C.c_void_p()
+        col, ent = C.c_int64(-1), C.c_int64(-1)
+        pairs = np.ascontiguousarray(np.asarray(val, np.complex128)).view(np.float64)
+        st = L.orc_zldlt_new(n, _a(ptr), _a(idx), pairs if pairs.size else np.zeros(2), int(max_entries),
+                             C.byref(self._h), C.byref(col), C.byref(ent))
+        if st != OK:
+            self._h = None
+            raise OracleError(st, col.value, ent.value)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().orc_free(self._h)
+            self._h = None
+
+    def solve(self, b):
+        x = np.array(b, dtype=np.complex128, copy=True)
+        if self.n:
+            lib().orc_zldlt_solve(self._h, x.view(np.float64), 1)
+        return x
+
+    def two_by_two(self) -> int:
+        p, q, t = C.c_int64(), C.c_int64(), C.c_int64()
+        lib().orc_ldlt_inertia(self._h, C.byref(p), C.byref(q), C.byref(t))
+        return int(t.value)
+
+    def values(self) -> np.ndarray:
+        L = lib()
+        e = L.orc_entries(self._h)
+        out = np.zeros(2 * max(e, 1))
+        L.orc_zexport_values(self._h, out)
+        return out[:2 * e].view(np.complex128)
