@@ -141,7 +141,8 @@ EXPORTS = ["fc_analyze", "fc_symbolic_info_get", "fc_symbolic_export", "fc_symbo
            "fc_factor_free", "fc_matrix_new", "fc_matrix_free", "fc_matrix_set_segments", "fc_solve_refined",
            "fc_solve_refined_device", "fc_solve_refined_many_device", "fc_assembly_new", "fc_assemble_device",
            "fc_assembly_info", "fc_assembly_free", "fc_lanczos_device", "fc_ldlt_numeric", "fc_ldlt_solve",
-           "fc_ldlt_inertia", "fc_ldlt_export", "fc_ldlt_free", "fc_version"]
+           "fc_ldlt_inertia", "fc_ldlt_export", "fc_ldlt_free", "fc_zldlt_numeric", "fc_zldlt_solve",
+           "fc_version"]
 
 #: kernel kinds of the numeric phase (FC_KIND_* in include/fc_chol.h)
 KINDS = ["extend_l", "extend_u", "update", "potrf", "trsm", "syrk", "reduce", "split"]
@@ -190,6 +191,8 @@ def lib():
         L.fc_ldlt_inertia.argtypes = [vp] + [C.POINTER(C.c_uint64)] * 3
         L.fc_ldlt_export.argtypes = [vp, _f64p, C.c_uint64, ep]
         L.fc_ldlt_free.argtypes = [vp]
+        L.fc_zldlt_numeric.argtypes = [vp, _f64p, C.c_uint64, C.c_int32, C.POINTER(vp), ep]
+        L.fc_zldlt_solve.argtypes = [vp, _f64p, C.c_uint64, ep]
         L.fc_ldlt_free.restype = None
         L.fc_version.restype = C.c_char_p
         _lib = L
@@ -563,6 +566,66 @@ class LdltFactor:
         st = lib().fc_ldlt_export(self._h, out, e, C.byref(err))
         _raise(st, err)
         return out[:e]
+
+
+class ZLdltFactor:
+    """LdltFactor::<Complex64> (crates/solver/src/ldlt.rs over scalar.rs's Complex64) on the GPU:
+    the complex-symmetric A = A^T factored as P A P^T = L D L^T with the real path's pivoting,
+    the modulus for its tests and num-complex's arithmetic (transposes, never conjugates), on a
+    Cholesky analysis of the pattern: ``solve``, ``solve_many``, ``two_by_two``, ``values``.
+    Raises ``Singular`` naming the original column."""
+
+    def __init__(self, symbolic: Symbolic, val, device: int = 0):
+        L = lib()
+        self.symbolic = symbolic
+        self.n = symbolic.n
+        self._L = L
+        self._h = C.c_void_p()
+        err = _Error()
+        pairs = np.ascontiguousarray(np.asarray(val, np.complex128)).view(np.float64)
+        st = L.fc_zldlt_numeric(symbolic._h, pairs if pairs.size else np.zeros(2), pairs.size // 2, int(device),
+                                C.byref(self._h), C.byref(err))
+        if st != _OK:
+            self._h = None
+            _raise(st, err)
+
+    @staticmethod
+    def new(n: int, ptr, idx, val, max_entries: int = 2 ** 63 - 1, **kw) -> "ZLdltFactor":
+        """LdltFactor::<Complex64>::new (ldlt.rs:374-376) from the upper columns (ptr, idx) and
+        their complex values."""
+        return ZLdltFactor(Symbolic(UpperColumns(n, np.asarray(ptr, np.int64), np.asarray(idx, np.int64),
+                                                np.zeros(len(idx))), max_entries), val, **kw)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._L.fc_ldlt_free(self._h)
+            self._h = None
+
+    def solve(self, b) -> np.ndarray:
+        x = np.array(b, dtype=np.complex128, copy=True)
+        err = _Error()
+        st = lib().fc_zldlt_solve(self._h, x.view(np.float64) if x.size else np.zeros(2), x.shape[0], C.byref(err))
+        _raise(st, err)
+        return x
+
+    def solve_many(self, bs) -> list:
+        xs = [np.array(b, dtype=np.complex128, copy=True) for b in bs]
+        if any(x.shape[0] != self.n for x in xs):
+            raise Internal("right-hand side length")
+        return [self.solve(x) for x in xs]
+
+    def two_by_two(self) -> int:
+        v = [C.c_uint64() for _ in range(3)]
+        lib().fc_ldlt_inertia(self._h, *[C.byref(x) for x in v])
+        return int(v[2].value)
+
+    def values(self) -> np.ndarray:
+        e = self.symbolic.info()["entries"]
+        out = np.zeros(2 * max(e, 1))
+        err = _Error()
+        st = lib().fc_ldlt_export(self._h, out, 2 * e, C.byref(err))
+        _raise(st, err)
+        return out[:2 * e].view(np.complex128)
 
 
 class Symmetric:

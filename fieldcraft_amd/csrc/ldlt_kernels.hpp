@@ -22,12 +22,41 @@ constexpr int LDL_PANEL = 64;                     // cholesky.rs:54
 constexpr double LDL_PIVOT_TOLERANCE = 1e-10;     // cholesky.rs:48
 enum : int8_t { PIV_ONE = 0, PIV_FIRST = 1, PIV_SECOND = 2 };
 
+// The complex scalar of complex-symmetric LDL^T (ldlt.rs over Complex64, scalar.rs:115-131) with
+// num-complex 0.4's arithmetic: products (ac - bd, ad + bc), quotients by the squared norm, sums
+// by parts; the modulus hypot(re, im) for the pivot tests. Transposes, never conjugates.
+struct zd {
+  double re, im;
+};
+__device__ __forceinline__ zd operator+(zd a, zd b) {
+#pragma clang fp contract(off)
+  return zd{a.re + b.re, a.im + b.im};
+}
+__device__ __forceinline__ zd operator-(zd a, zd b) {
+#pragma clang fp contract(off)
+  return zd{a.re - b.re, a.im - b.im};
+}
+__device__ __forceinline__ zd operator*(zd a, zd b) {
+#pragma clang fp contract(off)
+  return zd{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+__device__ __forceinline__ zd operator/(zd a, zd b) {
+#pragma clang fp contract(off)
+  const double nrm = b.re * b.re + b.im * b.im;
+  return zd{(a.re * b.re + a.im * b.im) / nrm, (a.im * b.re - a.re * b.im) / nrm};
+}
+__device__ __forceinline__ zd& operator+=(zd& a, zd b) { return a = a + b; }
+__device__ __forceinline__ zd& operator-=(zd& a, zd b) { return a = a - b; }
+__device__ __forceinline__ zd& operator/=(zd& a, zd b) { return a = a / b; }
+__device__ __forceinline__ double ldl_mod(double x) { return fabs(x); }
+__device__ __forceinline__ double ldl_mod(zd x) { return hypot(x.re, x.im); }
+
 struct LdlSupernode {
   int64_t voff;      // trapezoid
   int64_t rows;      // row structure (postorder columns) in the rows array
   int64_t first;     // first column (postorder)
   int64_t upd0, upd1;  // updates it receives
-  int64_t scratch;   // two work columns of m doubles
+  int64_t scratch;   // two work columns of m entries
   int32_t m, w;
 };
 
@@ -42,16 +71,16 @@ __device__ __forceinline__ int64_t ldl_offset(int64_t m, int64_t c) { return c *
 
 // Entry k of times_d(row) (cholesky.rs:460-484) for the row `row(j)` of a trapezoid t (m rows)
 // over its columns k0..: a 1 x 1 pivot scales, a 2 x 2 pivot mixes the pair with D's block.
-template <typename Row>
-__device__ __forceinline__ double times_d_at(const Row& row, const double* t, int64_t m, int64_t k0,
+template <typename T, typename Row>
+__device__ __forceinline__ T times_d_at(const Row& row, const T* t, int64_t m, int64_t k0,
                                              const int8_t* piv, int k, int len) {
 #pragma clang fp contract(off)
   if (piv[k] == PIV_FIRST) {
-    const double a = row(k), b = k + 1 < len ? row(k + 1) : 0.0;
+    const T a = row(k), b = k + 1 < len ? row(k + 1) : T{};
     return a * LDL_AT(t, m, k0 + k, k0 + k) + b * LDL_AT(t, m, k0 + k + 1, k0 + k);
   }
   if (piv[k] == PIV_SECOND && k > 0 && piv[k - 1] == PIV_FIRST) {
-    const double a = row(k - 1), b = row(k);
+    const T a = row(k - 1), b = row(k);
     return a * LDL_AT(t, m, k0 + k, k0 + k - 1) + b * LDL_AT(t, m, k0 + k, k0 + k);
   }
   return row(k) * LDL_AT(t, m, k0 + k, k0 + k);
@@ -67,7 +96,8 @@ constexpr int LDL_PP = 8;    // pairs per thread per batch
 // [rb, re): each (i, c) summed by one thread in k order from 0.0. The window's Y entries are
 // staged in LDS LDL_KC columns at a time; each thread carries LDL_PP sums across the stages.
 // All threads of the workgroup call it together.
-__device__ inline void ldl_update(double* own, int64_t m, const double* lk, int64_t mk, int64_t k0, int kw,
+template <typename T>
+__device__ inline void ldl_update(T* own, int64_t m, const T* lk, int64_t mk, int64_t k0, int kw,
                                   const int8_t* pk, int64_t p, int64_t mu, int64_t wu, const int32_t* rel, bool rel_id,
                                   int64_t rel0, int64_t cb, int64_t ce, int64_t rb, int64_t re) {
 #pragma clang fp contract(off)
@@ -93,7 +123,7 @@ __device__ inline void ldl_update(double* own, int64_t m, const double* lk, int6
   const int ncol = (int)(cz - ca);
   // pairs of column ca + t: rows max(ca + t, ia) .. iz; pre[t] = pairs before it
   __shared__ int64_t pre[LDL_YC + 1];
-  __shared__ double ys[LDL_YC * LDL_KC];
+  __shared__ T ys[LDL_YC * LDL_KC];
   if (threadIdx.x == 0) {
     int64_t acc = 0;
     for (int t = 0; t < ncol; ++t) {
@@ -109,11 +139,11 @@ __device__ inline void ldl_update(double* own, int64_t m, const double* lk, int6
   for (int64_t q0 = 0; q0 < pairs; q0 += (int64_t)blockDim.x * LDL_PP) {
     int64_t li[LDL_PP], tgt[LDL_PP];
     int yrow[LDL_PP];
-    double sum[LDL_PP];
+    T sum[LDL_PP];
 #pragma unroll
     for (int t = 0; t < LDL_PP; ++t) {
       const int64_t q = q0 + threadIdx.x + (int64_t)t * blockDim.x;
-      sum[t] = 0.0;
+      sum[t] = T{};
       yrow[t] = -1;
       li[t] = 0;
       tgt[t] = 0;
@@ -135,11 +165,11 @@ __device__ inline void ldl_update(double* own, int64_t m, const double* lk, int6
         const int t = e / kn, kk = e - t * kn, k = kb + kk;
         const int64_t r = p + ca + t;  // row of Y
         const int64_t dg = ldl_offset(mk, k0 + k);  // D(k0 + k, k0 + k)
-        const double a = lk[dg + r - (k0 + k)];
-        double y;
+        const T a = lk[dg + r - (k0 + k)];
+        T y;
         const int8_t pv = pk[k];
         if (pv == PIV_FIRST) {  // row(k) d11 + row(k + 1) d21
-          const double bb = k + 1 < kw ? LDL_AT(lk, mk, r, k0 + k + 1) : 0.0;
+          const T bb = k + 1 < kw ? LDL_AT(lk, mk, r, k0 + k + 1) : T{};
           y = a * lk[dg] + bb * lk[dg + 1];
         } else if (pv == PIV_SECOND && k > 0 && pk[k - 1] == PIV_FIRST) {  // row(k - 1) d21 + row(k) d22
           y = LDL_AT(lk, mk, r, k0 + k - 1) * LDL_AT(lk, mk, k0 + k, k0 + k - 1) + a * lk[dg];
@@ -152,9 +182,9 @@ __device__ inline void ldl_update(double* own, int64_t m, const double* lk, int6
 #pragma unroll
       for (int t = 0; t < LDL_PP; ++t) {
         if (yrow[t] < 0) continue;
-        const double* yv = ys + yrow[t] * LDL_KC;
+        const T* yv = ys + yrow[t] * LDL_KC;
         int64_t at = li[t];
-        double acc = sum[t];
+        T acc = sum[t];
         int64_t step = mk - k0 - kb - 1;  // entries of column k0 + k + 1 lie mk - k0 - k - 1 on
 #pragma unroll 4
         for (int kk = 0; kk < kn; ++kk) {
@@ -175,16 +205,18 @@ __device__ inline void ldl_update(double* own, int64_t m, const double* lk, int6
 
 // A's values into the trapezoids (the columns of A, cholesky.rs:1054-1068): every upper entry
 // has one slot; the trapezoids were zeroed.
-__global__ void k_ldlt_scatter(int64_t nnz, const double* __restrict__ aval, const int64_t* __restrict__ pos,
-                               double* __restrict__ val) {
+template <typename T>
+__global__ void k_ldlt_scatter(int64_t nnz, const T* __restrict__ aval, const int64_t* __restrict__ pos,
+                               T* __restrict__ val) {
 #pragma clang fp contract(off)
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e < nnz) val[pos[e]] = aval[e];
 }
 
-__global__ void k_ldlt_zero(int64_t n, double* __restrict__ v) {
+template <typename T>
+__global__ void k_ldlt_zero(int64_t n, T* __restrict__ v) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    v[i] = 0.0;
+    v[i] = T{};
 }
 
 // The descendants' updates of the supernodes of one level (the assemble of cholesky.rs:1069-1124
@@ -195,36 +227,37 @@ struct LdlTile {
   int64_t j, b, e;  // supernode, target columns [b, e)
   int64_t rb, re;   // target rows [rb, re)
 };
+template <typename T>
 __global__ __launch_bounds__(256) void k_ldlt_assemble(const LdlTile* __restrict__ tiles, const LdlSupernode* __restrict__ S,
                                                        const LdlUpdate* __restrict__ U, const int32_t* __restrict__ rel,
-                                                       double* __restrict__ val, const int8_t* __restrict__ piv) {
+                                                       T* __restrict__ val, const int8_t* __restrict__ piv) {
 #pragma clang fp contract(off)
-  const LdlTile T = tiles[blockIdx.x];
-  const LdlSupernode J = S[T.j];
-  double* own = val + J.voff;
+  const LdlTile tl = tiles[blockIdx.x];
+  const LdlSupernode J = S[tl.j];
+  T* own = val + J.voff;
   for (int64_t u = J.upd0; u < J.upd1; ++u) {
     const LdlUpdate up = U[u];
     const LdlSupernode K = S[up.k];
     ldl_update(own, J.m, val + K.voff, K.m, 0, K.w, piv + K.first, up.p1, K.m - up.p1, up.p2 - up.p1, rel + up.rel, false,
-               0, T.b, T.e, T.rb, T.re);
+               0, tl.b, tl.e, tl.rb, tl.re);
   }
 }
 
 // One supernode per workgroup (supernode list sn[blockIdx.x]), assembled: the panels
 // (ldlt.rs:256-358). fail[s] receives the failing pivot's postorder
 // column (or stays -1); two[s] the supernode's 2 x 2 pivots.
-template <bool ONE>
+template <typename T, bool ONE>
 __global__ __launch_bounds__(1024) void k_ldlt_supernode(const int64_t* __restrict__ sn, const LdlSupernode* __restrict__ S,
                                                         const LdlUpdate* __restrict__ U, const int32_t* __restrict__ rel,
-                                                        const double* __restrict__ scale, double* __restrict__ val,
+                                                        const double* __restrict__ scale, T* __restrict__ val,
                                                         int8_t* __restrict__ piv, int32_t* __restrict__ order,
-                                                        double* __restrict__ work, int64_t* __restrict__ fail,
+                                                        T* __restrict__ work, int64_t* __restrict__ fail,
                                                         int32_t* __restrict__ two, int64_t* __restrict__ state) {
 #pragma clang fp contract(off)
   const int64_t j = sn[blockIdx.x];
   const LdlSupernode J = S[j];
   const int64_t m = J.m, w = J.w;
-  double* own = val + J.voff;
+  T* own = val + J.voff;
   int8_t* pv = piv + J.first;
   int32_t* od = order + J.first;
   (void)U;
@@ -241,8 +274,8 @@ __global__ __launch_bounds__(1024) void k_ldlt_supernode(const int64_t* __restri
     if (threadIdx.x == 0) st[0] = st[1] = w;  // nothing left: the trailing update stays idle
     return;
   }
-  double* A = work + J.scratch;      // two work columns (swapped by pointer, as the reference's)
-  double* B = A + m;
+  T* A = work + J.scratch;      // two work columns (swapped by pointer, as the reference's)
+  T* B = A + m;
   const double* sc = scale + J.first;
   __shared__ double red_v[16];   // one entry per wave (up to 1024 threads)
   __shared__ int64_t red_i[16], red_n[16];
@@ -250,15 +283,15 @@ __global__ __launch_bounds__(1024) void k_ldlt_supernode(const int64_t* __restri
   __shared__ double shd[2];
   int twos = 0;
   // current(c) into out: column c rows k..m less the panel's factored columns c0..k (ldlt.rs:176-193)
-  __shared__ double ydc[LDL_PANEL + 2];
-  auto current = [&](int64_t c0, int64_t k, int64_t c, double* out) {
+  __shared__ T ydc[LDL_PANEL + 2];
+  auto current = [&](int64_t c0, int64_t k, int64_t c, T* out) {
     auto row = [&](int q) { return LDL_AT(own, m, c, c0 + q); };
     for (int64_t q = threadIdx.x; q < k - c0; q += blockDim.x)
       ydc[q] = times_d_at(row, own, m, c0, pv + c0, (int)q, (int)(k - c0));
     __syncthreads();
     const int64_t o0 = ldl_offset(m, c0) - c0;
     for (int64_t r = k + threadIdx.x; r < m; r += blockDim.x) {
-      double v = r >= c ? LDL_AT(own, m, r, c) : LDL_AT(own, m, c, r);
+      T v = r >= c ? LDL_AT(own, m, r, c) : LDL_AT(own, m, c, r);
       int64_t o = o0 + r;  // (r, q): the next column's entry lies m - q - 1 on
 #pragma unroll 8
       for (int64_t q = c0; q < k; ++q) {
@@ -271,12 +304,12 @@ __global__ __launch_bounds__(1024) void k_ldlt_supernode(const int64_t* __restri
   };
   // largest |v[i]| over [lo, hi) except skip (ldlt.rs:149-164): the first index of the largest,
   // or the first NaN's index with NaN
-  auto largest = [&](const double* v, int64_t lo, int64_t hi, int64_t skip, int64_t* at, double* value) {
+  auto largest = [&](const T* v, int64_t lo, int64_t hi, int64_t skip, int64_t* at, double* value) {
     double bv = 0.0;
     int64_t bi = -1, nan_i = -1;
     for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
       if (i == skip) continue;
-      const double x = fabs(v[i]);
+      const double x = ldl_mod(v[i]);
       if (isnan(x)) {
         if (nan_i < 0) nan_i = i;
       } else if (x > bv) {
@@ -336,7 +369,7 @@ __global__ __launch_bounds__(1024) void k_ldlt_supernode(const int64_t* __restri
     // between them, the rows below q
     const int64_t n1 = p, n2 = q - p - 1, n3 = m - q - 1;
     for (int64_t e = threadIdx.x; e < n1 + 1 + n2 + n3; e += blockDim.x) {
-      double *x, *y;
+      T *x, *y;
       if (e < n1) {
         x = &LDL_AT(own, m, p, e);
         y = &LDL_AT(own, m, q, e);
@@ -352,7 +385,7 @@ __global__ __launch_bounds__(1024) void k_ldlt_supernode(const int64_t* __restri
         x = &LDL_AT(own, m, r, p);
         y = &LDL_AT(own, m, r, q);
       }
-      const double t = *x;
+      const T t = *x;
       *x = *y;
       *y = t;
     }
@@ -360,7 +393,7 @@ __global__ __launch_bounds__(1024) void k_ldlt_supernode(const int64_t* __restri
       const int32_t t = od[p];
       od[p] = od[q];
       od[q] = t;
-      double x = A[p];
+      T x = A[p];
       A[p] = A[q];
       A[q] = x;
       x = B[p];
@@ -376,7 +409,7 @@ __global__ __launch_bounds__(1024) void k_ldlt_supernode(const int64_t* __restri
     int64_t k = c0;
     while (k < end) {
       current(c0, k, k, A);
-      const double akk = fabs(A[k]);
+      const double akk = ldl_mod(A[k]);
       int64_t r0;
       double omega_k;
       largest(A, k + 1, w, -1, &r0, &omega_k);
@@ -395,14 +428,14 @@ __global__ __launch_bounds__(1024) void k_ldlt_supernode(const int64_t* __restri
           int64_t s_at;
           double omega_r;
           largest(B, k, w, r, &s_at, &omega_r);
-          const double arr = fabs(B[r]);
+          const double arr = ldl_mod(B[r]);
           if (!isfinite(arr) || !isfinite(omega_r)) {
             failed = k;
             break;
           }
           step += 1;
           if (arr >= LDL_ALPHA * omega_r) {
-            double* t = A;
+            T* t = A;
             A = B;
             B = t;
             one_at = r;
@@ -414,7 +447,7 @@ __global__ __launch_bounds__(1024) void k_ldlt_supernode(const int64_t* __restri
             tr = r;
             break;
           }
-          double* t = A;
+          T* t = A;
           A = B;
           B = t;
           i = r;
@@ -425,12 +458,12 @@ __global__ __launch_bounds__(1024) void k_ldlt_supernode(const int64_t* __restri
       }
       if (!is_two) {
         bswap(k, one_at);
-        if (!(fabs(A[k]) > LDL_PIVOT_TOLERANCE * sc[od[k]])) {
+        if (!(ldl_mod(A[k]) > LDL_PIVOT_TOLERANCE * sc[od[k]])) {
           failed = k;
           break;
         }
         // one (ldlt.rs:207-221)
-        const double d = A[k];
+        const T d = A[k];
         const int64_t start = ldl_offset(m, k);
         for (int64_t r = k + threadIdx.x; r < m; r += blockDim.x) own[start + r - k] = r == k ? d : A[r] / d;
         if (threadIdx.x == 0) pv[k] = PIV_ONE;
@@ -440,18 +473,18 @@ __global__ __launch_bounds__(1024) void k_ldlt_supernode(const int64_t* __restri
         bswap(k, ti);
         const int64_t r = tr == k ? ti : tr;
         bswap(k + 1, r);
-        const double off = fabs(A[k + 1]);
+        const double off = ldl_mod(A[k + 1]);
         const double s0 = sc[od[k]], s1 = sc[od[k + 1]];
         if (!(off > LDL_PIVOT_TOLERANCE * (s0 > s1 ? s0 : s1))) {
           failed = k;
           break;
         }
         // two (ldlt.rs:224-253)
-        const double d11 = A[k], d21 = A[k + 1], d22 = B[k + 1];
-        const double det = d11 * d22 - d21 * d21;
+        const T d11 = A[k], d21 = A[k + 1], d22 = B[k + 1];
+        const T det = d11 * d22 - d21 * d21;
         const int64_t q0 = ldl_offset(m, k), q1 = ldl_offset(m, k + 1);
         for (int64_t rr = k + 2 + threadIdx.x; rr < m; rr += blockDim.x) {
-          const double x = A[rr], y = B[rr];
+          const T x = A[rr], y = B[rr];
           own[q0 + rr - k] = (x * d22 - y * d21) / det;
           own[q1 + rr - k - 1] = (y * d11 - x * d21) / det;
         }
@@ -521,18 +554,19 @@ __global__ void k_ldlt_state_init(const int64_t* __restrict__ sn, const LdlSuper
 // The trailing update after one panel (ldlt.rs:347-352) of a supernode factored one panel per
 // launch: workgroup t takes the target columns [tiles[t].b, tiles[t].e) that lie at or after the
 // panel's end.
+template <typename T>
 __global__ __launch_bounds__(256) void k_ldlt_trailing(const LdlTile* __restrict__ tiles, const LdlSupernode* __restrict__ S,
-                                                       double* __restrict__ val, const int8_t* __restrict__ piv,
+                                                       T* __restrict__ val, const int8_t* __restrict__ piv,
                                                        const int64_t* __restrict__ state) {
 #pragma clang fp contract(off)
-  const LdlTile T = tiles[blockIdx.x];
-  const LdlSupernode J = S[T.j];
-  const int64_t c0 = state[4 * T.j], c1 = state[4 * T.j + 1];
-  if (state[4 * T.j + 2] >= 0 || c1 >= J.w || c1 <= c0 || T.e <= c1) return;
+  const LdlTile tl = tiles[blockIdx.x];
+  const LdlSupernode J = S[tl.j];
+  const int64_t c0 = state[4 * tl.j], c1 = state[4 * tl.j + 1];
+  if (state[4 * tl.j + 2] >= 0 || c1 >= J.w || c1 <= c0 || tl.e <= c1) return;
   const int64_t m = J.m, w = J.w;
-  double* own = val + J.voff;
+  T* own = val + J.voff;
   ldl_update(own, m, own, m, c0, (int)(c1 - c0), piv + J.first + c0, c1, m - c1, w - c1, nullptr, true, c1,
-             T.b > c1 ? T.b : c1, T.e, T.rb, T.re);
+             tl.b > c1 ? tl.b : c1, tl.e, tl.rb, tl.re);
 }
 
 // D's blocks for the inertia: out[2 c] = D(c, c), out[2 c + 1] = D(c + 1, c) at the first column
@@ -553,28 +587,29 @@ __global__ void k_ldlt_diag(const LdlSupernode* __restrict__ S, const double* __
 // takes the columns [b, e) of supernode j and, update by update in increasing descendant order
 // (the order the one-pass sweep applies them), subtracts L_k(p1 + i, q) z_k(q) for each column q
 // of descendant k in order; z holds the descendants' forward values before D^-1.
+template <typename T>
 __global__ __launch_bounds__(256) void k_ldlt_fwd_gather(const LdlTile* __restrict__ tiles, const LdlSupernode* __restrict__ S,
                                                          const LdlUpdate* __restrict__ U, const int32_t* __restrict__ rel,
-                                                         const double* __restrict__ val, const double* __restrict__ z,
-                                                         double* __restrict__ x) {
+                                                         const T* __restrict__ val, const T* __restrict__ z,
+                                                         T* __restrict__ x) {
 #pragma clang fp contract(off)
-  const LdlTile T = tiles[blockIdx.x];
-  const LdlSupernode J = S[T.j];
+  const LdlTile tl = tiles[blockIdx.x];
+  const LdlSupernode J = S[tl.j];
   for (int64_t u = J.upd0; u < J.upd1; ++u) {
     const LdlUpdate up = U[u];
     const LdlSupernode K = S[up.k];
     const int32_t* rl = rel + up.rel;  // rl[i]: the column of J that row p1 + i of K lands on
     const int64_t cnt = up.p2 - up.p1;
     int64_t ia = 0, iz = cnt, hi = cnt;
-    while (ia < hi) { const int64_t mid = (ia + hi) / 2; if (rl[mid] < T.b) ia = mid + 1; else hi = mid; }
+    while (ia < hi) { const int64_t mid = (ia + hi) / 2; if (rl[mid] < tl.b) ia = mid + 1; else hi = mid; }
     hi = cnt;
     iz = ia;
-    while (iz < hi) { const int64_t mid = (iz + hi) / 2; if (rl[mid] < T.e) iz = mid + 1; else hi = mid; }
-    const double* lk = val + K.voff;
-    const double* zk = z + K.first;
+    while (iz < hi) { const int64_t mid = (iz + hi) / 2; if (rl[mid] < tl.e) iz = mid + 1; else hi = mid; }
+    const T* lk = val + K.voff;
+    const T* zk = z + K.first;
     for (int64_t i = ia + threadIdx.x; i < iz; i += blockDim.x) {
-      double* xi = x + J.first + rl[i];
-      double v = *xi;
+      T* xi = x + J.first + rl[i];
+      T v = *xi;
       int64_t at = up.p1 + i;  // L_k(p1 + i, 0); column q + 1's entry lies K.m - q - 1 on
       for (int64_t q = 0; q < K.w; ++q) {
         v -= lk[at] * zk[q];
@@ -589,24 +624,25 @@ __global__ __launch_bounds__(256) void k_ldlt_fwd_gather(const LdlTile* __restri
 // Then each supernode of the level (one workgroup each): its own unknowns in pivot order, the
 // unit lower triangle's forward substitution, z = the values before D^-1, D^-1 (1 x 1 and 2 x 2
 // blocks), the result left in pivot order in x. y is a work vector of n entries.
+template <typename T>
 __global__ __launch_bounds__(256) void k_ldlt_fwd_own(const int64_t* __restrict__ sn, const LdlSupernode* __restrict__ S,
-                                                      const double* __restrict__ val, const int8_t* __restrict__ piv,
-                                                      const int32_t* __restrict__ order, double* __restrict__ x,
-                                                      double* __restrict__ ywork, double* __restrict__ z) {
+                                                      const T* __restrict__ val, const int8_t* __restrict__ piv,
+                                                      const int32_t* __restrict__ order, T* __restrict__ x,
+                                                      T* __restrict__ ywork, T* __restrict__ z) {
 #pragma clang fp contract(off)
   const LdlSupernode K = S[sn[blockIdx.x]];
   const int64_t m = K.m, w = K.w;
-  const double* t = val + K.voff;
+  const T* t = val + K.voff;
   const int8_t* pv = piv + K.first;
   const int32_t* od = order + K.first;
-  double* own = x + K.first;
-  double* y = ywork + K.first;
-  double* zk = z + K.first;
+  T* own = x + K.first;
+  T* y = ywork + K.first;
+  T* zk = z + K.first;
   for (int64_t c = threadIdx.x; c < w; c += blockDim.x) y[c] = own[od[c]];
   __syncthreads();
   for (int64_t q = 0; q < w; ++q) {
     const int64_t end = q + (pv[q] == PIV_FIRST ? 2 : 1);
-    const double yq = y[q];
+    const T yq = y[q];
     for (int64_t r = end + threadIdx.x; r < w; r += blockDim.x) y[r] -= LDL_AT(t, m, r, q) * yq;
     __syncthreads();
   }
@@ -614,9 +650,9 @@ __global__ __launch_bounds__(256) void k_ldlt_fwd_own(const int64_t* __restrict_
   __syncthreads();
   for (int64_t c = threadIdx.x; c < w; c += blockDim.x) {
     if (pv[c] == PIV_FIRST) {
-      const double d11 = LDL_AT(t, m, c, c), d21 = LDL_AT(t, m, c + 1, c), d22 = LDL_AT(t, m, c + 1, c + 1);
-      const double det = d11 * d22 - d21 * d21;
-      const double p = y[c], q2 = y[c + 1];
+      const T d11 = LDL_AT(t, m, c, c), d21 = LDL_AT(t, m, c + 1, c), d22 = LDL_AT(t, m, c + 1, c + 1);
+      const T det = d11 * d22 - d21 * d21;
+      const T p = y[c], q2 = y[c + 1];
       y[c] = (d22 * p - d21 * q2) / det;
       y[c + 1] = (d11 * q2 - d21 * p) / det;
     } else if (pv[c] == PIV_ONE) {
@@ -666,6 +702,29 @@ __device__ __forceinline__ double ldl_chain(double sum, const double* b, int cnt
   return sum;
 }
 
+// The complex chain: sum - b[0] - ... - b[cnt - 1] by parts, one element per 16-byte read.
+__device__ __forceinline__ zd ldl_chain(zd sum, const zd* b, int cnt) {
+#pragma clang fp contract(off)
+  const int nb = cnt / 8;
+  const double2* p2 = reinterpret_cast<const double2*>(b);
+  double2 cur[8], nxt[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) cur[u] = p2[u];
+  for (int bt = 0; bt < nb; ++bt) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) nxt[u] = p2[(bt + 1) * 8 + u];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      sum.re -= cur[u].x;
+      sum.im -= cur[u].y;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) cur[u] = nxt[u];
+  }
+  for (int r = nb * 8; r < cnt; ++r) sum = sum - b[r];
+  return sum;
+}
+
 constexpr int LDL_BW_CAP = 3840;  // products of one column held in LDS (two such buffers)
 
 // The backward pass (cholesky.rs:982-1030) for the supernodes of one level (sn[blockIdx.x]),
@@ -675,31 +734,32 @@ constexpr int LDL_BW_CAP = 3840;  // products of one column held in LDS (two suc
 // products while lane 0 runs the current column's sum: the one product that needs the unknown
 // being finished (its row is the current column's q) is left to lane 0, which forms it first.
 // Larger supernodes stage each column in chunks before its sum.
+template <typename T>
 __global__ __launch_bounds__(256) void k_ldlt_solve_bwd(const int64_t* __restrict__ sn, const LdlSupernode* __restrict__ S,
-                                                       const int64_t* __restrict__ rows_all, const double* __restrict__ val,
+                                                       const int64_t* __restrict__ rows_all, const T* __restrict__ val,
                                                        const int8_t* __restrict__ piv, const int32_t* __restrict__ order,
-                                                       double* __restrict__ x, double* __restrict__ ywork, int cap) {
+                                                       T* __restrict__ x, T* __restrict__ ywork, int cap) {
 #pragma clang fp contract(off)
   const LdlSupernode K = S[sn[blockIdx.x]];
   const int64_t m = K.m, w = K.w;
-  const double* t = val + K.voff;
+  const T* t = val + K.voff;
   const int8_t* pv = piv + K.first;
   const int32_t* od = order + K.first;
   const int64_t* rows = rows_all + K.rows;
-  double* own = x + K.first;
-  double* y = ywork + K.first;
+  T* own = x + K.first;
+  T* y = ywork + K.first;
   if (cap < 16 || cap > LDL_BW_CAP) cap = LDL_BW_CAP;
-  __shared__ __align__(16) double P[2][LDL_BW_CAP + 16];
-  __shared__ double Lr[2];      // L(late row, q) of the staged column
+  __shared__ __align__(16) T P[2][LDL_BW_CAP + 16];
+  __shared__ T Lr[2];      // L(late row, q) of the staged column
   __shared__ int64_t late[2];   // position of its late product, or -1
-  __shared__ double acc;
+  __shared__ T acc;
   for (int64_t c = threadIdx.x; c < w; c += blockDim.x) y[c] = own[c];
   __syncthreads();
   if (m <= cap && blockDim.x > 64) {
     {
       int64_t q, end;
       ldl_bwd_col(pv, w - 1, q, end);  // end = w: every product reads an ancestor's unknown
-      const double* col = t + ldl_offset(m, q) - q;
+      const T* col = t + ldl_offset(m, q) - q;
       for (int64_t r = end + threadIdx.x; r < m; r += blockDim.x) P[0][r - end] = col[r] * (r < w ? y[r] : x[rows[r]]);
       if (threadIdx.x == 0) late[0] = -1;
     }
@@ -714,7 +774,7 @@ __global__ __launch_bounds__(256) void k_ldlt_solve_bwd(const int64_t* __restric
       } else if (threadIdx.x >= 64 && c > 0) {
         int64_t qn, en;
         ldl_bwd_col(pv, c - 1, qn, en);
-        const double* col = t + ldl_offset(m, qn) - qn;
+        const T* col = t + ldl_offset(m, qn) - qn;
         for (int64_t r = en + (threadIdx.x - 64); r < m; r += blockDim.x - 64)
           if (r != q) P[nx][r - en] = col[r] * (r < w ? y[r] : x[rows[r]]);
         if (threadIdx.x == 64) {
@@ -732,7 +792,7 @@ __global__ __launch_bounds__(256) void k_ldlt_solve_bwd(const int64_t* __restric
     for (int64_t c = w - 1; c >= 0; --c) {
       int64_t q, end;
       ldl_bwd_col(pv, c, q, end);
-      const double* col = t + ldl_offset(m, q) - q;  // col[r] = L(r, q)
+      const T* col = t + ldl_offset(m, q) - q;  // col[r] = L(r, q)
       if (threadIdx.x == 0) acc = y[q];
       for (int64_t r0 = end; r0 < m; r0 += cap) {
         const int64_t r1 = r0 + cap < m ? r0 + cap : m;

@@ -729,6 +729,7 @@ struct fc_ldlt {
   int32_t* order = nullptr;
   int64_t* sn = nullptr;           // supernodes by level
   std::vector<int64_t> lvl_ptr;
+  bool complex = false;            // values are (re, im) pairs: complex-symmetric LDL^T
   fc::LdlUpdate* U = nullptr;      // updates and their relative maps, for the forward gather
   int32_t* rel = nullptr;
   fc::LdlTile* ftiles = nullptr;   // the forward gather's tiles, ftile_ptr[l] .. ftile_ptr[l + 1] per level
@@ -1844,8 +1845,16 @@ extern "C" {
 
 const char* fc_version(void) { return VERSION; }
 
-int fc_ldlt_numeric(const fc_symbolic* h, const double* upper_val, uint64_t nnz, int32_t device, fc_ldlt** out,
-                    fc_error* err) {
+}  // extern "C"
+
+namespace {
+double host_mod(double x) { return std::fabs(x); }
+double host_mod(fc::zd x) { return std::hypot(x.re, x.im); }  // Complex64::norm
+
+template <typename T>
+int ldlt_numeric(const fc_symbolic* h, const T* upper_val, uint64_t nnz, int32_t device, fc_ldlt** out,
+                 fc_error* err) {
+  constexpr bool REAL = std::is_same<T, double>::value;
   if (out) *out = nullptr;
   if (!h || !out || (nnz && !upper_val)) {
     set_error(err, FC_INVALID_ARGUMENT, "bad arguments to fc_ldlt_numeric");
@@ -1857,6 +1866,7 @@ int fc_ldlt_numeric(const fc_symbolic* h, const double* upper_val, uint64_t nnz,
     return FC_INVALID_ARGUMENT;
   }
   std::unique_ptr<fc_ldlt> F(new fc_ldlt());
+  F->complex = !REAL;
   F->device = device;
   F->sym_keep = keep(h);
   F->sym = h;
@@ -1890,13 +1900,14 @@ int fc_ldlt_numeric(const fc_symbolic* h, const double* upper_val, uint64_t nnz,
     // NaN leaves it as it is, as f64::max does)
     std::vector<double> scale(std::max<i64>(s.n, 1), 0.0);
     for (i64 e = 0; e < (i64)nnz; ++e) {
-      const double x = std::fabs(upper_val[e]);
+      const double x = host_mod(upper_val[e]);
       if (x > scale[P.coord_r[e]]) scale[P.coord_r[e]] = x;
       if (x > scale[P.coord_c[e]]) scale[P.coord_c[e]] = x;
     }
     F->S = dupload(P.sn, stq);
     F->rows = dupload(s.rows, stq);
-    F->val = dalloc<double>((size_t)std::max<i64>(P.vals, 1));
+    F->val = dalloc<double>((size_t)std::max<i64>(P.vals, 1) * (sizeof(T) / sizeof(double)));
+    T* fval = reinterpret_cast<T*>(F->val);
     F->piv = dalloc<int8_t>((size_t)std::max<i64>(s.n, 1));
     F->order = dalloc<int32_t>((size_t)std::max<i64>(s.n, 1));
     fc::LdlUpdate* dU = dupload(P.up, stq);
@@ -1910,8 +1921,8 @@ int fc_ldlt_numeric(const fc_symbolic* h, const double* upper_val, uint64_t nnz,
     F->sn = dsn;
     F->lvl_ptr = P.lvl_ptr;
     double* dscale = dupload(scale, stq);
-    double* daval = dalloc<double>((size_t)std::max<uint64_t>(nnz, 1));
-    double* dwork = dalloc<double>((size_t)std::max<i64>(P.work, 1));
+    T* daval = dalloc<T>((size_t)std::max<uint64_t>(nnz, 1));
+    T* dwork = dalloc<T>((size_t)std::max<i64>(P.work, 1));
     int64_t* dfail = dalloc<int64_t>((size_t)std::max<i64>(F->ns, 1));
     int32_t* dtwo = dalloc<int32_t>((size_t)std::max<i64>(F->ns, 1));
     int64_t* dstate = dalloc<int64_t>((size_t)std::max<i64>(4 * F->ns, 1));
@@ -1921,9 +1932,9 @@ int fc_ldlt_numeric(const fc_symbolic* h, const double* upper_val, uint64_t nnz,
         for (void* q : p) dfree(q);
       }
     } tmp{{dpos, dscale, daval, dwork, dfail, dtwo, dstate}};
-    if (nnz) HIP_CHECK(hipMemcpyAsync(daval, upper_val, sizeof(double) * nnz, hipMemcpyHostToDevice, stq));
-    if (P.vals) fc::k_ldlt_zero<<<blocks_for(P.vals), 256, 0, stq>>>(P.vals, F->val);
-    if (nnz) fc::k_ldlt_scatter<<<(unsigned)((nnz + 255) / 256), 256, 0, stq>>>((i64)nnz, daval, dpos, F->val);
+    if (nnz) HIP_CHECK(hipMemcpyAsync(daval, upper_val, sizeof(T) * nnz, hipMemcpyHostToDevice, stq));
+    if (P.vals) fc::k_ldlt_zero<<<blocks_for(P.vals), 256, 0, stq>>>(P.vals, fval);
+    if (nnz) fc::k_ldlt_scatter<<<(unsigned)((nnz + 255) / 256), 256, 0, stq>>>((i64)nnz, daval, dpos, fval);
     fc::LdlTile* dtiles = dupload(P.tiles, stq);
     tmp.p.push_back(dtiles);
     fc::LdlTile* dptiles = dupload(P.pan_tiles, stq);
@@ -1933,19 +1944,19 @@ int fc_ldlt_numeric(const fc_symbolic* h, const double* upper_val, uint64_t nnz,
       const i64 nt = P.tile_ptr[l + 1] - P.tile_ptr[l];
       const int64_t* lsn = dsn + P.lvl_ptr[l];
       if (nt > 0)
-        fc::k_ldlt_assemble<<<(unsigned)nt, 256, 0, stq>>>(dtiles + P.tile_ptr[l], F->S, dU, drel, F->val, F->piv);
+        fc::k_ldlt_assemble<<<(unsigned)nt, 256, 0, stq>>>(dtiles + P.tile_ptr[l], F->S, dU, drel, fval, F->piv);
       // a level of few supernodes gives each 1024 threads; a crowded one 256
       if (narrow > 0)
-        fc::k_ldlt_supernode<false><<<(unsigned)narrow, narrow < 256 ? 1024 : 256, 0, stq>>>(lsn, F->S, dU, drel, dscale, F->val, F->piv,
+        fc::k_ldlt_supernode<T, false><<<(unsigned)narrow, narrow < 256 ? 1024 : 256, 0, stq>>>(lsn, F->S, dU, drel, dscale, fval, F->piv,
                                                                        F->order, dwork, dfail, dtwo, dstate);
       if (wide > 0) {
         fc::k_ldlt_state_init<<<(unsigned)wide, 256, 0, stq>>>(lsn + narrow, F->S, F->piv, F->order, dfail, dtwo, dstate);
         for (i64 i = P.pan_ptr[l]; i < P.pan_ptr[l + 1]; ++i) {
-          fc::k_ldlt_supernode<true><<<(unsigned)wide, 1024, 0, stq>>>(lsn + narrow, F->S, dU, drel, dscale, F->val,
+          fc::k_ldlt_supernode<T, true><<<(unsigned)wide, 1024, 0, stq>>>(lsn + narrow, F->S, dU, drel, dscale, fval,
                                                                       F->piv, F->order, dwork, dfail, dtwo, dstate);
           const i64 pt = P.pan_tile_ptr[i + 1] - P.pan_tile_ptr[i];
           if (pt > 0)
-            fc::k_ldlt_trailing<<<(unsigned)pt, 256, 0, stq>>>(dptiles + P.pan_tile_ptr[i], F->S, F->val, F->piv, dstate);
+            fc::k_ldlt_trailing<<<(unsigned)pt, 256, 0, stq>>>(dptiles + P.pan_tile_ptr[i], F->S, fval, F->piv, dstate);
         }
       }
     }
@@ -1961,7 +1972,7 @@ int fc_ldlt_numeric(const fc_symbolic* h, const double* upper_val, uint64_t nnz,
       if (fails[k] >= 0 && (failed < 0 || fails[k] < failed)) failed = fails[k];
       F->two_by_two += twos[k];
     }
-    if (failed < 0) {
+    if (REAL && failed < 0) {
       // inertia from D (ldlt.rs:412-445)
       std::vector<double> v((size_t)std::max<i64>(2 * s.n, 1));
       std::vector<int8_t> pv((size_t)std::max<i64>(s.n, 1));
@@ -2014,10 +2025,31 @@ int fc_ldlt_numeric(const fc_symbolic* h, const double* upper_val, uint64_t nnz,
   set_error(err, FC_OK, "");
   return FC_OK;
 }
+}  // namespace
 
-int fc_ldlt_solve(const fc_ldlt* f, double* b, uint64_t n, fc_error* err) {
+extern "C" {
+
+int fc_ldlt_numeric(const fc_symbolic* h, const double* upper_val, uint64_t nnz, int32_t device, fc_ldlt** out,
+                    fc_error* err) {
+  return ldlt_numeric<double>(h, upper_val, nnz, device, out, err);
+}
+
+int fc_zldlt_numeric(const fc_symbolic* h, const double* upper_val, uint64_t nnz, int32_t device, fc_ldlt** out,
+                     fc_error* err) {
+  return ldlt_numeric<fc::zd>(h, reinterpret_cast<const fc::zd*>(upper_val), nnz, device, out, err);
+}
+
+}  // extern "C"
+
+namespace {
+template <typename T>
+int ldlt_solve(const fc_ldlt* f, T* b, uint64_t n, fc_error* err) {
   if (!f || (n && !b)) {
     set_error(err, FC_INVALID_ARGUMENT, "bad arguments to fc_ldlt_solve");
+    return FC_INVALID_ARGUMENT;
+  }
+  if (f->complex != !std::is_same<T, double>::value) {
+    set_error(err, FC_INVALID_ARGUMENT, "a real solve of a complex factor or the reverse");
     return FC_INVALID_ARGUMENT;
   }
   if ((i64)n != f->n) {
@@ -2031,12 +2063,13 @@ int fc_ldlt_solve(const fc_ldlt* f, double* b, uint64_t n, fc_error* err) {
   const std::vector<i64>& perm = f->sym->s.perm;
   return guarded(err, "not enough device memory for the solve", [&] {
     HIP_CHECK(hipSetDevice(f->device));
-    std::vector<double> x(n);
+    const T* fv = reinterpret_cast<const T*>(f->val);
+    std::vector<T> x(n);
     for (uint64_t i = 0; i < n; ++i) x[i] = b[perm[i]];
-    double* dx = dalloc<double>(n);
-    double* dz = dalloc<double>(n);
-    HIP_CHECK(hipMemcpyAsync(dx, x.data(), sizeof(double) * n, hipMemcpyHostToDevice, f->stream));
-    double* dyw = dalloc<double>(n);
+    T* dx = dalloc<T>(n);
+    T* dz = dalloc<T>(n);
+    HIP_CHECK(hipMemcpyAsync(dx, x.data(), sizeof(T) * n, hipMemcpyHostToDevice, f->stream));
+    T* dyw = dalloc<T>(n);
     // rows of one backward column staged in LDS at a time (FC_LDLT_SOLVE_ROWS, 16..3840: a tuning
     // knob; larger supernodes take the chunked branch)
     int cap = fc::LDL_BW_CAP;
@@ -2047,26 +2080,35 @@ int fc_ldlt_solve(const fc_ldlt* f, double* b, uint64_t n, fc_error* err) {
     for (size_t l = 0; l + 1 < f->lvl_ptr.size(); ++l) {
       const i64 nt = f->ftile_ptr[l + 1] - f->ftile_ptr[l], cnt = f->lvl_ptr[l + 1] - f->lvl_ptr[l];
       if (nt > 0)
-        fc::k_ldlt_fwd_gather<<<(unsigned)nt, 256, 0, f->stream>>>(f->ftiles + f->ftile_ptr[l], f->S, f->U, f->rel, f->val,
+        fc::k_ldlt_fwd_gather<<<(unsigned)nt, 256, 0, f->stream>>>(f->ftiles + f->ftile_ptr[l], f->S, f->U, f->rel, fv,
                                                                    dz, dx);
       if (cnt > 0)
-        fc::k_ldlt_fwd_own<<<(unsigned)cnt, 256, 0, f->stream>>>(f->sn + f->lvl_ptr[l], f->S, f->val, f->piv, f->order, dx,
+        fc::k_ldlt_fwd_own<<<(unsigned)cnt, 256, 0, f->stream>>>(f->sn + f->lvl_ptr[l], f->S, fv, f->piv, f->order, dx,
                                                                  dyw, dz);
     }
     for (i64 l = (i64)f->lvl_ptr.size() - 2; l >= 0; --l) {
       const i64 cnt = f->lvl_ptr[l + 1] - f->lvl_ptr[l];
       if (cnt > 0)
-        fc::k_ldlt_solve_bwd<<<(unsigned)cnt, 256, 0, f->stream>>>(f->sn + f->lvl_ptr[l], f->S, f->rows, f->val, f->piv,
+        fc::k_ldlt_solve_bwd<<<(unsigned)cnt, 256, 0, f->stream>>>(f->sn + f->lvl_ptr[l], f->S, f->rows, fv, f->piv,
                                                                    f->order, dx, dyw, cap);
     }
     HIP_CHECK(hipGetLastError());
-    HIP_CHECK(hipMemcpyAsync(x.data(), dx, sizeof(double) * n, hipMemcpyDeviceToHost, f->stream));
+    HIP_CHECK(hipMemcpyAsync(x.data(), dx, sizeof(T) * n, hipMemcpyDeviceToHost, f->stream));
     HIP_CHECK(hipStreamSynchronize(f->stream));
     dfree(dx);
     dfree(dz);
     dfree(dyw);
     for (uint64_t i = 0; i < n; ++i) b[perm[i]] = x[i];
   });
+}
+}  // namespace
+
+extern "C" {
+
+int fc_ldlt_solve(const fc_ldlt* f, double* b, uint64_t n, fc_error* err) { return ldlt_solve<double>(f, b, n, err); }
+
+int fc_zldlt_solve(const fc_ldlt* f, double* b, uint64_t n, fc_error* err) {
+  return ldlt_solve<fc::zd>(f, reinterpret_cast<fc::zd*>(b), n, err);
 }
 
 int fc_ldlt_inertia(const fc_ldlt* f, uint64_t* positive, uint64_t* negative, uint64_t* two_by_two) {
@@ -2078,8 +2120,8 @@ int fc_ldlt_inertia(const fc_ldlt* f, uint64_t* positive, uint64_t* negative, ui
 }
 
 int fc_ldlt_export(const fc_ldlt* f, double* out, uint64_t len, fc_error* err) {
-  if (!f || (len && !out) || (i64)len != f->vals) {
-    set_error(err, FC_INVALID_ARGUMENT, "the output needs the factor's entry count");
+  if (!f || (len && !out) || (i64)len != f->vals * (f->complex ? 2 : 1)) {
+    set_error(err, FC_INVALID_ARGUMENT, "the output needs the factor's entry count (twice for a complex factor)");
     return FC_INVALID_ARGUMENT;
   }
   return guarded(err, "device", [&] {

@@ -294,9 +294,21 @@ int fc_ldlt_numeric(const fc_symbolic* s, const double* upper_val, uint64_t nnz,
 int fc_ldlt_solve(const fc_ldlt* f, double* b, uint64_t n, fc_error* err);
 /* LdltFactor::inertia and two_by_two (ldlt.rs:393-445). */
 int fc_ldlt_inertia(const fc_ldlt* f, uint64_t* positive, uint64_t* negative, uint64_t* two_by_two);
-/* The supernodes' trapezoids (L below the pivots, D on them) in the reference's layout. */
+/* The supernodes' trapezoids (L below the pivots, D on them) in the reference's layout: the
+ * factor's entry count of doubles, twice that (re, im pairs) for a complex factor. */
 int fc_ldlt_export(const fc_ldlt* f, double* out, uint64_t len, fc_error* err);
 void fc_ldlt_free(fc_ldlt* f);
+
+/* LdltFactor::<Complex64>::from_symbolic (ldlt.rs:378-391 over scalar.rs:115-131): the
+ * complex-symmetric A = A^T (never Hermitian) factored as P A P^T = L D L^T with the same pivoting,
+ * the modulus for its tests and num-complex's arithmetic; upper_val holds nnz (re, im) pairs in
+ * the analysed pattern. The factor and the solutions are the oracle's restatement bit for bit
+ * where the device's hypot gives the host's moduli. fc_ldlt_inertia gives only the 2 x 2 count
+ * (no inertia for complex values); fc_ldlt_export and fc_ldlt_free serve both kinds. */
+int fc_zldlt_numeric(const fc_symbolic* s, const double* upper_val, uint64_t nnz, int32_t device, fc_ldlt** out,
+                     fc_error* err);
+/* LdltFactor::<Complex64>::solve (ldlt.rs:404-408): b (host, n (re, im) pairs) in place. */
+int fc_zldlt_solve(const fc_ldlt* f, double* b, uint64_t n, fc_error* err);
 
 /* --- shift-invert block Lanczos ----------------------------------------------------------- */
 
