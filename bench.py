@@ -165,7 +165,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--contrast", action="store_true")
-    ap.add_argument("--workload", default="cube", choices=["cube", "batch", "shift", "assemble", "modes", "ldlt"])
+    ap.add_argument("--workload", default="cube", choices=["cube", "batch", "shift", "assemble", "modes", "ldlt",
+                                                                 "zldlt"])
     ap.add_argument("--ldlt-cells", type=int, default=12)
     ap.add_argument("--modes", type=int, default=20)
     ap.add_argument("--models", type=int, default=512)
@@ -198,6 +199,8 @@ def main():
         return modes_workload(args, CH, P)
     if args.workload == "ldlt":
         return ldlt_workload(args, CH, P)
+    if args.workload == "zldlt":
+        return zldlt_workload(args, CH, P)
     t0 = time.perf_counter()
     p = P.cube_hex8(args.cells)
     up = p.upper()
@@ -725,6 +728,77 @@ def ldlt_workload(args, CH, P):
                               note="the Cholesky flop count of the same structure over the factor's wall time (host "
                                    "plan cached by the warmup); LDL^T is bound by chains of dependent steps (pivot "
                                    "searches, per-entry sums in a fixed order), not by flops or bytes"),
+                cpu_baseline=cpu)
+    print(json.dumps(line), flush=True)
+
+
+def zldlt_workload(args, CH, P):
+    """Complex-symmetric LDL^T (LdltFactor<Complex64>, ldlt.rs over scalar.rs) of a harmonic
+    response's dynamic stiffness K - w^2 M + i w C on the tet10 block, C = 1e-4 K + 10 M
+    (Rayleigh damping), w = 2 pi 5 kHz (above the first modes of the 12^3 block: indefinite real
+    part). One step = the factorization + one solve. The CPU baseline is the oracle's complex
+    LDL^T of the same matrix."""
+    import torch
+
+    k = args.ldlt_cells
+    t0 = time.perf_counter()
+    p = P.block_tet10((k, k, k), (k / 100, k / 100, k / 100))
+    uptr, uidx, _ = p.upper()
+    sym = CH.Symbolic(CH.UpperColumns(p.n, uptr, uidx, np.zeros(len(uidx))))
+    info = sym.info()
+    w = 2 * np.pi * 5000.0
+    full = (p.kval - w * w * p.mval) + 1j * w * (1e-4 * p.kval + 10.0 * p.mval)
+    _, _, uval_re = p.upper(full.real)
+    _, _, uval_im = p.upper(full.imag)
+    uval = uval_re + 1j * uval_im
+    log(f"bench: tet10 {k}^3 harmonic, {p.n} equations (set up in {time.perf_counter() - t0:.1f} s)")
+    b = np.cos(np.arange(p.n) * 0.3) + 1j * np.sin(np.arange(p.n) * 0.2)
+    runs = []
+    for step in range(args.warmup + args.steps):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        f = CH.ZLdltFactor(sym, uval)
+        tf = time.perf_counter()
+        x = f.solve(b)
+        te = time.perf_counter()
+        runs.append(dict(total=te - t, factor=tf - t, solve=te - tf))
+        log(f"bench: step {step}: factor {tf - t:.3f} s, solve {te - tf:.3f} s")
+    timed = runs[args.warmup:]
+    mean = {key: float(np.mean([r[key] for r in timed])) for key in timed[0]}
+    rows = np.repeat(np.arange(p.n), np.diff(p.ptr))
+    ax = np.zeros(p.n, np.complex128)
+    np.add.at(ax, rows, full * x[p.idx])
+    res = float(np.abs(ax - b).max() / np.abs(b).max())
+    F = info["flops"]
+    achieved = 4 * F / mean["factor"] / 1e12  # a complex multiply-add is 4 real ones
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+
+        O.set_threads(args.cpu_threads)
+        t = time.perf_counter()
+        of = O.ZLdltFactor(p.n, uptr, uidx, uval)
+        xo = of.solve(b)
+        cs = time.perf_counter() - t
+        same = bool(np.array_equal(of.values().view(np.uint64), f.values().view(np.uint64)) and
+                    np.array_equal(xo.view(np.uint64), x.view(np.uint64)))
+        cpu = dict(value=cs, unit="s", cores=args.cpu_threads, kind="port",
+                   sample=f"oracle complex LDL^T factor + solve of the same matrix ({p.n} equations) on "
+                          f"{args.cpu_threads} threads: {cs:.2f} s; factor and solution bits equal to the device's: "
+                          f"{same}")
+        log(f"bench: cpu baseline {cs:.2f} s, same bits {same}")
+    line = dict(metric=f"complex-symmetric LDL^T of K - w^2 M + i w C, tet10 {k}^3: factor + solve s",
+                value=mean["total"], unit="s", n_gpus=1, steps=args.steps, warmup=args.warmup,
+                ms_per_step=mean["total"] * 1e3, higher_is_better=False, scaling="weak", vs_baseline=None,
+                dtype="c128", data="synthetic",
+                config=dict(workload=f"tet10 {k}^3 Kuhn block, steel, z=0 clamped, w = 2 pi 5 kHz, "
+                                     f"C = 1e-4 K + 10 M", equations=p.n, factor_entries=info["entries"],
+                            flops_real_equivalent=4 * F),
+                factor_s=mean["factor"], solve_s=mean["solve"], two_by_two=f.two_by_two(), residual=res,
+                roofline=dict(bound="mfma", achieved=achieved, peak=PEAK["fp64"], unit="TFLOP/s",
+                              frac=achieved / PEAK["fp64"], traffic=None, kernel="all LDL^T factor kernels",
+                              note="four times the Cholesky flop count of the structure over the factor's wall "
+                                   "time; bound by chains of dependent steps like the real LDL^T"),
                 cpu_baseline=cpu)
     print(json.dumps(line), flush=True)
 
