@@ -252,7 +252,7 @@ __global__ __launch_bounds__(1024) void k_ldlt_supernode(const int64_t* __restri
                                                         const double* __restrict__ scale, T* __restrict__ val,
                                                         int8_t* __restrict__ piv, int32_t* __restrict__ order,
                                                         T* __restrict__ work, int64_t* __restrict__ fail,
-                                                        int32_t* __restrict__ two, int64_t* __restrict__ state) {
+                                                        int32_t* __restrict__ two, int64_t* __restrict__ state, int lds_rows) {
 #pragma clang fp contract(off)
   const int64_t j = sn[blockIdx.x];
   const LdlSupernode J = S[j];
@@ -274,7 +274,10 @@ __global__ __launch_bounds__(1024) void k_ldlt_supernode(const int64_t* __restri
     if (threadIdx.x == 0) st[0] = st[1] = w;  // nothing left: the trailing update stays idle
     return;
   }
-  T* A = work + J.scratch;      // two work columns (swapped by pointer, as the reference's)
+  // two work columns (swapped by pointer, as the reference's), in dynamic LDS when the launch
+  // gave room for m rows (lds_rows), else in the work array
+  extern __shared__ __align__(16) unsigned char ldl_dyn[];
+  T* A = m <= lds_rows ? reinterpret_cast<T*>(ldl_dyn) : work + J.scratch;
   T* B = A + m;
   const double* sc = scale + J.first;
   __shared__ double red_v[16];   // one entry per wave (up to 1024 threads)

@@ -1435,6 +1435,7 @@ struct LdltPlan {
 constexpr int64_t LDL_TILE_COLS = fc::LDL_YC;  // target columns of one assemble or trailing workgroup
 constexpr int64_t LDL_TILE_ROWS = 512;         // and its target rows
 constexpr int64_t LDL_GATHER_COLS = 256;       // unknowns of one forward-gather workgroup
+constexpr size_t LDL_WORK_LDS = 48 * 1024;     // dynamic LDS for a panel kernel's two work columns
 
 // tiles of LDL_TILE_COLS target columns [b, e) of supernode k by bands of LDL_TILE_ROWS rows from b
 void ldl_tiles(std::vector<fc::LdlTile>& out, i64 k, i64 b, i64 e, i64 m) {
@@ -1946,14 +1947,26 @@ int ldlt_numeric(const fc_symbolic* h, const T* upper_val, uint64_t nnz, int32_t
       if (nt > 0)
         fc::k_ldlt_assemble<<<(unsigned)nt, 256, 0, stq>>>(dtiles + P.tile_ptr[l], F->S, dU, drel, fval, F->piv);
       // a level of few supernodes gives each 1024 threads; a crowded one 256
+      // the work columns of a launch's supernodes in LDS when two columns of its tallest fit
+      auto lds_for = [&](i64 q0, i64 q1, int* rows) {
+        i64 mm = 0;
+        for (i64 q = q0; q < q1; ++q) mm = std::max<i64>(mm, P.sn[P.lvl_sn[q]].m);
+        const size_t bytes = 2 * (size_t)mm * sizeof(T);
+        *rows = bytes <= LDL_WORK_LDS ? (int)mm : 0;
+        return bytes <= LDL_WORK_LDS ? (unsigned)bytes : 0u;
+      };
+      int nrows = 0, wrows = 0;
+      const unsigned nlds = lds_for(P.lvl_ptr[l], P.lvl_ptr[l] + narrow, &nrows);
+      const unsigned wlds = lds_for(P.lvl_ptr[l] + narrow, P.lvl_ptr[l + 1], &wrows);
       if (narrow > 0)
-        fc::k_ldlt_supernode<T, false><<<(unsigned)narrow, narrow < 256 ? 1024 : 256, 0, stq>>>(lsn, F->S, dU, drel, dscale, fval, F->piv,
-                                                                       F->order, dwork, dfail, dtwo, dstate);
+        fc::k_ldlt_supernode<T, false><<<(unsigned)narrow, narrow < 256 ? 1024 : 256, nlds, stq>>>(
+            lsn, F->S, dU, drel, dscale, fval, F->piv, F->order, dwork, dfail, dtwo, dstate, nrows);
       if (wide > 0) {
         fc::k_ldlt_state_init<<<(unsigned)wide, 256, 0, stq>>>(lsn + narrow, F->S, F->piv, F->order, dfail, dtwo, dstate);
         for (i64 i = P.pan_ptr[l]; i < P.pan_ptr[l + 1]; ++i) {
-          fc::k_ldlt_supernode<T, true><<<(unsigned)wide, 1024, 0, stq>>>(lsn + narrow, F->S, dU, drel, dscale, fval,
-                                                                      F->piv, F->order, dwork, dfail, dtwo, dstate);
+          fc::k_ldlt_supernode<T, true><<<(unsigned)wide, 1024, wlds, stq>>>(lsn + narrow, F->S, dU, drel, dscale, fval,
+                                                                         F->piv, F->order, dwork, dfail, dtwo, dstate,
+                                                                         wrows);
           const i64 pt = P.pan_tile_ptr[i + 1] - P.pan_tile_ptr[i];
           if (pt > 0)
             fc::k_ldlt_trailing<<<(unsigned)pt, 256, 0, stq>>>(dptiles + P.pan_tile_ptr[i], F->S, fval, F->piv, dstate);
