@@ -1436,6 +1436,7 @@ constexpr int64_t LDL_TILE_COLS = fc::LDL_YC;  // target columns of one assemble
 constexpr int64_t LDL_TILE_ROWS = 512;         // and its target rows
 constexpr int64_t LDL_GATHER_COLS = 256;       // unknowns of one forward-gather workgroup
 constexpr size_t LDL_WORK_LDS = 48 * 1024;     // dynamic LDS for a panel kernel's two work columns
+constexpr unsigned LDL_PANEL_THREADS = 1024;    // threads of a panel-search workgroup (512 measured no faster)
 
 // tiles of LDL_TILE_COLS target columns [b, e) of supernode k by bands of LDL_TILE_ROWS rows from b
 void ldl_tiles(std::vector<fc::LdlTile>& out, i64 k, i64 b, i64 e, i64 m) {
@@ -1959,12 +1960,12 @@ int ldlt_numeric(const fc_symbolic* h, const T* upper_val, uint64_t nnz, int32_t
       const unsigned nlds = lds_for(P.lvl_ptr[l], P.lvl_ptr[l] + narrow, &nrows);
       const unsigned wlds = lds_for(P.lvl_ptr[l] + narrow, P.lvl_ptr[l + 1], &wrows);
       if (narrow > 0)
-        fc::k_ldlt_supernode<T, false><<<(unsigned)narrow, narrow < 256 ? 1024 : 256, nlds, stq>>>(
+        fc::k_ldlt_supernode<T, false><<<(unsigned)narrow, narrow < 256 ? LDL_PANEL_THREADS : 256, nlds, stq>>>(
             lsn, F->S, dU, drel, dscale, fval, F->piv, F->order, dwork, dfail, dtwo, dstate, nrows);
       if (wide > 0) {
         fc::k_ldlt_state_init<<<(unsigned)wide, 256, 0, stq>>>(lsn + narrow, F->S, F->piv, F->order, dfail, dtwo, dstate);
         for (i64 i = P.pan_ptr[l]; i < P.pan_ptr[l + 1]; ++i) {
-          fc::k_ldlt_supernode<T, true><<<(unsigned)wide, 1024, wlds, stq>>>(lsn + narrow, F->S, dU, drel, dscale, fval,
+          fc::k_ldlt_supernode<T, true><<<(unsigned)wide, LDL_PANEL_THREADS, wlds, stq>>>(lsn + narrow, F->S, dU, drel, dscale, fval,
                                                                          F->piv, F->order, dwork, dfail, dtwo, dstate,
                                                                          wrows);
           const i64 pt = P.pan_tile_ptr[i + 1] - P.pan_tile_ptr[i];
