@@ -142,6 +142,7 @@ EXPORTS = ["fc_analyze", "fc_symbolic_info_get", "fc_symbolic_export", "fc_symbo
            "fc_solve_refined_device", "fc_solve_refined_many_device", "fc_assembly_new", "fc_assemble_device",
            "fc_assembly_info", "fc_assembly_free", "fc_lanczos_device", "fc_ldlt_numeric", "fc_ldlt_solve",
            "fc_ldlt_inertia", "fc_ldlt_export", "fc_ldlt_free", "fc_zldlt_numeric", "fc_zldlt_solve",
+           "fc_lanczos_device_ldlt",
            "fc_version"]
 
 #: kernel kinds of the numeric phase (FC_KIND_* in include/fc_chol.h)
@@ -185,6 +186,8 @@ def lib():
         L.fc_solve_refined_many_device.argtypes = [vp, vp, vp, vp, C.c_uint64, C.c_uint64, C.POINTER(_RefineOpts),
                                                    C.POINTER(_RefineReport), _f64p, ep]
         L.fc_lanczos_device.argtypes = [vp, vp, C.c_double, C.POINTER(_LanczosOpts), _f64p, vp,
+                                         C.POINTER(_LanczosReport), ep]
+        L.fc_lanczos_device_ldlt.argtypes = [vp, vp, C.c_double, C.POINTER(_LanczosOpts), _f64p, vp,
                                          C.POINTER(_LanczosReport), ep]
         L.fc_ldlt_numeric.argtypes = [vp, _f64p, C.c_uint64, C.c_int32, C.POINTER(vp), ep]
         L.fc_ldlt_solve.argtypes = [vp, _f64p, C.c_uint64, ep]
@@ -550,6 +553,27 @@ class LdltFactor:
         v = [C.c_uint64() for _ in range(3)]
         lib().fc_ldlt_inertia(self._h, *[C.byref(x) for x in v])
         return [int(x.value) for x in v]
+
+    def lanczos_device(self, mass: "Matrix", shift: float, count: int, lo: float = -np.inf, hi: float = np.inf,
+                       block: int = 6, max_basis: int | None = None, tolerance: float = 1e-9):
+        """Factor.lanczos_device with this LDL^T factor of K - shift M: a shift between modes
+        (indefinite K - shift M; the modes below it come back in ``negative``). Returns
+        dict(values, negative, basis)."""
+        import torch
+
+        if max_basis is None:
+            free, _ = torch.cuda.mem_get_info()
+            max_basis = max(min(3000, self.n, int(free / 4 / (16 * max(self.n, 1)))), min(block, self.n))
+        opts = _LanczosOpts(int(count), float(lo), float(hi), int(block), 0, int(max_basis), float(tolerance))
+        vals = np.zeros(max(int(count), 1))
+        rep = _LanczosReport()
+        err = _Error()
+        st = lib().fc_lanczos_device_ldlt(self._h, mass._h, float(shift), C.byref(opts), vals, None, C.byref(rep),
+                                          C.byref(err))
+        _raise(st, err)
+        found = int(rep.found)
+        return dict(values=vals[:found].copy(), negative=np.array(rep.negative[:min(int(rep.negative_count), 16)]),
+                    basis=int(rep.basis))
 
     def inertia(self) -> tuple:
         """(positive, negative) eigenvalue counts (Sylvester's law on D, ldlt.rs:412-445)."""

@@ -584,6 +584,15 @@ __global__ void k_ldlt_diag(const LdlSupernode* __restrict__ S, const double* __
   }
 }
 
+// dst[i] = src[perm[i]] (into postorder) or, inverse, dst[perm[i]] = src[i] (back).
+__global__ void k_ldlt_permute(int64_t n, const int64_t* __restrict__ perm, const double* __restrict__ src,
+                               double* __restrict__ dst, bool inverse) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (inverse) dst[perm[i]] = src[i];
+  else dst[i] = src[perm[i]];
+}
+
 // The forward pass of the solve (solve_group, cholesky.rs:904-981, with LDL^T's unit L, block
 // diagonal D and pivot orders) level by level on one right-hand side x (postorder, in place).
 // First the descendants' contributions to one level's unknowns, read as a gather: workgroup t

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <limits>
 #include <memory>
 #include <mutex>
@@ -730,6 +731,7 @@ struct fc_ldlt {
   int64_t* sn = nullptr;           // supernodes by level
   std::vector<int64_t> lvl_ptr;
   bool complex = false;            // values are (re, im) pairs: complex-symmetric LDL^T
+  int64_t* perm = nullptr;         // postorder position -> original equation (device)
   fc::LdlUpdate* U = nullptr;      // updates and their relative maps, for the forward gather
   int32_t* rel = nullptr;
   fc::LdlTile* ftiles = nullptr;   // the forward gather's tiles, ftile_ptr[l] .. ftile_ptr[l + 1] per level
@@ -739,7 +741,7 @@ struct fc_ldlt {
     (void)hipGetDevice(&cur);
     (void)hipSetDevice(device);
     for (void* p : {(void*)S, (void*)rows, (void*)val, (void*)piv, (void*)order, (void*)sn, (void*)U, (void*)rel,
-                    (void*)ftiles})
+                    (void*)ftiles, (void*)perm})
       if (p) (void)hipFree(p);
     if (stream) (void)hipStreamDestroy(stream);
     (void)hipSetDevice(cur);
@@ -1691,9 +1693,11 @@ struct LanczosResult {
 
 // eigen.rs:281-466 with the operator (K - shift M)^{-1} M: the factor's block solves and the
 // mass matrix's products on the device; the projected matrix T and its eigenproblem on the host.
-LanczosResult lanczos_run(const fc_factor* F, const fc_matrix* M, double shift, const fc_lanczos_opts& o,
-                          double* vectors, hipStream_t st) {
-  const i64 n = F->sym->s.n;
+// solve(W, b) overwrites the n x b block W (device, leading dimension n) with (K - shift M)^-1 W
+// on stream st; *cancel (if given) is polled once per block step.
+LanczosResult lanczos_run(i64 n, const std::function<void(double*, int)>& solve, const volatile int32_t* cancel,
+                          const fc_matrix* M, double shift, const fc_lanczos_opts& o, double* vectors, hipStream_t st) {
+  const fc_factor* F = nullptr;
   const int b = (int)std::max<i64>(1, std::min<i64>(o.block, n));
   const i64 max_basis = std::max<i64>(std::min<i64>((i64)o.max_basis, n), b);
   const i64 cap = max_basis + b;
@@ -1719,11 +1723,11 @@ LanczosResult lanczos_run(const fc_factor* F, const fc_matrix* M, double shift, 
   std::vector<double> T;  // m x m row-major
   i64 m = 0;
   for (;;) {
-    if (F->opts.cancel && *F->opts.cancel) throw LanczosError{"cancelled"};
+    if (cancel && *cancel) throw LanczosError{"cancelled"};
     const i64 j0 = m;
     // W = Op Q_j: the block's M products solved together
     HIP_CHECK(hipMemcpyAsync(L.W, L.mq(j0), sizeof(double) * n * b, hipMemcpyDeviceToDevice, st));
-    solve_device(F, L.W, n, b, n, st);
+    solve(L.W, b);
     // A_j = MQ_j^T W, symmetrized
     std::vector<double> a = L.dots(L.mq(j0), b, L.W, b);
     for (int i = 0; i < b; ++i)
@@ -1908,6 +1912,7 @@ int ldlt_numeric(const fc_symbolic* h, const T* upper_val, uint64_t nnz, int32_t
     }
     F->S = dupload(P.sn, stq);
     F->rows = dupload(s.rows, stq);
+    F->perm = dupload(s.perm, stq);
     F->val = dalloc<double>((size_t)std::max<i64>(P.vals, 1) * (sizeof(T) / sizeof(double)));
     T* fval = reinterpret_cast<T*>(F->val);
     F->piv = dalloc<int8_t>((size_t)std::max<i64>(s.n, 1));
@@ -2056,6 +2061,51 @@ int fc_zldlt_numeric(const fc_symbolic* h, const double* upper_val, uint64_t nnz
 }  // extern "C"
 
 namespace {
+// The forward and backward passes on x (postorder, device) with work vectors z and y of n entries.
+template <typename T>
+void ldlt_passes(const fc_ldlt* f, const T* fv, T* dx, T* dz, T* dyw, hipStream_t st) {
+  // rows of one backward column staged in LDS at a time (FC_LDLT_SOLVE_ROWS, 16..3840: a tuning
+  // knob; larger supernodes take the chunked branch)
+  int cap = fc::LDL_BW_CAP;
+  if (const char* e = std::getenv("FC_LDLT_SOLVE_ROWS")) {
+    const long v = std::strtol(e, nullptr, 10);
+    if (v >= 16 && v <= fc::LDL_BW_CAP) cap = (int)v;
+  }
+  for (size_t l = 0; l + 1 < f->lvl_ptr.size(); ++l) {
+    const i64 nt = f->ftile_ptr[l + 1] - f->ftile_ptr[l], cnt = f->lvl_ptr[l + 1] - f->lvl_ptr[l];
+    if (nt > 0)
+      fc::k_ldlt_fwd_gather<<<(unsigned)nt, 256, 0, st>>>(f->ftiles + f->ftile_ptr[l], f->S, f->U, f->rel, fv, dz, dx);
+    if (cnt > 0)
+      fc::k_ldlt_fwd_own<<<(unsigned)cnt, 256, 0, st>>>(f->sn + f->lvl_ptr[l], f->S, fv, f->piv, f->order, dx, dyw, dz);
+  }
+  for (i64 l = (i64)f->lvl_ptr.size() - 2; l >= 0; --l) {
+    const i64 cnt = f->lvl_ptr[l + 1] - f->lvl_ptr[l];
+    if (cnt > 0)
+      fc::k_ldlt_solve_bwd<<<(unsigned)cnt, 256, 0, st>>>(f->sn + f->lvl_ptr[l], f->S, f->rows, fv, f->piv, f->order, dx,
+                                                         dyw, cap);
+  }
+}
+
+// Real LDL^T solves of the n x nb block W (device, leading dimension n) in place on stream st,
+// each column as a lone solve (the same bits): permuted into postorder, the passes, permuted back.
+void ldlt_solve_block_device(const fc_ldlt* f, double* W, int nb, hipStream_t st) {
+  const i64 n = f->n;
+  double* dx = dalloc<double>((size_t)n);
+  double* dz = dalloc<double>((size_t)n);
+  double* dyw = dalloc<double>((size_t)n);
+  for (int c = 0; c < nb; ++c) {
+    double* col = W + (size_t)c * n;
+    fc::k_ldlt_permute<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, f->perm, col, dx, false);
+    ldlt_passes(f, f->val, dx, dz, dyw, st);
+    fc::k_ldlt_permute<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, f->perm, dx, col, true);
+  }
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(st));
+  dfree(dx);
+  dfree(dz);
+  dfree(dyw);
+}
+
 template <typename T>
 int ldlt_solve(const fc_ldlt* f, T* b, uint64_t n, fc_error* err) {
   if (!f || (n && !b)) {
@@ -2086,26 +2136,7 @@ int ldlt_solve(const fc_ldlt* f, T* b, uint64_t n, fc_error* err) {
     T* dyw = dalloc<T>(n);
     // rows of one backward column staged in LDS at a time (FC_LDLT_SOLVE_ROWS, 16..3840: a tuning
     // knob; larger supernodes take the chunked branch)
-    int cap = fc::LDL_BW_CAP;
-    if (const char* e = std::getenv("FC_LDLT_SOLVE_ROWS")) {
-      const long v = std::strtol(e, nullptr, 10);
-      if (v >= 16 && v <= fc::LDL_BW_CAP) cap = (int)v;
-    }
-    for (size_t l = 0; l + 1 < f->lvl_ptr.size(); ++l) {
-      const i64 nt = f->ftile_ptr[l + 1] - f->ftile_ptr[l], cnt = f->lvl_ptr[l + 1] - f->lvl_ptr[l];
-      if (nt > 0)
-        fc::k_ldlt_fwd_gather<<<(unsigned)nt, 256, 0, f->stream>>>(f->ftiles + f->ftile_ptr[l], f->S, f->U, f->rel, fv,
-                                                                   dz, dx);
-      if (cnt > 0)
-        fc::k_ldlt_fwd_own<<<(unsigned)cnt, 256, 0, f->stream>>>(f->sn + f->lvl_ptr[l], f->S, fv, f->piv, f->order, dx,
-                                                                 dyw, dz);
-    }
-    for (i64 l = (i64)f->lvl_ptr.size() - 2; l >= 0; --l) {
-      const i64 cnt = f->lvl_ptr[l + 1] - f->lvl_ptr[l];
-      if (cnt > 0)
-        fc::k_ldlt_solve_bwd<<<(unsigned)cnt, 256, 0, f->stream>>>(f->sn + f->lvl_ptr[l], f->S, f->rows, fv, f->piv,
-                                                                   f->order, dx, dyw, cap);
-    }
+    ldlt_passes(f, fv, dx, dz, dyw, f->stream);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipMemcpyAsync(x.data(), dx, sizeof(T) * n, hipMemcpyDeviceToHost, f->stream));
     HIP_CHECK(hipStreamSynchronize(f->stream));
@@ -2168,7 +2199,9 @@ int fc_lanczos_device(const fc_factor* f, const fc_matrix* m, double shift, cons
     HIP_CHECK(hipEventCreate(&e1));
     HIP_CHECK(hipEventRecord(e0, f->stream));
     try {
-      res = lanczos_run(f, m, shift, *opts, vectors_device, f->stream);
+      res = lanczos_run(
+          f->sym->s.n, [&](double* W, int nb) { solve_device(f, W, f->sym->s.n, nb, f->sym->s.n, f->stream); },
+          f->opts.cancel, m, shift, *opts, vectors_device, f->stream);
     } catch (const LanczosError& le) {
       stopped = le.msg;
     }
@@ -2195,6 +2228,47 @@ int fc_lanczos_device(const fc_factor* f, const fc_matrix* m, double shift, cons
     report->negative_count = res.negative.size();
     for (size_t i = 0; i < res.negative.size() && i < 16; ++i) report->negative[i] = res.negative[i];
     report->seconds = ms * 1e-3;
+  }
+  set_error(err, FC_OK, "");
+  return FC_OK;
+}
+
+int fc_lanczos_device_ldlt(const fc_ldlt* f, const fc_matrix* m, double shift, const fc_lanczos_opts* opts,
+                           double* values, double* vectors_device, fc_lanczos_report* report, fc_error* err) {
+  if (!f || f->complex || !m || !opts || (opts->count && !values) || m->n != f->n || opts->block < 1 ||
+      opts->block > 8 || !(opts->tolerance > 0.0) || !std::isfinite(shift)) {
+    set_error(err, FC_INVALID_ARGUMENT, "bad arguments to fc_lanczos_device_ldlt");
+    return FC_INVALID_ARGUMENT;
+  }
+  if (f->n == 0) {
+    set_error(err, FC_INVALID_ARGUMENT, "there are no free degrees of freedom");
+    return FC_INVALID_ARGUMENT;
+  }
+  LanczosResult res;
+  std::string stopped;
+  int st = guarded(err, "not enough device memory for the Lanczos basis", [&] {
+    HIP_CHECK(hipSetDevice(f->device));
+    try {
+      res = lanczos_run(
+          f->n, [&](double* W, int nb) { ldlt_solve_block_device(f, W, nb, f->stream); }, nullptr, m, shift, *opts,
+          vectors_device, f->stream);
+    } catch (const LanczosError& le) {
+      stopped = le.msg;
+    }
+    HIP_CHECK(hipStreamSynchronize(f->stream));
+  });
+  if (st == FC_OK && !stopped.empty()) {
+    set_error(err, FC_NOT_CONVERGED, stopped);
+    return FC_NOT_CONVERGED;
+  }
+  if (st != FC_OK) return st;
+  for (size_t i = 0; i < res.values.size(); ++i) values[i] = res.values[i];
+  if (report) {
+    std::memset(report, 0, sizeof *report);
+    report->found = res.values.size();
+    report->basis = (uint64_t)res.basis;
+    report->negative_count = res.negative.size();
+    for (size_t i = 0; i < res.negative.size() && i < 16; ++i) report->negative[i] = res.negative[i];
   }
   set_error(err, FC_OK, "");
   return FC_OK;

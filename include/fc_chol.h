@@ -342,6 +342,12 @@ typedef struct {
  * (the factor's cancel flag, polled once per block). */
 int fc_lanczos_device(const fc_factor* f, const fc_matrix* m, double shift, const fc_lanczos_opts* opts,
                       double* values, double* vectors_device, fc_lanczos_report* report, fc_error* err);
+/* The same Lanczos run with an LDL^T factor of K - shift M (fc_ldlt_numeric): a shift between
+ * modes, where K - shift M is indefinite (Symmetric with Kind::Auto under ShiftInvert,
+ * analysis.rs:291-293, eigen.rs:187-211). Modes below the shift come back in report->negative. The
+ * factor's stream runs the solves; calls on one factor must not overlap. */
+int fc_lanczos_device_ldlt(const fc_ldlt* f, const fc_matrix* m, double shift, const fc_lanczos_opts* opts,
+                           double* values, double* vectors_device, fc_lanczos_report* report, fc_error* err);
 
 /* Library version string. */
 const char* fc_version(void);

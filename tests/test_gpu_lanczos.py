@@ -108,3 +108,35 @@ def test_cancel_flag_stops_the_iteration(gpu):
     flag.value = 1
     with pytest.raises(CH.Cancelled):
         f.lanczos_device(M, -1.0, 5, block=6, max_basis=n)
+
+
+def test_indefinite_shift_through_ldlt_matches_the_oracle(gpu):
+    """A shift between the tet10 block's 4th and 5th modes: K - sigma M is indefinite, so the
+    modes come from an LDL^T factor (Symmetric with Kind::Auto under ShiftInvert): the four below
+    the shift as negative Ritz values, the next ones as found modes, against the oracle's Lanczos
+    over its LDL^T solves and scipy's eigsh."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as sla
+
+    from fieldcraft_amd import cholesky as CH
+    from fieldcraft_amd import problems as P
+    from oracle import eigen as EG
+    from oracle import oracle as O
+
+    p = P.block("tet10", (3, 2, 2), (0.3, 0.2, 0.2), 210e9, 0.3, clamp_face=4, jitter=0.1, seed=5, rho=7850.0)
+    K = sp.csr_matrix((p.kval, p.idx, p.ptr), shape=(p.n, p.n))
+    M = sp.csr_matrix((p.mval, p.idx, p.ptr), shape=(p.n, p.n))
+    low = np.sort(sla.eigsh(K.tocsc(), k=8, M=M.tocsc(), sigma=-1.0e3, which="LM", tol=1e-13)[0])
+    sigma = 0.5 * (low[3] + low[4])
+    up = p.upper(p.kval - sigma * p.mval)
+    f = CH.LdltFactor.new(CH.UpperColumns(p.n, *up))
+    assert f.inertia()[1] == 4
+    res = f.lanczos_device(CH.Matrix(p.n, p.ptr, p.idx, p.mval), sigma, 4, block=6, max_basis=p.n)
+    of = O.LdltFactor(p.n, *up)
+    op = EG.Operator(p.n, lambda x: M @ x, lambda X: np.stack([of.solve(X[:, c]) for c in range(X.shape[1])], 1))
+    want = EG.lanczos(op, sigma, 4, block=6, max_basis=p.n, tolerance=1e-9)
+    assert np.abs(res["values"] - want["values"]).max() <= 1e-9 * low.max()
+    assert np.abs(res["values"] - low[4:8]).max() <= 1e-9 * low.max()
+    assert len(res["negative"]) == len(want["negative"])
+    if len(res["negative"]):
+        assert np.abs(np.sort(res["negative"]) - np.sort(want["negative"])).max() <= 1e-9 * low.max()
