@@ -1435,8 +1435,8 @@ struct LdltPlan {
 };
 
 constexpr int64_t LDL_TILE_COLS = fc::LDL_YC;  // target columns of one assemble or trailing workgroup
-constexpr int64_t LDL_TILE_ROWS = 256;         // and its target rows
-constexpr int64_t LDL_GATHER_COLS = 128;       // unknowns of one forward-gather workgroup
+constexpr int64_t LDL_TILE_ROWS = 128;         // and its target rows
+constexpr int64_t LDL_GATHER_COLS = 64;        // unknowns of one forward-gather workgroup
 constexpr size_t LDL_WORK_LDS = 48 * 1024;     // dynamic LDS for a panel kernel's two work columns
 constexpr unsigned LDL_PANEL_THREADS = 1024;    // threads of a panel-search workgroup (512 measured no faster)
 
