@@ -100,7 +100,7 @@ class _Error(C.Structure):
 class _Opts(C.Structure):
     _fields_ = [("precision", C.c_int32), ("device", C.c_int32), ("stream", C.c_void_p),
                 ("cancel", C.POINTER(C.c_int32)), ("bf16_terms", C.c_int32), ("use_graph", C.c_int32),
-                ("pivot_fallback", C.c_double), ("profile_mask", C.c_int32), ("pad", C.c_int32)]
+                ("fallback", C.c_int32), ("pad0", C.c_int32), ("profile_mask", C.c_int32), ("pad", C.c_int32)]
 
 
 class _Info(C.Structure):
@@ -137,7 +137,7 @@ _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 #: every symbol include/fc_chol.h declares
 EXPORTS = ["fc_analyze", "fc_symbolic_info_get", "fc_symbolic_export", "fc_symbolic_free", "fc_factor_numeric",
            "fc_factor_numeric_device", "fc_factor_new", "fc_refactor", "fc_refactor_device", "fc_entries", "fc_solve",
-           "fc_solve_many", "fc_solve_device", "fc_factor_export", "fc_factor_seconds", "fc_factor_profile",
+           "fc_solve_many", "fc_solve_device", "fc_factor_export", "fc_factor_seconds", "fc_factor_precision", "fc_factor_profile",
            "fc_factor_free", "fc_matrix_new", "fc_matrix_free", "fc_matrix_set_segments", "fc_solve_refined",
            "fc_solve_refined_device", "fc_solve_refined_many_device", "fc_assembly_new", "fc_assemble_device",
            "fc_assembly_info", "fc_assembly_free", "fc_lanczos_device", "fc_ldlt_numeric", "fc_ldlt_solve",
@@ -173,6 +173,8 @@ def lib():
         L.fc_factor_export.argtypes = [vp, _f64p, C.c_uint64, ep]
         L.fc_factor_seconds.argtypes = [vp]
         L.fc_factor_seconds.restype = C.c_double
+        L.fc_factor_precision.argtypes = [vp]
+        L.fc_factor_precision.restype = C.c_int32
         L.fc_factor_free.argtypes = [vp]
         L.fc_factor_free.restype = None
         L.fc_matrix_new.argtypes = [C.c_uint64, _u64p, _u64p, _f64p, C.c_int32, C.POINTER(vp), ep]
@@ -322,13 +324,16 @@ class Factor:
     """A Cholesky factor on the GPU (Factor, cholesky.rs:1532-1570)."""
 
     def __init__(self, symbolic: Symbolic, val=None, precision: str | int = "fp64", device: int = 0,
-                 bf16_terms: int = 0, cancel: C.c_int32 | None = None, pivot_fallback: float = 0.0,
+                 bf16_terms: int = 0, cancel: C.c_int32 | None = None, fallback: int = 0,
                  use_graph: int = 0, profile: tuple = (), device_values: int | None = None,
                  stream: int | None = None):
         """Factor the matrix with values `val` (host array) or `device_values` (a device
         pointer to the upper columns' values, nnz doubles). bf16_terms (mixed form): 1 or 3
         BF16 products per operand pair, 0 for the library's default (3). use_graph: bit 0
-        replays the factorization from a hipGraph, bit 1 the solves."""
+        replays the factorization from a hipGraph, bit 1 the solves. fallback (mixed, fp32):
+        0 refactors in the next precision when a pivot fails in the reduced one (mixed -> fp32
+        -> fp64) and raises NotPositiveDefinite only when FP64 fails too; 1 raises at once.
+        ``precision_used()`` tells which precision the factor holds."""
         L = lib()
         self.symbolic = symbolic
         self.n = symbolic.n
@@ -339,7 +344,7 @@ class Factor:
             mask |= 1 << (KINDS.index(k) if isinstance(k, str) else int(k))
         opts = _Opts(self.precision, int(device), C.c_void_p(stream) if stream else None,
                      C.pointer(cancel) if cancel is not None else None, int(bf16_terms), int(use_graph),
-                     float(pivot_fallback), mask, 0)
+                     int(fallback), 0, mask, 0)
         self._h = C.c_void_p()
         err = _Error()
         if device_values is not None:
@@ -390,6 +395,12 @@ class Factor:
 
     def seconds(self) -> float:
         return float(lib().fc_factor_seconds(self._h))
+
+    def precision_used(self) -> str:
+        """The precision of the last successful factorization ("fp64", "mixed", "fp32"): the one
+        asked for, or the one a fallback reached."""
+        p = int(lib().fc_factor_precision(self._h))
+        return {v: k for k, v in PRECISIONS.items()}[p]
 
     def solve(self, b) -> np.ndarray:
         """Factor::solve (cholesky.rs:1562-1564); returns x (b is not modified)."""

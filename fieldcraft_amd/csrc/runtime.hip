@@ -42,9 +42,20 @@ struct DeviceError {
     if (e_ != hipSuccess) throw DeviceError{e_, #x}; \
   } while (0)
 
+// The largest single device allocation the library makes (FC_MAX_DEVICE_ALLOC bytes; unset or
+// 0: no limit): a caller can keep the library's buffers below what other work on the GPU needs,
+// and a test can make any allocation fail. An allocation above it fails as a short device would.
+size_t max_device_alloc() {
+  const char* e = std::getenv("FC_MAX_DEVICE_ALLOC");
+  if (!e) return 0;
+  return (size_t)std::strtoull(e, nullptr, 10);
+}
+
 template <typename T>
 T* dalloc(size_t count) {
   if (count == 0) count = 1;
+  const size_t cap = max_device_alloc();
+  if (cap && count * sizeof(T) > cap) throw std::bad_alloc();
   void* p = nullptr;
   hipError_t e = hipMalloc(&p, count * sizeof(T));
   if (e != hipSuccess) {
@@ -622,7 +633,9 @@ struct fc_factor {
   const fc_symbolic* sym = nullptr;
   std::shared_ptr<DevicePlan> plan;
   int device = 0;
-  int precision = FC_FP64;
+  int precision = FC_FP64;  // the current factorization's (fallbacks change it)
+  int requested = FC_FP64;  // opts.precision
+  int graph_precision = -1; // the precision the captured graphs were recorded for
   fc_factor_opts opts{};
   hipStream_t stream = nullptr;
   bool own_stream = false;
@@ -681,6 +694,18 @@ struct fc_factor {
     if (side) (void)hipStreamDestroy(side);
     if (own_stream && stream) (void)hipStreamDestroy(stream);
     (void)hipSetDevice(cur);
+  }
+  // Graphs recorded for another precision's kernels are dropped (recaptured on next use).
+  void drop_graphs() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    graph_exec = nullptr;
+    graph = nullptr;
+    for (auto& sg : sgraphs) {
+      (void)hipGraphExecDestroy(sg.exec);
+      (void)hipGraphDestroy(sg.g);
+    }
+    sgraphs.clear();
   }
   // The side stream and its events, made once (before any capture) on the factor's device.
   void make_side() {
@@ -750,13 +775,23 @@ struct fc_ldlt {
 
 namespace {
 
-// Registry so a factor can keep its analysis alive after fc_symbolic_free.
-std::mutex g_reg_mu;
-std::vector<std::shared_ptr<fc_symbolic>> g_symbolics;
+// Registry of the live analyses, so a factor can keep its analysis alive after
+// fc_symbolic_free. It is allocated once and never destroyed: an analysis the caller has not
+// freed by process exit is left to the driver, since its device plan's hipFree would otherwise
+// run from a static destructor, after the HIP runtime has been torn down.
+struct Registry {
+  std::mutex mu;
+  std::vector<std::shared_ptr<fc_symbolic>> live;
+};
+Registry& registry() {
+  static Registry* r = new Registry();
+  return *r;
+}
 
 std::shared_ptr<fc_symbolic> keep(const fc_symbolic* s) {
-  std::lock_guard<std::mutex> g(g_reg_mu);
-  for (auto& p : g_symbolics)
+  Registry& R = registry();
+  std::lock_guard<std::mutex> g(R.mu);
+  for (auto& p : R.live)
     if (p.get() == s) return p;
   return nullptr;
 }
@@ -951,16 +986,52 @@ int factor_device(fc_factor* F, const double* dval, uint64_t nnz, fc_error* err)
     HIP_CHECK(hipSetDevice(F->device));
     hipStream_t st = F->stream;
     F->valid = false;
-    if (F->precision == FC_FP64) run_factor<double>(F, dval, (i64)nnz, st);
-    else run_factor<float>(F, dval, (i64)nnz, st);
-    unsigned long long fail = 0;
-    HIP_CHECK(hipMemcpyAsync(&fail, F->fail, sizeof fail, hipMemcpyDeviceToHost, st));
-    HIP_CHECK(hipStreamSynchronize(st));
-    if (fail != ~0ull) {
-      const uint64_t col = (uint64_t)s.perm[(i64)fail];
-      set_error(err, FC_NOT_POSITIVE_DEFINITE, "the matrix is not positive definite", col, s.entries);
-      return FC_NOT_POSITIVE_DEFINITE;
+    // each factorization starts from the precision asked for, unless a fallback has moved the
+    // factor to FP64 storage
+    if (F->precision != FC_FP64 || F->requested == FC_FP64) F->precision = F->requested;
+    if (!F->L) {  // a failed fallback could not restore the storage: allocate it again
+      const size_t es = F->precision == FC_FP64 ? sizeof(double) : sizeof(float);
+      F->L = (void*)dalloc<char>((size_t)std::max<i64>(s.l_size, 1) * es);
     }
+    double seconds = 0.0;
+    for (;;) {
+      if (F->graph_precision != F->precision) {
+        F->drop_graphs();
+        F->graph_precision = F->precision;
+      }
+      if (F->precision == FC_FP64) run_factor<double>(F, dval, (i64)nnz, st);
+      else run_factor<float>(F, dval, (i64)nnz, st);
+      seconds += F->seconds;
+      unsigned long long fail = 0;
+      HIP_CHECK(hipMemcpyAsync(&fail, F->fail, sizeof fail, hipMemcpyDeviceToHost, st));
+      HIP_CHECK(hipStreamSynchronize(st));
+      if (fail == ~0ull) break;
+      if (F->precision == FC_FP64 || F->opts.fallback != 0) {
+        F->seconds = seconds;
+        const uint64_t col = (uint64_t)s.perm[(i64)fail];
+        set_error(err, FC_NOT_POSITIVE_DEFINITE, "the matrix is not positive definite", col, s.entries);
+        return FC_NOT_POSITIVE_DEFINITE;
+      }
+      // the reduced precision lost definiteness: the next precision, mixed -> FP32 on the same
+      // FP32 storage, FP32 -> FP64 on new FP64 storage (kept for later factorizations)
+      if (F->precision == FC_MIXED) {
+        F->precision = FC_FP32;
+      } else {
+        dfree(F->L);
+        F->L = nullptr;
+        F->U.release();
+        F->inv.release();
+        F->part.release();
+        try {
+          F->L = (void*)dalloc<double>((size_t)std::max<i64>(s.l_size, 1));
+        } catch (const std::bad_alloc&) {
+          F->L = (void*)dalloc<float>((size_t)std::max<i64>(s.l_size, 1));  // the storage just freed
+          throw;
+        }
+        F->precision = FC_FP64;
+      }
+    }
+    F->seconds = seconds;
     F->valid = true;
     set_error(err, FC_OK, "");
     return FC_OK;
@@ -1574,7 +1645,6 @@ struct LanczosError {
 // The device side of one Lanczos run: the basis Q and its M products MQ (n x cap), a block W and
 // MW (n x 8), reduction buffers.
 struct LanczosWork {
-  const fc_factor* F;
   const fc_matrix* M;
   hipStream_t st;
   i64 n;
@@ -1697,11 +1767,10 @@ struct LanczosResult {
 // on stream st; *cancel (if given) is polled once per block step.
 LanczosResult lanczos_run(i64 n, const std::function<void(double*, int)>& solve, const volatile int32_t* cancel,
                           const fc_matrix* M, double shift, const fc_lanczos_opts& o, double* vectors, hipStream_t st) {
-  const fc_factor* F = nullptr;
   const int b = (int)std::max<i64>(1, std::min<i64>(o.block, n));
   const i64 max_basis = std::max<i64>(std::min<i64>((i64)o.max_basis, n), b);
   const i64 cap = max_basis + b;
-  LanczosWork L{F, M, st, n, cap};
+  LanczosWork L{M, st, n, cap};
   L.chunks = (int)std::max<i64>(1, (n + fc::TS_ROWS - 1) / fc::TS_ROWS);
   L.Q = dalloc<double>((size_t)n * cap);
   L.MQ = dalloc<double>((size_t)n * cap);
@@ -1946,6 +2015,14 @@ int ldlt_numeric(const fc_symbolic* h, const T* upper_val, uint64_t nnz, int32_t
     tmp.p.push_back(dtiles);
     fc::LdlTile* dptiles = dupload(P.pan_tiles, stq);
     tmp.p.push_back(dptiles);
+    // the LDS budget of the panel kernels' work columns (FC_LDLT_WORK_LDS bytes, 0..LDL_WORK_LDS:
+    // a tuning knob; a launch whose tallest supernode's two columns do not fit keeps them in
+    // device memory instead)
+    size_t work_lds = LDL_WORK_LDS;
+    if (const char* e = std::getenv("FC_LDLT_WORK_LDS")) {
+      const long v = std::strtol(e, nullptr, 10);
+      if (v >= 0 && (size_t)v <= LDL_WORK_LDS) work_lds = (size_t)v;
+    }
     for (size_t l = 0; l + 1 < P.lvl_ptr.size(); ++l) {
       const i64 narrow = P.lvl_narrow[l], wide = P.lvl_ptr[l + 1] - P.lvl_ptr[l] - narrow;
       const i64 nt = P.tile_ptr[l + 1] - P.tile_ptr[l];
@@ -1958,8 +2035,8 @@ int ldlt_numeric(const fc_symbolic* h, const T* upper_val, uint64_t nnz, int32_t
         i64 mm = 0;
         for (i64 q = q0; q < q1; ++q) mm = std::max<i64>(mm, P.sn[P.lvl_sn[q]].m);
         const size_t bytes = 2 * (size_t)mm * sizeof(T);
-        *rows = bytes <= LDL_WORK_LDS ? (int)mm : 0;
-        return bytes <= LDL_WORK_LDS ? (unsigned)bytes : 0u;
+        *rows = bytes <= work_lds ? (int)mm : 0;
+        return bytes <= work_lds ? (unsigned)bytes : 0u;
       };
       int nrows = 0, wrows = 0;
       const unsigned nlds = lds_for(P.lvl_ptr[l], P.lvl_ptr[l] + narrow, &nrows);
@@ -2088,11 +2165,12 @@ void ldlt_passes(const fc_ldlt* f, const T* fv, T* dx, T* dz, T* dyw, hipStream_
 
 // Real LDL^T solves of the n x nb block W (device, leading dimension n) in place on stream st,
 // each column as a lone solve (the same bits): permuted into postorder, the passes, permuted back.
-void ldlt_solve_block_device(const fc_ldlt* f, double* W, int nb, hipStream_t st) {
+// `work` holds the three work vectors (allocated on first use, kept by the caller).
+void ldlt_solve_block_device(const fc_ldlt* f, double* W, int nb, hipStream_t st, Buffer& work) {
   const i64 n = f->n;
-  double* dx = dalloc<double>((size_t)n);
-  double* dz = dalloc<double>((size_t)n);
-  double* dyw = dalloc<double>((size_t)n);
+  double* dx = (double*)work.get((size_t)3 * n * sizeof(double));
+  double* dz = dx + n;
+  double* dyw = dz + n;
   for (int c = 0; c < nb; ++c) {
     double* col = W + (size_t)c * n;
     fc::k_ldlt_permute<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, f->perm, col, dx, false);
@@ -2100,10 +2178,6 @@ void ldlt_solve_block_device(const fc_ldlt* f, double* W, int nb, hipStream_t st
     fc::k_ldlt_permute<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, f->perm, dx, col, true);
   }
   HIP_CHECK(hipGetLastError());
-  HIP_CHECK(hipStreamSynchronize(st));
-  dfree(dx);
-  dfree(dz);
-  dfree(dyw);
 }
 
 template <typename T>
@@ -2246,12 +2320,16 @@ int fc_lanczos_device_ldlt(const fc_ldlt* f, const fc_matrix* m, double shift, c
   }
   LanczosResult res;
   std::string stopped;
+  struct Work {  // the block solves' work vectors, freed however the run ends
+    Buffer b;
+    ~Work() { b.release(); }
+  } work;
   int st = guarded(err, "not enough device memory for the Lanczos basis", [&] {
     HIP_CHECK(hipSetDevice(f->device));
     try {
       res = lanczos_run(
-          f->n, [&](double* W, int nb) { ldlt_solve_block_device(f, W, nb, f->stream); }, nullptr, m, shift, *opts,
-          vectors_device, f->stream);
+          f->n, [&](double* W, int nb) { ldlt_solve_block_device(f, W, nb, f->stream, work.b); }, nullptr, m, shift,
+          *opts, vectors_device, f->stream);
     } catch (const LanczosError& le) {
       stopped = le.msg;
     }
@@ -2301,8 +2379,9 @@ int fc_analyze(uint64_t n, const uint64_t* ptr, const uint64_t* idx, uint64_t ma
     }
     h->prog = build_program(s);
     {
-      std::lock_guard<std::mutex> g(g_reg_mu);
-      g_symbolics.push_back(h);
+      Registry& R = registry();
+      std::lock_guard<std::mutex> g(R.mu);
+      R.live.push_back(h);
     }
     *out = h.get();
     set_error(err, FC_OK, "");
@@ -2366,10 +2445,19 @@ int fc_symbolic_export(const fc_symbolic* h, uint64_t* perm, uint64_t* first, ui
 }
 
 void fc_symbolic_free(fc_symbolic* h) {
-  std::lock_guard<std::mutex> g(g_reg_mu);
-  g_symbolics.erase(std::remove_if(g_symbolics.begin(), g_symbolics.end(),
-                                   [h](const std::shared_ptr<fc_symbolic>& p) { return p.get() == h; }),
-                    g_symbolics.end());
+  // the last reference may be this one: the analysis (and its device plans) is destroyed
+  // outside the registry's lock
+  std::shared_ptr<fc_symbolic> last;
+  {
+    Registry& R = registry();
+    std::lock_guard<std::mutex> g(R.mu);
+    for (auto it = R.live.begin(); it != R.live.end(); ++it)
+      if (it->get() == h) {
+        last = std::move(*it);
+        R.live.erase(it);
+        break;
+      }
+  }
 }
 
 static int factor_create(const fc_symbolic* h, const fc_factor_opts* opts, std::unique_ptr<fc_factor>& F,
@@ -2385,11 +2473,15 @@ static int factor_create(const fc_symbolic* h, const fc_factor_opts* opts, std::
     set_error(err, FC_INVALID_ARGUMENT, "bf16_terms must be 1 or 3 (0: the default, 3)");
     return FC_INVALID_ARGUMENT;
   }
+  if (o.fallback != 0 && o.fallback != 1) {
+    set_error(err, FC_INVALID_ARGUMENT, "fallback must be 0 (refactor in the next precision) or 1 (report)");
+    return FC_INVALID_ARGUMENT;
+  }
   F.reset(new fc_factor());
   F->sym_keep = keep(h);
   F->sym = h;
   F->device = o.device;
-  F->precision = o.precision;
+  F->precision = F->requested = o.precision;
   F->opts = o;
   try {
     HIP_CHECK(hipSetDevice(o.device));
@@ -2476,6 +2568,8 @@ int fc_refactor_device(fc_factor* f, const double* upper_val_device, uint64_t nn
 uint64_t fc_entries(const fc_factor* f) { return f ? (uint64_t)f->sym->s.entries : 0; }
 
 double fc_factor_seconds(const fc_factor* f) { return f ? f->seconds : 0.0; }
+
+int fc_factor_precision(const fc_factor* f) { return f ? f->precision : -1; }
 
 int fc_factor_profile(const fc_factor* f, int32_t kind, double* launches, double* seconds, double* flops,
                       double* entries) {

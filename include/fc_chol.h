@@ -55,9 +55,10 @@ typedef struct fc_matrix fc_matrix;
 /* Arithmetic of the numeric phase. */
 typedef enum {
   FC_FP64 = 0,  /* the plain method: every step in FP64 on the FP64 matrix cores           */
-  FC_MIXED = 1, /* Schur updates in BF16 (power-of-two scaled, FP32 accumulation), diagonal
-                   blocks and TRSM in FP32 (FP64 where a scaled pivot is below threshold);
-                   the factor is stored in FP32; solutions are refined in FP64            */
+  FC_MIXED = 1, /* Schur updates in BF16 (power-of-two scaled, FP32 accumulation), the rows
+                   below a panel (TRSM) on the FP32 matrix cores, diagonal blocks factored
+                   and inverted with FP64 arithmetic; the factor is stored in FP32; solutions
+                   are refined in FP64                                                    */
   FC_FP32 = 2   /* every step in FP32, factor stored in FP32                               */
 } fc_precision;
 
@@ -72,8 +73,15 @@ typedef struct {
                                      on the first call and replay it (the cancel flag is then
                                      polled once per call and no kind is timed); bit 1: the
                                      same for the solves, one graph per right-hand-side count */
-  double pivot_fallback;          /* FC_MIXED: relative pivot below which a diagonal block
-                                     is refactored in FP64 (0: the default, 1e-4)           */
+  int32_t fallback;               /* FC_MIXED, FC_FP32: when a pivot fails the test in the
+                                     reduced precision (an SPD matrix can lose definiteness
+                                     there), 0 (the default) refactors in the next precision
+                                     (mixed -> FP32 -> FP64, the last on FP64 storage) and
+                                     reports FC_NOT_POSITIVE_DEFINITE only when the FP64 factor
+                                     fails too, naming its column; 1 reports the first
+                                     precision's failure as it is. fc_factor_precision tells
+                                     which precision the factor holds                      */
+  int32_t pad0;
   int32_t profile_mask;           /* bit k: time every launch of kernel kind k with HIP events
                                      (fc_factor_profile); 0: none                           */
   int32_t pad;
@@ -168,8 +176,15 @@ int fc_solve_device(const fc_factor* f, double* b, uint64_t n, uint64_t nrhs, ui
  * order; `out` holds `entries` doubles. For parity checks. */
 int fc_factor_export(const fc_factor* f, double* out, uint64_t len, fc_error* err);
 
-/* Seconds the last numeric factorization took on the device (events around the launches). */
+/* Seconds the last numeric factorization took on the device (events around the launches),
+ * fallback refactorizations included. */
 double fc_factor_seconds(const fc_factor* f);
+
+/* The precision (fc_precision) of the factor's last successful factorization: the one asked for,
+ * or the one a fallback (fc_factor_opts.fallback) reached; -1 for NULL. Each factorization
+ * starts from the precision asked for, except that a factor moved to FP64 storage by a fallback
+ * stays in FP64. */
+int fc_factor_precision(const fc_factor* f);
 
 /* Kernel kind `kind` in the last numeric factorization: launches timed and their summed
  * device seconds (when opts.profile_mask selected the kind), and the kind's algorithmic flops
@@ -344,8 +359,9 @@ int fc_lanczos_device(const fc_factor* f, const fc_matrix* m, double shift, cons
                       double* values, double* vectors_device, fc_lanczos_report* report, fc_error* err);
 /* The same Lanczos run with an LDL^T factor of K - shift M (fc_ldlt_numeric): a shift between
  * modes, where K - shift M is indefinite (Symmetric with Kind::Auto under ShiftInvert,
- * analysis.rs:291-293, eigen.rs:187-211). Modes below the shift come back in report->negative. The
- * factor's stream runs the solves; calls on one factor must not overlap. */
+ * analysis.rs:291-293, eigen.rs:187-211). report->negative receives the converged negative Ritz
+ * values as eigenvalues, largest |theta| first (eigen.rs:413-426): possibly fewer than the modes
+ * below the shift. The factor's stream runs the solves; calls on one factor must not overlap. */
 int fc_lanczos_device_ldlt(const fc_ldlt* f, const fc_matrix* m, double shift, const fc_lanczos_opts* opts,
                            double* values, double* vectors_device, fc_lanczos_report* report, fc_error* err);
 

@@ -112,9 +112,9 @@ def test_cancel_flag_stops_the_iteration(gpu):
 
 def test_indefinite_shift_through_ldlt_matches_the_oracle(gpu):
     """A shift between the tet10 block's 4th and 5th modes: K - sigma M is indefinite, so the
-    modes come from an LDL^T factor (Symmetric with Kind::Auto under ShiftInvert): the four below
-    the shift as negative Ritz values, the next ones as found modes, against the oracle's Lanczos
-    over its LDL^T solves and scipy's eigsh."""
+    modes come from an LDL^T factor (Symmetric with Kind::Auto under ShiftInvert): the next four
+    modes as found ones, and the converged negative Ritz values (some of the four modes below the
+    shift, eigen.rs:413-426), against the oracle's Lanczos over its LDL^T solves and scipy's eigsh."""
     import scipy.sparse as sp
     import scipy.sparse.linalg as sla
 
@@ -137,6 +137,8 @@ def test_indefinite_shift_through_ldlt_matches_the_oracle(gpu):
     want = EG.lanczos(op, sigma, 4, block=6, max_basis=p.n, tolerance=1e-9)
     assert np.abs(res["values"] - want["values"]).max() <= 1e-9 * low.max()
     assert np.abs(res["values"] - low[4:8]).max() <= 1e-9 * low.max()
+    assert len(want["negative"]) >= 1
     assert len(res["negative"]) == len(want["negative"])
-    if len(res["negative"]):
-        assert np.abs(np.sort(res["negative"]) - np.sort(want["negative"])).max() <= 1e-9 * low.max()
+    assert np.abs(np.sort(res["negative"]) - np.sort(want["negative"])).max() <= 1e-9 * low.max()
+    for v in res["negative"]:  # each one of the modes below the shift
+        assert np.abs(low[:4] - v).min() <= 1e-9 * low.max()

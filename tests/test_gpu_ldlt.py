@@ -228,3 +228,33 @@ def test_backward_solve_in_chunks_gives_the_same_bits(gpu, monkeypatch, rows):
     assert np.array_equal(whole.view(np.uint64), chunked.view(np.uint64))
     o = O.LdltFactor(n, *O.from_symmetric(n, ptr, idx, val))
     assert np.array_equal(chunked.view(np.uint64), o.solve(b).view(np.uint64))
+
+
+@pytest.mark.parametrize("kind", ["real", "complex"])
+def test_panel_work_columns_in_device_memory_give_the_same_bits(gpu, monkeypatch, kind):
+    """The panel kernels keep a supernode's two work columns in LDS when they fit
+    (FC_LDLT_WORK_LDS bytes) and in device memory otherwise. With the budget at 0 every launch
+    takes the device-memory branch (production-size supernodes do: two columns of 3072 rows or
+    more); its factor and solution must be the oracle's bits, real and complex, on a grid whose
+    supernodes are several panels wide."""
+    from test_zldlt_oracle import complex_grid, upper as zupper
+
+    monkeypatch.setenv("FC_LDLT_WORK_LDS", "0")
+    if kind == "real":
+        a, _ = shifted_grid([16, 16, 14], 4.1)
+        n, ptr, idx, val = a
+        f = device_factor(a)
+        o = O.LdltFactor(n, *O.from_symmetric(n, ptr, idx, val))
+        b = np.cos(np.arange(n) * 0.3)
+        assert f.inertia() == o.inertia()
+    else:
+        a = complex_grid([16, 16, 14], 4.1)
+        n = a[0]
+        f = CH.ZLdltFactor.new(n, *zupper(a))
+        o = O.ZLdltFactor(n, *zupper(a))
+        b = np.cos(np.arange(n) * 0.3) + 0.5j * np.sin(np.arange(n) * 0.7)
+    assert f.two_by_two() == o.two_by_two()
+    assert np.array_equal(np.ascontiguousarray(f.values()).view(np.uint64),
+                          np.ascontiguousarray(o.values()).view(np.uint64))
+    assert np.array_equal(np.ascontiguousarray(f.solve(b)).view(np.uint64),
+                          np.ascontiguousarray(o.solve(b)).view(np.uint64))
