@@ -607,6 +607,9 @@ __global__ __launch_bounds__(256, 1) void k_potrf_blk(const Item* __restrict__ i
       *(v8bf*)(Ll + o) = l;
     }
   };
+  // xd beyond the block's width is read by the products below with a 0 weight: it must hold a
+  // finite value, not what an earlier kernel left in LDS (a NaN there times 0 is a NaN)
+  if (tid < NB) xd[tid] = 0.0;
   {  // the pw x pw square (its upper triangle is zeros in L, in bounds): 64 independent loads
     T v[NB / 2];
 #pragma unroll

@@ -51,16 +51,43 @@ size_t max_device_alloc() {
   return (size_t)std::strtoull(e, nullptr, 10);
 }
 
+// FC_POISON_ALLOC=1: every device allocation starts filled with 0xFF bytes (a NaN in FP64 and
+// FP32), so that a kernel reading memory no step of the current call has written shows up as a
+// NaN in the results instead of whatever the allocator hands back. FC_POISON_ALLOC=2 also puts
+// POISON_GUARD bytes of the same filling before and after every allocation, so that a read just
+// outside a buffer shows up the same way. A test facility; off unless the variable is set.
+int poison_alloc() {
+  const char* e = std::getenv("FC_POISON_ALLOC");
+  return e && (e[0] == '1' || e[0] == '2') ? e[0] - '0' : 0;
+}
+constexpr size_t POISON_GUARD = (size_t)4 << 20;
+std::mutex g_guard_mu;
+std::vector<void*>* g_guarded = new std::vector<void*>();  // the guarded allocations' user pointers
+
 template <typename T>
 T* dalloc(size_t count) {
   if (count == 0) count = 1;
   const size_t cap = max_device_alloc();
   if (cap && count * sizeof(T) > cap) throw std::bad_alloc();
+  const int poison = poison_alloc();
+  const size_t guard = poison == 2 ? POISON_GUARD : 0;
   void* p = nullptr;
-  hipError_t e = hipMalloc(&p, count * sizeof(T));
+  hipError_t e = hipMalloc(&p, count * sizeof(T) + 2 * guard);
   if (e != hipSuccess) {
     (void)hipGetLastError();
     throw std::bad_alloc();
+  }
+  if (poison) {
+    // the null stream does not order the library's non-blocking streams: wait for the fill
+    if (hipMemset(p, 0xFF, count * sizeof(T) + 2 * guard) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+      (void)hipFree(p);
+      throw DeviceError{hipErrorUnknown, "poisoning a new allocation"};
+    }
+  }
+  if (guard) {
+    p = (char*)p + guard;
+    std::lock_guard<std::mutex> g(g_guard_mu);
+    g_guarded->push_back(p);
   }
   return (T*)p;
 }
@@ -73,7 +100,16 @@ T* dupload(const std::vector<T>& v, hipStream_t st) {
 }
 
 void dfree(void* p) {
-  if (p) (void)hipFree(p);
+  if (!p) return;
+  {
+    std::lock_guard<std::mutex> g(g_guard_mu);
+    auto it = std::find(g_guarded->begin(), g_guarded->end(), p);
+    if (it != g_guarded->end()) {
+      g_guarded->erase(it);
+      p = (char*)p - POISON_GUARD;
+    }
+  }
+  (void)hipFree(p);
 }
 
 // A device buffer that grows on demand and is kept (workspaces reused across calls).
@@ -767,7 +803,7 @@ struct fc_ldlt {
     (void)hipSetDevice(device);
     for (void* p : {(void*)S, (void*)rows, (void*)val, (void*)piv, (void*)order, (void*)sn, (void*)U, (void*)rel,
                     (void*)ftiles, (void*)perm})
-      if (p) (void)hipFree(p);
+      dfree(p);
     if (stream) (void)hipStreamDestroy(stream);
     (void)hipSetDevice(cur);
   }

@@ -92,8 +92,10 @@ __device__ __forceinline__ void inv_apply_t(const T* __restrict__ inv, int pw, c
     for (int q = 0; q < KB; ++q) {
       const int k = 64 * h + k0 + q;
       const bool in = k < pw;
+      // Z past the block's width is not written by every caller: select it away (a 0 weight on
+      // a NaN left in LDS by an earlier kernel would still give a NaN)
 #pragma unroll
-      for (int r = 0; r < R; ++r) s[r] = fma(in ? a[q] : 0.0, Z[min(k, SNB - 1)][r], s[r]);
+      for (int r = 0; r < R; ++r) s[r] = fma(in ? a[q] : 0.0, in ? Z[min(k, SNB - 1)][r] : 0.0, s[r]);
     }
   }
 #pragma unroll

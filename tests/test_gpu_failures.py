@@ -175,6 +175,25 @@ def nearly_singular_pairs(p, delta=1e-9, pairs=3):
     return n, ptr, idx, val
 
 
+def check_fp64(f, ref, up, b, xr):
+    """The FP64 factor of the nearly singular matrix: its values within 1e-12 of the oracle's
+    (normwise) and a backward-stable solve. The solutions themselves differ from the oracle's by
+    up to cond(A) eps ~ 2e9 x 1.1e-16 along the near-null directions (two backward-stable solves
+    with different rounding), so they are held to that bound, not to 1e-12."""
+    assert np.linalg.norm(f.values() - ref.values()) <= 1e-12 * np.linalg.norm(ref.values())
+    x = f.solve(b)
+    n = len(b)
+    rows = np.repeat(np.arange(n), np.diff(up.ptr))
+    off = up.idx != rows  # the upper columns hold each off-diagonal entry once
+    ax = np.bincount(up.idx, up.val * x[rows], minlength=n) + np.bincount(rows[off], up.val[off] * x[up.idx[off]],
+                                                                          minlength=n)
+    norm = np.max(np.bincount(up.idx, np.abs(up.val), minlength=n) + np.bincount(rows[off], np.abs(up.val[off]),
+                                                                                 minlength=n))
+    berr = np.abs(b - ax).max() / (norm * np.abs(x).max() + np.abs(b).max())
+    assert berr <= 1e-15, berr
+    assert np.linalg.norm(x - xr) <= 1e-6 * np.linalg.norm(xr)
+
+
 @pytest.mark.parametrize("precision", ["fp32", "mixed"])
 def test_reduced_precision_falls_back_to_fp64_storage(gpu, monkeypatch, precision):
     """A matrix SPD in FP64 but singular in FP32: the FP32 (and mixed) factor fails, the fallback
@@ -192,7 +211,7 @@ def test_reduced_precision_falls_back_to_fp64_storage(gpu, monkeypatch, precisio
     f = CH.Factor(sym, up.val, precision=precision)
     assert f.precision_used() == "fp64"
     xr = ref.solve(b)
-    assert np.linalg.norm(f.solve(b) - xr) <= 1e-12 * np.linalg.norm(xr)
+    check_fp64(f, ref, up, b, xr)
     g = CH.Factor(sym, up.val * 0 + np.where(up.idx == np.repeat(np.arange(n), np.diff(up.ptr)), 1.0, 0.0),
                   precision=precision)  # the identity: no fallback, FP32 storage
     assert g.precision_used() == precision
@@ -202,4 +221,4 @@ def test_reduced_precision_falls_back_to_fp64_storage(gpu, monkeypatch, precisio
     monkeypatch.delenv("FC_MAX_DEVICE_ALLOC")
     g.refactor(up.val)
     assert g.precision_used() == "fp64"
-    assert np.linalg.norm(g.solve(b) - xr) <= 1e-12 * np.linalg.norm(xr)
+    check_fp64(g, ref, up, b, xr)

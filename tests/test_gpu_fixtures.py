@@ -2,7 +2,7 @@
 C ABI) against the reference's blessed end-to-end results (tests/golden/fixture_*.json, data from
 the reference's tests/oracle cases): the constant-stress patch tests on distorted hex8 and tet10
 meshes and the tet10 cantilever's tip deflection, in FP64 and in the mixed form refined to
-berr <= 1e-12, each within the fixture's own band."""
+berr <= 1e-15, each within the fixture's own band."""
 import numpy as np
 import pytest
 
@@ -32,7 +32,10 @@ def device_solve(kind, xyz, conn, young, poisson, eq_dof, uptr, uidx, rhs, preci
     order = np.lexsort((cc, rows))
     ptr = np.zeros(n + 1, np.int64)
     np.cumsum(np.bincount(rows, minlength=n), out=ptr[1:])
-    x, rep = f.solve_refined(CH.Matrix(n, ptr, cc[order], vv[order]), rhs, target_berr=1e-12)
+    # berr <= 1e-12 is the mixed form's bar, but the slender cantilever's forward error at that
+    # backward error (~5e-7 relative) is wider than its fixture's band: refine to 1e-15, where
+    # the refined solution is as accurate as the FP64 factor's
+    x, rep = f.solve_refined(CH.Matrix(n, ptr, cc[order], vv[order]), rhs, target_berr=1e-15)
     assert rep["converged"], rep
     return x
 
