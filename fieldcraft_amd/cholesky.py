@@ -108,7 +108,8 @@ class _Info(C.Structure):
                                            "levels", "stored"]] + \
                [("flops", C.c_double), ("front_flops", C.c_double), ("l_bytes_fp64", C.c_uint64),
                 ("u_bytes_fp64", C.c_uint64), ("by_blocks", C.c_int32), ("pad", C.c_int32),
-                ("launches", C.c_uint64), ("kind_flops", C.c_double * 8), ("kind_entries", C.c_double * 8)]
+                ("launches", C.c_uint64), ("kind_flops", C.c_double * 8), ("kind_entries", C.c_double * 8),
+                ("solve_bytes", C.c_uint64)]
 
 
 class _LanczosOpts(C.Structure):

@@ -568,11 +568,8 @@ Program build_program(const fc::Symbolic& s) {
       sadd(p.sbwd, S_BWD_STEP, l, i0);
     }
   }
-  p.uoff.resize(s.nf);
-  for (i64 f = 0; f < s.nf; ++f) {
-    p.uoff[f] = p.u_solve;
-    p.u_solve += s.f_m[f] - s.f_w[f];
-  }
+  p.uoff.assign(s.f_soff.begin(), s.f_soff.end());
+  p.u_solve = s.s_size;
   return p;
 }
 
@@ -2453,6 +2450,7 @@ int fc_symbolic_info_get(const fc_symbolic* h, fc_symbolic_info* info) {
     info->kind_flops[k] = h->prog.work[k][0];
     info->kind_entries[k] = h->prog.work[k][1];
   }
+  info->solve_bytes = (uint64_t)(2 * s.n + h->prog.u_solve) * sizeof(double);
   return FC_OK;
 }
 

@@ -375,6 +375,56 @@ Failure analyze(i64 n, const i64* ptr, const i64* idx, i64 max_entries, const Pl
     }
     s.u_size = arena.end;
   }
+  // The solves' per-front vectors (M - W entries a right-hand side) in an arena of their own,
+  // with the same level lifetimes: a front's contribution is written at its level and read by
+  // its parent (forward); on the way back a level's blocks are written and read within the level.
+  // 8-entry blocks keep every front's vector aligned to 64 bytes.
+  s.f_soff.assign(nf, 0);
+  {
+    Arena arena;
+    auto ssize = [&](i64 f) { return round_up(s.f_m[f] - s.f_w[f], 8); };
+    for (i64 l = 0; l < s.levels; ++l) {
+      for (i64 t = s.lvl_ptr[l]; t < s.lvl_ptr[l + 1]; ++t) {
+        const i64 f = s.lvl_fronts[t];
+        if (s.f_m[f] > s.f_w[f]) s.f_soff[f] = arena.take(ssize(f));
+      }
+      for (i64 t = s.lvl_ptr[l]; t < s.lvl_ptr[l + 1]; ++t) {
+        const i64 f = s.lvl_fronts[t];
+        for (i64 u = s.f_kptr[f]; u < s.f_kptr[f + 1]; ++u) {
+          const i64 c = s.f_kids[u];
+          if (s.f_m[c] > s.f_w[c]) arena.give(s.f_soff[c], ssize(c));
+        }
+      }
+    }
+    s.s_size = arena.end;
+  }
+  // both arenas checked: the blocks live at one level (a front's from its level to its
+  // parent's) never overlap. Levels in order: blocks whose lifetime ended leave an ordered map of
+  // the live ones, each new block is checked against its neighbours there.
+  {
+    std::vector<std::vector<i64>> born(s.levels + 1), dead(s.levels + 2);
+    for (i64 f = 0; f < nf; ++f) {
+      if (s.f_m[f] == s.f_w[f]) continue;
+      born[s.f_level[f]].push_back(f);
+      dead[(s.f_parent[f] == NONE ? s.levels : s.f_level[s.f_parent[f]]) + 1].push_back(f);
+    }
+    auto disjoint = [&](const std::vector<i64>& off, bool square) {
+      std::map<i64, i64> live;  // offset -> end
+      for (i64 l = 0; l < s.levels; ++l) {
+        for (i64 f : dead[l]) live.erase(off[f]);
+        for (i64 f : born[l]) {
+          const i64 mu = s.f_m[f] - s.f_w[f], b = off[f], e = b + (square ? mu * mu : mu);
+          auto nx = live.lower_bound(b);
+          if (nx != live.end() && nx->first < e) return false;
+          if (nx != live.begin() && std::prev(nx)->second > b) return false;
+          live.emplace(b, e);
+        }
+      }
+      return true;
+    };
+    if (!disjoint(s.f_upos, true) || !disjoint(s.f_soff, false))
+      return internal("overlapping blocks in a work arena");
+  }
   // where each entry of A lands in the L storage
   s.a_pos.resize(nnz);
   {

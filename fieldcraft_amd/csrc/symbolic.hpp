@@ -66,6 +66,9 @@ struct Symbolic {
   std::vector<i64> f_lpos, f_ld;    // storage of the front's L part (M x W, column-major)
   std::vector<i64> f_upos;          // storage of the update matrix ((M-W) x (M-W), ld M-W), -1 if none
   i64 l_size = 0, u_size = 0;       // elements
+  std::vector<i64> f_soff;          // a solve's M-W entries of the front (its contribution up the
+                                    // tree, then its rows' solution values) per right-hand side
+  i64 s_size = 0;                   // the solve arena per right-hand side (entries)
   i64 levels = 0;
   std::vector<i64> lvl_ptr, lvl_fronts;  // fronts of each level
   std::vector<i64> a_pos;           // per upper entry: position in the L storage

@@ -111,6 +111,9 @@ typedef struct {
   uint64_t launches;               /* kernel launches of one numeric factorization          */
   double kind_flops[8];            /* algorithmic flops per kernel kind (FC_KIND_*)          */
   double kind_entries[8];          /* update-matrix entries added per extend-add kind       */
+  uint64_t solve_bytes;            /* device work bytes of a solve per right-hand side: the
+                                      permuted vector, the forward result and the fronts'
+                                      contributions (fc_solve_device_noalloc's measure)      */
 } fc_symbolic_info;
 
 /* --- analysis ---------------------------------------------------------------------------- */
