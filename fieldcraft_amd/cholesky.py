@@ -138,7 +138,7 @@ _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 #: every symbol include/fc_chol.h declares
 EXPORTS = ["fc_analyze", "fc_symbolic_info_get", "fc_symbolic_export", "fc_symbolic_free", "fc_factor_numeric",
            "fc_factor_numeric_device", "fc_factor_new", "fc_refactor", "fc_refactor_device", "fc_entries", "fc_solve",
-           "fc_solve_many", "fc_solve_device", "fc_factor_export", "fc_factor_seconds", "fc_factor_precision", "fc_factor_profile",
+           "fc_solve_many", "fc_solve_device", "fc_solve_device_noalloc", "fc_factor_export", "fc_factor_seconds", "fc_factor_precision", "fc_factor_profile",
            "fc_factor_free", "fc_matrix_new", "fc_matrix_free", "fc_matrix_set_segments", "fc_solve_refined",
            "fc_solve_refined_device", "fc_solve_refined_many_device", "fc_assembly_new", "fc_assemble_device",
            "fc_assembly_info", "fc_assembly_free", "fc_lanczos_device", "fc_ldlt_numeric", "fc_ldlt_solve",
@@ -171,6 +171,7 @@ def lib():
         L.fc_solve.argtypes = [vp, _f64p, C.c_uint64, ep]
         L.fc_solve_many.argtypes = [vp, C.POINTER(C.c_void_p), C.c_uint64, C.c_uint64, ep]
         L.fc_solve_device.argtypes = [vp, vp, C.c_uint64, C.c_uint64, C.c_uint64, ep]
+        L.fc_solve_device_noalloc.argtypes = [vp, vp, C.c_uint64, C.c_uint64, C.c_uint64, ep]
         L.fc_factor_export.argtypes = [vp, _f64p, C.c_uint64, ep]
         L.fc_factor_seconds.argtypes = [vp]
         L.fc_factor_seconds.restype = C.c_double
@@ -425,6 +426,14 @@ class Factor:
         """Solve in place a column-major block of nrhs right-hand sides in device memory."""
         err = _Error()
         st = lib().fc_solve_device(self._h, C.c_void_p(b_ptr), self.n, nrhs, ldb or self.n, C.byref(err))
+        _raise(st, err)
+
+    def solve_device_noalloc(self, b_ptr: int, nrhs: int, ldb: int | None = None) -> None:
+        """solve_device without any device allocation: the work vectors live in the library's
+        fixed per-device array (fc_solve_device_noalloc); raises TooLarge when one right-hand
+        side's work does not fit it."""
+        err = _Error()
+        st = lib().fc_solve_device_noalloc(self._h, C.c_void_p(b_ptr), self.n, nrhs, ldb or self.n, C.byref(err))
         _raise(st, err)
 
     def values(self) -> np.ndarray:

@@ -2634,6 +2634,109 @@ int fc_solve_device(const fc_factor* f, double* b, uint64_t n, uint64_t nrhs, ui
   });
 }
 
+// ---------------------------------------------------------------------------------------------
+// The solve that allocates nothing (fc_solve_device_noalloc): its work vectors (the permuted
+// right-hand sides, the forward result and the fronts' contributions) live in `scratch`, one
+// copy per device, held by every device the library runs this entry point on until the process
+// ends. Host code uses the device address hipGetSymbolAddress gives for it (kept per device),
+// never the array's name. The solve kernels write every work entry before reading it, so nothing
+// has to be cleared between calls.
+
+}  // extern "C"
+
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)128 << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int NOALLOC_DEVICES = 64;
+struct NoallocDevice {
+  std::mutex mu;        // held through a call: calls on one device never overlap
+  void* ptr = nullptr;  // this device's copy of scratch
+};
+// never destroyed: no HIP call or lock may run from a static destructor
+NoallocDevice* const g_noalloc = new NoallocDevice[NOALLOC_DEVICES];
+
+// The scratch bytes a call may use: NOALLOC_SCRATCH_BYTES, or less when FC_NOALLOC_LIMIT sets a
+// lower figure (a test of the size check needs no large matrix).
+size_t noalloc_limit() {
+  size_t cap = NOALLOC_SCRATCH_BYTES;
+  if (const char* e = std::getenv("FC_NOALLOC_LIMIT")) {
+    const size_t v = (size_t)std::strtoull(e, nullptr, 10);
+    if (v > 0 && v < cap) cap = v;
+  }
+  return cap;
+}
+
+size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+// Bytes of scratch a pass over nr right-hand sides takes: x and y (n x nr each) and the
+// contributions, each block 256-byte aligned.
+size_t noalloc_need(i64 n, i64 u_solve, int nr) {
+  return 2 * align256((size_t)n * nr * sizeof(double)) + align256((size_t)std::max<i64>(u_solve * nr, 1) * sizeof(double));
+}
+
+}  // namespace
+
+extern "C" {
+
+int fc_solve_device_noalloc(const fc_factor* f, double* b, uint64_t n, uint64_t nrhs, uint64_t ldb, fc_error* err) {
+  if (!f || !f->valid) {
+    set_error(err, FC_INVALID_ARGUMENT, "no valid factor");
+    return FC_INVALID_ARGUMENT;
+  }
+  if ((i64)n != f->sym->s.n || ldb < n) {
+    set_error(err, FC_INTERNAL, "right-hand side length");
+    return FC_INTERNAL;
+  }
+  if (f->device < 0 || f->device >= NOALLOC_DEVICES) {
+    set_error(err, FC_INVALID_ARGUMENT, "device ordinal beyond the allocation-free solve's table");
+    return FC_INVALID_ARGUMENT;
+  }
+  const i64 us = f->sym->prog.u_solve;
+  const size_t cap = noalloc_limit(), one = noalloc_need((i64)n, us, 1);
+  if (one > cap) {
+    set_error(err, FC_TOO_LARGE,
+              "the solve needs " + std::to_string(one) + " bytes of device work per right-hand side; the "
+              "allocation-free solve holds " + std::to_string(cap),
+              0, one, cap);
+    return FC_TOO_LARGE;
+  }
+  if (n == 0 || nrhs == 0) {
+    set_error(err, FC_OK, "");
+    return FC_OK;
+  }
+  // as many right-hand sides a pass as fit (at most MAX_RHS, the solve's own pass width)
+  int per = (int)std::min<uint64_t>(nrhs, (uint64_t)fc::MAX_RHS);
+  while (per > 1 && noalloc_need((i64)n, us, per) > cap) --per;
+  NoallocDevice& D = g_noalloc[f->device];
+  std::lock_guard<std::mutex> gd(D.mu);
+  std::lock_guard<std::mutex> g(f->solve_mu);
+  return guarded(err, "the allocation-free solve allocated", [&] {
+    HIP_CHECK(hipSetDevice(f->device));
+    if (!D.ptr) {  // once per device, with the device current
+      void* p = nullptr;
+      HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+      D.ptr = p;
+    }
+    hipStream_t st = f->stream;
+    for (uint64_t r0 = 0; r0 < nrhs; r0 += (uint64_t)per) {
+      const int nr = (int)std::min<uint64_t>((uint64_t)per, nrhs - r0);
+      double* br = b + r0 * ldb;
+      char* base = (char*)D.ptr;
+      double* xp = (double*)base;
+      double* yp = (double*)(base + align256((size_t)n * nr * sizeof(double)));
+      double* u = (double*)(base + 2 * align256((size_t)n * nr * sizeof(double)));
+      fc::k_permute<<<blocks_for((i64)n), 256, 0, st>>>(br, xp, f->plan->perm, (i64)n, nr, (i64)ldb, (i64)n, 0);
+      if (f->precision == FC_FP64) run_solve<double>(f, xp, yp, u, (i64)n, nr, st);  // eager: no graph
+      else run_solve<float>(f, xp, yp, u, (i64)n, nr, st);
+      fc::k_permute<<<blocks_for((i64)n), 256, 0, st>>>(xp, br, f->plan->perm, (i64)n, nr, (i64)n, (i64)ldb, 1);
+    }
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(st));
+  });
+}
+
 int fc_solve_many(const fc_factor* f, double* const* bs, uint64_t nrhs, uint64_t n, fc_error* err) {
   if (!f || !f->valid) {
     set_error(err, FC_INVALID_ARGUMENT, "no valid factor");

@@ -174,6 +174,18 @@ int fc_solve_many(const fc_factor* f, double* const* bs, uint64_t nrhs, uint64_t
  * ldb, solved in place on the factor's stream. */
 int fc_solve_device(const fc_factor* f, double* b, uint64_t n, uint64_t nrhs, uint64_t ldb, fc_error* err);
 
+/* No counterpart in the reference: fc_solve_device for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The same solution, bit for bit, with no hipMalloc, hipFree or allocating library call: the
+ * work vectors live in a fixed 128 MiB device array of the library (one per device it runs on,
+ * held until the process ends). A solve needs fc_symbolic_info.solve_bytes of it per right-hand
+ * side (about 36 bytes an equation); right-hand sides are taken as many a pass as fit, and a
+ * factor needing more than the array for one right-hand side is refused with FC_TOO_LARGE
+ * (err->entries: the bytes needed, err->limit: the array's). Calls on one device share the array
+ * and are serialised by the library; launches are eager (never a graph), on the factor's stream,
+ * which is synchronised before the call returns. */
+int fc_solve_device_noalloc(const fc_factor* f, double* b, uint64_t n, uint64_t nrhs, uint64_t ldb, fc_error* err);
+
 /* The factor values in the reference layout (cholesky.rs:393-420): every supernode's lower
  * trapezoid, column by column (column c holds rows c..m of its row structure), supernodes in
  * order; `out` holds `entries` doubles. For parity checks. */
