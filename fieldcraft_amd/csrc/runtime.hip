@@ -159,6 +159,9 @@ struct Program {
   i64 w_size = 0;                      // entries of the FP64 TRSM workspace
   std::vector<fc::Item> sitems;       // solve work items
   std::vector<fc::Launch> sfwd, sbwd; // forward launches (leaves first), backward (root first)
+  std::vector<fc::Launch> sfwd1, sbwd1;  // the same for one right-hand side: each level's block
+                                         // chain in one launch (k_fwd_front / k_bwd_front)
+
   std::vector<i64> uoff;              // per front: start of its solve contribution
   i64 u_solve = 0;                    // total contribution length (one right-hand side)
   double work[fc::KINDS][2] = {};     // per kind: flops, entries
@@ -180,7 +183,16 @@ constexpr int32_t L_WAIT = 101;
 constexpr int SYNC_SLOTS = 4;
 
 // kinds of solve launches
-enum SolveKind : int32_t { S_GATHER = 0, S_FWD_STEP = 1, S_BWD_GATHER = 2, S_BWD_BELOW = 3, S_BWD_FIRST = 4, S_BWD_STEP = 5 };
+enum SolveKind : int32_t {
+  S_GATHER = 0,
+  S_FWD_STEP = 1,
+  S_BWD_GATHER = 2,
+  S_BWD_BELOW = 3,
+  S_BWD_FIRST = 4,
+  S_BWD_STEP = 5,
+  S_FWD_FRONT = 6,
+  S_BWD_FRONT = 7
+};
 static_assert(fc::SNB == fc::NB, "solve blocks are the factorization's panels");
 static_assert(fc::BCOLS == fc::SNB, "a backward column tile is one block");
 
@@ -568,6 +580,69 @@ Program build_program(const fc::Symbolic& s) {
       sadd(p.sbwd, S_BWD_STEP, l, i0);
     }
   }
+  // one right-hand side: per level the gather, then the whole block chain in one launch; the
+  // backward's gather, below and first steps, then the chain. A front's forward workgroups are
+  // in increasing rows and its backward ones in decreasing blocks: each waits only on earlier ones.
+  for (i64 l = 0; l < s.levels; ++l) {
+    const i64* fl = &s.lvl_fronts[s.lvl_ptr[l]];
+    const i64 nl = s.lvl_ptr[l + 1] - s.lvl_ptr[l];
+    i64 i0 = (i64)p.sitems.size();
+    for (i64 t = 0; t < nl; ++t) p.sitems.push_back(Item{(int32_t)fl[t], 0, 0, 0, 0, 0, 0, 0});
+    sadd(p.sfwd1, S_GATHER, l, i0);
+    i0 = (i64)p.sitems.size();
+    bool chains = false;  // a front of several blocks: the launch forms block solutions
+    i64 tiles = 0;
+    for (i64 t = 0; t < nl; ++t) {
+      chains |= s.f_w[fl[t]] > fc::SNB;
+      tiles = std::max<i64>(tiles, (s.f_m[fl[t]] + fc::FROWS - 1) / fc::FROWS);
+    }
+    if (!chains) {  // one block a front: the step launch of the many-right-hand-side lists
+      for (i64 t = 0; t < nl; ++t) {
+        const i64 f = fl[t], c1 = s.f_w[f];
+        for (i64 r = c1; r < s.f_m[f]; r += fc::SROWS)
+          p.sitems.push_back(Item{(int32_t)f, 0, (int32_t)r, 0, 0, 0, 0, 0});
+      }
+      sadd(p.sfwd1, S_FWD_STEP, l, i0);
+      continue;
+    }
+    // tile-major across the level's fronts: every front's chain starts in the first workgroups
+    for (i64 k = 0; k < tiles; ++k)
+      for (i64 t = 0; t < nl; ++t) {
+        const i64 f = fl[t], c1 = std::min<i64>(fc::SNB, s.f_w[f]), r0 = k * fc::FROWS;
+        if (r0 < s.f_m[f] && c1 < s.f_m[f] && r0 + fc::FROWS - 1 >= c1)  // a row of the tile below the first block
+          p.sitems.push_back(Item{(int32_t)f, (int32_t)r0, 0, 0, 0, 0, 0, 0});
+      }
+    sadd(p.sfwd1, S_FWD_FRONT, l, i0);
+  }
+  for (i64 l = s.levels - 1; l >= 0; --l) {
+    const i64* fl = &s.lvl_fronts[s.lvl_ptr[l]];
+    const i64 nl = s.lvl_ptr[l + 1] - s.lvl_ptr[l];
+    i64 i0 = (i64)p.sitems.size();
+    for (i64 t = 0; t < nl; ++t)
+      if (s.f_m[fl[t]] > s.f_w[fl[t]]) p.sitems.push_back(Item{(int32_t)fl[t], 0, 0, 0, 0, 0, 0, 0});
+    sadd(p.sbwd1, S_BWD_GATHER, l, i0);
+    i0 = (i64)p.sitems.size();
+    for (i64 t = 0; t < nl; ++t) {
+      const i64 f = fl[t];
+      if (s.f_m[f] == s.f_w[f]) continue;
+      for (i64 j = 0; j < s.f_w[f]; j += fc::SCOLS) p.sitems.push_back(Item{(int32_t)f, (int32_t)j, 0, 0, 0, 0, 0, 0});
+    }
+    sadd(p.sbwd1, S_BWD_BELOW, l, i0);
+    i0 = (i64)p.sitems.size();
+    for (i64 t = 0; t < nl; ++t) p.sitems.push_back(Item{(int32_t)fl[t], 0, 0, 0, 0, 0, 0, 0});
+    sadd(p.sbwd1, S_BWD_FIRST, l, i0);
+    // by distance from the front's last block, across the level's fronts: every chain starts in
+    // the first workgroups, and a block's x is formed by an earlier workgroup than its users
+    i0 = (i64)p.sitems.size();
+    i64 depth = 0;
+    for (i64 t = 0; t < nl; ++t) depth = std::max<i64>(depth, (s.f_w[fl[t]] + fc::SNB - 1) / fc::SNB - 1);
+    for (i64 k = 0; k < depth; ++k)
+      for (i64 t = 0; t < nl; ++t) {
+        const i64 f = fl[t], jb = (s.f_w[f] + fc::SNB - 1) / fc::SNB - 2 - k;
+        if (jb >= 0) p.sitems.push_back(Item{(int32_t)f, (int32_t)jb, 0, 0, 0, 0, 0, 0});
+      }
+    sadd(p.sbwd1, S_BWD_FRONT, l, i0);
+  }
   p.uoff.assign(s.f_soff.begin(), s.f_soff.end());
   p.u_solve = s.s_size;
   return p;
@@ -686,14 +761,14 @@ struct fc_factor {
   hipGraphExec_t graph_exec = nullptr;
   struct SolveGraph {                  // opts.use_graph bit 1: a captured solve
     int nr;
-    void *xp, *yp, *u;
+    void *xp, *yp, *u, *err;
     hipGraph_t g;
     hipGraphExec_t exec;
   };
   mutable std::vector<SolveGraph> sgraphs;
   Buffer vals;             // the matrix values on the device (host-value entry points)
   mutable std::mutex solve_mu;
-  mutable Buffer xp, yp, usolve, refine_work;  // solve workspaces
+  mutable Buffer xp, yp, usolve, serr, refine_work;  // solve workspaces
   double seconds = 0.0;
   double kind_seconds[fc::KINDS] = {};
   double kind_launches[fc::KINDS] = {};
@@ -721,6 +796,7 @@ struct fc_factor {
     xp.release();
     yp.release();
     usolve.release();
+    serr.release();
     refine_work.release();
     for (auto e : sync_ev)
       if (e) (void)hipEventDestroy(e);
@@ -1095,18 +1171,36 @@ int factor_host(fc_factor* F, const double* upper_val, uint64_t nnz, fc_error* e
   }
 }
 
+// A solve pass's device work: x and y (n x nr), the fronts' contributions, and the error word of
+// the one-right-hand-side chains (set when a wait gave up).
+struct SolveWork {
+  double *xp, *yp, *u;
+  int32_t* err;
+};
+
+// The polls a chained solve's wait may take before it gives up (FC_SOLVE_SPIN_LIMIT; negative:
+// every wait gives up at once, a test's way to reach the failure path).
+int64_t solve_spin_limit() {
+  if (const char* e = std::getenv("FC_SOLVE_SPIN_LIMIT")) return (int64_t)std::strtoll(e, nullptr, 10);
+  return (int64_t)1 << 22;
+}
+
 template <typename T, int R>
-void run_solve_r(const fc_factor* F, double* xp, double* yp, double* u, i64 ldx, int nrhs, hipStream_t st) {
+void run_solve_r(const fc_factor* F, const SolveWork& w, i64 ldx, int nrhs, hipStream_t st) {
   const Program& prog = F->sym->prog;
   DevicePlan& P = *F->plan;
   const T* L = (const T*)F->L;
   const T* Inv = (const T*)F->inv.p;
+  double *xp = w.xp, *yp = w.yp, *u = w.u;
+  int32_t* err = w.err;
+  const int64_t limit = solve_spin_limit();
+  const int mark = R == 1;  // the chained passes: the gathers mark the block solutions to come
   auto run = [&](const fc::Launch& l) {
     const fc::Item* it = P.sitems + l.item0;
     dim3 g(l.count), b(256);
     switch (l.kind) {
       case S_GATHER:
-        fc::k_fwd_gather<T, R><<<g, b, 0, st>>>(it, P.fronts, P.kids, P.rel, Inv, xp, yp, ldx, nrhs, u, P.uoff);
+        fc::k_fwd_gather<T, R><<<g, b, 0, st>>>(it, P.fronts, P.kids, P.rel, Inv, xp, yp, ldx, nrhs, u, P.uoff, mark);
         break;
       case S_FWD_STEP:
         fc::k_fwd_step<T, R><<<g, b, 0, st>>>(it, P.fronts, L, Inv, xp, yp, ldx, nrhs, u, P.uoff);
@@ -1118,40 +1212,52 @@ void run_solve_r(const fc_factor* F, double* xp, double* yp, double* u, i64 ldx,
         fc::k_bwd_below<T, R><<<g, b, 0, st>>>(it, P.fronts, L, u, P.uoff, yp, ldx, nrhs);
         break;
       case S_BWD_FIRST:
-        fc::k_bwd_first<T, R><<<g, b, 0, st>>>(it, P.fronts, Inv, yp, xp, ldx, nrhs);
+        fc::k_bwd_first<T, R><<<g, b, 0, st>>>(it, P.fronts, Inv, yp, xp, ldx, nrhs, mark);
         break;
       case S_BWD_STEP:
         if constexpr (R == 1) fc::k_bwd_step1<T><<<g, b, 0, st>>>(it, P.fronts, L, Inv, yp, xp);
         else fc::k_bwd_step<T, R><<<g, b, 0, st>>>(it, P.fronts, L, Inv, yp, xp, ldx, nrhs);
         break;
+      case S_FWD_FRONT:
+        fc::k_fwd_front<T><<<g, b, fc::SNB * fc::SNB * sizeof(T), st>>>(it, P.fronts, L, Inv, xp, yp, u, P.uoff, err,
+                                                                         limit);
+        break;
+      case S_BWD_FRONT:
+        fc::k_bwd_front<T><<<g, b, 0, st>>>(it, P.fronts, L, Inv, yp, xp, err, limit);
+        break;
     }
   };
-  for (const fc::Launch& l : prog.sfwd) run(l);
-  for (const fc::Launch& l : prog.sbwd) run(l);
+  if constexpr (R == 1) {
+    for (const fc::Launch& l : prog.sfwd1) run(l);
+    for (const fc::Launch& l : prog.sbwd1) run(l);
+  } else {
+    for (const fc::Launch& l : prog.sfwd) run(l);
+    for (const fc::Launch& l : prog.sbwd) run(l);
+  }
   HIP_CHECK(hipGetLastError());
 }
 
 template <typename T>
-void run_solve(const fc_factor* F, double* xp, double* yp, double* u, i64 ldx, int nrhs, hipStream_t st) {
-  if (nrhs == 1) run_solve_r<T, 1>(F, xp, yp, u, ldx, nrhs, st);
-  else if (nrhs <= fc::RC / 2) run_solve_r<T, fc::RC / 2>(F, xp, yp, u, ldx, nrhs, st);  // a Lanczos block
-  else run_solve_r<T, fc::RC>(F, xp, yp, u, ldx, nrhs, st);
+void run_solve(const fc_factor* F, const SolveWork& w, i64 ldx, int nrhs, hipStream_t st) {
+  if (nrhs == 1) run_solve_r<T, 1>(F, w, ldx, nrhs, st);
+  else if (nrhs <= fc::RC / 2) run_solve_r<T, fc::RC / 2>(F, w, ldx, nrhs, st);  // a Lanczos block
+  else run_solve_r<T, fc::RC>(F, w, ldx, nrhs, st);
 }
 
-// The solve's launches for nr right-hand sides in xp/yp/u, replayed from a hipGraph when
+// The solve's launches for nr right-hand sides in w, replayed from a hipGraph when
 // opts.use_graph asks for it (bit 1): captured on first use for each (nr, workspaces).
-void solve_launches(const fc_factor* F, double* xp, double* yp, double* u, i64 n, int nr, hipStream_t st) {
+void solve_launches(const fc_factor* F, const SolveWork& w, i64 n, int nr, hipStream_t st) {
   auto eager = [&]() {
-    if (F->precision == FC_FP64) run_solve<double>(F, xp, yp, u, n, nr, st);
-    else run_solve<float>(F, xp, yp, u, n, nr, st);
+    if (F->precision == FC_FP64) run_solve<double>(F, w, n, nr, st);
+    else run_solve<float>(F, w, n, nr, st);
   };
   if (!(F->opts.use_graph & 2)) return eager();
   for (const auto& sg : F->sgraphs)
-    if (sg.nr == nr && sg.xp == xp && sg.yp == yp && sg.u == u) {
+    if (sg.nr == nr && sg.xp == w.xp && sg.yp == w.yp && sg.u == w.u && sg.err == w.err) {
       HIP_CHECK(hipGraphLaunch(sg.exec, st));
       return;
     }
-  fc_factor::SolveGraph sg{nr, xp, yp, u, nullptr, nullptr};
+  fc_factor::SolveGraph sg{nr, w.xp, w.yp, w.u, w.err, nullptr, nullptr};
   HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
   try {
     eager();
@@ -1166,21 +1272,42 @@ void solve_launches(const fc_factor* F, double* xp, double* yp, double* u, i64 n
   HIP_CHECK(hipGraphLaunch(sg.exec, st));
 }
 
+// The error word of a chained solve: read (waiting for the stream), cleared when set; a set word
+// means a wait gave up and the solution is not valid.
+void solve_check(int32_t* err, hipStream_t st) {
+  int32_t e = 0;
+  HIP_CHECK(hipMemcpyAsync(&e, err, sizeof e, hipMemcpyDeviceToHost, st));
+  HIP_CHECK(hipStreamSynchronize(st));
+  if (e) {
+    fc::k_fill<int32_t><<<1, 64, 0, st>>>(err, 1, 0);
+    HIP_CHECK(hipStreamSynchronize(st));
+    throw DeviceError{hipErrorLaunchTimeOut, "a solve's wait for a block solution ran past its limit"};
+  }
+}
+
 // b (device, n x nrhs, ldb) solved in place, MAX_RHS right-hand sides at a time; the caller
-// holds solve_mu
+// holds solve_mu. Returns once the stream has finished (the error word is read).
 void solve_device(const fc_factor* F, double* b, i64 n, int nrhs, i64 ldb, hipStream_t st) {
   if (n == 0 || nrhs == 0) return;
+  if (!F->serr.p) {  // the error word starts clear (and is cleared whenever it is found set)
+    F->serr.get(sizeof(int32_t));
+    fc::k_fill<int32_t><<<1, 64, 0, st>>>((int32_t*)F->serr.p, 1, 0);
+  }
+  int32_t* err = (int32_t*)F->serr.p;
   for (int r0 = 0; r0 < nrhs; r0 += fc::MAX_RHS) {
     const int nr = std::min(fc::MAX_RHS, nrhs - r0);
     double* br = b + (i64)r0 * ldb;
-    double* xp = (double*)F->xp.get((size_t)n * nr * sizeof(double));
-    double* yp = (double*)F->yp.get((size_t)n * nr * sizeof(double));
-    double* u = (double*)F->usolve.get((size_t)std::max<i64>(F->sym->prog.u_solve * nr, 1) * sizeof(double));
-    fc::k_permute<<<blocks_for(n), 256, 0, st>>>(br, xp, F->plan->perm, n, nr, ldb, n, 0);
-    solve_launches(F, xp, yp, u, n, nr, st);
-    fc::k_permute<<<blocks_for(n), 256, 0, st>>>(xp, br, F->plan->perm, n, nr, n, ldb, 1);
+    SolveWork w;
+    w.xp = (double*)F->xp.get((size_t)n * nr * sizeof(double));
+    w.yp = (double*)F->yp.get((size_t)n * nr * sizeof(double));
+    w.u = (double*)F->usolve.get((size_t)std::max<i64>(F->sym->prog.u_solve * nr, 1) * sizeof(double));
+    w.err = err;
+    fc::k_permute<<<blocks_for(n), 256, 0, st>>>(br, w.xp, F->plan->perm, n, nr, ldb, n, 0);
+    solve_launches(F, w, n, nr, st);
+    fc::k_permute<<<blocks_for(n), 256, 0, st>>>(w.xp, br, F->plan->perm, n, nr, n, ldb, 1);
   }
   HIP_CHECK(hipGetLastError());
+  solve_check(err, st);
 }
 
 constexpr i64 RED_ROWS = 8192;  // rows of a reduction block
@@ -2422,6 +2549,9 @@ int fc_analyze(uint64_t n, const uint64_t* ptr, const uint64_t* idx, uint64_t ma
   } catch (const std::bad_alloc&) {
     set_error(err, FC_OUT_OF_MEMORY, "not enough host memory for the analysis");
     return FC_OUT_OF_MEMORY;
+  } catch (const fc::Failure& f) {
+    set_error(err, f.status, f.message);
+    return f.status;
   }
 }
 
@@ -2450,7 +2580,7 @@ int fc_symbolic_info_get(const fc_symbolic* h, fc_symbolic_info* info) {
     info->kind_flops[k] = h->prog.work[k][0];
     info->kind_entries[k] = h->prog.work[k][1];
   }
-  info->solve_bytes = (uint64_t)(2 * s.n + h->prog.u_solve) * sizeof(double);
+  info->solve_bytes = (uint64_t)(2 * s.n + h->prog.u_solve) * sizeof(double) + 256;
   return FC_OK;
 }
 
@@ -2670,10 +2800,12 @@ size_t noalloc_limit() {
 
 size_t align256(size_t b) { return (b + 255) / 256 * 256; }
 
-// Bytes of scratch a pass over nr right-hand sides takes: x and y (n x nr each) and the
-// contributions, each block 256-byte aligned.
+// Bytes of scratch a pass over nr right-hand sides takes: x and y (n x nr each), the
+// contributions, and the chained solve's error word (the last 256 bytes), each block 256-byte
+// aligned.
 size_t noalloc_need(i64 n, i64 u_solve, int nr) {
-  return 2 * align256((size_t)n * nr * sizeof(double)) + align256((size_t)std::max<i64>(u_solve * nr, 1) * sizeof(double));
+  return 2 * align256((size_t)n * nr * sizeof(double)) + align256((size_t)std::max<i64>(u_solve * nr, 1) * sizeof(double)) +
+         256;
 }
 
 }  // namespace
@@ -2720,20 +2852,25 @@ int fc_solve_device_noalloc(const fc_factor* f, double* b, uint64_t n, uint64_t 
       D.ptr = p;
     }
     hipStream_t st = f->stream;
+    // the chained solve's error word at the array's end, cleared first
+    int32_t* serr = (int32_t*)((char*)D.ptr + cap - 256);
+    fc::k_fill<int32_t><<<1, 64, 0, st>>>(serr, 1, 0);
     for (uint64_t r0 = 0; r0 < nrhs; r0 += (uint64_t)per) {
       const int nr = (int)std::min<uint64_t>((uint64_t)per, nrhs - r0);
       double* br = b + r0 * ldb;
       char* base = (char*)D.ptr;
-      double* xp = (double*)base;
-      double* yp = (double*)(base + align256((size_t)n * nr * sizeof(double)));
-      double* u = (double*)(base + 2 * align256((size_t)n * nr * sizeof(double)));
-      fc::k_permute<<<blocks_for((i64)n), 256, 0, st>>>(br, xp, f->plan->perm, (i64)n, nr, (i64)ldb, (i64)n, 0);
-      if (f->precision == FC_FP64) run_solve<double>(f, xp, yp, u, (i64)n, nr, st);  // eager: no graph
-      else run_solve<float>(f, xp, yp, u, (i64)n, nr, st);
-      fc::k_permute<<<blocks_for((i64)n), 256, 0, st>>>(xp, br, f->plan->perm, (i64)n, nr, (i64)n, (i64)ldb, 1);
+      SolveWork w;
+      w.xp = (double*)base;
+      w.yp = (double*)(base + align256((size_t)n * nr * sizeof(double)));
+      w.u = (double*)(base + 2 * align256((size_t)n * nr * sizeof(double)));
+      w.err = serr;
+      fc::k_permute<<<blocks_for((i64)n), 256, 0, st>>>(br, w.xp, f->plan->perm, (i64)n, nr, (i64)ldb, (i64)n, 0);
+      if (f->precision == FC_FP64) run_solve<double>(f, w, (i64)n, nr, st);  // eager: no graph
+      else run_solve<float>(f, w, (i64)n, nr, st);
+      fc::k_permute<<<blocks_for((i64)n), 256, 0, st>>>(w.xp, br, f->plan->perm, (i64)n, nr, (i64)n, (i64)ldb, 1);
     }
     HIP_CHECK(hipGetLastError());
-    HIP_CHECK(hipStreamSynchronize(st));
+    solve_check(serr, st);
   });
 }
 

@@ -29,6 +29,9 @@ constexpr int SCOLS = 32;     // columns of a below-the-front workgroup (backwar
 constexpr int BCOLS = 128;    // columns of a backward block-step workgroup
 constexpr int RC = 16;        // right-hand sides of the multi-column kernels
 constexpr int MAX_RHS = 64;   // right-hand sides one solve pass handles (the host splits larger blocks)
+// A block solution not yet formed (the chained one-right-hand-side solve below): a signalling
+// NaN, which no arithmetic result is.
+constexpr uint64_t PENDING = 0x7FF0DEAD0000BEEFull;
 
 // Where a thread's loads would otherwise wait on branches, they are made in batches from clamped
 // (always in-bounds) addresses and the out-of-range values are selected away afterwards, so a
@@ -113,11 +116,15 @@ template <typename T, int R>
 __global__ __launch_bounds__(256, 2) void k_fwd_gather(const Item* __restrict__ items, const FrontDev* __restrict__ fronts,
                                                     const int32_t* __restrict__ kids, const int32_t* __restrict__ rel,
                                                     const T* __restrict__ Inv, double* v, double* __restrict__ y,
-                                                    int64_t ldx, int nrhs, double* u, const int64_t* __restrict__ uoff) {
+                                                    int64_t ldx, int nrhs, double* u, const int64_t* __restrict__ uoff,
+                                                    int mark) {
   __shared__ double Vb[SNB][R + 1];
   const int f = items[blockIdx.x].f;
   const FrontDev F = fronts[f];
   const int w = F.w, mu = F.m - F.w, pw = min(SNB, w);
+  if (mark)  // one right-hand side, chained: y of the later blocks still to come (k_fwd_front)
+    for (int e = pw + threadIdx.x; e < w; e += blockDim.x)
+      reinterpret_cast<uint64_t*>(y)[F.col0 + e] = PENDING;
   double* uf = u + uoff[f] * nrhs;
   for (int e = threadIdx.x; e < mu * nrhs; e += blockDim.x) uf[e] = 0.0;
   __syncthreads();
@@ -354,11 +361,13 @@ __global__ __launch_bounds__(256) void k_bwd_below(const Item* __restrict__ item
 template <typename T, int R>
 __global__ __launch_bounds__(256) void k_bwd_first(const Item* __restrict__ items, const FrontDev* __restrict__ fronts,
                                                    const T* __restrict__ Inv, const double* __restrict__ z,
-                                                   double* __restrict__ x, int64_t ldx, int nrhs) {
+                                                   double* __restrict__ x, int64_t ldx, int nrhs, int mark) {
   __shared__ double Zb[SNB][R + 1];
   __shared__ double H[2][SNB][R + 1];
   const FrontDev F = fronts[items[blockIdx.x].f];
   const int cl = (F.w - 1) / SNB * SNB, pw = F.w - cl;
+  if (mark)  // one right-hand side, chained: x of the earlier blocks still to come (k_bwd_front)
+    for (int e = threadIdx.x; e < cl; e += blockDim.x) reinterpret_cast<uint64_t*>(x)[F.col0 + e] = PENDING;
   for (int q0 = 0; q0 < nrhs; q0 += R) {
     const int nr = min(R, nrhs - q0);
     if (threadIdx.x < pw) {
@@ -540,6 +549,252 @@ __global__ __launch_bounds__(256) void k_bwd_step1(const Item* __restrict__ item
     H[hh][b] = t;
     __syncthreads();
     if (tid < SNB) x[F.col0 + j0 + tid] = H[0][tid] + H[1][tid];
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// One right-hand side: a level's chain of blocks in one launch each way.
+//
+// The blocks of a front depend on each other in sequence (y of block b needs every earlier
+// block's product with its rows; x of block b every later block's), which the launches above
+// follow with one launch per block. Here every workgroup of a level's launch stays resident and
+// takes its rows (forward) or columns (backward) through the blocks in order. A block's solution
+// formed by another workgroup is waited for value by value: the launch before (k_fwd_gather /
+// k_bwd_first with `mark`) fills the block solutions still to come with PENDING, a signalling NaN
+// that no arithmetic produces, the forming workgroup writes each value with a device-coherent
+// atomic store, and a waiting thread polls its own value with device-coherent atomic loads until
+// it is not PENDING. No fence is needed: each value is its own signal. A workgroup waits only on
+// workgroups of its own front placed earlier in the launch (lower ids), so the waits resolve in
+// dispatch order; every wait is bounded, and one that runs past `limit` polls sets *err and gives
+// up (the host reports the solve as failed), so every wave exits. The loads that do not depend on
+// a block solution (the rows' L values, the next inverse) are issued before the wait. Per row
+// (forward) and column (backward) the arithmetic and its order are those of k_fwd_step and
+// k_bwd_step1 / k_bwd_first: the solution is the same bit for bit.
+
+constexpr int FROWS = 2 * SNB;  // rows of a forward workgroup (one a thread)
+
+__device__ __forceinline__ void publish(double* p, double v) {
+  __hip_atomic_store(reinterpret_cast<uint64_t*>(p), __builtin_bit_cast(uint64_t, v), __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The value at p once published; bounded (limit < 0: give up at once, a test's way in).
+__device__ __forceinline__ double await_value(double* p, int64_t limit, int32_t* err) {
+  uint64_t* q = reinterpret_cast<uint64_t*>(p);
+  uint64_t b = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (limit < 0) {
+    __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __builtin_bit_cast(double, b);
+  }
+  int64_t n = 0;
+  while (b == PENDING) {
+    ++n;
+    if (n > limit || ((n & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
+      __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      break;
+    }
+    __builtin_amdgcn_s_sleep(1);
+    b = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  return __builtin_bit_cast(double, b);
+}
+
+// Forward, item (f, a = r0): rows [r0, r0 + FROWS) of front f through every block above them.
+// y of block 0 comes from k_fwd_gather (an earlier launch); y of a block whose rows are the
+// workgroup's is formed here, kept in LDS for the next block and published. A row's L values of
+// the next block are loaded while the current block is multiplied (PF of them; all of a block's
+// in FP32). Dynamic LDS: SNB^2 T for the inverse of a block to form, on launches that form any
+// (0 otherwise, so the launches of levels without chains keep their occupancy).
+template <typename T>
+__global__ __launch_bounds__(256) void k_fwd_front(const Item* __restrict__ items, const FrontDev* __restrict__ fronts,
+                                                   const T* __restrict__ L, const T* __restrict__ Inv, double* v,
+                                                   double* y, double* u, const int64_t* __restrict__ uoff,
+                                                   int32_t* err, int64_t limit) {
+  constexpr int IQ = SNB * SNB / 256;          // inverse entries a thread stages
+  constexpr int PF = sizeof(T) == 4 ? 64 : 32;  // L values of the next block loaded ahead
+  __shared__ double Yb[SNB][2];
+  __shared__ double Vn[SNB][2];
+  extern __shared__ __align__(16) unsigned char fwd_dyn[];
+  T* Is = reinterpret_cast<T*>(fwd_dyn);  // the next block's inverse, column-major as in Inv
+  const Item it = items[blockIdx.x];
+  const FrontDev F = fronts[it.f];
+  const int r0 = it.a, w = F.w, m = F.m;
+  const int tid = threadIdx.x, i = r0 + tid;
+  const bool row = i < m;
+  double* p = i < w ? v + F.col0 + i : u + uoff[it.f] + i - w;  // the row's entry of v (or of u below)
+  double val = row ? *p : 0.0;
+  const uint32_t lo = min(i, m - 1) * sizeof(T);  // rows past the front load its last row (not stored)
+  const int nb = (w + SNB - 1) / SNB;
+  // the row's L values of block b (first PF of them), clamped to the block's width
+  auto fetch = [&](int b, T (&dst)[PF]) {
+    const int c0 = b * SNB, pw = min(SNB, w - c0);
+    const __amdgpu_buffer_rsrc_t rs = raw_rsrc(L + F.lpos + (int64_t)c0 * F.ld);
+#pragma unroll
+    for (int q = 0; q < PF; ++q) dst[q] = buf_load<T>(rs, lo, (uint32_t)min(q, pw - 1) * F.ld * sizeof(T));
+  };
+  T cur[PF], nxt[PF];
+  fetch(0, cur);
+  for (int b = 0; b < nb; ++b) {
+    const int c0 = b * SNB, pw = min(SNB, w - c0), c1 = c0 + pw;
+    if (c1 >= m || c1 > r0 + FROWS - 1) break;  // no row of the workgroup lies below this block (or a later one)
+    const int pn = min(SNB, w - c1);
+    const bool form = c1 < w && c1 >= r0;  // the next block's rows are the workgroup's: its y is formed here
+    // ahead of any wait: the next block's L values, and the inverse of the block to form
+    const bool more = b + 1 < nb && c1 + pn < m && c1 + pn <= r0 + FROWS - 1;
+    if (more) fetch(b + 1, nxt);
+    T ia[IQ];
+    if (form) {
+      const T* inv = Inv + F.inv0 + (int64_t)(c1 / SNB) * SNB * SNB;
+#pragma unroll
+      for (int q = 0; q < IQ; ++q) ia[q] = inv[min(tid + 256 * q, pn * pn - 1)];
+    }
+    if (c0 < r0 || b == 0) {  // y of the block formed elsewhere: k_fwd_gather (b = 0) or an earlier workgroup
+      if (tid < pw) Yb[tid][0] = b == 0 ? y[F.col0 + c0 + tid] : await_value(y + F.col0 + c0 + tid, limit, err);
+      __syncthreads();
+    }
+    // the row's sum over the block, j ascending (k_fwd_step's order)
+    double s = 0.0;
+    if (pw == SNB) {
+      // the rest of the row's values, in batches of RB issued one ahead of the batch multiplied
+      constexpr int RB = sizeof(T) == 4 ? 32 : 16, NR = (SNB - PF) / RB;
+      static_assert(NR * RB == SNB - PF, "the batches cover the block");
+      const __amdgpu_buffer_rsrc_t rs = raw_rsrc(L + F.lpos + (int64_t)c0 * F.ld);
+      T rb[2][RB];
+#pragma unroll
+      for (int q = 0; q < RB; ++q) rb[0][q] = buf_load<T>(rs, lo, (uint32_t)(PF + q) * F.ld * sizeof(T));
+#pragma unroll
+      for (int q = 0; q < PF; ++q) s = fma((double)cur[q], Yb[q][0], s);
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        if (r + 1 < NR)
+#pragma unroll
+          for (int q = 0; q < RB; ++q) rb[(r + 1) & 1][q] = buf_load<T>(rs, lo, (uint32_t)(PF + (r + 1) * RB + q) * F.ld * sizeof(T));
+#pragma unroll
+        for (int q = 0; q < RB; ++q) s = fma((double)rb[r & 1][q], Yb[PF + r * RB + q][0], s);
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < PF; ++q)
+        if (q < pw) s = fma((double)cur[q], Yb[q][0], s);
+      if (PF < SNB) {
+        const __amdgpu_buffer_rsrc_t rs = raw_rsrc(L + F.lpos + (int64_t)c0 * F.ld);
+        for (int j0 = PF; j0 < pw; j0 += 32) {
+          T l[32];
+#pragma unroll
+          for (int q = 0; q < 32; ++q) l[q] = buf_load<T>(rs, lo, (uint32_t)min(j0 + q, pw - 1) * F.ld * sizeof(T));
+#pragma unroll
+          for (int q = 0; q < 32; ++q)
+            if (j0 + q < pw) s = fma((double)l[q], Yb[j0 + q][0], s);
+        }
+      }
+    }
+    if (row && i >= c1) val -= s;
+    if (form) {
+      if (row && i >= c1 && i < c1 + pn) Vn[i - c1][0] = val;
+#pragma unroll
+      for (int q = 0; q < IQ; ++q)
+        if (tid + 256 * q < pn * pn) Is[tid + 256 * q] = ia[q];
+      __syncthreads();  // (every read of Yb above is done too)
+      if (tid < pn) {
+        double yn = 0.0;  // inv_apply's sum for one right-hand side: k ascending, one fma each
+        if (pn == SNB) {  // batches of LDS reads issued ahead of their products
+#pragma unroll
+          for (int k0 = 0; k0 < SNB; k0 += 32) {
+            T a[32];
+            double vv[32];
+#pragma unroll
+            for (int q = 0; q < 32; ++q) {
+              a[q] = Is[(k0 + q) * SNB + tid];
+              vv[q] = Vn[k0 + q][0];
+            }
+#pragma unroll
+            for (int q = 0; q < 32; ++q) yn = fma((double)a[q], vv[q], yn);
+          }
+        } else {
+          for (int k = 0; k < pn; ++k) yn = fma((double)Is[k * pn + tid], Vn[k][0], yn);
+        }
+        Yb[tid][0] = yn;
+        publish(y + F.col0 + c1 + tid, yn);
+      }
+    }
+    __syncthreads();  // Yb whole for the next block, and every read of it done
+    if (more)
+#pragma unroll
+      for (int q = 0; q < PF; ++q) cur[q] = nxt[q];
+  }
+  if (row) *p = val;
+}
+
+// Backward, item (f, a = jb): the columns of block jb (a full block, not the front's last)
+// through every later block, last first; then x of block jb, published. x of the front's last
+// block comes from k_bwd_first (an earlier launch). A front's items are in decreasing jb, so the
+// workgroup forming a block's x precedes every workgroup waiting for it.
+template <typename T>
+__global__ __launch_bounds__(256) void k_bwd_front(const Item* __restrict__ items, const FrontDev* __restrict__ fronts,
+                                                   const T* __restrict__ L, const T* __restrict__ Inv, double* z,
+                                                   double* x, int32_t* err, int64_t limit) {
+  __shared__ T Lt[SNB][SNB + 1];  // Lt[j][k] = L(c0 + k, j0 + j)
+  __shared__ double Xb[SNB];
+  __shared__ double Zn[SNB];
+  __shared__ double part[2][4][64];
+  __shared__ double H[2][SNB];
+  const Item it = items[blockIdx.x];
+  const FrontDev F = fronts[it.f];
+  const int jb = it.a, j0 = jb * SNB;
+  const int nb = (F.w + SNB - 1) / SNB;
+  const int tid = threadIdx.x, jq = tid & 63, g = tid >> 6;
+  const int lk = tid & (SNB - 1), jh = tid >> 7;
+  // half jh of column lk of the block's inverse, for x of the block at the end
+  T ca[64];
+  {
+    const T* col = Inv + F.inv0 + (int64_t)jb * SNB * SNB + (int64_t)lk * SNB + 64 * jh;
+#pragma unroll
+    for (int q = 0; q < 64; ++q) ca[q] = col[q];
+  }
+  double zv = tid < SNB ? z[F.col0 + j0 + tid] : 0.0;
+  for (int b = nb - 1; b > jb; --b) {
+    const int c0 = b * SNB, pw = min(SNB, F.w - c0);
+    const __amdgpu_buffer_rsrc_t rs = raw_rsrc(L + F.lpos + (int64_t)j0 * F.ld + c0);
+    const uint32_t lo = min(lk, pw - 1) * sizeof(T);
+    {  // the tile into LDS before any wait (it does not depend on x)
+      T vt[SNB / 2];
+#pragma unroll
+      for (int q = 0; q < SNB / 2; ++q) vt[q] = buf_load<T>(rs, lo, (uint32_t)(jh + 2 * q) * F.ld * sizeof(T));
+#pragma unroll
+      for (int q = 0; q < SNB / 2; ++q) Lt[jh + 2 * q][lk] = vt[q];
+    }
+    if (tid < SNB) {
+      const double xv = b == nb - 1 ? x[F.col0 + c0 + min(tid, pw - 1)]
+                                    : await_value(x + F.col0 + c0 + min(tid, pw - 1), limit, err);
+      Xb[tid] = tid < pw ? xv : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      double t = 0.0;
+#pragma unroll
+      for (int r = 0; r < 32; ++r)
+        if (32 * g + r < pw) t = fma((double)Lt[64 * h + jq][32 * g + r], Xb[32 * g + r], t);
+      part[h][g][jq] = t;
+    }
+    __syncthreads();
+    if (tid < SNB) {
+      const int h = tid >> 6, jj = tid & 63;
+      const double d = (part[h][0][jj] + part[h][1][jj]) + (part[h][2][jj] + part[h][3][jj]);
+      zv = zv - d;
+      Zn[tid] = zv;
+    }
+    __syncthreads();
+  }
+  // x of block jb = Inv_jb^T z_jb (k_bwd_step1's designated item)
+  double t = 0.0;
+#pragma unroll
+  for (int q = 0; q < 64; ++q) t = fma((double)ca[q], Zn[64 * jh + q], t);
+  H[jh][lk] = t;
+  __syncthreads();
+  if (tid < SNB) {
+    z[F.col0 + j0 + tid] = zv;
+    publish(x + F.col0 + j0 + tid, H[0][tid] + H[1][tid]);
   }
 }
 
