@@ -605,13 +605,17 @@ Program build_program(const fc::Symbolic& s) {
       sadd(p.sfwd1, S_FWD_STEP, l, i0);
       continue;
     }
-    // tile-major across the level's fronts: every front's chain starts in the first workgroups
-    for (i64 k = 0; k < tiles; ++k)
-      for (i64 t = 0; t < nl; ++t) {
-        const i64 f = fl[t], c1 = std::min<i64>(fc::SNB, s.f_w[f]), r0 = k * fc::FROWS;
-        if (r0 < s.f_m[f] && c1 < s.f_m[f] && r0 + fc::FROWS - 1 >= c1)  // a row of the tile below the first block
-          p.sitems.push_back(Item{(int32_t)f, (int32_t)r0, 0, 0, 0, 0, 0, 0});
-      }
+    // tile-major across the level's fronts, the tiles of the fronts' own rows (the chains) first:
+    // every chain starts in the first workgroups, and the tiles below, which only follow the
+    // chains, never hold a slot a chain tile waits for
+    for (int below = 0; below < 2; ++below)
+      for (i64 k = 0; k < tiles; ++k)
+        for (i64 t = 0; t < nl; ++t) {
+          const i64 f = fl[t], c1 = std::min<i64>(fc::SNB, s.f_w[f]), r0 = k * fc::FROWS;
+          if ((r0 >= s.f_w[f]) != (below == 1)) continue;
+          if (r0 < s.f_m[f] && c1 < s.f_m[f] && r0 + fc::FROWS - 1 >= c1)  // a row of the tile below the first block
+            p.sitems.push_back(Item{(int32_t)f, (int32_t)r0, 0, 0, 0, 0, 0, 0});
+        }
     sadd(p.sfwd1, S_FWD_FRONT, l, i0);
   }
   for (i64 l = s.levels - 1; l >= 0; --l) {
