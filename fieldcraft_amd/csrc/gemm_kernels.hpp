@@ -547,6 +547,90 @@ __device__ inline void wave_potrf_inv(double (*S)[NB + 1], double* xd, int o, in
     }
 }
 
+// The same wave-local Cholesky and inverse in FP32 arithmetic, for the FP32-storage forms
+// (their diagonal blocks are FP32 work; the block's values are FP32 already). The steps are
+// those of wpi_steps, with the broadcast column zeroed outside the rows and columns a step
+// changes (L(i, j) for i <= j, L(k, j) for k <= j): the trailing updates then need no selects,
+// an update with a zero weight leaving its entry as it was. Entries above the diagonal of L's
+// diagonal register blocks take meaningless (finite) values and are never read or stored.
+// A pivot not above PIVOT_FP32 |A_jj| (or not finite) sets fb: the caller then factors the
+// whole block again in FP64, where the reference's pivot test decides.
+constexpr float PIVOT_FP32 = 1e-3f;
+
+__device__ __forceinline__ float readlane_f(float v, int l) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+}
+
+template <int CB>
+__device__ __forceinline__ void wpi32_steps(float (&a)[8][8], float (&x)[8][8], int h, int lane, float dl, bool& fb) {
+  const int ti = lane & 7, tk = lane >> 3;
+  for (int cc = 0; cc < 8; ++cc) {
+    const int j = 8 * CB + cc;
+    if (j >= h) return;
+    const float p = readlane_f(a[CB][CB], cc + 8 * cc);  // the pivot, on the step's dependent chain
+    const float a0 = readlane_f(dl, j);
+    fb = fb || !(p > PIVOT_FP32 * a0) || !(p < __builtin_huge_valf());
+    const float y = __builtin_amdgcn_rsqf(p);
+    const float rd = y * fmaf(-0.5f * p * y, y, 1.5f);  // 1 / sqrt(p), one Newton step
+    const float fc = tk == cc ? rd : 1.0f, fr = ti == cc ? rd : 1.0f;
+#pragma unroll
+    for (int r = CB; r < 8; ++r) a[r][CB] *= fc;  // column j of L (the pivot becomes p rd = L(j, j))
+#pragma unroll
+    for (int c = 0; c <= CB; ++c) x[CB][c] *= fr;  // row j of X divided by L(j, j)
+    float li[8], lk[8], xj[8];
+#pragma unroll
+    for (int r = CB; r < 8; ++r) li[r] = __shfl(a[r][CB], ti + 8 * cc);  // L(ti + 8r, j)
+#pragma unroll
+    for (int c = CB; c < 8; ++c) lk[c] = __shfl(a[c][CB], tk + 8 * cc);  // L(tk + 8c, j)
+#pragma unroll
+    for (int c = 0; c <= CB; ++c) xj[c] = __shfl(x[CB][c], cc + 8 * tk);  // X(j, tk + 8c)
+    li[CB] = ti > cc ? li[CB] : 0.0f;
+    lk[CB] = tk > cc ? lk[CB] : 0.0f;
+#pragma unroll
+    for (int r = CB; r < 8; ++r) {
+#pragma unroll
+      for (int c = CB; c <= r; ++c) a[r][c] = fmaf(-li[r], lk[c], a[r][c]);  // L(i, k) -= L(i, j) L(k, j)
+#pragma unroll
+      for (int c = 0; c <= CB; ++c) x[r][c] = fmaf(-li[r], xj[c], x[r][c]);  // X(i, :) -= L(i, j) X(j, :)
+    }
+  }
+}
+
+__device__ inline void wave_potrf_inv32(double (*S)[NB + 1], double* xd, int o, int h, int lane,
+                                        const double* __restrict__ dg, bool& fb) {
+  const int ti = lane & 7, tk = lane >> 3;
+  const float dl = lane < h ? (float)fabs(dg[lane]) : 0.0f;
+  float a[8][8], x[8][8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r)
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int i = ti + 8 * r, k = tk + 8 * c;
+      const float v = (float)S[o + i][o + k];  // in bounds for every lane (o + 64 <= NB)
+      a[r][c] = (i < h && k < h && i >= k) ? v : (i == k ? 1.0f : 0.0f);
+      x[r][c] = i == k ? 1.0f : 0.0f;
+    }
+  wpi32_steps<0>(a, x, h, lane, dl, fb);
+  wpi32_steps<1>(a, x, h, lane, dl, fb);
+  wpi32_steps<2>(a, x, h, lane, dl, fb);
+  wpi32_steps<3>(a, x, h, lane, dl, fb);
+  wpi32_steps<4>(a, x, h, lane, dl, fb);
+  wpi32_steps<5>(a, x, h, lane, dl, fb);
+  wpi32_steps<6>(a, x, h, lane, dl, fb);
+  wpi32_steps<7>(a, x, h, lane, dl, fb);
+#pragma unroll
+  for (int r = 0; r < 8; ++r)
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int i = ti + 8 * r, k = tk + 8 * c;
+      if (i < h && k < h) {
+        if (i >= k) S[o + i][o + k] = a[r][c];
+        if (i > k) S[o + k][o + i] = x[r][c];
+        if (i == k) xd[o + i] = x[r][c];
+      }
+    }
+}
+
 // One wave's 64 x 16 strip of a product over k in [0, 64) on the FP64 MFMA:
 //   acc[ib][v] += sum_k col(t, k) row(i, k),  i = 16 ib + lane % 16,  t = (lane / 16) + 4 v
 // where col(r, k) and row(i, k) return in-bounds LDS values (col's r is the strip column in
@@ -575,6 +659,7 @@ __global__ __launch_bounds__(256, 1) void k_potrf_blk(const Item* __restrict__ i
   constexpr int H = 64;
   __shared__ double S[NB][NB + 1];
   __shared__ double xd[NB];
+  __shared__ int redo;  // the FP32 attempt met a small pivot: factor again in FP64
   const Item it = items[blockIdx.x];
   const FrontDev F = fronts[it.f];
   const int c0 = it.a, pw = min(NB, F.w - c0);
@@ -607,10 +692,11 @@ __global__ __launch_bounds__(256, 1) void k_potrf_blk(const Item* __restrict__ i
       *(v8bf*)(Ll + o) = l;
     }
   };
+  // the pw x pw square (its upper triangle is zeros in L, in bounds): 64 independent loads.
   // xd beyond the block's width is read by the products below with a 0 weight: it must hold a
   // finite value, not what an earlier kernel left in LDS (a NaN there times 0 is a NaN)
-  if (tid < NB) xd[tid] = 0.0;
-  {  // the pw x pw square (its upper triangle is zeros in L, in bounds): 64 independent loads
+  auto load = [&]() {
+    if (tid < NB) xd[tid] = 0.0;
     T v[NB / 2];
 #pragma unroll
     for (int q = 0; q < NB / 2; ++q) {
@@ -619,61 +705,91 @@ __global__ __launch_bounds__(256, 1) void k_potrf_blk(const Item* __restrict__ i
     }
 #pragma unroll
     for (int q = 0; q < NB / 2; ++q) S[rr][jp + 2 * q] = (double)v[q];
-  }
-  __syncthreads();
+  };
+  // each wave: strip columns t0 + [0, 16) of a 64 x 64 product, rows 16 ib + lane % 16
+  const int r = lane & 15, q = lane >> 4, t0 = 16 * wave;
+  // L11 and X11; for a wide panel then L21 = A21 X11^T, A22 -= L21 L21^T, L22 and X22. F32:
+  // the wave steps in FP32 arithmetic (a small pivot sets redo); otherwise FP64 with the
+  // reference's pivot test.
   bool bad = false;
-  if (wave == 0) wave_potrf_inv<sizeof(T) == 8>(S, xd, 0, h0, lane, dg, col, fail, bad);  // L11, X11
-  __syncthreads();
-  if (h1 > 0) {
-    // each wave: strip columns t0 + [0, 16) of a 64 x 64 product, rows 16 ib + lane % 16
-    const int r = lane & 15, q = lane >> 4, t0 = 16 * wave;
-    {  // L21 = A21 X11^T: L21(i, t) = sum_{k <= t} A21(i, k) X11(t, k), X11(t, k) = S[k][t] (t > k)
-      v4d acc[4] = {};
-      strip64(
-          acc, lane,
-          [&](int c, int k) {  // 0/1 weights, not selects: the loads stay unconditional
-            const int t = t0 + c;
-            return S[k][t] * (t > k ? 1.0 : 0.0) + xd[k] * (t == k ? 1.0 : 0.0);
-          },
-          [&](int i, int k) { return S[h0 + min(i, h1 - 1)][k]; });
-      __syncthreads();  // every A21 read before L21 is stored over it
-#pragma unroll
-      for (int ib = 0; ib < 4; ++ib)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int i = 16 * ib + r;
-          if (i < h1) S[h0 + i][t0 + q + 4 * v] = acc[ib][v];
-        }
+  auto factor = [&](auto f32) {
+    constexpr bool F32 = decltype(f32)::value;
+    bool fb = false;
+    if (wave == 0) {
+      if constexpr (F32) wave_potrf_inv32(S, xd, 0, h0, lane, dg, fb);
+      else wave_potrf_inv<sizeof(T) == 8>(S, xd, 0, h0, lane, dg, col, fail, bad);
     }
     __syncthreads();
-    {  // A22 -= L21 L21^T over the lower triangle (reads the left half, writes the right)
-      v4d acc[4] = {};
-      strip64(
-          acc, lane, [&](int c, int k) { return S[h0 + min(t0 + c, h1 - 1)][k]; },
-          [&](int i, int k) { return S[h0 + min(i, h1 - 1)][k]; });
+    if (h1 > 0) {
+      {  // L21 = A21 X11^T: L21(i, t) = sum_{k <= t} A21(i, k) X11(t, k), X11(t, k) = S[k][t] (t > k)
+        v4d acc[4] = {};
+        strip64(
+            acc, lane,
+            [&](int c, int k) {  // 0/1 weights, not selects: the loads stay unconditional
+              const int t = t0 + c;
+              return S[k][t] * (t > k ? 1.0 : 0.0) + xd[k] * (t == k ? 1.0 : 0.0);
+            },
+            [&](int i, int k) { return S[h0 + min(i, h1 - 1)][k]; });
+        __syncthreads();  // every A21 read before L21 is stored over it
 #pragma unroll
-      for (int ib = 0; ib < 4; ++ib)
+        for (int ib = 0; ib < 4; ++ib)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int i = 16 * ib + r, j = t0 + q + 4 * v;
-          if (i < h1 && j <= i) S[h0 + i][h0 + j] -= acc[ib][v];
-        }
-    }
-    __syncthreads();
-    if (wave == 0) wave_potrf_inv<sizeof(T) == 8>(S, xd, h0, h1, lane, dg + h0, col + h0, fail, bad);  // L22, X22
-    __syncthreads();
-    // L is final: write it out (reads first, then stores), and its copies
-    {
-      T v[NB / 2];
+          for (int v = 0; v < 4; ++v) {
+            const int i = 16 * ib + r;
+            if (i < h1) S[h0 + i][t0 + q + 4 * v] = acc[ib][v];
+          }
+      }
+      __syncthreads();
+      {  // A22 -= L21 L21^T over the lower triangle (reads the left half, writes the right)
+        v4d acc[4] = {};
+        strip64(
+            acc, lane, [&](int c, int k) { return S[h0 + min(t0 + c, h1 - 1)][k]; },
+            [&](int i, int k) { return S[h0 + min(i, h1 - 1)][k]; });
 #pragma unroll
-      for (int qq = 0; qq < NB / 2; ++qq) v[qq] = (T)S[rr][jp + 2 * qq];
+        for (int ib = 0; ib < 4; ++ib)
 #pragma unroll
-      for (int qq = 0; qq < NB / 2; ++qq) {
-        const int j = jp + 2 * qq;
-        if (rr < pw && j <= rr) base[(int64_t)j * F.ld + rr] = v[qq];
+          for (int v = 0; v < 4; ++v) {
+            const int i = 16 * ib + r, j = t0 + q + 4 * v;
+            if (i < h1 && j <= i) S[h0 + i][h0 + j] -= acc[ib][v];
+          }
+      }
+      __syncthreads();
+      if (wave == 0) {
+        if constexpr (F32) wave_potrf_inv32(S, xd, h0, h1, lane, dg + h0, fb);
+        else wave_potrf_inv<sizeof(T) == 8>(S, xd, h0, h1, lane, dg + h0, col + h0, fail, bad);
       }
     }
-    copies();
+    if constexpr (F32) {
+      if (wave == 0 && lane == 0) redo = fb ? 1 : 0;
+    }
+    __syncthreads();
+  };
+  load();
+  __syncthreads();
+  if constexpr (sizeof(T) == 4) {
+    factor(std::true_type{});
+    if (redo) {  // uniform: read by every thread after the barrier that ends factor()
+      __syncthreads();  // every thread has read redo before S is loaded again
+      load();
+      __syncthreads();
+      factor(std::false_type{});
+    }
+  } else {
+    factor(std::false_type{});
+  }
+  // L is final: write it out (reads first, then stores), and its copies
+  {
+    T v[NB / 2];
+#pragma unroll
+    for (int qq = 0; qq < NB / 2; ++qq) v[qq] = (T)S[rr][jp + 2 * qq];
+#pragma unroll
+    for (int qq = 0; qq < NB / 2; ++qq) {
+      const int j = jp + 2 * qq;
+      if (rr < pw && j <= rr) base[(int64_t)j * F.ld + rr] = v[qq];
+    }
+  }
+  copies();
+  if (h1 > 0) {
     {  // T = L21 X11: T(i, t) = sum_{k >= t} L21(i, k) X11(k, t), X11(k, t) = S[t][k] (k > t)
       v4d acc[4] = {};
       strip64(
@@ -708,16 +824,6 @@ __global__ __launch_bounds__(256, 1) void k_potrf_blk(const Item* __restrict__ i
           if (ii < h1) S[t0 + q + 4 * v][h0 + ii] = -acc[ib][v];
         }
     }
-  } else {
-    T v[NB / 2];
-#pragma unroll
-    for (int q = 0; q < NB / 2; ++q) v[q] = (T)S[rr][jp + 2 * q];
-#pragma unroll
-    for (int q = 0; q < NB / 2; ++q) {
-      const int j = jp + 2 * q;
-      if (rr < pw && j <= rr) base[(int64_t)j * F.ld + rr] = v[q];
-    }
-    copies();
   }
   __syncthreads();
   {  // the inverse, column-major with ld = pw, zeros above the diagonal: reads, then stores

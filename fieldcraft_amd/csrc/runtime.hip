@@ -145,6 +145,18 @@ void set_error(fc_error* err, int kind, const std::string& msg, uint64_t column 
   std::snprintf(err->message, sizeof err->message, "%s", msg.c_str());
 }
 
+// The factorization's schedule: how many panels the side stream updates together and how far
+// GEMM launches are split along k. The defaults are measured at hex8 80^3; the tuner
+// (fc_tune) times candidates on the caller's matrix and keeps the fastest.
+struct Schedule {
+  int32_t slab = 4;                // panels of a side-stream slab update (1: one-panel lookahead)
+  int32_t split_target = 1024;     // tiles the Schur-complement launches are split up to
+  // The side stream's slab updates only have to keep ahead of the main stream's panel chain:
+  // they are split less, so they leave CUs to the chain's kernels and need fewer partial sums.
+  int32_t split_side_fp64 = 384;
+  int32_t split_side_fp32 = 256;
+};
+
 // The launches of the numeric phase and of the solves, built once per analysis.
 struct Program {
   std::vector<fc::Item> items;         // extend-add and diagonal-block items
@@ -165,6 +177,7 @@ struct Program {
   std::vector<i64> uoff;              // per front: start of its solve contribution
   i64 u_solve = 0;                    // total contribution length (one right-hand side)
   double work[fc::KINDS][2] = {};     // per kind: flops, entries
+  Schedule sched;                     // the schedule the launches were built with
 };
 
 // TRSM launches (Launch.pad): the in-place product run by the FP32 and mixed factors, and the
@@ -212,12 +225,6 @@ struct Tile {
 // Output tiles a launch should have to fill the GPU (256 CUs, two workgroups each, and some
 // slack); a launch with fewer is split along k into pieces of at least SPLIT_MIN_K.
 constexpr i64 SPLIT_TARGET = 1024;
-// The side stream's panel updates only have to keep ahead of the main stream's panel chain: they
-// are split less, so they leave CUs to the chain's kernels and need fewer partial sums. How much
-// less depends on the chain's length: measured at 80^3, the FP64 factor is as fast at 384 tiles
-// as at 1024, the FP32 and mixed ones are fastest from 192 to 384 (slower at 96 and below).
-constexpr i64 SPLIT_TARGET_SIDE_FP64 = 384;
-constexpr i64 SPLIT_TARGET_SIDE_FP32 = 256;
 constexpr i64 SPLIT_MIN_K = 512;
 
 // One GEMM launch of `tiles` (kind: UPDATE, TRSM or SYRK), split along k when the tiles are
@@ -272,10 +279,11 @@ void add_gemm(Program& p, const std::vector<Tile>& tiles, int kind, i64 level, i
   p.part_size = std::max(p.part_size, part);
 }
 
-Program build_program(const fc::Symbolic& s) {
+Program build_program(const fc::Symbolic& s, const Schedule& sched) {
   using fc::Item;
   using fc::Launch;
   Program p;
+  p.sched = sched;
   double flops = 0, entries = 0;
   auto add = [&](int kind, i64 level, i64 item0) {
     const i64 count = (i64)p.items.size() - item0;
@@ -332,28 +340,30 @@ Program build_program(const fc::Symbolic& s) {
     i64 wmax = 0;
     for (i64 t = 0; t < nl; ++t) wmax = std::max(wmax, s.f_w[fl[t]]);
     std::vector<Tile> tiles;
-    // The panel at c0 minus its products with the front's columns [k0, k1) (all before c0).
-    auto update = [&](i64 c0, i64 k0, i64 k1, i64 split_target) {
+    // The columns [c0, c1) (whole panels) minus their products with the front's columns
+    // [k0, k1) (all before c0): one tile column per panel, from the panel's diagonal down.
+    auto update = [&](i64 c0, i64 c1, i64 k0, i64 k1, i64 split_target) {
       tiles.clear();
       for (i64 t = 0; t < nl; ++t) {
         const i64 f = fl[t];
-        if (s.f_w[f] <= c0) continue;
-        const i64 c1 = std::min(c0 + fc::NB, s.f_w[f]);
-        for (i64 r = c0; r < s.f_m[f]; r += fc::GB) {
-          const i64 m = std::min<i64>(fc::GB, s.f_m[f] - r);
-          fc::GemmItem g{};
-          g.a = s.f_lpos[f] + k0 * s.f_ld[f] + r;
-          g.b = s.f_lpos[f] + k0 * s.f_ld[f] + c0;
-          g.c = s.f_lpos[f] + c0 * s.f_ld[f] + r;
-          g.lda = g.ldb = g.ldc = (int32_t)s.f_ld[f];
-          g.m = (int32_t)m;
-          g.n = (int32_t)(c1 - c0);
-          g.k = (int32_t)(k1 - k0);
-          g.mode = fc::G_SUB;
-          g.diag = (int32_t)(r - c0);
-          g.asrc = g.bsrc = g.csrc = fc::S_L;
-          g.front = (int32_t)f;
-          tiles.push_back(Tile{g, 2.0 * (double)(k1 - k0) * lower_pairs(r, r + m, c0, c1)});
+        for (i64 b0 = c0; b0 < std::min(c1, s.f_w[f]); b0 += fc::NB) {
+          const i64 b1 = std::min(b0 + fc::NB, s.f_w[f]);
+          for (i64 r = b0; r < s.f_m[f]; r += fc::GB) {
+            const i64 m = std::min<i64>(fc::GB, s.f_m[f] - r);
+            fc::GemmItem g{};
+            g.a = s.f_lpos[f] + k0 * s.f_ld[f] + r;
+            g.b = s.f_lpos[f] + k0 * s.f_ld[f] + b0;
+            g.c = s.f_lpos[f] + b0 * s.f_ld[f] + r;
+            g.lda = g.ldb = g.ldc = (int32_t)s.f_ld[f];
+            g.m = (int32_t)m;
+            g.n = (int32_t)(b1 - b0);
+            g.k = (int32_t)(k1 - k0);
+            g.mode = fc::G_SUB;
+            g.diag = (int32_t)(r - b0);
+            g.asrc = g.bsrc = g.csrc = fc::S_L;
+            g.front = (int32_t)f;
+            tiles.push_back(Tile{g, 2.0 * (double)(k1 - k0) * lower_pairs(r, r + m, b0, b1)});
+          }
         }
       }
       add_gemm(p, tiles, fc::K_UPDATE, l, split_target);
@@ -447,40 +457,47 @@ Program build_program(const fc::Symbolic& s) {
       q.slot = slot;
       p.launches.push_back(q);
     };
-    // Panels of NB columns, left-looking inside each front (cholesky.rs:1138-1186), one panel
-    // of lookahead: the side stream takes panel k + 1's products with panels 0 .. k - 1 while
-    // the main stream factors panel k; the main stream then subtracts panel k's product from
-    // panel k + 1 and factors it. The side update starts only once the main stream's own short
-    // product of the step is done (it needs panel k - 1, which that product follows), so that
-    // product never competes with it for the CUs. Events: slot k % 2 marks "the main stream's
-    // product of panel k done", slot 2 + k % 2 "panel k's side update done"; each is waited on
-    // before it is recorded again.
+    // Panels of NB columns, left-looking inside each front (cholesky.rs:1138-1186), grouped in
+    // slabs of S panels (SW = S NB columns) with one slab of lookahead: the side stream takes
+    // slab j + 1's products with the columns before slab j (one product SW columns wide) while
+    // the main stream factors slab j; the main stream then subtracts slab j's product from
+    // slab j + 1 and factors its panels, each after the products with the slab's earlier
+    // panels. The side update starts only once the main stream's own product with slab j - 1 is
+    // done (that product follows slab j - 1's panels), so the two never compete for the CUs.
+    // Events: slot j % 2 marks "the main stream's product of slab j done", slot 2 + j % 2 "slab
+    // j's side update done"; each is waited on before it is recorded again. S = 1 is the plain
+    // one-panel lookahead.
     const i64 np = (wmax + fc::NB - 1) / fc::NB;
-    auto side = [&](i64 k) {  // panel k's products with panels [0, k - 1)
-      sync(L_WAIT, 1, (int)((k - 1) % 2));
+    const i64 S = std::max<i64>(1, sched.slab), SW = S * fc::NB, nsl = (np + S - 1) / S;
+    auto side = [&](i64 j) {  // slab j's products with the columns [0, (j - 1) SW)
+      sync(L_WAIT, 1, (int)((j - 1) % 2));
       const double before = p.work[fc::K_UPDATE][0];
       for (const int variant : {SIDE_FP64, SIDE_FP32}) {
         const i64 from = (i64)p.launches.size();
-        update(k * fc::NB, 0, (k - 1) * fc::NB, variant == SIDE_FP64 ? SPLIT_TARGET_SIDE_FP64 : SPLIT_TARGET_SIDE_FP32);
+        update(j * SW, (j + 1) * SW, 0, (j - 1) * SW, variant == SIDE_FP64 ? sched.split_side_fp64 : sched.split_side_fp32);
         for (i64 q = from; q < (i64)p.launches.size(); ++q) {
           p.launches[q].stream = 1;
           p.launches[q].pad = variant;
         }
         if (variant == SIDE_FP64) p.work[fc::K_UPDATE][0] = before;  // count the products once
       }
-      sync(L_RECORD, 1, 2 + (int)(k % 2));
+      sync(L_RECORD, 1, 2 + (int)(j % 2));
     };
-    for (i64 k = 0; k < np; ++k) {
-      const i64 c0 = k * fc::NB;
-      if (k >= 2) sync(L_WAIT, 0, 2 + (int)(k % 2));
-      if (k >= 1) {
-        update(c0, c0 - fc::NB, c0, SPLIT_TARGET);  // the product with the panel just finished
-        if (k + 1 < np) {
-          sync(L_RECORD, 0, (int)(k % 2));
-          side(k + 1);
+    for (i64 j = 0; j < nsl; ++j) {
+      const i64 s0 = j * SW;
+      if (j >= 2) sync(L_WAIT, 0, 2 + (int)(j % 2));
+      if (j >= 1) {
+        update(s0, s0 + SW, s0 - SW, s0, 0);  // the product with the slab just finished
+        if (j + 1 < nsl) {
+          sync(L_RECORD, 0, (int)(j % 2));
+          side(j + 1);
         }
       }
-      chain(c0);
+      for (i64 k = j * S; k < std::min((j + 1) * S, np); ++k) {
+        const i64 c0 = k * fc::NB;
+        if (c0 > s0) update(c0, c0 + fc::NB, s0, c0, 0);  // the slab's earlier panels
+        chain(c0);
+      }
     }
     // update matrices U = -L21 L21^T
     tiles.clear();
@@ -507,7 +524,7 @@ Program build_program(const fc::Symbolic& s) {
           tiles.push_back(Tile{g, 2.0 * (double)w * lower_pairs(a, a + m, b, b + nn)});
         }
     }
-    add_gemm(p, tiles, fc::K_SYRK, l);
+    add_gemm(p, tiles, fc::K_SYRK, l, sched.split_target);
     i0 = (i64)p.items.size();
     for (i64 t = 0; t < nl; ++t) {
       const i64 f = fl[t];
@@ -2541,7 +2558,7 @@ int fc_analyze(uint64_t n, const uint64_t* ptr, const uint64_t* idx, uint64_t ma
         for (i64 t = s.upper_ptr[k]; t < s.upper_ptr[k + 1]; ++t)
           if (s.upper_idx[t] == k) h->diag_src[inv[k]] = t;
     }
-    h->prog = build_program(s);
+    h->prog = build_program(s, Schedule{});
     {
       Registry& R = registry();
       std::lock_guard<std::mutex> g(R.mu);
