@@ -172,6 +172,7 @@ def main():
     ap.add_argument("--models", type=int, default=512)
     ap.add_argument("--tet-cells", type=int, default=50)
     ap.add_argument("--calibrate-copy", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default schedule (no autotuning)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU: the factorization of one matrix is not split across GPUs here "
@@ -220,8 +221,24 @@ def main():
     # timed with events in the timed steps: the kinds of the headline's dominant kernel only
     profile = ["syrk", "update"] if args.form == "mixed" else ["syrk", "update", "trsm"]
     legs = {}
+    tuned = {}
+
+    def tune(leg):
+        # the autotuner (fc_tune) times candidate schedules on this matrix, outside the timed
+        # steps, and installs the fastest for the leg's factorizations
+        if args.no_tune:
+            tuned[leg] = dict(chosen=sym.schedule(), tuned=False)
+            return
+        t0 = time.perf_counter()
+        rep = sym.tune(dvals.data_ptr(), precision=leg, repeats=1)
+        tuned[leg] = dict(chosen=rep["chosen"], tuned=True, seconds_start=rep["seconds_start"],
+                          seconds_chosen=rep["seconds_chosen"], candidates=len(rep["tried"]),
+                          tune_s=time.perf_counter() - t0)
+        log(f"bench: {leg} schedule {tuned[leg]}")
+
     if not args.no_fp64:
         log("bench: fp64 leg")
+        tune("fp64")
         legs["fp64"] = run_leg(CH, sym, dvals.data_ptr(), A, b.data_ptr(), x.data_ptr(), p.n, "fp64", args.steps,
                                args.warmup, [], breakdown=breakdown)
         x64 = x.cpu().numpy().copy()
@@ -229,6 +246,7 @@ def main():
     form = args.form
     if form != "fp64":
         log(f"bench: {form} leg")
+        tune(form)
         legs[form] = run_leg(CH, sym, dvals.data_ptr(), A, b.data_ptr(), x.data_ptr(), p.n, form, args.steps,
                              args.warmup, profile, bf16_terms=args.bf16_terms, breakdown=breakdown)
         xf = x.cpu().numpy()
@@ -289,6 +307,7 @@ def main():
                             equations=p.n, factor_entries=info["entries"], flops=F, form=form,
                             target_berr=1e-12),
                 roofline=roof)
+    line["schedule"] = tuned  # per leg: the schedule its factorizations ran (the tuner's pick)
     # The factor's bound over its precision classes: Schur products (SYRK, UPDATE) at the MFMA
     # peak of their class (three BF16 products per flop in the mixed form), TRSM at the FP32
     # (FP64) peak, diagonal blocks (FP64 arithmetic) at the FP64 peak.

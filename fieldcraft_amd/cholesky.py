@@ -131,6 +131,29 @@ class _RefineReport(C.Structure):
                 ("berr_history", C.c_double * 64)]
 
 
+class _Schedule(C.Structure):
+    _fields_ = [("slab", C.c_int32), ("split_target", C.c_int32), ("split_side_fp64", C.c_int32),
+                ("split_side_fp32", C.c_int32)]
+
+
+class _TuneOpts(C.Structure):
+    _fields_ = [("precision", C.c_int32), ("device", C.c_int32), ("stream", C.c_void_p), ("repeats", C.c_int32),
+                ("pad", C.c_int32)]
+
+
+TUNE_MAX = 32
+
+
+class _TuneReport(C.Structure):
+    _fields_ = [("chosen", _Schedule), ("seconds_start", C.c_double), ("seconds_chosen", C.c_double),
+                ("candidates", C.c_int32), ("pad", C.c_int32), ("tried", _Schedule * TUNE_MAX),
+                ("tried_seconds", C.c_double * TUNE_MAX)]
+
+
+def _sched_dict(c) -> dict:
+    return {k: int(getattr(c, k)) for k, _ in _Schedule._fields_}
+
+
 _lib = None
 _u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -143,7 +166,7 @@ EXPORTS = ["fc_analyze", "fc_symbolic_info_get", "fc_symbolic_export", "fc_symbo
            "fc_solve_refined_device", "fc_solve_refined_many_device", "fc_assembly_new", "fc_assemble_device",
            "fc_assembly_info", "fc_assembly_free", "fc_lanczos_device", "fc_ldlt_numeric", "fc_ldlt_solve",
            "fc_ldlt_inertia", "fc_ldlt_export", "fc_ldlt_free", "fc_zldlt_numeric", "fc_zldlt_solve",
-           "fc_lanczos_device_ldlt",
+           "fc_lanczos_device_ldlt", "fc_symbolic_schedule_get", "fc_symbolic_schedule_set", "fc_tune",
            "fc_version"]
 
 #: kernel kinds of the numeric phase (FC_KIND_* in include/fc_chol.h)
@@ -161,6 +184,9 @@ def lib():
         L.fc_symbolic_export.argtypes = [vp] + [vp] * 8
         L.fc_symbolic_free.argtypes = [vp]
         L.fc_symbolic_free.restype = None
+        L.fc_symbolic_schedule_get.argtypes = [vp, C.POINTER(_Schedule)]
+        L.fc_symbolic_schedule_set.argtypes = [vp, C.POINTER(_Schedule), ep]
+        L.fc_tune.argtypes = [vp, vp, C.c_uint64, C.POINTER(_TuneOpts), C.POINTER(_TuneReport), ep]
         L.fc_factor_numeric.argtypes = [vp, _f64p, C.c_uint64, C.POINTER(_Opts), C.POINTER(vp), ep]
         L.fc_factor_numeric_device.argtypes = [vp, vp, C.c_uint64, C.POINTER(_Opts), C.POINTER(vp), ep]
         L.fc_refactor.argtypes = [vp, _f64p, C.c_uint64, ep]
@@ -212,6 +238,16 @@ def _raise(status: int, err: _Error):
     cls = _KINDS.get(status, Internal)
     msg = err.message.decode(errors="replace")
     raise cls(msg or cls.kind, column=int(err.column), entries=int(err.entries), limit=int(err.limit))
+
+
+def _raise_status(status: int):
+    if status != _OK:
+        cls = _KINDS.get(status, Internal)
+        raise cls(cls.kind)
+
+
+def _precision(p) -> int:
+    return PRECISIONS[p] if isinstance(p, str) else int(p)
 
 
 def _u64(a):
@@ -272,6 +308,36 @@ class Symbolic:
         out["kind_flops"] = {KINDS[k]: i.kind_flops[k] for k in range(len(KINDS))}
         out["kind_entries"] = {KINDS[k]: i.kind_entries[k] for k in range(len(KINDS))}
         return out
+
+    def schedule(self) -> dict:
+        """The analysis's schedule: slab (panels a side-stream update covers), split targets."""
+        c = _Schedule()
+        _raise_status(lib().fc_symbolic_schedule_get(self._h, C.byref(c)))
+        return _sched_dict(c)
+
+    def set_schedule(self, **kw) -> None:
+        """Rebuild the launch program for a new schedule (keys of schedule(); the others keep
+        their values). Factors made afterwards use it."""
+        cur = self.schedule()
+        unknown = set(kw) - set(cur)
+        if unknown:
+            raise ValueError(f"unknown schedule keys {sorted(unknown)}")
+        cur.update(kw)
+        c = _Schedule(**cur)
+        err = _Error()
+        _raise(lib().fc_symbolic_schedule_set(self._h, C.byref(c), C.byref(err)), err)
+
+    def tune(self, device_values: int, nnz: int | None = None, precision: str | int = "mixed", device: int = 0,
+             repeats: int = 0, stream: int | None = None) -> dict:
+        """Time candidate schedules on this matrix (upper values in device memory) and install
+        the fastest; returns the report (chosen, seconds_start, seconds_chosen, tried)."""
+        o = _TuneOpts(_precision(precision), int(device), C.c_void_p(stream) if stream else None, int(repeats), 0)
+        r = _TuneReport()
+        err = _Error()
+        _raise(lib().fc_tune(self._h, C.c_void_p(device_values), self.nnz if nnz is None else int(nnz), C.byref(o),
+                             C.byref(r), C.byref(err)), err)
+        return dict(chosen=_sched_dict(r.chosen), seconds_start=r.seconds_start, seconds_chosen=r.seconds_chosen,
+                    tried=[dict(_sched_dict(r.tried[q]), seconds=r.tried_seconds[q]) for q in range(r.candidates)])
 
     def export(self) -> dict:
         """The reference's symbolic outputs: perm, first, rptr, rows, uptr, updates (k, p1, p2),

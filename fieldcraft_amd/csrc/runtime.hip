@@ -670,8 +670,34 @@ Program build_program(const fc::Symbolic& s, const Schedule& sched) {
 }
 
 // The plan on one device.
+// A device's copy of the analysis's structure (fronts, maps, orderings), shared by every
+// program of the analysis on that device.
+struct DeviceStruct {
+  int device = -1;
+  fc::FrontDev* fronts = nullptr;
+  int32_t* kids = nullptr;
+  int32_t* rel = nullptr;
+  int32_t* rows = nullptr;
+  int64_t* a_pos = nullptr;
+  int64_t* diag_src = nullptr;
+  int64_t* perm = nullptr;
+  ~DeviceStruct() {
+    if (device < 0) return;
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(device);
+    for (void* q : {(void*)fronts, (void*)kids, (void*)rel, (void*)rows, (void*)a_pos, (void*)diag_src, (void*)perm})
+      dfree(q);
+    (void)hipSetDevice(cur);
+  }
+};
+
+// One program's items on one device, with the device's structure arrays (held, and their
+// pointers copied here for the launches).
 struct DevicePlan {
   int device = -1;
+  const Program* prog = nullptr;  // the program the items come from (held by its users)
+  std::shared_ptr<DeviceStruct> st;
   fc::FrontDev* fronts = nullptr;
   int32_t* kids = nullptr;
   int32_t* rel = nullptr;
@@ -689,9 +715,7 @@ struct DevicePlan {
     int cur = 0;
     (void)hipGetDevice(&cur);
     (void)hipSetDevice(device);
-    for (void* q : {(void*)fronts, (void*)kids, (void*)rel, (void*)rows, (void*)a_pos, (void*)diag_src, (void*)perm,
-                    (void*)items, (void*)gitems, (void*)ritems, (void*)sitems, (void*)uoff})
-      dfree(q);
+    for (void* q : {(void*)items, (void*)gitems, (void*)ritems, (void*)sitems, (void*)uoff}) dfree(q);
     (void)hipSetDevice(cur);
   }
 };
@@ -707,52 +731,87 @@ struct fc_symbolic {
   // LDL^T's host plan, built on the first factorization and shared by the later ones
   mutable std::mutex ldlt_mu;
   mutable std::shared_ptr<const LdltPlan> ldlt_cache;
-  Program prog;
   std::vector<int64_t> diag_src;  // per postorder column: the upper entry of its diagonal
-  std::mutex mu;
+  mutable std::mutex mu;
+  std::shared_ptr<const Program> prog;  // the current program (a factor keeps the one it began with)
+  std::vector<std::shared_ptr<DeviceStruct>> structs;
   std::vector<std::shared_ptr<DevicePlan>> plans;
 
-  std::shared_ptr<DevicePlan> plan(int device, hipStream_t st) {
+  std::shared_ptr<const Program> program() const {
     std::lock_guard<std::mutex> g(mu);
-    for (auto& p : plans)
-      if (p->device == device) return p;
+    return prog;
+  }
+  // A new current program: the cached device plans of other programs are dropped (factors
+  // that use them keep them alive).
+  void install(std::shared_ptr<const Program> p) {
+    std::lock_guard<std::mutex> g(mu);
+    prog = std::move(p);
+    plans.erase(std::remove_if(plans.begin(), plans.end(),
+                               [&](const std::shared_ptr<DevicePlan>& q) { return q->prog != prog.get(); }),
+                plans.end());
+  }
+
+  // The device plan of program pr (cached unless `cache` is false: the tuner's candidates).
+  std::shared_ptr<DevicePlan> plan(int device, const std::shared_ptr<const Program>& pr, hipStream_t st,
+                                   bool cache = true) {
+    std::lock_guard<std::mutex> g(mu);
+    if (cache)
+      for (auto& p : plans)
+        if (p->device == device && p->prog == pr.get()) return p;
+    std::shared_ptr<DeviceStruct> ds;
+    for (auto& d : structs)
+      if (d->device == device) ds = d;
+    if (!ds) {
+      ds = std::make_shared<DeviceStruct>();
+      ds->device = device;
+      const auto& S = s;
+      std::vector<fc::FrontDev> fd(S.nf);
+      for (i64 f = 0; f < S.nf; ++f) {
+        fc::FrontDev& d = fd[f];
+        d.lpos = S.f_lpos[f];
+        d.upos = S.f_upos[f] < 0 ? 0 : S.f_upos[f];
+        d.rel_off = S.f_rel_off[f];
+        d.rows_off = S.f_rptr[f];
+        d.inv0 = pr->f_inv0[f];  // per-front layouts: the same in every program of the analysis
+        d.bpos = pr->f_bpos[f];
+        d.ld = (int32_t)S.f_ld[f];
+        d.w = (int32_t)S.f_w[f];
+        d.m = (int32_t)S.f_m[f];
+        d.col0 = (int32_t)S.f_col0[f];
+        d.parent = (int32_t)S.f_parent[f];
+        d.kid0 = (int32_t)S.f_kptr[f];
+        d.kid1 = (int32_t)S.f_kptr[f + 1];
+        d.level = (int32_t)S.f_level[f];
+      }
+      std::vector<int32_t> kids(S.f_kids.begin(), S.f_kids.end());
+      std::vector<int32_t> rows(S.f_rows.begin(), S.f_rows.end());
+      ds->fronts = dupload(fd, st);
+      ds->kids = dupload(kids, st);
+      ds->rel = dupload(S.rel, st);
+      ds->rows = dupload(rows, st);
+      ds->a_pos = dupload(S.a_pos, st);
+      ds->diag_src = dupload(diag_src, st);
+      ds->perm = dupload(S.perm, st);
+      structs.push_back(ds);
+    }
     auto p = std::make_shared<DevicePlan>();
     p->device = device;
-    const auto& S = s;
-    std::vector<fc::FrontDev> fd(S.nf);
-    for (i64 f = 0; f < S.nf; ++f) {
-      fc::FrontDev& d = fd[f];
-      d.lpos = S.f_lpos[f];
-      d.upos = S.f_upos[f] < 0 ? 0 : S.f_upos[f];
-      d.rel_off = S.f_rel_off[f];
-      d.rows_off = S.f_rptr[f];
-      d.inv0 = prog.f_inv0[f];
-      d.bpos = prog.f_bpos[f];
-      d.ld = (int32_t)S.f_ld[f];
-      d.w = (int32_t)S.f_w[f];
-      d.m = (int32_t)S.f_m[f];
-      d.col0 = (int32_t)S.f_col0[f];
-      d.parent = (int32_t)S.f_parent[f];
-      d.kid0 = (int32_t)S.f_kptr[f];
-      d.kid1 = (int32_t)S.f_kptr[f + 1];
-      d.level = (int32_t)S.f_level[f];
-    }
-    std::vector<int32_t> kids(S.f_kids.begin(), S.f_kids.end());
-    std::vector<int32_t> rows(S.f_rows.begin(), S.f_rows.end());
-    p->fronts = dupload(fd, st);
-    p->kids = dupload(kids, st);
-    p->rel = dupload(S.rel, st);
-    p->rows = dupload(rows, st);
-    p->a_pos = dupload(S.a_pos, st);
-    p->diag_src = dupload(diag_src, st);
-    p->perm = dupload(S.perm, st);
-    p->items = dupload(prog.items, st);
-    p->gitems = dupload(prog.gitems, st);
-    p->ritems = dupload(prog.ritems, st);
-    p->sitems = dupload(prog.sitems, st);
-    p->uoff = dupload(prog.uoff, st);
+    p->prog = pr.get();
+    p->st = ds;
+    p->fronts = ds->fronts;
+    p->kids = ds->kids;
+    p->rel = ds->rel;
+    p->rows = ds->rows;
+    p->a_pos = ds->a_pos;
+    p->diag_src = ds->diag_src;
+    p->perm = ds->perm;
+    p->items = dupload(pr->items, st);
+    p->gitems = dupload(pr->gitems, st);
+    p->ritems = dupload(pr->ritems, st);
+    p->sitems = dupload(pr->sitems, st);
+    p->uoff = dupload(pr->uoff, st);
     HIP_CHECK(hipStreamSynchronize(st));
-    plans.push_back(p);
+    if (cache) plans.push_back(p);
     return p;
   }
 };
@@ -760,6 +819,7 @@ struct fc_symbolic {
 struct fc_factor {
   std::shared_ptr<fc_symbolic> sym_keep;  // keeps the analysis alive
   const fc_symbolic* sym = nullptr;
+  std::shared_ptr<const Program> prog;  // the program this factor runs (its analysis's at creation)
   std::shared_ptr<DevicePlan> plan;
   int device = 0;
   int precision = FC_FP64;  // the current factorization's (fallbacks change it)
@@ -951,7 +1011,7 @@ void enqueue_factor(fc_factor* F, const double* dval, i64 nnz, hipStream_t st, b
   DevicePlan& P = *F->plan;
   T* L = (T*)F->L;
   T* U = (T*)F->U.get((size_t)std::max<i64>(s.u_size, 1) * sizeof(T));
-  const auto& pr = F->sym->prog;
+  const auto& pr = *F->prog;
   T* Inv = (T*)F->inv.get((size_t)std::max<i64>(pr.inv_size, 1) * sizeof(T));
   T* Part = (T*)F->part.get((size_t)std::max<i64>(pr.part_size, 1) * sizeof(T));
   T* W = F->precision == FC_FP64 ? (T*)F->trsm_w.get((size_t)std::max<i64>(pr.w_size, 1) * sizeof(T)) : nullptr;
@@ -963,14 +1023,14 @@ void enqueue_factor(fc_factor* F, const double* dval, i64 nnz, hipStream_t st, b
   int32_t* fexp = nullptr;
   __bf16 *Lh = nullptr, *Ll = nullptr;
   if (mixed) {
-    const size_t nb = (size_t)std::max<i64>(F->sym->prog.b_size, 1);
+    const size_t nb = (size_t)std::max<i64>(F->prog->b_size, 1);
     Lh = (__bf16*)F->lb.get(2 * nb * sizeof(__bf16));
     Ll = Lh + nb;
     fexp = (int32_t*)F->fexp.get((size_t)std::max<i64>(s.nf, 1) * sizeof(int32_t));
     fc::k_front_scale<<<(int)std::min<i64>(std::max<i64>(s.nf, 1), 1 << 16), 256, 0, st>>>(P.fronts, P.rows, F->diag,
                                                                                           s.nf, fexp);
   }
-  const auto& prog = F->sym->prog;
+  const auto& prog = *F->prog;
   const int mask = eager ? F->opts.profile_mask : 0;
   int level = -1;
   for (const fc::Launch& l : prog.launches) {
@@ -1062,7 +1122,7 @@ void run_factor(fc_factor* F, const double* dval, i64 nnz, hipStream_t st) {
     if (!F->graph_exec) {
       // every workspace is allocated before the capture (enqueue_factor then finds them)
       const fc::Symbolic& s = F->sym->s;
-      const auto& pr = F->sym->prog;
+      const auto& pr = *F->prog;
       (void)F->U.get((size_t)std::max<i64>(s.u_size, 1) * sizeof(T));
       (void)F->inv.get((size_t)std::max<i64>(pr.inv_size, 1) * sizeof(T));
       (void)F->part.get((size_t)std::max<i64>(pr.part_size, 1) * sizeof(T));
@@ -1208,7 +1268,7 @@ int64_t solve_spin_limit() {
 
 template <typename T, int R>
 void run_solve_r(const fc_factor* F, const SolveWork& w, i64 ldx, int nrhs, hipStream_t st) {
-  const Program& prog = F->sym->prog;
+  const Program& prog = *F->prog;
   DevicePlan& P = *F->plan;
   const T* L = (const T*)F->L;
   const T* Inv = (const T*)F->inv.p;
@@ -1321,7 +1381,7 @@ void solve_device(const fc_factor* F, double* b, i64 n, int nrhs, i64 ldb, hipSt
     SolveWork w;
     w.xp = (double*)F->xp.get((size_t)n * nr * sizeof(double));
     w.yp = (double*)F->yp.get((size_t)n * nr * sizeof(double));
-    w.u = (double*)F->usolve.get((size_t)std::max<i64>(F->sym->prog.u_solve * nr, 1) * sizeof(double));
+    w.u = (double*)F->usolve.get((size_t)std::max<i64>(F->prog->u_solve * nr, 1) * sizeof(double));
     w.err = err;
     fc::k_permute<<<blocks_for(n), 256, 0, st>>>(br, w.xp, F->plan->perm, n, nr, ldb, n, 0);
     solve_launches(F, w, n, nr, st);
@@ -2558,7 +2618,7 @@ int fc_analyze(uint64_t n, const uint64_t* ptr, const uint64_t* idx, uint64_t ma
         for (i64 t = s.upper_ptr[k]; t < s.upper_ptr[k + 1]; ++t)
           if (s.upper_idx[t] == k) h->diag_src[inv[k]] = t;
     }
-    h->prog = build_program(s, Schedule{});
+    h->prog = std::make_shared<const Program>(build_program(s, Schedule{}));
     {
       Registry& R = registry();
       std::lock_guard<std::mutex> g(R.mu);
@@ -2596,12 +2656,13 @@ int fc_symbolic_info_get(const fc_symbolic* h, fc_symbolic_info* info) {
   info->u_bytes_fp64 = (uint64_t)s.u_size * 8;
   info->by_blocks = s.by_blocks;
   info->launches = 0;
-  for (const fc::Launch& l : h->prog.launches) info->launches += l.kind < fc::KINDS;  // not the ordering steps
+  const auto prog = h->program();
+  for (const fc::Launch& l : prog->launches) info->launches += l.kind < fc::KINDS;  // not the ordering steps
   for (int k = 0; k < fc::KINDS && k < 8; ++k) {
-    info->kind_flops[k] = h->prog.work[k][0];
-    info->kind_entries[k] = h->prog.work[k][1];
+    info->kind_flops[k] = prog->work[k][0];
+    info->kind_entries[k] = prog->work[k][1];
   }
-  info->solve_bytes = (uint64_t)(2 * s.n + h->prog.u_solve) * sizeof(double) + 256;
+  info->solve_bytes = (uint64_t)(2 * s.n + prog->u_solve) * sizeof(double) + 256;
   return FC_OK;
 }
 
@@ -2676,7 +2737,8 @@ static int factor_create(const fc_symbolic* h, const fc_factor_opts* opts, std::
       HIP_CHECK(hipStreamCreateWithFlags(&F->stream, hipStreamNonBlocking));
       F->own_stream = true;
     }
-    F->plan = const_cast<fc_symbolic*>(h)->plan(o.device, F->stream);
+    F->prog = h->program();
+    F->plan = const_cast<fc_symbolic*>(h)->plan(o.device, F->prog, F->stream);
     const fc::Symbolic& s = h->s;
     const size_t es = o.precision == FC_FP64 ? sizeof(double) : sizeof(float);
     F->L = (void*)dalloc<char>((size_t)std::max<i64>(s.l_size, 1) * es);
@@ -2761,12 +2823,164 @@ int fc_factor_profile(const fc_factor* f, int32_t kind, double* launches, double
   if (!f || kind < 0 || kind >= fc::KINDS) return FC_INVALID_ARGUMENT;
   if (launches) *launches = f->kind_launches[kind];
   if (seconds) *seconds = f->kind_seconds[kind];
-  if (flops) *flops = f->sym->prog.work[kind][0];
-  if (entries) *entries = f->sym->prog.work[kind][1];
+  if (flops) *flops = f->prog->work[kind][0];
+  if (entries) *entries = f->prog->work[kind][1];
   return FC_OK;
 }
 
 void fc_factor_free(fc_factor* f) { delete f; }
+
+namespace {
+bool schedule_ok(const fc_schedule& c) {
+  auto in = [](int32_t v, int32_t lo, int32_t hi) { return v >= lo && v <= hi; };
+  return in(c.slab, 1, 16) && in(c.split_target, 1, 65536) && in(c.split_side_fp64, 1, 65536) &&
+         in(c.split_side_fp32, 1, 65536);
+}
+Schedule to_schedule(const fc_schedule& c) {
+  Schedule r;
+  r.slab = c.slab;
+  r.split_target = c.split_target;
+  r.split_side_fp64 = c.split_side_fp64;
+  r.split_side_fp32 = c.split_side_fp32;
+  return r;
+}
+fc_schedule from_schedule(const Schedule& r) {
+  fc_schedule c;
+  c.slab = r.slab;
+  c.split_target = r.split_target;
+  c.split_side_fp64 = r.split_side_fp64;
+  c.split_side_fp32 = r.split_side_fp32;
+  return c;
+}
+bool same(const Schedule& a, const Schedule& b) {
+  return a.slab == b.slab && a.split_target == b.split_target && a.split_side_fp64 == b.split_side_fp64 &&
+         a.split_side_fp32 == b.split_side_fp32;
+}
+}  // namespace
+
+int fc_symbolic_schedule_get(const fc_symbolic* h, fc_schedule* out) {
+  if (!h || !out) return FC_INVALID_ARGUMENT;
+  *out = from_schedule(h->program()->sched);
+  return FC_OK;
+}
+
+int fc_symbolic_schedule_set(fc_symbolic* h, const fc_schedule* c, fc_error* err) {
+  if (!h || !c) {
+    set_error(err, FC_INVALID_ARGUMENT, "null argument");
+    return FC_INVALID_ARGUMENT;
+  }
+  if (!schedule_ok(*c)) {
+    set_error(err, FC_INVALID_ARGUMENT, "schedule out of range (slab 1..16, split targets 1..65536)");
+    return FC_INVALID_ARGUMENT;
+  }
+  try {
+    h->install(std::make_shared<const Program>(build_program(h->s, to_schedule(*c))));
+  } catch (const std::bad_alloc&) {
+    set_error(err, FC_OUT_OF_MEMORY, "not enough host memory for the program");
+    return FC_OUT_OF_MEMORY;
+  }
+  set_error(err, FC_OK, "");
+  return FC_OK;
+}
+
+int fc_tune(fc_symbolic* h, const double* dval, uint64_t nnz, const fc_tune_opts* opts, fc_tune_report* report,
+            fc_error* err) {
+  if (!h || (nnz && !dval)) {
+    set_error(err, FC_INVALID_ARGUMENT, "null argument");
+    return FC_INVALID_ARGUMENT;
+  }
+  fc_tune_opts o{};
+  if (opts) o = *opts;
+  const int repeats = o.repeats > 0 ? o.repeats : 2;
+  fc_factor_opts fo{};
+  fo.precision = o.precision;
+  fo.device = o.device;
+  fo.stream = o.stream;
+  std::unique_ptr<fc_factor> F;
+  int st = factor_create(h, &fo, F, err);
+  if (st != FC_OK) return st;
+  const std::shared_ptr<const Program> start = F->prog;
+  struct Tried {
+    Schedule sc;
+    double seconds;
+    std::shared_ptr<const Program> prog;
+  };
+  std::vector<Tried> tried;
+  // the fastest of `repeats` factorizations (after an untimed one) with schedule sc
+  auto time = [&](const Schedule& sc, double& secs) -> int {
+    for (const Tried& t : tried)
+      if (same(t.sc, sc)) {
+        secs = t.seconds;
+        return FC_OK;
+      }
+    std::shared_ptr<const Program> pr;
+    try {
+      pr = same(sc, start->sched) ? start : std::make_shared<const Program>(build_program(h->s, sc));
+      F->prog = pr;
+      F->plan = h->plan(F->device, pr, F->stream, pr == start);
+    } catch (const std::bad_alloc&) {
+      set_error(err, FC_OUT_OF_MEMORY, "not enough memory for a candidate's program");
+      return FC_OUT_OF_MEMORY;
+    } catch (const DeviceError& d) {
+      set_error(err, FC_DEVICE, std::string(d.what) + ": " + hipGetErrorString(d.e));
+      return FC_DEVICE;
+    }
+    secs = 1e300;
+    for (int r = 0; r <= repeats; ++r) {
+      const int s2 = factor_device(F.get(), dval, nnz, err);
+      if (s2 != FC_OK) return s2;
+      if (r > 0) secs = std::min(secs, F->seconds);
+    }
+    tried.push_back(Tried{sc, secs, pr});
+    return FC_OK;
+  };
+  Schedule best = start->sched;
+  double best_s = 0;
+  if ((st = time(best, best_s)) != FC_OK) return st;
+  const double start_s = best_s;
+  auto search = [&](int32_t Schedule::*field, std::initializer_list<int32_t> values) -> int {
+    const Schedule base = best;
+    for (const int32_t v : values) {
+      if (tried.size() >= FC_TUNE_MAX) break;
+      Schedule c = base;
+      c.*field = v;
+      double t = 0;
+      const int s2 = time(c, t);
+      if (s2 != FC_OK) return s2;
+      if (t < best_s) {
+        best_s = t;
+        best = c;
+      }
+      for (Tried& q : tried)  // only the fastest candidate's program is kept
+        if (!same(q.sc, best)) q.prog.reset();
+    }
+    return FC_OK;
+  };
+  if ((st = search(&Schedule::slab, {1, 2, 4, 8})) != FC_OK) return st;
+  if ((st = search(o.precision == FC_FP64 ? &Schedule::split_side_fp64 : &Schedule::split_side_fp32,
+                   {128, 256, 384, 512})) != FC_OK)
+    return st;
+  if ((st = search(&Schedule::split_target, {512, 1024, 2048})) != FC_OK) return st;
+  for (const Tried& t : tried)
+    if (same(t.sc, best)) {
+      if (t.prog != start) h->install(t.prog);
+      break;
+    }
+  F.reset();  // the factor and the candidates' device plans go before the report is written
+  if (report) {
+    std::memset(report, 0, sizeof *report);
+    report->chosen = from_schedule(best);
+    report->seconds_start = start_s;
+    report->seconds_chosen = best_s;
+    report->candidates = (int32_t)tried.size();
+    for (size_t q = 0; q < tried.size() && q < FC_TUNE_MAX; ++q) {
+      report->tried[q] = from_schedule(tried[q].sc);
+      report->tried_seconds[q] = tried[q].seconds;
+    }
+  }
+  set_error(err, FC_OK, "");
+  return FC_OK;
+}
 
 int fc_solve_device(const fc_factor* f, double* b, uint64_t n, uint64_t nrhs, uint64_t ldb, fc_error* err) {
   if (!f || !f->valid) {
@@ -2846,7 +3060,7 @@ int fc_solve_device_noalloc(const fc_factor* f, double* b, uint64_t n, uint64_t 
     set_error(err, FC_INVALID_ARGUMENT, "device ordinal beyond the allocation-free solve's table");
     return FC_INVALID_ARGUMENT;
   }
-  const i64 us = f->sym->prog.u_solve;
+  const i64 us = f->prog->u_solve;
   const size_t cap = noalloc_limit(), one = noalloc_need((i64)n, us, 1);
   if (one > cap) {
     set_error(err, FC_TOO_LARGE,

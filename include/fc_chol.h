@@ -57,9 +57,10 @@ typedef enum {
   FC_FP64 = 0,  /* the plain method: every step in FP64 on the FP64 matrix cores           */
   FC_MIXED = 1, /* Schur updates in BF16 (power-of-two scaled, FP32 accumulation), the rows
                    below a panel (TRSM) on the FP32 matrix cores, diagonal blocks factored
-                   and inverted with FP64 arithmetic; the factor is stored in FP32; solutions
-                   are refined in FP64                                                    */
-  FC_FP32 = 2   /* every step in FP32, factor stored in FP32                               */
+                   and inverted in FP32 arithmetic (a block with a pivot below 1e-3 |A_jj|
+                   is factored again in FP64 arithmetic, where the reference's pivot test
+                   decides); the factor is stored in FP32; solutions are refined in FP64   */
+  FC_FP32 = 2   /* every step in FP32 (diagonal blocks as FC_MIXED's), factor stored in FP32 */
 } fc_precision;
 
 typedef struct {
@@ -135,6 +136,63 @@ int fc_symbolic_export(const fc_symbolic* s, uint64_t* perm, uint64_t* first, ui
                        uint64_t* uptr, uint64_t* updates, uint64_t* tasks, uint64_t* top);
 
 void fc_symbolic_free(fc_symbolic* s);
+
+/* --- schedule and tuner ------------------------------------------------------------------ */
+
+/* No counterpart in the reference (its CPU factorization has no launch schedule; its own
+ * work-division constants are fixed, cholesky.rs:48-97): the numeric phase's schedule on the
+ * GPU, a property of the analysis. Panels of 128 columns are factored left-looking inside
+ * each front; while the caller's stream factors a slab of `slab` panels, a second stream
+ * subtracts from the next slab its products with the columns before the current one (one
+ * product slab x 128 columns wide). GEMM launches with fewer output tiles than a target are
+ * split along k (pieces of at least 512) and their pieces summed in a fixed order. The
+ * schedule changes launch shapes and the order of floating-point sums, not the algorithm:
+ * every schedule computes the same factor to rounding. */
+typedef struct {
+  int32_t slab;            /* panels per side-stream slab update, 1..16 (1: one-panel lookahead) */
+  int32_t split_target;    /* tiles the update-matrix (SYRK) launches are split up to, 1..65536 */
+  int32_t split_side_fp64; /* the same for the FP64 factor's side-stream slab updates           */
+  int32_t split_side_fp32; /* ... and for the FP32 and mixed factors'                           */
+} fc_schedule;
+
+/* The analysis's current schedule (the built-in default until set or tuned). */
+int fc_symbolic_schedule_get(const fc_symbolic* s, fc_schedule* out);
+
+/* Rebuilds the analysis's launch program for a new schedule. Factors created afterwards use
+ * it; a factor already made keeps the program it was created with. FC_INVALID_ARGUMENT for
+ * values out of range. */
+int fc_symbolic_schedule_set(fc_symbolic* s, const fc_schedule* sched, fc_error* err);
+
+typedef struct {
+  int32_t precision; /* fc_precision the candidates are timed in                           */
+  int32_t device;    /* HIP device ordinal                                                  */
+  void* stream;      /* hipStream_t; NULL: a stream of the tuner's own                      */
+  int32_t repeats;   /* timed factorizations per candidate, the fastest counting (0: 2); one
+                        untimed factorization precedes them                                  */
+  int32_t pad;
+} fc_tune_opts;
+
+#define FC_TUNE_MAX 32
+typedef struct {
+  fc_schedule chosen;              /* installed in the analysis when the call succeeds          */
+  double seconds_start;            /* the schedule the analysis had before the call              */
+  double seconds_chosen;
+  int32_t candidates;              /* schedules timed (tried[0] is the starting one)             */
+  int32_t pad;
+  fc_schedule tried[FC_TUNE_MAX];
+  double tried_seconds[FC_TUNE_MAX];
+} fc_tune_report;
+
+/* No counterpart in the reference: an autotuner for the schedule. It factors the caller's
+ * matrix (upper values in device memory, as fc_factor_numeric_device) once per candidate
+ * schedule, timed with HIP events, searching one parameter at a time from the analysis's
+ * current schedule: slab in {1, 2, 4, 8}, then the side-stream split target of the precision
+ * in {128, 256, 384, 512}, then the SYRK split target in {512, 1024, 2048}; each parameter keeps
+ * its fastest value. The fastest schedule is installed in the analysis (as
+ * fc_symbolic_schedule_set) and reported. Errors of a factorization (FC_NOT_POSITIVE_DEFINITE,
+ * FC_OUT_OF_MEMORY, ...) end the search and are returned; the analysis then keeps its schedule. */
+int fc_tune(fc_symbolic* s, const double* upper_val_device, uint64_t nnz, const fc_tune_opts* opts,
+            fc_tune_report* report, fc_error* err);
 
 /* --- numeric phase ----------------------------------------------------------------------- */
 
