@@ -2463,6 +2463,35 @@ int ldlt_solve(const fc_ldlt* f, T* b, uint64_t n, fc_error* err) {
 }
 }  // namespace
 
+// The schedule between its C and C++ forms.
+namespace {
+bool schedule_ok(const fc_schedule& c) {
+  auto in = [](int32_t v, int32_t lo, int32_t hi) { return v >= lo && v <= hi; };
+  return in(c.slab, 1, 16) && in(c.split_target, 1, 65536) && in(c.split_side_fp64, 1, 65536) &&
+         in(c.split_side_fp32, 1, 65536);
+}
+Schedule to_schedule(const fc_schedule& c) {
+  Schedule r;
+  r.slab = c.slab;
+  r.split_target = c.split_target;
+  r.split_side_fp64 = c.split_side_fp64;
+  r.split_side_fp32 = c.split_side_fp32;
+  return r;
+}
+fc_schedule from_schedule(const Schedule& r) {
+  fc_schedule c;
+  c.slab = r.slab;
+  c.split_target = r.split_target;
+  c.split_side_fp64 = r.split_side_fp64;
+  c.split_side_fp32 = r.split_side_fp32;
+  return c;
+}
+bool same(const Schedule& a, const Schedule& b) {
+  return a.slab == b.slab && a.split_target == b.split_target && a.split_side_fp64 == b.split_side_fp64 &&
+         a.split_side_fp32 == b.split_side_fp32;
+}
+}  // namespace
+
 extern "C" {
 
 int fc_ldlt_solve(const fc_ldlt* f, double* b, uint64_t n, fc_error* err) { return ldlt_solve<double>(f, b, n, err); }
@@ -2830,33 +2859,6 @@ int fc_factor_profile(const fc_factor* f, int32_t kind, double* launches, double
 
 void fc_factor_free(fc_factor* f) { delete f; }
 
-namespace {
-bool schedule_ok(const fc_schedule& c) {
-  auto in = [](int32_t v, int32_t lo, int32_t hi) { return v >= lo && v <= hi; };
-  return in(c.slab, 1, 16) && in(c.split_target, 1, 65536) && in(c.split_side_fp64, 1, 65536) &&
-         in(c.split_side_fp32, 1, 65536);
-}
-Schedule to_schedule(const fc_schedule& c) {
-  Schedule r;
-  r.slab = c.slab;
-  r.split_target = c.split_target;
-  r.split_side_fp64 = c.split_side_fp64;
-  r.split_side_fp32 = c.split_side_fp32;
-  return r;
-}
-fc_schedule from_schedule(const Schedule& r) {
-  fc_schedule c;
-  c.slab = r.slab;
-  c.split_target = r.split_target;
-  c.split_side_fp64 = r.split_side_fp64;
-  c.split_side_fp32 = r.split_side_fp32;
-  return c;
-}
-bool same(const Schedule& a, const Schedule& b) {
-  return a.slab == b.slab && a.split_target == b.split_target && a.split_side_fp64 == b.split_side_fp64 &&
-         a.split_side_fp32 == b.split_side_fp32;
-}
-}  // namespace
 
 int fc_symbolic_schedule_get(const fc_symbolic* h, fc_schedule* out) {
   if (!h || !out) return FC_INVALID_ARGUMENT;

@@ -76,33 +76,40 @@ __device__ __forceinline__ void inv_apply(const T* __restrict__ inv, int pw, con
 
 // x[b] = sum_k Inv(k, b) z[k] (the transpose): thread (b, h) sums the half h of column b of Inv
 // (contiguous), then the halves are added in a fixed order. Z: LDS, [k][r]; H: LDS scratch.
+// Several right-hand sides are summed four at a time (each one's sum in the same order as
+// alone): sixteen at once would not fit in registers.
 template <typename T, int R>
 __device__ __forceinline__ void inv_apply_t(const T* __restrict__ inv, int pw, const double (*Z)[R + 1], int nr,
                                             double* __restrict__ out, int64_t ldx, double (*H)[SNB][R + 1]) {
   const int b = threadIdx.x & (SNB - 1), h = threadIdx.x >> 7, bb = min(b, pw - 1);
   const __amdgpu_buffer_rsrc_t rs = raw_rsrc(inv);
   const uint32_t co = bb * pw * sizeof(T);  // column bb of Inv
-  double s[R];
-#pragma unroll
-  for (int r = 0; r < R; ++r) s[r] = 0.0;
-  constexpr int KB = R == 1 ? 64 : 16;  // loads per batch (one batch for a single right-hand side)
+  constexpr int RG = R < 4 ? R : 4;         // right-hand sides summed at a time
+  static_assert(R % RG == 0, "whole groups of right-hand sides");
+  constexpr int KB = R == 1 ? 64 : 16;      // loads per batch (one batch for a single right-hand side)
 #pragma unroll 1
-  for (int k0 = 0; k0 < 64; k0 += KB) {
-    double a[KB];
+  for (int r0 = 0; r0 < R; r0 += RG) {
+    double s[RG];
 #pragma unroll
-    for (int q = 0; q < KB; ++q) a[q] = (double)buf_load<T>(rs, co, min(64 * h + k0 + q, pw - 1) * sizeof(T));
+    for (int r = 0; r < RG; ++r) s[r] = 0.0;
+#pragma unroll 1
+    for (int k0 = 0; k0 < 64; k0 += KB) {
+      double a[KB];
 #pragma unroll
-    for (int q = 0; q < KB; ++q) {
-      const int k = 64 * h + k0 + q;
-      const bool in = k < pw;
-      // Z past the block's width is not written by every caller: select it away (a 0 weight on
-      // a NaN left in LDS by an earlier kernel would still give a NaN)
+      for (int q = 0; q < KB; ++q) a[q] = (double)buf_load<T>(rs, co, min(64 * h + k0 + q, pw - 1) * sizeof(T));
 #pragma unroll
-      for (int r = 0; r < R; ++r) s[r] = fma(in ? a[q] : 0.0, in ? Z[min(k, SNB - 1)][r] : 0.0, s[r]);
+      for (int q = 0; q < KB; ++q) {
+        const int k = 64 * h + k0 + q;
+        const bool in = k < pw;
+        // Z past the block's width is not written by every caller: select it away (a 0 weight on
+        // a NaN left in LDS by an earlier kernel would still give a NaN)
+#pragma unroll
+        for (int r = 0; r < RG; ++r) s[r] = fma(in ? a[q] : 0.0, in ? Z[min(k, SNB - 1)][r0 + r] : 0.0, s[r]);
+      }
     }
-  }
 #pragma unroll
-  for (int r = 0; r < R; ++r) H[h][b][r] = s[r];
+    for (int r = 0; r < RG; ++r) H[h][b][r0 + r] = s[r];
+  }
   __syncthreads();
   for (int e = threadIdx.x; e < pw * nr; e += blockDim.x) {
     const int c = e % pw, r = e / pw;
