@@ -558,7 +558,7 @@ Program build_program(const fc::Symbolic& s, const Schedule& sched) {
         if (s.f_w[f] <= c0) continue;
         const i64 c1 = std::min(c0 + fc::SNB, s.f_w[f]);
         // the first row tile also forms the next block's solution
-        for (i64 r = c1; r < s.f_m[f]; r += fc::SROWS)
+        for (i64 r = c1; r < s.f_m[f]; r += fc::SROWS_M)
           p.sitems.push_back(Item{(int32_t)f, (int32_t)c0, (int32_t)r, (r == c1 && c1 < s.f_w[f]) ? 1 : 0, 0, 0, 0, 0});
       }
       sadd(p.sfwd, S_FWD_STEP, l, i0);

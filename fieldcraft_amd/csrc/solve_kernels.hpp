@@ -24,8 +24,9 @@
 namespace fc {
 
 constexpr int SNB = 128;      // columns of a solve block (the factorization's panel width)
-constexpr int SROWS = 256;    // rows of a forward workgroup
-constexpr int SCOLS = 32;     // columns of a below-the-front workgroup (backward)
+constexpr int SROWS = 256;    // rows of a forward workgroup (one right-hand side)
+constexpr int SROWS_M = 128;  // ... of several: each row's right-hand sides split between two threads
+constexpr int SCOLS = 64;     // columns of a below-the-front workgroup (backward)
 constexpr int BCOLS = 128;    // columns of a backward block-step workgroup
 constexpr int RC = 16;        // right-hand sides of the multi-column kernels
 constexpr int MAX_RHS = 64;   // right-hand sides one solve pass handles (the host splits larger blocks)
@@ -39,14 +40,16 @@ constexpr uint64_t PENDING = 0x7FF0DEAD0000BEEFull;
 
 // y[b] = sum_k Inv(b, k) v[k] for b < pw (Inv column-major, ld = pw, zero above the diagonal):
 // thread b of the first 128, loads coalesced along b and independent along k. V: LDS, [k][r].
+// Several right-hand sides: thread (b, hh) of the 256 takes half hh of them.
 template <typename T, int R>
 __device__ __forceinline__ void inv_apply(const T* __restrict__ inv, int pw, const double (*V)[R + 1], int nr,
                                           double* __restrict__ out, int64_t ldx) {
-  const int b = threadIdx.x;
+  constexpr int RH = R > 1 ? R / 2 : 1;  // right-hand sides a thread sums
+  const int b = R > 1 ? (int)(threadIdx.x & (SNB - 1)) : (int)threadIdx.x, r0 = R > 1 ? (int)(threadIdx.x >> 7) * RH : 0;
   if (b >= pw) return;
-  double s[R];
+  double s[RH];
 #pragma unroll
-  for (int r = 0; r < R; ++r) s[r] = 0.0;
+  for (int r = 0; r < RH; ++r) s[r] = 0.0;
   if (R == 1 && pw == SNB) {  // a full block, one right-hand side: the whole row in flight at once
     constexpr int KB = sizeof(T) == 4 ? SNB : SNB / 2;
     const __amdgpu_buffer_rsrc_t rs = raw_rsrc(inv);
@@ -68,10 +71,10 @@ __device__ __forceinline__ void inv_apply(const T* __restrict__ inv, int pw, con
       for (int q = 0; q < 32; ++q)
         if (k0 + q < pw)
 #pragma unroll
-          for (int r = 0; r < R; ++r) s[r] = fma((double)a[q], V[k0 + q][r], s[r]);
+          for (int r = 0; r < RH; ++r) s[r] = fma((double)a[q], V[k0 + q][r0 + r], s[r]);
     }
   }
-  for (int r = 0; r < nr; ++r) out[(int64_t)r * ldx + b] = s[r];
+  for (int r = 0; r < RH && r0 + r < nr; ++r) out[(int64_t)(r0 + r) * ldx + b] = s[r];
 }
 
 // x[b] = sum_k Inv(k, b) z[k] (the transpose): thread (b, h) sums the half h of column b of Inv
@@ -193,7 +196,10 @@ __global__ __launch_bounds__(256, 2) void k_fwd_step(const Item* __restrict__ it
   const T* lf = L + F.lpos + (int64_t)c0 * F.ld;
   const int mu = F.m - F.w;
   double* uf = u + uoff[it.f] * nrhs;
-  const int i = r0 + threadIdx.x;
+  // several right-hand sides: SROWS_M rows, thread (row, hh) taking half hh of them
+  constexpr int RH = R > 1 ? R / 2 : 1, RPI = R > 1 ? SROWS_M : SROWS;
+  const int rh0 = R > 1 ? (int)(threadIdx.x / RPI) * RH : 0;
+  const int i = r0 + (int)(threadIdx.x % RPI);
   const bool row = i < F.m;
   const __amdgpu_buffer_rsrc_t rs = raw_rsrc(lf);
   const uint32_t lo = min(i, F.m - 1) * sizeof(T);  // rows past the front load its last row (not stored)
@@ -203,10 +209,10 @@ __global__ __launch_bounds__(256, 2) void k_fwd_step(const Item* __restrict__ it
     const int nr = min(R, nrhs - q0);
     // Loads that do not wait on y go first: the row's old values and, for a full block and one
     // right-hand side, its first (FP32: only) batch of L.
-    double old[R];
+    double old[RH];
     if (row)
 #pragma unroll
-      for (int r = 0; r < R; ++r) old[r] = p[(int64_t)(q0 + min(r, nr - 1)) * ldp];
+      for (int r = 0; r < RH; ++r) old[r] = p[(int64_t)(q0 + min(rh0 + r, nr - 1)) * ldp];
     // (KP columns of L per batch: the first batch goes out before the barrier, each later one
     // before the previous is multiplied)
     constexpr int KP = sizeof(T) == 4 ? 64 : 32;
@@ -223,9 +229,9 @@ __global__ __launch_bounds__(256, 2) void k_fwd_step(const Item* __restrict__ it
       for (int r = 0; r < R; ++r) Yb[threadIdx.x][r] = yv[r];
     }
     __syncthreads();
-    double s[R];
+    double s[RH];
 #pragma unroll
-    for (int r = 0; r < R; ++r) s[r] = 0.0;
+    for (int r = 0; r < RH; ++r) s[r] = 0.0;
     if (full1) {  // a full block, one right-hand side
 #pragma unroll
       for (int j0 = 0; j0 < SNB; j0 += KP) {
@@ -245,14 +251,14 @@ __global__ __launch_bounds__(256, 2) void k_fwd_step(const Item* __restrict__ it
         for (int q = 0; q < 32; ++q)
           if (j0 + q < pw)
 #pragma unroll
-            for (int r = 0; r < R; ++r) s[r] = fma((double)l[q], Yb[j0 + q][r], s[r]);
+            for (int r = 0; r < RH; ++r) s[r] = fma((double)l[q], Yb[j0 + q][rh0 + r], s[r]);
       }
     }
     if (row)
-      for (int r = 0; r < nr; ++r) {
+      for (int r = 0; r < RH && rh0 + r < nr; ++r) {
         const double nv = old[r] - s[r];
-        p[(int64_t)(q0 + r) * ldp] = nv;
-        if (it.c && i >= c1 && i < c1 + pn) Vn[i - c1][r] = nv;
+        p[(int64_t)(q0 + rh0 + r) * ldp] = nv;
+        if (it.c && i >= c1 && i < c1 + pn) Vn[i - c1][rh0 + r] = nv;
       }
     if (it.c) {
       __syncthreads();
@@ -287,80 +293,87 @@ __global__ __launch_bounds__(256) void k_bwd_gather(const Item* __restrict__ ite
 }
 
 // Backward: z_j -= L21(:, j)^T x_below for the columns [a, a + SCOLS) of a front (z in y,
-// x_below gathered in u). A wave takes four columns (two with several right-hand sides) at a
-// time with lanes along the rows, so each x_below value is loaded once for them; the rows go
-// in batches of 64 NQ, the next batch's loads issued before the current one is multiplied. A
-// column's sum runs over the rows in the same order (lane-strided, then across the wave) for any
-// number of right-hand sides; every lane ends the butterfly with the same sum, and lane c R + r
-// applies the pair (column c, right-hand side r). Item f, a = j0.
+// x_below gathered in u). The rows go in tiles of 64, each tile of L (64 rows x SCOLS columns)
+// and of x_below (64 rows x R) staged through LDS with coalesced loads, the next tile's loads
+// issued before the current one is multiplied. Wave w takes the columns a + 16 w + [0, 16);
+// lane (g, r, h) = (lane / 16, lane % 16 % R, lane % 16 / R) sums, for right-hand side r and the
+// R columns 16 w + h R + [0, R), the rows of residue g mod 4 in increasing order; the four
+// residue sums are then added as (S0 + S1) + (S2 + S3). Every (column, right-hand side) sum is
+// formed in that order for any R, so a right-hand side's result does not depend on how many
+// are solved together. Item f, a = j0.
 template <typename T, int R>
 __global__ __launch_bounds__(256) void k_bwd_below(const Item* __restrict__ items, const FrontDev* __restrict__ fronts,
                                                    const T* __restrict__ L, const double* __restrict__ u,
                                                    const int64_t* __restrict__ uoff, double* __restrict__ z,
                                                    int64_t ldx, int nrhs) {
-  constexpr int JC = R == 1 ? 4 : 2;  // columns a wave takes at a time
-  constexpr int NQ = R == 1 ? 8 : 2;  // rows a lane loads per batch
-  static_assert(JC * R <= 64, "one lane per (column, right-hand side) pair");
+  static_assert(SCOLS == 64 && 16 % R == 0, "four waves of 16 columns; R divides 16");
+  constexpr int TR = 64;                // rows of a tile
+  constexpr int XE = (TR * R + 255) / 256;  // x_below entries a thread stages per tile
+  __shared__ T Ls[TR][SCOLS + 1];
+  __shared__ double Xs[TR][R + 1];
   const Item it = items[blockIdx.x];
   const FrontDev F = fronts[it.f];
   const int mu = F.m - F.w;
-  const T* lf = L + F.lpos + F.w;
+  const int j0 = it.a, nc = min(SCOLS, F.w - j0);
+  const T* lf = L + F.lpos + F.w;  // L21: row i of column j at lf[j ld + i]
   const double* uf = u + uoff[it.f] * nrhs;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int j1 = min(F.w, it.a + SCOLS);
-  for (int jb = it.a + JC * wave; jb < j1; jb += 4 * JC) {
-    const int nc = min(JC, j1 - jb);
-    for (int q0 = 0; q0 < nrhs; q0 += R) {
-      const int nr = min(R, nrhs - q0);
-      double s[JC][R];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int g = lane >> 4, r = (lane & 15) % R, h = (lane & 15) / R;
+  const int cb = 16 * wave + h * R;  // the lane's first column (of the item's SCOLS)
+  const int lr = tid & (TR - 1), lc = tid >> 6;  // staging: row lr, columns lc + 4 q
+  for (int q0 = 0; q0 < nrhs; q0 += R) {
+    const int nr = min(R, nrhs - q0);
+    T lv[SCOLS / 4];
+    double xv[XE];
+    auto load = [&](int t0) {  // clamped in bounds; rows past mu and columns past nc staged as 0
+      const int i = min(t0 + lr, mu - 1);
 #pragma unroll
-      for (int c = 0; c < JC; ++c)
+      for (int q = 0; q < SCOLS / 4; ++q) lv[q] = lf[(int64_t)(j0 + min(lc + 4 * q, nc - 1)) * F.ld + i];
 #pragma unroll
-        for (int r = 0; r < R; ++r) s[c][r] = 0.0;
-      // rows b0 + lane + 64 q (clamped in bounds; rows past mu weigh zero)
-      auto load = [&](int b0, double (&xv)[NQ][R], double (&l)[NQ][JC]) {
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-          const int i = min(b0 + lane + 64 * q, mu - 1);
-#pragma unroll
-          for (int r = 0; r < R; ++r) xv[q][r] = uf[(int64_t)(q0 + min(r, nr - 1)) * mu + i];
-#pragma unroll
-          for (int c = 0; c < JC; ++c) l[q][c] = (double)lf[(int64_t)(jb + min(c, nc - 1)) * F.ld + i];
-        }
-      };
-      auto mul = [&](int b0, const double (&xv)[NQ][R], const double (&l)[NQ][JC]) {
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-          const bool in = b0 + lane + 64 * q < mu;
-#pragma unroll
-          for (int c = 0; c < JC; ++c)
-#pragma unroll
-            for (int r = 0; r < R; ++r) s[c][r] = fma(in ? l[q][c] : 0.0, xv[q][r], s[c][r]);
-        }
-      };
-      double xa[NQ][R], la[NQ][JC], xb[NQ][R], lb[NQ][JC];
-      constexpr int BS = 64 * NQ;
-      if (mu > 0) load(0, xa, la);
-      for (int b0 = 0; b0 < mu; b0 += 2 * BS) {
-        if (b0 + BS < mu) load(b0 + BS, xb, lb);
-        mul(b0, xa, la);
-        if (b0 + BS >= mu) break;
-        if (b0 + 2 * BS < mu) load(b0 + 2 * BS, xa, la);
-        mul(b0 + BS, xb, lb);
+      for (int q = 0; q < XE; ++q) {
+        const int e = min(tid + 256 * q, TR * R - 1), k = e / R, rr = e % R;
+        xv[q] = uf[(int64_t)(q0 + min(rr, nr - 1)) * mu + min(t0 + k, mu - 1)];
       }
-      double mine = 0.0;
+    };
+    auto stage = [&](int t0) {
+      const bool rin = t0 + lr < mu;
 #pragma unroll
-      for (int c = 0; c < JC; ++c)
+      for (int q = 0; q < SCOLS / 4; ++q) {
+        Ls[lr][lc + 4 * q] = (rin && lc + 4 * q < nc) ? lv[q] : T(0);
+      }
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-          double t = s[c][r];
+      for (int q = 0; q < XE; ++q) {
+        const int e = tid + 256 * q, k = e / R, rr = e % R;
+        if (e < TR * R) Xs[k][rr] = (t0 + k < mu && rr < nr) ? xv[q] : 0.0;
+      }
+    };
+    double acc[R];
 #pragma unroll
-          for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
-          mine = lane == c * R + r ? t : mine;
+    for (int c = 0; c < R; ++c) acc[c] = 0.0;
+    if (mu > 0) load(0);
+    for (int t0 = 0; t0 < mu; t0 += TR) {
+      __syncthreads();  // the previous tile's reads are done
+      stage(t0);
+      if (t0 + TR < mu) load(t0 + TR);
+      __syncthreads();
+#pragma unroll 4
+      for (int kk = 0; kk < TR / 4; ++kk) {
+        const int k = 4 * kk + g;
+        const double x = Xs[k][r];
+#pragma unroll
+        for (int c = 0; c < R; ++c) {
+          acc[c] = fma((double)Ls[k][cb + c], x, acc[c]);
         }
-      const int c = lane / R, r = lane % R;
-      if (c < nc && r < nr) z[(int64_t)(q0 + r) * ldx + F.col0 + jb + c] -= mine;
+      }
     }
+#pragma unroll
+    for (int c = 0; c < R; ++c) {
+      double t = acc[c];
+      t += __shfl_xor(t, 16);  // (S0 + S1), (S2 + S3)
+      t += __shfl_xor(t, 32);  // (S0 + S1) + (S2 + S3)
+      if (g == 0 && cb + c < nc && r < nr) z[(int64_t)(q0 + r) * ldx + F.col0 + j0 + cb + c] -= t;
+    }
+    __syncthreads();  // LDS reused by the next group of right-hand sides
   }
 }
 
