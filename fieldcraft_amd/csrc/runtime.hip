@@ -1276,6 +1276,7 @@ void run_solve_r(const fc_factor* F, const SolveWork& w, i64 ldx, int nrhs, hipS
   int32_t* err = w.err;
   const int64_t limit = solve_spin_limit();
   const int mark = R == 1;  // the chained passes: the gathers mark the block solutions to come
+  const int groups = (nrhs + fc::RSTEP - 1) / fc::RSTEP;  // block steps: right-hand-side groups
   auto run = [&](const fc::Launch& l) {
     const fc::Item* it = P.sitems + l.item0;
     dim3 g(l.count), b(256);
@@ -1283,8 +1284,9 @@ void run_solve_r(const fc_factor* F, const SolveWork& w, i64 ldx, int nrhs, hipS
       case S_GATHER:
         fc::k_fwd_gather<T, R><<<g, b, 0, st>>>(it, P.fronts, P.kids, P.rel, Inv, xp, yp, ldx, nrhs, u, P.uoff, mark);
         break;
-      case S_FWD_STEP:
-        fc::k_fwd_step<T, R><<<g, b, 0, st>>>(it, P.fronts, L, Inv, xp, yp, ldx, nrhs, u, P.uoff);
+      case S_FWD_STEP:  // every right-hand side in one workgroup (groups side by side would each
+                        // read the step's rows of L again: measured slower)
+        fc::k_fwd_step<T, R><<<g, b, 0, st>>>(it, P.fronts, L, Inv, xp, yp, ldx, nrhs, u, P.uoff, l.count);
         break;
       case S_BWD_GATHER:
         fc::k_bwd_gather<<<g, b, 0, st>>>(it, P.fronts, P.rows, xp, ldx, nrhs, u, P.uoff);
@@ -1297,7 +1299,9 @@ void run_solve_r(const fc_factor* F, const SolveWork& w, i64 ldx, int nrhs, hipS
         break;
       case S_BWD_STEP:
         if constexpr (R == 1) fc::k_bwd_step1<T><<<g, b, 0, st>>>(it, P.fronts, L, Inv, yp, xp);
-        else fc::k_bwd_step<T, R><<<g, b, 0, st>>>(it, P.fronts, L, Inv, yp, xp, ldx, nrhs);
+        else  // several right-hand sides: groups of RSTEP side by side (a step's tile is small)
+          fc::k_bwd_step<T, fc::RSTEP><<<dim3(l.count * groups), b, 0, st>>>(it, P.fronts, L, Inv, yp, xp, ldx, nrhs,
+                                                                             l.count);
         break;
       case S_FWD_FRONT:
         fc::k_fwd_front<T><<<g, b, fc::SNB * fc::SNB * sizeof(T), st>>>(it, P.fronts, L, Inv, xp, yp, u, P.uoff, err,

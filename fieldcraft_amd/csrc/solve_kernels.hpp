@@ -29,6 +29,8 @@ constexpr int SROWS_M = 128;  // ... of several: each row's right-hand sides spl
 constexpr int SCOLS = 64;     // columns of a below-the-front workgroup (backward)
 constexpr int BCOLS = 128;    // columns of a backward block-step workgroup
 constexpr int RC = 16;        // right-hand sides of the multi-column kernels
+constexpr int RSTEP = 4;      // right-hand sides of a backward block-step workgroup: a step's launch
+                              // holds one copy of its items per group of them, run side by side
 constexpr int MAX_RHS = 64;   // right-hand sides one solve pass handles (the host splits larger blocks)
 // A block solution not yet formed (the chained one-right-hand-side solve below): a signalling
 // NaN, which no arithmetic result is.
@@ -186,10 +188,12 @@ template <typename T, int R>
 __global__ __launch_bounds__(256, 2) void k_fwd_step(const Item* __restrict__ items, const FrontDev* __restrict__ fronts,
                                                   const T* __restrict__ L, const T* __restrict__ Inv, double* v,
                                                   double* __restrict__ y, int64_t ldx, int nrhs, double* __restrict__ u,
-                                                  const int64_t* __restrict__ uoff) {
+                                                  const int64_t* __restrict__ uoff, int nitems) {
   __shared__ double Yb[SNB][R + 1];
   __shared__ double Vn[SNB][R + 1];
-  const Item it = items[blockIdx.x];
+  // workgroup b takes item b % nitems for the right-hand-side groups b / nitems + k gridDim / nitems
+  const int grp = blockIdx.x / nitems, ngrp = gridDim.x / nitems;
+  const Item it = items[blockIdx.x - grp * nitems];
   const FrontDev F = fronts[it.f];
   const int c0 = it.a, pw = min(SNB, F.w - c0), c1 = c0 + pw, r0 = it.b;
   const int pn = min(SNB, F.w - c1);  // width of the next block (designated item)
@@ -205,7 +209,7 @@ __global__ __launch_bounds__(256, 2) void k_fwd_step(const Item* __restrict__ it
   const uint32_t lo = min(i, F.m - 1) * sizeof(T);  // rows past the front load its last row (not stored)
   double* p = i < F.w ? v + F.col0 + i : uf + i - F.w;  // the row's entry of v (or of u below the front)
   const int64_t ldp = i < F.w ? ldx : mu;
-  for (int q0 = 0; q0 < nrhs; q0 += R) {
+  for (int q0 = grp * R; q0 < nrhs; q0 += R * ngrp) {
     const int nr = min(R, nrhs - q0);
     // Loads that do not wait on y go first: the row's old values and, for a full block and one
     // right-hand side, its first (FP32: only) batch of L.
@@ -415,20 +419,22 @@ __global__ __launch_bounds__(256) void k_bwd_first(const Item* __restrict__ item
 template <typename T, int R>
 __global__ __launch_bounds__(256) void k_bwd_step(const Item* __restrict__ items, const FrontDev* __restrict__ fronts,
                                                   const T* __restrict__ L, const T* __restrict__ Inv, double* z,
-                                                  double* __restrict__ x, int64_t ldx, int nrhs) {
+                                                  double* __restrict__ x, int64_t ldx, int nrhs, int nitems) {
   constexpr int HC = 64;  // columns staged at a time
   __shared__ T Lt[HC][SNB + 1];
   __shared__ double Xb[SNB][R + 1];
   __shared__ double Zn[SNB][R + 1];
   __shared__ double part[4][HC][R + 1];  // also the halves of inv_apply_t (2 x SNB = 4 x HC)
   static_assert(2 * SNB == 4 * HC, "the halves fit the partial sums");
-  const Item it = items[blockIdx.x];
+  // workgroup b takes item b % nitems for the right-hand-side groups b / nitems + k gridDim / nitems
+  const int grp = blockIdx.x / nitems, ngrp = gridDim.x / nitems;
+  const Item it = items[blockIdx.x - grp * nitems];
   const FrontDev F = fronts[it.f];
   const int c0 = it.a, pw = min(SNB, F.w - c0), j0 = it.b;
   const T* lf = L + F.lpos + c0;
   const int tid = threadIdx.x, jq = tid & (HC - 1), g = tid >> 6;
   const int nj = min(BCOLS, c0 - j0);
-  for (int q0 = 0; q0 < nrhs; q0 += R) {
+  for (int q0 = grp * R; q0 < nrhs; q0 += R * ngrp) {
     const int nr = min(R, nrhs - q0);
     if (tid < pw) {
       double xv[R];
