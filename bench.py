@@ -173,6 +173,8 @@ def main():
     ap.add_argument("--tet-cells", type=int, default=50)
     ap.add_argument("--calibrate-copy", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default schedule (no autotuning)")
+    ap.add_argument("--schedule", default="", help="run this schedule instead of tuning, e.g. "
+                                                   "slab=2,split_target=2048,split_side_fp32=384")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU: the factorization of one matrix is not split across GPUs here "
@@ -226,7 +228,9 @@ def main():
     def tune(leg):
         # the autotuner (fc_tune) times candidate schedules on this matrix, outside the timed
         # steps, and installs the fastest for the leg's factorizations
-        if args.no_tune:
+        if args.schedule:
+            sym.set_schedule(**{k: int(v) for k, v in (kv.split("=") for kv in args.schedule.split(","))})
+        if args.no_tune or args.schedule:
             tuned[leg] = dict(chosen=sym.schedule(), tuned=False)
             return
         t0 = time.perf_counter()
