@@ -313,8 +313,8 @@ def main():
                 roofline=roof)
     line["schedule"] = tuned  # per leg: the schedule its factorizations ran (the tuner's pick)
     # The factor's bound over its precision classes: Schur products (SYRK, UPDATE) at the MFMA
-    # peak of their class (three BF16 products per flop in the mixed form), TRSM at the FP32
-    # (FP64) peak, diagonal blocks (FP64 arithmetic) at the FP64 peak.
+    # peak of their class (three BF16 products per flop in the mixed form), TRSM and diagonal
+    # blocks at the FP32 (FP64) peak.
     def factor_bound(leg, form_):
         k = legs[leg]["breakdown"]
         schur = k["syrk"]["flops"] + k["update"]["flops"]
@@ -323,7 +323,7 @@ def main():
         else:
             t_schur = schur / (PEAK["fp64" if form_ == "fp64" else "fp32"] * 1e12)
         t_trsm = k["trsm"]["flops"] / (PEAK["fp64" if form_ == "fp64" else "fp32"] * 1e12)
-        t_diag = k["potrf"]["flops"] / (PEAK["fp64"] * 1e12)
+        t_diag = k["potrf"]["flops"] / (PEAK["fp64" if form_ == "fp64" else "fp32"] * 1e12)
         b = t_schur + t_trsm + t_diag
         return dict(bound_s=b, factor_s=legs[leg]["factor_s"], frac=b / legs[leg]["factor_s"],
                     schur_flops=schur, trsm_flops=k["trsm"]["flops"], diag_flops=k["potrf"]["flops"])
