@@ -71,7 +71,14 @@ struct Launch {
 
 constexpr int KINDS = 8;
 
+// A child's run of update-matrix columns landing in an extend-add item's parent columns:
+// columns [jc0, jc1) of child front `child`.
+struct EaRange {
+  int32_t child, jc0, jc1, pad;
+};
+
 constexpr int EA_BATCH = 8;   // rows an extend-add thread moves at a time
-constexpr int EA_COLS = 8;     // columns of the parent one extend-add workgroup owns
+constexpr int EA_COLS = 8;     // columns of the parent an extend-add window spans
+constexpr int EA_ENTRIES = 8192;  // child entries an extend-add workgroup takes (whole windows)
 
 }  // namespace fc

@@ -68,29 +68,26 @@ __global__ void k_fill(T* __restrict__ p, int64_t len, T v) {
 // Extend-add: the workgroup owns the parent's columns [a, b) and adds every child's update
 // matrix entries that land there, child after child (in increasing order), so each entry
 // receives its contributions in a fixed order and no two workgroups write the same entry.
+// Item: f, a, b, and at ranges[c .. c + d) the children's column runs that land in [a, b),
+// window of EA_COLS parent columns after window, children in increasing order within a
+// window (children with none are not listed; the runs are found when the launches are planned).
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_extend_add(const Item* __restrict__ items, const FrontDev* __restrict__ fronts,
-                                                    const int32_t* __restrict__ kids, const int32_t* __restrict__ rel,
-                                                    T* __restrict__ L, const T* U, T* Uw) {
+__global__ __launch_bounds__(256) void k_extend_add(const Item* __restrict__ items, const EaRange* __restrict__ ranges,
+                                                    const FrontDev* __restrict__ fronts,
+                                                    const int32_t* __restrict__ rel, T* __restrict__ L, const T* U,
+                                                    T* Uw) {
   // U (children, read) and Uw (this front, written) are disjoint blocks of one arena
   const Item it = items[blockIdx.x];
   const FrontDev P = fronts[it.f];
-  const int j0 = it.a, j1 = it.b;
   const int pw = P.w;
   const int64_t pmu = P.m - P.w;
-  for (int q = P.kid0; q < P.kid1; ++q) {
-    const FrontDev C = fronts[kids[q]];
+  for (int q = it.c; q < it.c + it.d; ++q) {
+    const EaRange rg = ranges[q];
+    const FrontDev C = fronts[rg.child];
     const int cmu = C.m - C.w;
-    if (cmu == 0) continue;
     const int32_t* r = rel + C.rel_off;
-    // child columns whose parent column lies in [j0, j1): a contiguous run (rel rises)
-    int lo = 0, hi = cmu;
-    while (lo < hi) { int mid = (lo + hi) >> 1; if (r[mid] < j0) lo = mid + 1; else hi = mid; }
-    int jc0 = lo;
-    lo = jc0; hi = cmu;
-    while (lo < hi) { int mid = (lo + hi) >> 1; if (r[mid] < j1) lo = mid + 1; else hi = mid; }
-    const int jc1 = lo;
+    const int jc0 = rg.jc0, jc1 = rg.jc1;  // child columns whose parent column lies in [a, b)
     const T* cu = U + C.upos;
     // The child's entries that land in [j0, j1): columns jc0 + t (t < nc <= EA_COLS, rel is
     // injective), rows jc0 + t .. cmu - 1 of each, taken as one flat range so that small

@@ -163,6 +163,7 @@ struct Program {
   std::vector<fc::GemmItem> gitems;    // GEMM tiles (UPDATE, TRSM, SYRK launches)
   std::vector<fc::ReduceItem> ritems;  // split-K reductions
   std::vector<fc::Launch> launches;
+  std::vector<fc::EaRange> earanges;   // extend-add items' child column runs (item.c, item.d)
   std::vector<i64> f_inv0;             // per front: its diagonal-block inverses in the Inv buffer
   std::vector<i64> f_bpos;             // per front: its L part in the BF16 copies (mixed form)
   i64 b_size = 0;                      // entries of each BF16 copy
@@ -324,6 +325,35 @@ Program build_program(const fc::Symbolic& s, const Schedule& sched) {
         for (i64 jc = 0; jc < cmu; ++jc) colent[s.rel[s.f_rel_off[c] + jc]] += (double)(cmu - jc);
       }
     };
+    // Extend-add items: windows of EA_COLS parent columns, each with the runs of its children's
+    // columns that land there (rel rises, so each child's is one run), children in increasing
+    // order. Consecutive windows of one front share an item until it holds EA_ENTRIES child
+    // entries, so that small fronts still give a workgroup enough to move.
+    i64 ea_r0 = -1, ea_n = 0;
+    int32_t ea_f = -1, ea_a = 0, ea_b = 0;
+    auto ea_close = [&]() {
+      if (ea_r0 >= 0 && (i64)p.earanges.size() > ea_r0)
+        p.items.push_back(Item{ea_f, ea_a, ea_b, (int32_t)ea_r0, (int32_t)((i64)p.earanges.size() - ea_r0), 0, 0, 0});
+      ea_r0 = -1;
+      ea_n = 0;
+    };
+    auto ea_window = [&](i64 f, i64 a, i64 b) {
+      if (ea_r0 >= 0 && (ea_f != (int32_t)f || ea_n >= fc::EA_ENTRIES)) ea_close();
+      if (ea_r0 < 0) {
+        ea_r0 = (i64)p.earanges.size();
+        ea_f = (int32_t)f;
+        ea_a = (int32_t)a;
+      }
+      ea_b = (int32_t)b;
+      for (i64 q = s.f_kptr[f]; q < s.f_kptr[f + 1]; ++q) {
+        const i64 c = s.f_kids[q], cmu = s.f_m[c] - s.f_w[c];
+        const int32_t* rc = s.rel.data() + s.f_rel_off[c];
+        const i64 jc0 = std::lower_bound(rc, rc + cmu, (int32_t)a) - rc;
+        const i64 jc1 = std::lower_bound(rc + jc0, rc + cmu, (int32_t)b) - rc;
+        if (jc1 > jc0) p.earanges.push_back(fc::EaRange{(int32_t)c, (int32_t)jc0, (int32_t)jc1, 0});
+        for (i64 jc = jc0; jc < jc1; ++jc) ea_n += cmu - jc;
+      }
+    };
     // extend-add into the L parts
     i64 i0 = (i64)p.items.size();
     for (i64 t = 0; t < nl; ++t) {
@@ -332,10 +362,11 @@ Program build_program(const fc::Symbolic& s, const Schedule& sched) {
       column_entries(f);
       for (i64 a = 0; a < s.f_w[f]; a += fc::EA_COLS) {
         const i64 b = std::min(a + fc::EA_COLS, s.f_w[f]);
-        p.items.push_back(Item{(int32_t)f, (int32_t)a, (int32_t)b, 0, 0, 0, 0, 0});
+        ea_window(f, a, b);
         for (i64 j = a; j < b; ++j) entries += colent[j];
       }
     }
+    ea_close();
     add(fc::K_EXTEND_L, l, i0);
     i64 wmax = 0;
     for (i64 t = 0; t < nl; ++t) wmax = std::max(wmax, s.f_w[fl[t]]);
@@ -532,10 +563,11 @@ Program build_program(const fc::Symbolic& s, const Schedule& sched) {
       column_entries(f);
       for (i64 a = s.f_w[f]; a < s.f_m[f]; a += fc::EA_COLS) {
         const i64 b = std::min(a + fc::EA_COLS, s.f_m[f]);
-        p.items.push_back(Item{(int32_t)f, (int32_t)a, (int32_t)b, 0, 0, 0, 0, 0});
+        ea_window(f, a, b);
         for (i64 j = a; j < b; ++j) entries += colent[j];
       }
     }
+    ea_close();
     add(fc::K_EXTEND_U, l, i0);
   }
   // solves: per level, forward leaves first, backward root first
@@ -706,6 +738,7 @@ struct DevicePlan {
   int64_t* diag_src = nullptr;
   int64_t* perm = nullptr;
   fc::Item* items = nullptr;
+  fc::EaRange* earanges = nullptr;
   fc::GemmItem* gitems = nullptr;
   fc::ReduceItem* ritems = nullptr;
   fc::Item* sitems = nullptr;
@@ -715,7 +748,7 @@ struct DevicePlan {
     int cur = 0;
     (void)hipGetDevice(&cur);
     (void)hipSetDevice(device);
-    for (void* q : {(void*)items, (void*)gitems, (void*)ritems, (void*)sitems, (void*)uoff}) dfree(q);
+    for (void* q : {(void*)items, (void*)earanges, (void*)gitems, (void*)ritems, (void*)sitems, (void*)uoff}) dfree(q);
     (void)hipSetDevice(cur);
   }
 };
@@ -806,6 +839,7 @@ struct fc_symbolic {
     p->diag_src = ds->diag_src;
     p->perm = ds->perm;
     p->items = dupload(pr->items, st);
+    p->earanges = dupload(pr->earanges, st);
     p->gitems = dupload(pr->gitems, st);
     p->ritems = dupload(pr->ritems, st);
     p->sitems = dupload(pr->sitems, st);
@@ -1060,7 +1094,7 @@ void enqueue_factor(fc_factor* F, const double* dval, i64 nnz, hipStream_t st, b
     switch (l.kind) {
       case fc::K_EXTEND_L:
       case fc::K_EXTEND_U:
-        fc::k_extend_add<T><<<g, b, 0, ls>>>(it, P.fronts, P.kids, P.rel, L, U, U);
+        fc::k_extend_add<T><<<g, b, 0, ls>>>(it, P.earanges, P.fronts, P.rel, L, U, U);
         break;
       case fc::K_UPDATE:
       case fc::K_SYRK:
