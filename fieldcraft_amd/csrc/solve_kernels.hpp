@@ -29,7 +29,7 @@ constexpr int SROWS_M = 128;  // ... of several: each row's right-hand sides spl
 constexpr int SCOLS = 64;     // columns of a below-the-front workgroup (backward)
 constexpr int BCOLS = 128;    // columns of a backward block-step workgroup
 constexpr int RC = 16;        // right-hand sides of the multi-column kernels
-constexpr int RSTEP = 4;      // right-hand sides of a backward block-step workgroup: a step's launch
+constexpr int RSTEP = 8;      // right-hand sides of a backward block-step workgroup: a step's launch
                               // holds one copy of its items per group of them, run side by side
 constexpr int MAX_RHS = 64;   // right-hand sides one solve pass handles (the host splits larger blocks)
 // A block solution not yet formed (the chained one-right-hand-side solve below): a signalling
@@ -81,17 +81,17 @@ __device__ __forceinline__ void inv_apply(const T* __restrict__ inv, int pw, con
 
 // x[b] = sum_k Inv(k, b) z[k] (the transpose): thread (b, h) sums the half h of column b of Inv
 // (contiguous), then the halves are added in a fixed order. Z: LDS, [k][r]; H: LDS scratch.
-// Several right-hand sides are summed four at a time (each one's sum in the same order as
-// alone): sixteen at once would not fit in registers.
+// Several right-hand sides are summed eight at a time, with eight loads of Inv in flight (each
+// one's sum in the same order as alone): sixteen at once, or sixteen loads, spill registers.
 template <typename T, int R>
 __device__ __forceinline__ void inv_apply_t(const T* __restrict__ inv, int pw, const double (*Z)[R + 1], int nr,
                                             double* __restrict__ out, int64_t ldx, double (*H)[SNB][R + 1]) {
   const int b = threadIdx.x & (SNB - 1), h = threadIdx.x >> 7, bb = min(b, pw - 1);
   const __amdgpu_buffer_rsrc_t rs = raw_rsrc(inv);
   const uint32_t co = bb * pw * sizeof(T);  // column bb of Inv
-  constexpr int RG = R < 4 ? R : 4;         // right-hand sides summed at a time
+  constexpr int RG = R < 8 ? R : 8;         // right-hand sides summed at a time
   static_assert(R % RG == 0, "whole groups of right-hand sides");
-  constexpr int KB = R == 1 ? 64 : 16;      // loads per batch (one batch for a single right-hand side)
+  constexpr int KB = R == 1 ? 64 : 8;       // loads per batch (one batch for a single right-hand side)
 #pragma unroll 1
   for (int r0 = 0; r0 < R; r0 += RG) {
     double s[RG];
