@@ -219,7 +219,7 @@ def main():
     x = torch.zeros_like(b)
     A = CH.Matrix(p.n, p.ptr, p.idx, p.kval)
     torch.cuda.synchronize()
-    breakdown = ["syrk", "update", "extend_l", "extend_u", "trsm", "potrf", "reduce"]
+    breakdown = ["syrk", "update", "extend_l", "extend_u", "trsm", "potrf", "reduce", "small"]
     # timed with events in the timed steps: the kinds of the headline's dominant kernel only
     profile = ["syrk", "update"] if args.form == "mixed" else ["syrk", "update", "trsm"]
     legs = {}
@@ -275,6 +275,8 @@ def main():
             name, bound, peak = f"k_extend_add<{el}>", "hbm", HBM_PEAK
         elif kind == "potrf":
             name, bound, peak = f"k_potrf_blk<{el}>", "latency", None
+        elif kind == "small":
+            name, bound, peak = f"k_small_fronts<{el}>", "latency", None
         else:
             name, bound, peak = f"k_reduce_parts<{el}>", "hbm", HBM_PEAK
         g = funcs.setdefault(name, dict(kinds=[], seconds=0.0, launches=0.0, flops=0.0, entries=0.0, bound=bound,
@@ -314,7 +316,7 @@ def main():
     line["schedule"] = tuned  # per leg: the schedule its factorizations ran (the tuner's pick)
     # The factor's bound over its precision classes: Schur products (SYRK, UPDATE) at the MFMA
     # peak of their class (three BF16 products per flop in the mixed form), TRSM and diagonal
-    # blocks at the FP32 (FP64) peak.
+    # blocks at the FP32 (FP64) peak, the small fronts (FP64 arithmetic) at the FP64 peak.
     def factor_bound(leg, form_):
         k = legs[leg]["breakdown"]
         schur = k["syrk"]["flops"] + k["update"]["flops"]
@@ -324,9 +326,11 @@ def main():
             t_schur = schur / (PEAK["fp64" if form_ == "fp64" else "fp32"] * 1e12)
         t_trsm = k["trsm"]["flops"] / (PEAK["fp64" if form_ == "fp64" else "fp32"] * 1e12)
         t_diag = k["potrf"]["flops"] / (PEAK["fp64" if form_ == "fp64" else "fp32"] * 1e12)
-        b = t_schur + t_trsm + t_diag
+        t_small = k["small"]["flops"] / (PEAK["fp64"] * 1e12)
+        b = t_schur + t_trsm + t_diag + t_small
         return dict(bound_s=b, factor_s=legs[leg]["factor_s"], frac=b / legs[leg]["factor_s"],
-                    schur_flops=schur, trsm_flops=k["trsm"]["flops"], diag_flops=k["potrf"]["flops"])
+                    schur_flops=schur, trsm_flops=k["trsm"]["flops"], diag_flops=k["potrf"]["flops"],
+                    small_flops=k["small"]["flops"])
     line["factor_roofline"] = {leg: factor_bound(leg, leg) for leg in legs}
     if "fp64" in legs:
         f64 = legs["fp64"]

@@ -170,7 +170,7 @@ EXPORTS = ["fc_analyze", "fc_symbolic_info_get", "fc_symbolic_export", "fc_symbo
            "fc_version"]
 
 #: kernel kinds of the numeric phase (FC_KIND_* in include/fc_chol.h)
-KINDS = ["extend_l", "extend_u", "update", "potrf", "trsm", "syrk", "reduce", "split"]
+KINDS = ["extend_l", "extend_u", "update", "potrf", "trsm", "syrk", "reduce", "small"]
 
 
 def lib():

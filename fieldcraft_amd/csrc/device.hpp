@@ -23,9 +23,9 @@ struct FrontDev {
 };
 static_assert(sizeof(FrontDev) == 80, "FrontDev layout");
 
-// Strided runs of loads through a buffer resource: the lane's part of the address is one VGPR
-// offset and each load's part a scalar, so a run of independent loads issued back to back needs
-// no 64-bit address per load. Raw resource (byte offsets, stride 0); every access made through
+// Strided runs of loads (and stores) through a buffer resource: the lane's part of the address
+// is one VGPR offset and each access's part a scalar, so a run of independent accesses needs no
+// 64-bit address per access. Raw resource (byte offsets, stride 0); every access made through
 // it is in bounds by construction, the record count only has to cover the lane offsets.
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t raw_rsrc(const void* base) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, 0x7fffffff, 0x00020000);
@@ -41,6 +41,18 @@ __device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t r, uin
   return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, lane_off, off, 0));
 }
 
+template <typename T>
+__device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, T v, uint32_t lane_off, uint32_t off);
+template <>
+__device__ __forceinline__ void buf_store<float>(__amdgpu_buffer_rsrc_t r, float v, uint32_t lane_off, uint32_t off) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, lane_off, off, 0);
+}
+template <>
+__device__ __forceinline__ void buf_store<double>(__amdgpu_buffer_rsrc_t r, double v, uint32_t lane_off, uint32_t off) {
+  typedef unsigned int u32x2 __attribute__((__vector_size__(2 * sizeof(unsigned int))));
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, lane_off, off, 0);
+}
+
 // A unit of work of one launch; the meaning of a, b, c, d depends on the kernel.
 struct Item {
   int32_t f, a, b, c, d, e, pad0, pad1;
@@ -54,7 +66,7 @@ enum Kind : int32_t {
   K_TRSM = 4,      // rows below a panel against its diagonal block
   K_SYRK = 5,      // update matrix U = -L21 L21^T
   K_REDUCE = 6,    // sums of the pieces of a GEMM split along k
-  K_SPLIT = 7,     // mixed form: a finished panel's BF16 (hi, lo) copies
+  K_SMALL = 7,     // small fronts' whole partial factorizations, one workgroup a front
 };
 
 struct Launch {

@@ -22,6 +22,7 @@
 #include "gemm_kernels.hpp"
 #include "lanczos_kernels.hpp"
 #include "ldlt_kernels.hpp"
+#include "small_kernels.hpp"
 #include "solve_kernels.hpp"
 #include "symbolic.hpp"
 
@@ -280,6 +281,14 @@ void add_gemm(Program& p, const std::vector<Tile>& tiles, int kind, i64 level, i
   p.part_size = std::max(p.part_size, part);
 }
 
+// Fronts whose dense steps one workgroup of k_small_fronts takes: at most SMALL_W columns (one
+// wave's diagonal block), and an update matrix of at most SMALL_SYRK multiply-adds (the part
+// that sets a front's time there: a front above it is faster on the tiled launches).
+bool is_small(const fc::Symbolic& s, i64 f) {
+  const double w = (double)s.f_w[f], mu = (double)(s.f_m[f] - s.f_w[f]);
+  return s.f_w[f] <= fc::SMALL_W && s.f_m[f] - s.f_w[f] <= fc::SMALL_MU && 0.5 * mu * mu * w <= fc::SMALL_SYRK;
+}
+
 Program build_program(const fc::Symbolic& s, const Schedule& sched) {
   using fc::Item;
   using fc::Launch;
@@ -295,14 +304,16 @@ Program build_program(const fc::Symbolic& s, const Schedule& sched) {
     }
     flops = entries = 0;
   };
-  // the inverses of every front's diagonal blocks (NB x NB, the last one pw x pw), kept for
-  // the solves
   // the BF16 copies of the mixed form: each front's L part by chunks of BK columns
+  // (none for small fronts: no product of the tiled launches reads them)
   p.f_bpos.resize(s.nf);
   for (i64 f = 0; f < s.nf; ++f) {
     p.f_bpos[f] = p.b_size;
-    p.b_size += (s.f_ld[f] * ((s.f_w[f] + fc::BK - 1) / fc::BK) * fc::BK + 63) / 64 * 64;
+    if (!is_small(s, f))
+      p.b_size += (s.f_ld[f] * ((s.f_w[f] + fc::BK - 1) / fc::BK) * fc::BK + 63) / 64 * 64;
   }
+  // the inverses of every front's diagonal blocks (NB x NB, the last one pw x pw), kept for
+  // the solves
   p.f_inv0.resize(s.nf);
   for (i64 f = 0; f < s.nf; ++f) {
     p.f_inv0[f] = p.inv_size;
@@ -311,8 +322,14 @@ Program build_program(const fc::Symbolic& s, const Schedule& sched) {
   }
   std::vector<double> colent;  // child entries landing in each parent column
   for (i64 l = 0; l < s.levels; ++l) {
-    const i64* fl = &s.lvl_fronts[s.lvl_ptr[l]];
-    const i64 nl = s.lvl_ptr[l + 1] - s.lvl_ptr[l];
+    // the level's fronts (every one takes the extend-add launches), its small fronts
+    // (k_small_fronts) and the others (the tiled launches)
+    const i64* fall = &s.lvl_fronts[s.lvl_ptr[l]];
+    const i64 nall = s.lvl_ptr[l + 1] - s.lvl_ptr[l];
+    std::vector<i64> reg, sm;
+    for (i64 t = 0; t < nall; ++t) (is_small(s, fall[t]) ? sm : reg).push_back(fall[t]);
+    const i64* fl = reg.data();
+    const i64 nl = (i64)reg.size();
     auto has_kids_below = [&](i64 f) {
       for (i64 q = s.f_kptr[f]; q < s.f_kptr[f + 1]; ++q)
         if (s.f_m[s.f_kids[q]] > s.f_w[s.f_kids[q]]) return true;
@@ -356,8 +373,8 @@ Program build_program(const fc::Symbolic& s, const Schedule& sched) {
     };
     // extend-add into the L parts
     i64 i0 = (i64)p.items.size();
-    for (i64 t = 0; t < nl; ++t) {
-      const i64 f = fl[t];
+    for (i64 t = 0; t < nall; ++t) {
+      const i64 f = fall[t];
       if (!has_kids_below(f)) continue;
       column_entries(f);
       for (i64 a = 0; a < s.f_w[f]; a += fc::EA_COLS) {
@@ -368,6 +385,21 @@ Program build_program(const fc::Symbolic& s, const Schedule& sched) {
     }
     ea_close();
     add(fc::K_EXTEND_L, l, i0);
+    // the small fronts' diagonal blocks, rows below and update matrices, one item (workgroup)
+    // each, costliest first (the first workgroups to start take them)
+    auto small_cost = [&](i64 f) {
+      const double w = (double)s.f_w[f], mu = (double)(s.f_m[f] - s.f_w[f]);
+      return w * w * w / 3 + mu * w * w + mu * mu * w;
+    };
+    std::stable_sort(sm.begin(), sm.end(), [&](i64 x, i64 y) { return small_cost(x) > small_cost(y); });
+    i0 = (i64)p.items.size();
+    for (const i64 f : sm) {
+      const i64 w = s.f_w[f], mu = s.f_m[f] - w;
+      p.items.push_back(Item{(int32_t)f, 0, 0, 0, 0, 0, 0, 0});
+      // as the tiled launches count them
+      flops += (double)w * (w + 1) * (2 * w + 1) / 6.0 + (double)mu * w * w + 2.0 * w * lower_pairs(0, mu, 0, mu);
+    }
+    add(fc::K_SMALL, l, i0);
     i64 wmax = 0;
     for (i64 t = 0; t < nl; ++t) wmax = std::max(wmax, s.f_w[fl[t]]);
     std::vector<Tile> tiles;
@@ -557,8 +589,8 @@ Program build_program(const fc::Symbolic& s, const Schedule& sched) {
     }
     add_gemm(p, tiles, fc::K_SYRK, l, sched.split_target);
     i0 = (i64)p.items.size();
-    for (i64 t = 0; t < nl; ++t) {
-      const i64 f = fl[t];
+    for (i64 t = 0; t < nall; ++t) {
+      const i64 f = fall[t];
       if (!has_kids_below(f)) continue;
       column_entries(f);
       for (i64 a = s.f_w[f]; a < s.f_m[f]; a += fc::EA_COLS) {
@@ -1127,6 +1159,9 @@ void enqueue_factor(fc_factor* F, const double* dval, i64 nnz, hipStream_t st, b
         break;
       case fc::K_POTRF:
         fc::k_potrf_blk<T><<<g, dim3(256), 0, ls>>>(it, P.fronts, L, F->diag, F->fail, Inv, Lh, Ll, fexp);
+        break;
+      case fc::K_SMALL:
+        fc::k_small_fronts<T><<<g, b, 0, ls>>>(it, P.fronts, L, U, F->diag, F->fail, Inv);
         break;
       default:
         throw fc::Failure{fc::INTERNAL, -1, 0, 0, "unknown launch"};

@@ -97,7 +97,9 @@ enum {
   FC_KIND_TRSM = 4,     /* rows below a panel against its diagonal block                     */
   FC_KIND_SYRK = 5,     /* update matrices U = -L21 L21^T                                    */
   FC_KIND_REDUCE = 6,   /* sums of the pieces of a GEMM split along k                        */
-  FC_KIND_SPLIT = 7,    /* FC_MIXED: a finished panel's scaled BF16 (hi, lo) copies          */
+  FC_KIND_SMALL = 7,    /* small fronts (<= 64 columns, <= 768 rows below, an update matrix  */
+                        /* of <= 2e6 multiply-adds): diagonal block, rows below and update   */
+                        /* matrix of each by one workgroup, one launch a level              */
   FC_KINDS = 8
 };
 

@@ -140,21 +140,29 @@ def test_allocation_failure_is_the_factors_lack_of_memory(gpu, cube16, monkeypat
 
 
 def test_mixed_form_falls_back_instead_of_failing_an_spd_matrix(gpu):
-    """The single-product BF16 form loses definiteness on the slender cantilever (condition
-    number ~1e9). With the fallback (the default) the factorization moves to FP32 and the
-    refined solve reaches berr <= 1e-12; without it, NotPositiveDefinite as before."""
-    p = P.cantilever_hex8()
-    up = p.upper()
-    sym = CH.Symbolic(CH.UpperColumns(p.n, *up))
+    """The single-product BF16 form (relative error about 4e-3 a product) loses definiteness on
+    a dense Gram matrix B B^T / 199 + 1e-5 I of 200 equations (one front of two panels, whose
+    second panel's update is a BF16 product; smallest eigenvalue 1e-5). With the fallback (the
+    default) the factorization moves to FP32 and the refined solve reaches berr <= 1e-12;
+    without it, NotPositiveDefinite."""
+    from matrices import dense_to_csr, upper
+    rng = np.random.default_rng(3)
+    b = rng.standard_normal((200, 199))
+    a = b @ b.T / 199 + 1e-5 * np.eye(200)
+    n, ptr, idx, val = dense_to_csr(a)
+    up = upper(n, ptr, idx, val)
+    sym = CH.Symbolic(CH.UpperColumns(n, *up))
+    assert sym.info()["kind_flops"]["small"] == 0  # the tiled launches, BF16 products included
     with pytest.raises(CH.NotPositiveDefinite):
         CH.Factor(sym, up[2], precision="mixed", bf16_terms=1, fallback=1)
     f = CH.Factor(sym, up[2], precision="mixed", bf16_terms=1)
     assert f.precision_used() == "fp32"
-    A = CH.Matrix(p.n, p.ptr, p.idx, p.kval)
-    x, rep = f.solve_refined(A, p.rhs, target_berr=1e-12)
+    A = CH.Matrix(n, ptr, idx, val)
+    rhs = np.cos(np.arange(n) * 0.1)
+    x, rep = f.solve_refined(A, rhs, target_berr=1e-12)
     assert rep["converged"], rep
-    r = p.mul(x) - p.rhs
-    assert np.abs(r).max() / (p.norm_inf() * np.abs(x).max() + np.abs(p.rhs).max()) <= 1e-12
+    r = a @ x - rhs
+    assert np.abs(r).max() / (np.abs(a).sum(axis=1).max() * np.abs(x).max() + np.abs(rhs).max()) <= 1e-12
 
 
 def nearly_singular_pairs(p, delta=1e-9, pairs=3):
