@@ -561,8 +561,11 @@ __device__ __forceinline__ float readlane_f(float v, int l) {
   return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
 }
 
-template <int CB>
-__device__ __forceinline__ void wpi32_steps(float (&a)[8][8], float (&x)[8][8], int h, int lane, float dl, bool& fb) {
+// NBK x NBK register blocks a lane (the wave covers blocks of up to 8 NBK columns): 8 for the
+// 64-column halves, 4 for 32-column quarters, whose steps touch a quarter of the registers.
+template <int CB, int NBK>
+__device__ __forceinline__ void wpi32_steps(float (&a)[NBK][NBK], float (&x)[NBK][NBK], int h, int lane, float dl,
+                                            bool& fb) {
   const int ti = lane & 7, tk = lane >> 3;
   for (int cc = 0; cc < 8; ++cc) {
     const int j = 8 * CB + cc;
@@ -574,20 +577,20 @@ __device__ __forceinline__ void wpi32_steps(float (&a)[8][8], float (&x)[8][8], 
     const float rd = y * fmaf(-0.5f * p * y, y, 1.5f);  // 1 / sqrt(p), one Newton step
     const float fc = tk == cc ? rd : 1.0f, fr = ti == cc ? rd : 1.0f;
 #pragma unroll
-    for (int r = CB; r < 8; ++r) a[r][CB] *= fc;  // column j of L (the pivot becomes p rd = L(j, j))
+    for (int r = CB; r < NBK; ++r) a[r][CB] *= fc;  // column j of L (the pivot becomes p rd = L(j, j))
 #pragma unroll
     for (int c = 0; c <= CB; ++c) x[CB][c] *= fr;  // row j of X divided by L(j, j)
-    float li[8], lk[8], xj[8];
+    float li[NBK], lk[NBK], xj[NBK];
 #pragma unroll
-    for (int r = CB; r < 8; ++r) li[r] = __shfl(a[r][CB], ti + 8 * cc);  // L(ti + 8r, j)
+    for (int r = CB; r < NBK; ++r) li[r] = __shfl(a[r][CB], ti + 8 * cc);  // L(ti + 8r, j)
 #pragma unroll
-    for (int c = CB; c < 8; ++c) lk[c] = __shfl(a[c][CB], tk + 8 * cc);  // L(tk + 8c, j)
+    for (int c = CB; c < NBK; ++c) lk[c] = __shfl(a[c][CB], tk + 8 * cc);  // L(tk + 8c, j)
 #pragma unroll
     for (int c = 0; c <= CB; ++c) xj[c] = __shfl(x[CB][c], cc + 8 * tk);  // X(j, tk + 8c)
     li[CB] = ti > cc ? li[CB] : 0.0f;
     lk[CB] = tk > cc ? lk[CB] : 0.0f;
 #pragma unroll
-    for (int r = CB; r < 8; ++r) {
+    for (int r = CB; r < NBK; ++r) {
 #pragma unroll
       for (int c = CB; c <= r; ++c) a[r][c] = fmaf(-li[r], lk[c], a[r][c]);  // L(i, k) -= L(i, j) L(k, j)
 #pragma unroll
@@ -596,32 +599,35 @@ __device__ __forceinline__ void wpi32_steps(float (&a)[8][8], float (&x)[8][8], 
   }
 }
 
+template <int NBK = 8>
 __device__ inline void wave_potrf_inv32(double (*S)[NB + 1], double* xd, int o, int h, int lane,
                                         const double* __restrict__ dg, bool& fb) {
   const int ti = lane & 7, tk = lane >> 3;
   const float dl = lane < h ? (float)fabs(dg[lane]) : 0.0f;
-  float a[8][8], x[8][8];
+  float a[NBK][NBK], x[NBK][NBK];
 #pragma unroll
-  for (int r = 0; r < 8; ++r)
+  for (int r = 0; r < NBK; ++r)
 #pragma unroll
-    for (int c = 0; c < 8; ++c) {
+    for (int c = 0; c < NBK; ++c) {
       const int i = ti + 8 * r, k = tk + 8 * c;
-      const float v = (float)S[o + i][o + k];  // in bounds for every lane (o + 64 <= NB)
+      const float v = (float)S[o + i][o + k];  // in bounds for every lane (o + 8 NBK <= NB)
       a[r][c] = (i < h && k < h && i >= k) ? v : (i == k ? 1.0f : 0.0f);
       x[r][c] = i == k ? 1.0f : 0.0f;
     }
-  wpi32_steps<0>(a, x, h, lane, dl, fb);
-  wpi32_steps<1>(a, x, h, lane, dl, fb);
-  wpi32_steps<2>(a, x, h, lane, dl, fb);
-  wpi32_steps<3>(a, x, h, lane, dl, fb);
-  wpi32_steps<4>(a, x, h, lane, dl, fb);
-  wpi32_steps<5>(a, x, h, lane, dl, fb);
-  wpi32_steps<6>(a, x, h, lane, dl, fb);
-  wpi32_steps<7>(a, x, h, lane, dl, fb);
+  wpi32_steps<0, NBK>(a, x, h, lane, dl, fb);
+  wpi32_steps<1, NBK>(a, x, h, lane, dl, fb);
+  wpi32_steps<2, NBK>(a, x, h, lane, dl, fb);
+  wpi32_steps<3, NBK>(a, x, h, lane, dl, fb);
+  if constexpr (NBK > 4) {
+    wpi32_steps<4, NBK>(a, x, h, lane, dl, fb);
+    wpi32_steps<5, NBK>(a, x, h, lane, dl, fb);
+    wpi32_steps<6, NBK>(a, x, h, lane, dl, fb);
+    wpi32_steps<7, NBK>(a, x, h, lane, dl, fb);
+  }
 #pragma unroll
-  for (int r = 0; r < 8; ++r)
+  for (int r = 0; r < NBK; ++r)
 #pragma unroll
-    for (int c = 0; c < 8; ++c) {
+    for (int c = 0; c < NBK; ++c) {
       const int i = ti + 8 * r, k = tk + 8 * c;
       if (i < h && k < h) {
         if (i >= k) S[o + i][o + k] = a[r][c];
@@ -629,6 +635,87 @@ __device__ inline void wave_potrf_inv32(double (*S)[NB + 1], double* xd, int o, 
         if (i == k) xd[o + i] = x[r][c];
       }
     }
+}
+
+// One wave's 16 x 16 tile of a product over k in [0, 32) on the FP64 MFMA:
+//   acc[v] = sum_k col(t, k) row(i, k),  i = lane % 16,  t = lane / 16 + 4 v
+// where col(t, k) and row(i, k) return in-bounds LDS values for t, i in [0, 16).
+template <class FC, class FR>
+__device__ __forceinline__ v4d tile16(int lane, FC col, FR row) {
+  const int r = lane & 15, q = lane >> 4;
+  v4d acc = {};
+#pragma unroll
+  for (int k0 = 0; k0 < 32; k0 += 4) acc = Mma<double>::step(col(r, k0 + q), row(r, k0 + q), acc);
+  return acc;
+}
+
+// The FP32 attempt on the h x h block at S[o.., o..] (h <= 64) by the whole workgroup, with the
+// layout wave_potrf_inv32 leaves: quarters of 32 columns, each factored and inverted inside one
+// wave (wave_potrf_inv32<4>, whose steps cost under half the 64-column ones'), the products
+// between them on the FP64 MFMA, a 16 x 16 tile a wave (tile rows 16 (w % 2), columns
+// 16 (w / 2)), operands from LDS, triangular ones as 0/1 weights:
+//   L21 = A21 X11^T, A22 -= L21 L21^T, (L22, X22) in one wave, T = L21 X11, X21 = -X22 T.
+// T is 32 x 32 of LDS. Every thread calls it; it ends on a barrier.
+__device__ inline void half_potrf_inv32(double (*S)[NB + 1], double* xd, double (*T)[33], int o, int h, int tid,
+                                        const double* __restrict__ dg, bool& fb) {
+  const int lane = tid & 63, wave = tid >> 6;
+  const int h0 = min(32, h), h1 = h - h0, p = o + 32;
+  if (wave == 0) wave_potrf_inv32<4>(S, xd, o, h0, lane, dg, fb);
+  __syncthreads();
+  if (h1 <= 0) return;  // (uniform) h0 = 32 below
+  const int bi = 16 * (wave & 1), bt = 16 * (wave >> 1);  // the wave's tile
+  const int ri = lane & 15;
+  auto rowq = [&](int i) { return p + min(bi + i, h1 - 1); };  // a row of the second quarter, clamped
+  {  // L21(i, t) = sum_{k <= t} A21(i, k) X11(t, k); X11(t, k) = S[o + k][o + t] (t > k), xd[o + t]
+    const v4d acc = tile16(
+        lane,
+        [&](int t, int k) {
+          const int c = bt + t;
+          return S[o + k][o + c] * (c > k ? 1.0 : 0.0) + xd[o + c] * (c == k ? 1.0 : 0.0);
+        },
+        [&](int i, int k) { return S[rowq(i)][o + k]; });
+    __syncthreads();  // every A21 read before L21 is stored over it
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+      if (bi + ri < h1) S[p + bi + ri][o + bt + (lane >> 4) + 4 * v] = acc[v];
+  }
+  __syncthreads();
+  if (bt <= bi) {  // A22(i, j) -= sum_k L21(i, k) L21(j, k), j <= i (reads the left quarter, writes the right)
+    const v4d acc = tile16(
+        lane, [&](int t, int k) { return S[p + min(bt + t, h1 - 1)][o + k]; }, [&](int i, int k) { return S[rowq(i)][o + k]; });
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int i = bi + ri, j = bt + (lane >> 4) + 4 * v;
+      if (i < h1 && j <= i) S[p + i][p + j] -= acc[v];
+    }
+  }
+  __syncthreads();
+  if (wave == 0) wave_potrf_inv32<4>(S, xd, p, h1, lane, dg + 32, fb);
+  {  // T(i, c) = sum_{k >= c} L21(i, k) X11(k, c); X11(k, c) = S[o + c][o + k] (k > c), xd[o + c]
+    const v4d acc = tile16(
+        lane,
+        [&](int t, int k) {
+          const int c = bt + t;
+          return S[o + c][o + k] * (k > c ? 1.0 : 0.0) + xd[o + c] * (k == c ? 1.0 : 0.0);
+        },
+        [&](int i, int k) { return S[rowq(i)][o + k]; });
+#pragma unroll
+    for (int v = 0; v < 4; ++v) T[bi + ri][bt + (lane >> 4) + 4 * v] = acc[v];
+  }
+  __syncthreads();  // T, and the second quarter's factor and inverse, done
+  {  // X21(i, c) = -sum_{k <= i} X22(i, k) T(k, c); X22(i, k) = S[p + k][p + i] (i > k), xd[p + i];
+     // stored at S[o + c][p + i]. T's rows past h1 hold finite values (clamped rows) times 0
+    const v4d acc = tile16(
+        lane, [&](int t, int k) { return T[k][bt + t]; },
+        [&](int i, int k) {
+          const int ii = bi + i;
+          return S[p + k][p + ii] * (ii > k ? 1.0 : 0.0) + xd[p + ii] * (ii == k ? 1.0 : 0.0);
+        });
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+      if (bi + ri < h1) S[o + bt + (lane >> 4) + 4 * v][p + bi + ri] = -acc[v];
+  }
+  __syncthreads();
 }
 
 // One wave's 64 x 16 strip of a product over k in [0, 64) on the FP64 MFMA:
@@ -660,6 +747,7 @@ __global__ __launch_bounds__(256, 1) void k_potrf_blk(const Item* __restrict__ i
   __shared__ double S[NB][NB + 1];
   __shared__ double xd[NB];
   __shared__ int redo;  // the FP32 attempt met a small pivot: factor again in FP64
+  __shared__ double Tq[32][33];  // the FP32 attempt's quarter products (half_potrf_inv32)
   const Item it = items[blockIdx.x];
   const FrontDev F = fronts[it.f];
   const int c0 = it.a, pw = min(NB, F.w - c0);
@@ -692,16 +780,22 @@ __global__ __launch_bounds__(256, 1) void k_potrf_blk(const Item* __restrict__ i
       *(v8bf*)(Ll + o) = l;
     }
   };
-  // the pw x pw square (its upper triangle is zeros in L, in bounds): 64 independent loads.
+  // the pw x pw square (its upper triangle is zeros in L, in bounds): 64 independent loads,
+  // through a buffer resource (the block spans far below 2 GiB of L): an entry outside the square
+  // takes the lane offset 2^31, past the resource's 2^31 - 1 bytes, and reads 0 from the buffer
+  // unit's range check, with no branch (a branch around each load made the compiler wait for
+  // each one in turn: 64 round trips, 28 us of a 73 us block).
   // xd beyond the block's width is read by the products below with a 0 weight: it must hold a
   // finite value, not what an earlier kernel left in LDS (a NaN there times 0 is a NaN)
+  const auto rs = raw_rsrc(base);
+  const uint32_t ldb = (uint32_t)F.ld * sizeof(T);
   auto load = [&]() {
     if (tid < NB) xd[tid] = 0.0;
     T v[NB / 2];
 #pragma unroll
     for (int q = 0; q < NB / 2; ++q) {
       const int j = jp + 2 * q;
-      v[q] = (rr < pw && j < pw) ? base[(int64_t)j * F.ld + rr] : T(0);
+      v[q] = buf_load<T>(rs, (rr < pw && j < pw) ? (uint32_t)(rr * sizeof(T)) : 0x80000000u, j * ldb);
     }
 #pragma unroll
     for (int q = 0; q < NB / 2; ++q) S[rr][jp + 2 * q] = (double)v[q];
@@ -709,16 +803,14 @@ __global__ __launch_bounds__(256, 1) void k_potrf_blk(const Item* __restrict__ i
   // each wave: strip columns t0 + [0, 16) of a 64 x 64 product, rows 16 ib + lane % 16
   const int r = lane & 15, q = lane >> 4, t0 = 16 * wave;
   // L11 and X11; for a wide panel then L21 = A21 X11^T, A22 -= L21 L21^T, L22 and X22. F32:
-  // the wave steps in FP32 arithmetic (a small pivot sets redo); otherwise FP64 with the
-  // reference's pivot test.
+  // each half by the whole workgroup in 32-column quarters, the wave steps in FP32 arithmetic
+  // (a small pivot sets redo); otherwise one wave a half, FP64 with the reference's pivot test.
   bool bad = false;
   auto factor = [&](auto f32) {
     constexpr bool F32 = decltype(f32)::value;
     bool fb = false;
-    if (wave == 0) {
-      if constexpr (F32) wave_potrf_inv32(S, xd, 0, h0, lane, dg, fb);
-      else wave_potrf_inv<sizeof(T) == 8>(S, xd, 0, h0, lane, dg, col, fail, bad);
-    }
+    if constexpr (F32) half_potrf_inv32(S, xd, Tq, 0, h0, tid, dg, fb);
+    else if (wave == 0) wave_potrf_inv<sizeof(T) == 8>(S, xd, 0, h0, lane, dg, col, fail, bad);
     __syncthreads();
     if (h1 > 0) {
       {  // L21 = A21 X11^T: L21(i, t) = sum_{k <= t} A21(i, k) X11(t, k), X11(t, k) = S[k][t] (t > k)
@@ -754,10 +846,8 @@ __global__ __launch_bounds__(256, 1) void k_potrf_blk(const Item* __restrict__ i
           }
       }
       __syncthreads();
-      if (wave == 0) {
-        if constexpr (F32) wave_potrf_inv32(S, xd, h0, h1, lane, dg + h0, fb);
-        else wave_potrf_inv<sizeof(T) == 8>(S, xd, h0, h1, lane, dg + h0, col + h0, fail, bad);
-      }
+      if constexpr (F32) half_potrf_inv32(S, xd, Tq, h0, h1, tid, dg + h0, fb);
+      else if (wave == 0) wave_potrf_inv<sizeof(T) == 8>(S, xd, h0, h1, lane, dg + h0, col + h0, fail, bad);
     }
     if constexpr (F32) {
       if (wave == 0 && lane == 0) redo = fb ? 1 : 0;

@@ -292,6 +292,12 @@ bool is_small(const fc::Symbolic& s, i64 f) {
 Program build_program(const fc::Symbolic& s, const Schedule& sched) {
   using fc::Item;
   using fc::Launch;
+  // The diagonal-block kernels read a panel's columns through one buffer resource, at byte
+  // offsets below 2^31: a front of 2^21 rows or more is refused (the same for every schedule,
+  // so only the analysis's first program can meet it).
+  for (i64 f = 0; f < s.nf; ++f)
+    if (s.f_ld[f] >= (i64(1) << 21))
+      throw fc::Failure{fc::TOO_LARGE, -1, s.f_ld[f], (i64(1) << 21) - 1, "a front of 2^21 rows or more"};
   Program p;
   p.sched = sched;
   double flops = 0, entries = 0;

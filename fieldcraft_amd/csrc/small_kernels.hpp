@@ -110,14 +110,17 @@ __global__ __launch_bounds__(256, 2) void k_small_fronts(const Item* __restrict_
   const int64_t col = F.col0;
   // 1. the diagonal block: every entry of S's first 64 x 64 written (zeros outside the lower
   // triangle of the w x w block: the row products read them past w), and xd cleared
-  // (thread t: row t % 64, columns t / 64 + 4 q; the 16 loads issued before any is used)
+  // (thread t: row t % 64, columns t / 64 + 4 q; the 16 loads issued before any is used, those
+  // outside the triangle reading 0 through the buffer unit's range check, as in small_rows)
+  const auto rs = raw_rsrc(base);
+  const uint32_t ldb = (uint32_t)F.ld * sizeof(T);
   auto load = [&]() {
     const int i = tid & (SMALL_W - 1), j0 = tid >> 6;
     T v[SMALL_W / 4];
 #pragma unroll
     for (int q = 0; q < SMALL_W / 4; ++q) {
       const int j = j0 + 4 * q;
-      v[q] = (i < w && j <= i) ? base[(int64_t)j * F.ld + i] : T(0);
+      v[q] = buf_load<T>(rs, (i < w && j <= i) ? (uint32_t)(i * sizeof(T)) : 0x80000000u, j * ldb);
     }
 #pragma unroll
     for (int q = 0; q < SMALL_W / 4; ++q) S[i][j0 + 4 * q] = (double)v[q];
@@ -213,14 +216,14 @@ __global__ __launch_bounds__(256, 2) void k_small_fronts(const Item* __restrict_
   double* As = &S[0][0];
   double* Bs = As + SMALL_W * SMALL_W;
   // rows r0 + t % 64 of columns t / 64 + 4 q (q < 16, those below w) into D[k][r], rows past mu
-  // as 0; the loads issued before any is used
+  // as 0 (read through the range check); the loads issued before any is used
   auto stage = [&](double* D, int r0) {
     const int r = tid & (SMALL_W - 1), k0 = tid >> 6;
     T v[SMALL_W / 4];
 #pragma unroll
     for (int q = 0; q < SMALL_W / 4; ++q) {
       const int k = k0 + 4 * q;
-      v[q] = (k < w && r0 + r < mu) ? l21[(int64_t)k * F.ld + r0 + r] : T(0);
+      v[q] = buf_load<T>(rs, (k < w && r0 + r < mu) ? (uint32_t)((w + r0 + r) * sizeof(T)) : 0x80000000u, k * ldb);
     }
 #pragma unroll
     for (int q = 0; q < SMALL_W / 4; ++q)
