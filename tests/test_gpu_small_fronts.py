@@ -86,3 +86,25 @@ def test_a_failing_pivot_in_a_small_front_is_named(gpu, precision):
     with pytest.raises(CH.NotPositiveDefinite) as g:
         CH.Factor(sym, up[2], precision=precision)
     assert g.value.column == o.value.column
+
+
+@pytest.mark.parametrize("n", [65, 96, 97, 129, 160, 168, 200, 250])
+@pytest.mark.parametrize("precision", ["fp32", "mixed"])
+def test_fp32_diagonal_blocks_of_every_quarter_shape(gpu, n, precision):
+    """A dense front of n columns (wider than a small front) factored by the tiled kernels: its
+    128-column panels' FP32 diagonal blocks go in 32-column quarters, and these widths give a
+    last panel of 1, 32, 33, 40, 72, 122 columns and halves of every shape (a lone quarter, a
+    quarter and a part, two whole ones). Every value within 2e-6 of the oracle's factor of the
+    FP32-rounded matrix (normwise; 1e-4 for the mixed form, whose panel updates are BF16 split
+    products), and its inverse blocks right: the refined solve reaches berr <= 1e-12. A wrong
+    quarter product would be off by O(1)."""
+    a = random_spd(n, 1.0, 100 + n)
+    sym, _, f, (nn, ptr, idx, val) = factor_both(a, precision)
+    assert sym.info()["kind_flops"]["small"] == 0
+    ref32 = O.Factor(nn, *upper(nn, ptr, idx, val.astype(np.float32).astype(np.float64)))
+    assert rel(f.values(), ref32.values()) <= (2e-6 if precision == "fp32" else 1e-4)
+    A = CH.Matrix(nn, ptr, idx, val)
+    b = np.sin(np.arange(nn) * 0.9)
+    x, rep = f.solve_refined(A, b, target_berr=1e-12)
+    assert rep["converged"], rep
+    assert rel(x, np.linalg.solve(a, b)) <= 1e-10
