@@ -149,22 +149,38 @@ __global__ __launch_bounds__(256, 2) void k_gemm(const GemmItem* __restrict__ it
     return;
   }
   T* C = base(it.csrc) + it.c;
+  // By row blocks: G_SUB and G_ADD read the block's 16 old values first, every load in flight at
+  // once, through a buffer resource (a tile spans far below 2 GiB of its array; an entry outside
+  // the tile reads 0 from the range check and is not stored). An update and its load in one
+  // branch made the compiler wait for each load in turn: 64 round trips a tile.
+  const bool rmw = it.mode == G_SUB || it.mode == G_ADD;
+  const auto rc = raw_rsrc(C);
+  const uint32_t ldcb = (uint32_t)it.ldc * sizeof(T);
 #pragma unroll
-  for (int bi = 0; bi < 4; ++bi)
+  for (int bi = 0; bi < 4; ++bi) {
+    T old[4][4] = {};
+    if (rmw) {
+#pragma unroll
+      for (int bj = 0; bj < 4; ++bj)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int i = wi + 16 * bi + il, j = wj + 16 * bj + Mma<T>::row(lane, v);
+          const bool in = i < it.m && j < it.n && i - j + it.diag >= 0;
+          old[bj][v] = buf_load<T>(rc, in ? (uint32_t)(i * sizeof(T)) : 0x80000000u, (uint32_t)j * ldcb);
+        }
+    }
 #pragma unroll
     for (int bj = 0; bj < 4; ++bj)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int i = wi + 16 * bi + il, j = wj + 16 * bj + Mma<T>::row(lane, v);
         if (i < it.m && j < it.n && i - j + it.diag >= 0) {
-          T* p = C + (int64_t)j * it.ldc + i;
           const T x = acc[bi][bj][v];
-          if (it.mode == G_SUB) *p -= x;
-          else if (it.mode == G_NEG) *p = -x;
-          else if (it.mode == G_ADD) *p += x;
-          else *p = x;
+          C[(int64_t)j * it.ldc + i] =
+              it.mode == G_SUB ? old[bj][v] - x : (it.mode == G_ADD ? old[bj][v] + x : (it.mode == G_NEG ? -x : x));
         }
       }
+  }
   if constexpr (SPLIT) {
     static_assert(sizeof(T) == 4, "copies of an FP32 result");
     // a lane's four values of a block are consecutive columns of one row: one 8-byte store a
@@ -360,21 +376,33 @@ __global__ __launch_bounds__(256, 2) void k_gemm_b16(const GemmItem* __restrict_
     return;
   }
   float* C = (it.csrc == S_L ? L : (it.csrc == S_U ? U : Part)) + it.c;
+  // by row blocks, G_SUB's 16 old values of a block loaded first (as in k_gemm)
+  const auto rc = raw_rsrc(C);
+  const uint32_t ldcb = (uint32_t)it.ldc * sizeof(float);
 #pragma unroll
-  for (int bi = 0; bi < 4; ++bi)
+  for (int bi = 0; bi < 4; ++bi) {
+    float old[4][4] = {};
+    if (it.mode == G_SUB) {
+#pragma unroll
+      for (int bj = 0; bj < 4; ++bj)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int i = wi + 16 * bi + il, j = wj + 16 * bj + Mma<float>::row(lane, v);
+          const bool in = i < it.m && j < it.n && i - j + it.diag >= 0;
+          old[bj][v] = buf_load<float>(rc, in ? (uint32_t)(i * sizeof(float)) : 0x80000000u, (uint32_t)j * ldcb);
+        }
+    }
 #pragma unroll
     for (int bj = 0; bj < 4; ++bj)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int i = wi + 16 * bi + il, j = wj + 16 * bj + Mma<float>::row(lane, v);
         if (i < it.m && j < it.n && i - j + it.diag >= 0) {
-          float* p = C + (int64_t)j * it.ldc + i;
           const float x = acc[bi][bj][v] * unscale;
-          if (it.mode == G_SUB) *p -= x;
-          else if (it.mode == G_NEG) *p = -x;
-          else *p = x;
+          C[(int64_t)j * it.ldc + i] = it.mode == G_SUB ? old[bj][v] - x : (it.mode == G_NEG ? -x : x);
         }
       }
+  }
 }
 
 // Sum of the partial tiles of a split product, in order, applied to C as its mode says. Each
