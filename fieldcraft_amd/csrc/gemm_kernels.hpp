@@ -419,11 +419,20 @@ __global__ __launch_bounds__(256) void k_reduce_parts(const ReduceItem* __restri
   for (int e = e0 + threadIdx.x; e < e0 + CHUNK; e += blockDim.x) {
     const int i = e % GB, j = e / GB;
     if (i >= it.m || j >= it.n || i - j + it.diag < 0) continue;
-    T s = 0;
-    for (int q = 0; q < it.parts; ++q) s += Part[it.part0 + (int64_t)q * GB * GB + e];
     T* p = C + (int64_t)j * it.ldc + i;
-    if (it.mode == G_SUB) *p -= s;
-    else *p = -s;
+    const T old = it.mode == G_SUB ? *p : T(0);
+    // the pieces in order, eight loads in flight at a time (one round trip per piece otherwise)
+    T s = 0;
+    int q = 0;
+    for (; q + 8 <= it.parts; q += 8) {
+      T v[8];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) v[t] = Part[it.part0 + (int64_t)(q + t) * GB * GB + e];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) s += v[t];
+    }
+    for (; q < it.parts; ++q) s += Part[it.part0 + (int64_t)q * GB * GB + e];
+    *p = it.mode == G_SUB ? old - s : -s;
   }
 }
 
