@@ -481,8 +481,8 @@ __device__ __forceinline__ double readlane_d(double v, int l) {
 // Otherwise (FP32 storage) the step takes rd from the hardware reciprocal square root with two
 // Newton steps and multiplies: the values are rounded to FP32 afterwards, and the step's
 // dependent chain (the part that sets this kernel's time) is shorter.
-template <int CB, bool EXACT>
-__device__ __forceinline__ void wpi_steps(double (&a)[8][8], double (&x)[8][8], int h, int lane, double dl,
+template <int CB, bool EXACT, int NBK>
+__device__ __forceinline__ void wpi_steps(double (&a)[NBK][NBK], double (&x)[NBK][NBK], int h, int lane, double dl,
                                           int64_t col, unsigned long long* __restrict__ fail, bool& bad) {
   const int ti = lane & 7, tk = lane >> 3;
   for (int cc = 0; cc < 8; ++cc) {
@@ -507,7 +507,7 @@ __device__ __forceinline__ void wpi_steps(double (&a)[8][8], double (&x)[8][8], 
     {  // column j of L: rows below scaled, the pivot replaced by d (lanes tk == cc own it)
       const bool own = tk == cc;
 #pragma unroll
-      for (int r = CB; r < 8; ++r) {
+      for (int r = CB; r < NBK; ++r) {
         const bool below = r > CB || ti > cc, diag = r == CB && ti == cc;
         const double sc = EXACT ? a[r][CB] / d : a[r][CB] * rd;
         a[r][CB] = own ? (below ? sc : (diag ? d : a[r][CB])) : a[r][CB];
@@ -518,16 +518,16 @@ __device__ __forceinline__ void wpi_steps(double (&a)[8][8], double (&x)[8][8], 
 #pragma unroll
       for (int c = 0; c <= CB; ++c) x[CB][c] = own ? x[CB][c] * rd : x[CB][c];
     }
-    double li[8], lk[8], xj[8];
+    double li[NBK], lk[NBK], xj[NBK];
 #pragma unroll
-    for (int r = CB; r < 8; ++r) li[r] = __shfl(a[r][CB], ti + 8 * cc);  // L(ti + 8r, j)
+    for (int r = CB; r < NBK; ++r) li[r] = __shfl(a[r][CB], ti + 8 * cc);  // L(ti + 8r, j)
 #pragma unroll
-    for (int c = CB; c < 8; ++c) lk[c] = __shfl(a[c][CB], tk + 8 * cc);  // L(tk + 8c, j)
+    for (int c = CB; c < NBK; ++c) lk[c] = __shfl(a[c][CB], tk + 8 * cc);  // L(tk + 8c, j)
 #pragma unroll
     for (int c = 0; c <= CB; ++c) xj[c] = __shfl(x[CB][c], cc + 8 * tk);  // X(j, tk + 8c)
     const bool kj = tk > cc, ik = ti >= tk, ij = ti > cc;
 #pragma unroll
-    for (int r = CB; r < 8; ++r) {
+    for (int r = CB; r < NBK; ++r) {
 #pragma unroll
       for (int c = CB; c <= r; ++c) {  // L(i, k) -= L(i, j) L(k, j) for i >= k > j
         const bool upd = (c > CB || kj) && (r > c || ik);
@@ -544,37 +544,40 @@ __device__ __forceinline__ void wpi_steps(double (&a)[8][8], double (&x)[8][8], 
   }
 }
 
-template <bool EXACT>
+// NBK x NBK register blocks a lane, as wave_potrf_inv32 (h <= 8 NBK).
+template <bool EXACT, int NBK = 8>
 __device__ inline void wave_potrf_inv(double (*S)[NB + 1], double* xd, int o, int h, int lane,
                                       const double* __restrict__ dg, int64_t col,
                                       unsigned long long* __restrict__ fail, bool& bad) {
   const int ti = lane & 7, tk = lane >> 3;
   // the reference diagonal of every column, read once: the pivot test then costs a shuffle
   const double dl = lane < h ? fabs(dg[lane]) : 0.0;
-  double a[8][8], x[8][8];
+  double a[NBK][NBK], x[NBK][NBK];
 #pragma unroll
-  for (int r = 0; r < 8; ++r)
+  for (int r = 0; r < NBK; ++r)
 #pragma unroll
-    for (int c = 0; c < 8; ++c) {
+    for (int c = 0; c < NBK; ++c) {
       const int i = ti + 8 * r, k = tk + 8 * c;
-      const double v = S[o + i][o + k];  // in bounds for every lane (o + 64 <= NB)
+      const double v = S[o + i][o + k];  // in bounds for every lane (o + 8 NBK <= NB)
       a[r][c] = (i < h && k < h && i >= k) ? v : (i == k ? 1.0 : 0.0);
       x[r][c] = i == k ? 1.0 : 0.0;
     }
-  wpi_steps<0, EXACT>(a, x, h, lane, dl, col, fail, bad);
-  wpi_steps<1, EXACT>(a, x, h, lane, dl, col, fail, bad);
-  wpi_steps<2, EXACT>(a, x, h, lane, dl, col, fail, bad);
-  wpi_steps<3, EXACT>(a, x, h, lane, dl, col, fail, bad);
-  wpi_steps<4, EXACT>(a, x, h, lane, dl, col, fail, bad);
-  wpi_steps<5, EXACT>(a, x, h, lane, dl, col, fail, bad);
-  wpi_steps<6, EXACT>(a, x, h, lane, dl, col, fail, bad);
-  wpi_steps<7, EXACT>(a, x, h, lane, dl, col, fail, bad);
+  wpi_steps<0, EXACT, NBK>(a, x, h, lane, dl, col, fail, bad);
+  wpi_steps<1, EXACT, NBK>(a, x, h, lane, dl, col, fail, bad);
+  wpi_steps<2, EXACT, NBK>(a, x, h, lane, dl, col, fail, bad);
+  wpi_steps<3, EXACT, NBK>(a, x, h, lane, dl, col, fail, bad);
+  if constexpr (NBK > 4) {
+    wpi_steps<4, EXACT, NBK>(a, x, h, lane, dl, col, fail, bad);
+    wpi_steps<5, EXACT, NBK>(a, x, h, lane, dl, col, fail, bad);
+    wpi_steps<6, EXACT, NBK>(a, x, h, lane, dl, col, fail, bad);
+    wpi_steps<7, EXACT, NBK>(a, x, h, lane, dl, col, fail, bad);
+  }
   // L below and on the diagonal, X above it (transposed), X's diagonal in xd; every lane
   // stores to in-bounds addresses, the entries outside the h x h block are not written
 #pragma unroll
-  for (int r = 0; r < 8; ++r)
+  for (int r = 0; r < NBK; ++r)
 #pragma unroll
-    for (int c = 0; c < 8; ++c) {
+    for (int c = 0; c < NBK; ++c) {
       const int i = ti + 8 * r, k = tk + 8 * c;
       if (i < h && k < h) {
         if (i >= k) S[o + i][o + k] = a[r][c];
@@ -686,18 +689,26 @@ __device__ __forceinline__ v4d tile16(int lane, FC col, FR row) {
   return acc;
 }
 
-// The FP32 attempt on the h x h block at S[o.., o..] (h <= 64) by the whole workgroup, with the
-// layout wave_potrf_inv32 leaves: quarters of 32 columns, each factored and inverted inside one
-// wave (wave_potrf_inv32<4>, whose steps cost under half the 64-column ones'), the products
+// The h x h block at S[o.., o..] (h <= 64) by the whole workgroup, with the layout the wave
+// functions leave: quarters of 32 columns, each factored and inverted inside one wave (4 x 4
+// register blocks a lane, whose steps cost under half the 64-column ones'), the products
 // between them on the FP64 MFMA, a 16 x 16 tile a wave (tile rows 16 (w % 2), columns
 // 16 (w / 2)), operands from LDS, triangular ones as 0/1 weights:
 //   L21 = A21 X11^T, A22 -= L21 L21^T, (L22, X22) in one wave, T = L21 X11, X21 = -X22 T.
 // T is 32 x 32 of LDS. Every thread calls it; it ends on a barrier.
-__device__ inline void half_potrf_inv32(double (*S)[NB + 1], double* xd, double (*T)[33], int o, int h, int tid,
-                                        const double* __restrict__ dg, bool& fb) {
+template <bool F32, bool EXACT>
+__device__ inline void half_potrf_inv(double (*S)[NB + 1], double* xd, double (*T)[33], int o, int h, int tid,
+                                      const double* __restrict__ dg, bool& fb, int64_t col,
+                                      unsigned long long* __restrict__ fail, bool& bad) {
   const int lane = tid & 63, wave = tid >> 6;
   const int h0 = min(32, h), h1 = h - h0, p = o + 32;
-  if (wave == 0) wave_potrf_inv32<4>(S, xd, o, h0, lane, dg, fb);
+  // one quarter's chain in wave 0: FP32 arithmetic (a small pivot sets fb), or FP64 with the
+  // reference's pivot test
+  auto chain = [&](int oq, int hq, int d) {
+    if constexpr (F32) wave_potrf_inv32<4>(S, xd, oq, hq, lane, dg + d, fb);
+    else wave_potrf_inv<EXACT, 4>(S, xd, oq, hq, lane, dg + d, col + d, fail, bad);
+  };
+  if (wave == 0) chain(o, h0, 0);
   __syncthreads();
   if (h1 <= 0) return;  // (uniform) h0 = 32 below
   const int bi = 16 * (wave & 1), bt = 16 * (wave >> 1);  // the wave's tile
@@ -727,7 +738,7 @@ __device__ inline void half_potrf_inv32(double (*S)[NB + 1], double* xd, double 
     }
   }
   __syncthreads();
-  if (wave == 0) wave_potrf_inv32<4>(S, xd, p, h1, lane, dg + 32, fb);
+  if (wave == 0) chain(p, h1, 32);
   {  // T(i, c) = sum_{k >= c} L21(i, k) X11(k, c); X11(k, c) = S[o + c][o + k] (k > c), xd[o + c]
     const v4d acc = tile16(
         lane,
@@ -784,7 +795,7 @@ __global__ __launch_bounds__(256, 1) void k_potrf_blk(const Item* __restrict__ i
   __shared__ double S[NB][NB + 1];
   __shared__ double xd[NB];
   __shared__ int redo;  // the FP32 attempt met a small pivot: factor again in FP64
-  __shared__ double Tq[32][33];  // the FP32 attempt's quarter products (half_potrf_inv32)
+  __shared__ double Tq[32][33];  // the quarter products' scratch (half_potrf_inv)
   const Item it = items[blockIdx.x];
   const FrontDev F = fronts[it.f];
   const int c0 = it.a, pw = min(NB, F.w - c0);
@@ -839,15 +850,14 @@ __global__ __launch_bounds__(256, 1) void k_potrf_blk(const Item* __restrict__ i
   };
   // each wave: strip columns t0 + [0, 16) of a 64 x 64 product, rows 16 ib + lane % 16
   const int r = lane & 15, q = lane >> 4, t0 = 16 * wave;
-  // L11 and X11; for a wide panel then L21 = A21 X11^T, A22 -= L21 L21^T, L22 and X22. F32:
-  // each half by the whole workgroup in 32-column quarters, the wave steps in FP32 arithmetic
-  // (a small pivot sets redo); otherwise one wave a half, FP64 with the reference's pivot test.
+  // L11 and X11; for a wide panel then L21 = A21 X11^T, A22 -= L21 L21^T, L22 and X22, each half
+  // by the whole workgroup in 32-column quarters. F32: the wave steps in FP32 arithmetic (a
+  // small pivot sets redo); otherwise FP64 with the reference's pivot test.
   bool bad = false;
   auto factor = [&](auto f32) {
     constexpr bool F32 = decltype(f32)::value;
     bool fb = false;
-    if constexpr (F32) half_potrf_inv32(S, xd, Tq, 0, h0, tid, dg, fb);
-    else if (wave == 0) wave_potrf_inv<sizeof(T) == 8>(S, xd, 0, h0, lane, dg, col, fail, bad);
+    half_potrf_inv<F32, sizeof(T) == 8>(S, xd, Tq, 0, h0, tid, dg, fb, col, fail, bad);
     __syncthreads();
     if (h1 > 0) {
       {  // L21 = A21 X11^T: L21(i, t) = sum_{k <= t} A21(i, k) X11(t, k), X11(t, k) = S[k][t] (t > k)
@@ -883,8 +893,7 @@ __global__ __launch_bounds__(256, 1) void k_potrf_blk(const Item* __restrict__ i
           }
       }
       __syncthreads();
-      if constexpr (F32) half_potrf_inv32(S, xd, Tq, h0, h1, tid, dg + h0, fb);
-      else if (wave == 0) wave_potrf_inv<sizeof(T) == 8>(S, xd, h0, h1, lane, dg + h0, col + h0, fail, bad);
+      half_potrf_inv<F32, sizeof(T) == 8>(S, xd, Tq, h0, h1, tid, dg + h0, fb, col + h0, fail, bad);
     }
     if constexpr (F32) {
       if (wave == 0 && lane == 0) redo = fb ? 1 : 0;
