@@ -6,9 +6,10 @@
 // The small fronts of a level (no wider than SMALL_W columns, their update matrices at most
 // SMALL_SYRK multiply-adds) take one launch between the level's two extend-add launches, in
 // which each workgroup does one front's dense steps:
-//   1. its diagonal block, factored and inverted by one wave as in k_potrf_blk (FP32 storage:
-//      FP32 arithmetic, the block redone in FP64 below the pivot threshold; FP64: the
-//      reference's pivot test), L11 and the inverse written out;
+//   1. its diagonal block, factored and inverted by one wave with the wave functions of
+//      k_potrf_blk (64-column register layout; FP32 storage: FP32 arithmetic, the block redone
+//      in FP64 below the pivot threshold; FP64: the reference's pivot test), L11 and the
+//      inverse written out;
 //   2. the rows below, X = B Linv^T with FP64 sums (the FP64 factor refines the product once,
 //      as the tiled path does);
 //   3. its update matrix U = -L21 L21^T (lower triangle), 64 x 64 tiles through LDS, FP64 sums.
