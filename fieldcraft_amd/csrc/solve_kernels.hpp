@@ -19,6 +19,8 @@
 // per R right-hand sides, with independent loads kept in flight.
 #pragma once
 
+#include <type_traits>
+
 #include "device.hpp"
 
 namespace fc {
@@ -847,15 +849,38 @@ __global__ __launch_bounds__(256) void k_spmm(int64_t n, const int64_t* __restri
                                               double* __restrict__ y, int k, int64_t ld) {
   const int sub = threadIdx.x & 7;
   for (int64_t r = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 3; r < n; r += ((int64_t)gridDim.x * blockDim.x) >> 3) {
+    const int64_t t0 = ptr[r], t1 = ptr[r + 1];
     for (int c0 = 0; c0 < k; c0 += 8) {
       double s[8] = {};
-      for (int64_t t = ptr[r] + sub; t < ptr[r + 1]; t += 8) {
-        const double v = val[t];
-        const int64_t j = idx[t];
+      // The lane's entries t, t + 8, t + 16, t + 24 at a time: their values and columns, then
+      // their x in every column of the chunk, all in flight together rather than each entry's
+      // loads waited on in turn. Entries past the row read its last entry and add nothing; the
+      // sums keep entry order. A chunk of eight columns takes no test per column.
+      auto sweep = [&](auto full) {
+        for (int64_t t = t0 + sub; t < t1; t += 32) {
+          double v[4];
+          int64_t j[4];
 #pragma unroll
-        for (int q = 0; q < 8; ++q)
-          if (c0 + q < k) s[q] += v * x[(int64_t)(c0 + q) * ld + j];
-      }
+          for (int u = 0; u < 4; ++u) {
+            const int64_t tt = min(t + 8 * u, t1 - 1);
+            v[u] = val[tt];
+            j[u] = idx[tt];
+          }
+#pragma unroll
+          for (int q = 0; q < 8; ++q)
+            if (decltype(full)::value || c0 + q < k) {
+              double xv[4];
+#pragma unroll
+              for (int u = 0; u < 4; ++u) xv[u] = x[(int64_t)(c0 + q) * ld + j[u]];
+#pragma unroll
+              for (int u = 0; u < 4; ++u) s[q] = t + 8 * u < t1 ? s[q] + v[u] * xv[u] : s[q];
+            }
+        }
+      };
+      if (c0 + 8 <= k)
+        sweep(std::true_type{});
+      else
+        sweep(std::false_type{});
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
         s[q] += __shfl_xor(s[q], 1);
