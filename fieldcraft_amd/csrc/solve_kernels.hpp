@@ -855,8 +855,9 @@ __global__ __launch_bounds__(256) void k_spmm(int64_t n, const int64_t* __restri
       // The lane's entries t, t + 8, t + 16, t + 24 at a time: their values and columns, then
       // their x in every column of the chunk, all in flight together rather than each entry's
       // loads waited on in turn. Entries past the row read its last entry and add nothing; the
-      // sums keep entry order. A chunk of eight columns takes no test per column.
-      auto sweep = [&](auto full) {
+      // sums keep entry order. The chunk's column count is a constant of the sweep, so no
+      // column takes a test of its own.
+      auto sweep = [&](auto nc) {
         for (int64_t t = t0 + sub; t < t1; t += 32) {
           double v[4];
           int64_t j[4];
@@ -868,7 +869,7 @@ __global__ __launch_bounds__(256) void k_spmm(int64_t n, const int64_t* __restri
           }
 #pragma unroll
           for (int q = 0; q < 8; ++q)
-            if (decltype(full)::value || c0 + q < k) {
+            if (q < decltype(nc)::value) {
               double xv[4];
 #pragma unroll
               for (int u = 0; u < 4; ++u) xv[u] = x[(int64_t)(c0 + q) * ld + j[u]];
@@ -877,10 +878,16 @@ __global__ __launch_bounds__(256) void k_spmm(int64_t n, const int64_t* __restri
             }
         }
       };
-      if (c0 + 8 <= k)
-        sweep(std::true_type{});
-      else
-        sweep(std::false_type{});
+      switch (min(k - c0, 8)) {
+        case 1: sweep(std::integral_constant<int, 1>{}); break;
+        case 2: sweep(std::integral_constant<int, 2>{}); break;
+        case 3: sweep(std::integral_constant<int, 3>{}); break;
+        case 4: sweep(std::integral_constant<int, 4>{}); break;
+        case 5: sweep(std::integral_constant<int, 5>{}); break;
+        case 6: sweep(std::integral_constant<int, 6>{}); break;
+        case 7: sweep(std::integral_constant<int, 7>{}); break;
+        default: sweep(std::integral_constant<int, 8>{}); break;
+      }
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
         s[q] += __shfl_xor(s[q], 1);
