@@ -213,17 +213,32 @@ __global__ __launch_bounds__(256, 2) void k_gemm(const GemmItem* __restrict__ it
   }
 }
 
-// A tile's entries C(i, j) = A(i, j) (i < m, j < n): the rows below a panel saved before the
-// product that overwrites them (the FP64 TRSM refines its result against them).
+// A tile's entries C(i, j) = A(i, j) (i < m <= GB, j < n <= GB): the rows below a panel saved
+// before the product that overwrites them (the FP64 TRSM refines its result against them).
+// Wave v copies columns v, v + 4, ...; lane l rows l and l + 64 of each, so every wave load and
+// store covers 64 consecutive entries of one column. CB columns' loads are issued before any
+// store (rows and columns past the tile load its last entry, which is not stored).
 template <typename T>
 __global__ __launch_bounds__(256) void k_copy_tiles(const GemmItem* __restrict__ items, const T* __restrict__ L,
                                                     T* __restrict__ W) {
+  constexpr int CB = 8;
+  static_assert(GB <= 128, "two rows a lane cover a tile's rows");
   const GemmItem it = items[blockIdx.x];
   const T* A = L + it.a;
   T* C = W + it.c;
-  for (int e = threadIdx.x; e < it.m * it.n; e += blockDim.x) {
-    const int i = e % it.m, j = e / it.m;
-    C[(int64_t)j * it.ldc + i] = A[(int64_t)j * it.lda + i];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int j0 = wv; j0 < it.n; j0 += 4 * CB) {
+    T v[CB][2];
+#pragma unroll
+    for (int c = 0; c < CB; ++c)
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+        v[c][h] = A[(int64_t)min(j0 + 4 * c, it.n - 1) * it.lda + min(lane + 64 * h, it.m - 1)];
+#pragma unroll
+    for (int c = 0; c < CB; ++c)
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+        if (j0 + 4 * c < it.n && lane + 64 * h < it.m) C[(int64_t)(j0 + 4 * c) * it.ldc + lane + 64 * h] = v[c][h];
   }
 }
 
